@@ -1,0 +1,216 @@
+/*
+ * libatssim_hip.h -- C ABI of the MI355X-native VBD cloth backend.
+ *
+ * This is the GPU device slot of the reference's heterogeneous task
+ * scheduler (ChengzhuUwU/libAtsSim).  In the reference the slot is the
+ * lambda registered as `Launcher::Implementation(DeviceTypeGpu, ...)`
+ * (src/heterogeneous/launcher.h:556-579) that forwards each scheduled task to
+ * `GpuSolver::fn_dispatch(const Launcher::LaunchParam&)`
+ * (examples/example3_hybrid_cpu_gpu.cpp:1744-1920), which encodes Metal
+ * kernels into a command list.  Here the same tasks are HIP kernels on one
+ * MI355X (gfx950), reached through plain C entry points: plain pointers and
+ * sizes, integer status codes, a thread-local last-error string.
+ *
+ * All positions/velocities crossing this boundary are float[3 * num_verts]
+ * in vertex-id order (the reference's AoS Float3 with 12-byte stride).
+ * Device pointers are only exposed by the lats_vbd_device_* accessors.
+ */
+#ifndef LIBATSSIM_HIP_H
+#define LIBATSSIM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (the reference prints and continues; here every call
+ *      returns one of these and sets lats_last_error()) ------------------- */
+#define LATS_OK 0
+#define LATS_ERR_INVALID_ARGUMENT 1
+#define LATS_ERR_HIP 2
+#define LATS_ERR_NO_DEVICE 3
+#define LATS_ERR_UNSUPPORTED 4
+#define LATS_ERR_OUT_OF_MEMORY 5
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* lats_last_error(void);
+
+/* ======================================================================
+ * Topology: replaces SimMesh::read_mesh_file's edge extraction
+ * (src/Utils/mesh_reader.h:206-298) and the adjacency / rest-state /
+ * colouring set-up of init_mesh + CpuSolver::init_xpbd_system
+ * (examples/example2_async_iter.cpp:229-374, 572-646).
+ * ====================================================================== */
+typedef struct lats_topology lats_topology;
+
+/* which-array selectors for lats_topology_size / lats_topology_copy */
+#define LATS_TOPO_EDGES 0            /* int32[2E]  sorted (min,max) edges            */
+#define LATS_TOPO_BENDING 1          /* int32[4B]  (e0,e1,opp_lowfid,opp_highfid)    */
+#define LATS_TOPO_REST_LENGTH 2      /* float[E]                                     */
+#define LATS_TOPO_BENDING_Q 3        /* float[16B] Q[b][col][row]                    */
+#define LATS_TOPO_ADJ_EDGES_CSR 4    /* uint32 packed CSR (V+1 offsets, then ids)    */
+#define LATS_TOPO_ADJ_BENDING_CSR 5  /* uint32 packed CSR                            */
+#define LATS_TOPO_CLUSTERS_CSR 6     /* uint32 packed CSR of colour clusters         */
+#define LATS_TOPO_COLOUR 7           /* uint8[V] colour id per vertex                */
+
+int lats_topology_build(const float* x, uint32_t num_verts, const int32_t* faces, uint32_t num_faces,
+                        lats_topology** out);
+void lats_topology_destroy(lats_topology* t);
+/* counts[0..3] = num_verts, num_edges, num_bending_edges, num_clusters */
+int lats_topology_counts(const lats_topology* t, uint32_t counts[4]);
+/* number of elements (not bytes) of array `which` */
+int lats_topology_size(const lats_topology* t, int which, uint64_t* num_elements);
+int lats_topology_copy(const lats_topology* t, int which, void* dst);
+
+/* ======================================================================
+ * Seeded synthetic scenes (grid cloths and per-frame wind).
+ * ====================================================================== */
+/* x: float[3*nx*ny]; faces: int32[6*(nx-1)*(ny-1)] */
+int lats_scene_grid(uint32_t nx, uint32_t ny, float spacing, float jitter, uint64_t seed, float* x,
+                    int32_t* faces);
+/* out: float[3*num_frames], force per vertex for each frame */
+int lats_scene_wind(uint32_t num_frames, uint32_t num_verts, float max_total_force, uint64_t seed, float* out);
+
+/* ======================================================================
+ * Single-cloth VBD context: the GPU slot's state (the reference's
+ * BasicMeshData + XpbdData SharedArrays, example3_hybrid_cpu_gpu.cpp:16-92)
+ * held in HBM.
+ * ====================================================================== */
+
+/* Subset of SceneParams (src/SharedDefine/scene_params.h:9-163) read by the
+ * VBD path. */
+typedef struct lats_scene_params {
+    float implicit_dt;                 /* scene_params.h:94 (1/60)                       */
+    uint32_t num_substep;              /* scene_params.h:68                               */
+    uint32_t constraint_iter_count;    /* scene_params.h:71                               */
+    float stiffness_stretch_spring;    /* scene_params.h:101                              */
+    float stiffness_quadratic_bending; /* get_stiffness_quadratic_bending() (k * bending_ui), :160 */
+    float damping_cloth;               /* scene_params.h:115                              */
+    float gravity[3];                  /* scene_params.h:145; predict uses (0,-9.8,0) as the reference */
+} lats_scene_params;
+
+/* Per-mesh arrays, consumed as-is (colouring and clusters are not
+ * recomputed).  Formats are those of lats_topology_copy. */
+typedef struct lats_mesh_desc {
+    uint32_t num_verts, num_edges, num_bending_edges, num_clusters;
+    const float* rest_x;               /* float[3V]: initial state, v = 0                 */
+    const int32_t* edges;              /* int32[2E]                                       */
+    const int32_t* bending_edges;      /* int32[4B]                                       */
+    const float* edges_rest_length;    /* float[E]                                        */
+    const float* bending_Q;            /* float[16B]                                      */
+    const uint32_t* vert_adj_edges_csr;
+    const uint32_t* vert_adj_bending_csr;
+    const uint32_t* clusters_csr;
+    const float* vert_mass;            /* float[V]                                        */
+    const uint8_t* is_fixed;           /* uint8[V]                                        */
+} lats_mesh_desc;
+
+typedef struct lats_vbd lats_vbd;
+
+/* Frame-execution strategies of lats_vbd_physics_step / lats_vbd_run_frames. */
+#define LATS_STRATEGY_AUTO 0        /* pick the fastest for the mesh                    */
+#define LATS_STRATEGY_GRAPH 1       /* one kernel per colour, frame captured in a hipGraph */
+#define LATS_STRATEGY_PERSISTENT 2  /* one cooperative kernel per frame, grid barrier per colour */
+
+/* Creates the context on the current HIP device; uploads everything. */
+int lats_vbd_create(const lats_mesh_desc* mesh, const lats_scene_params* params, lats_vbd** out);
+int lats_vbd_destroy(lats_vbd* ctx);
+int lats_vbd_set_strategy(lats_vbd* ctx, int strategy);
+/* Include the bending term in the recorded energy (the reference's GPU
+ * energy does, example3.metal:369-395; its CPU energy drops it,
+ * example2_async_iter.cpp:832-846).  Default 1. */
+int lats_vbd_set_energy_bending(lats_vbd* ctx, int include_bending);
+
+/* Frame state in vertex-id order (≙ sa_x_frame_end / sa_v_frame_end). */
+int lats_vbd_set_state(lats_vbd* ctx, const float* x, const float* v);
+int lats_vbd_get_state(lats_vbd* ctx, float* x, float* v);
+/* Uniform external force per vertex, added to m*g in the inertia term. */
+int lats_vbd_set_external_force(lats_vbd* ctx, const float force[3]);
+
+/* One frame (≙ SolverInterface::physics_step(SolverTypeVBD) +
+ * GpuSolver::physics_step_vbd, example3_hybrid_cpu_gpu.cpp:1510-1576).
+ * record_energy != 0: one substep and the energy history of
+ * constraint_iter_count + 1 values (before iteration 0, then after each). */
+int lats_vbd_physics_step(lats_vbd* ctx, int record_energy);
+int lats_vbd_get_energy(lats_vbd* ctx, float* out, uint32_t capacity, uint32_t* count);
+
+/* num_frames frames back to back on the device, external force per frame
+ * from forces[3*f] (NULL: keep the current one).  No host synchronisation
+ * until the call returns. */
+int lats_vbd_run_frames(lats_vbd* ctx, uint32_t num_frames, const float* forces);
+/* GPU time (HIP events on the context's stream) of the last
+ * lats_vbd_physics_step / lats_vbd_run_frames call, in ms. */
+int lats_vbd_last_elapsed_ms(lats_vbd* ctx, float* ms);
+int lats_vbd_synchronize(lats_vbd* ctx);
+
+/* ---- scheduler slot: GpuSolver::fn_dispatch ------------------------- */
+/* Function ids of the tasks the VBD DAG registers
+ * (example3_hybrid_cpu_gpu.cpp:1921-1988; FunctionID launcher.h:15-221). */
+#define LATS_FN_ADDITIONAL_ROOT 0
+#define LATS_FN_ADDITIONAL_TERMINAL 1
+#define LATS_FN_PREDICT_POSITION 2      /* id_xpbd_predict_position     */
+#define LATS_FN_VBD_ALL_IN_ONE 3        /* id_vbd_all_in_one            */
+#define LATS_FN_CONSTRAINT_LAST_NODE 4  /* id_xpbd_constraint_last_node */
+#define LATS_FN_UPDATE_VELOCITY 5       /* id_xpbd_update_velocity      */
+#define LATS_FN_RESET_CONSTRAINS 6      /* id_xpbd_reset_constrains (no state in VBD) */
+
+#define LATS_BUFFER_DEFAULT 0x0FFFFFFFu /* default_buffer_mask / input_buffer_mask, launcher.h:836-837 */
+#define LATS_BUFFER_NONE 0xFFFFFFFFu    /* -1u                                   */
+#define LATS_NUM_ASYNC_BUFFERS 32       /* example3_hybrid_cpu_gpu.cpp:1747        */
+
+/* Mirror of Launcher::LaunchParam (launcher.h:435-539) fields the slot reads. */
+typedef struct lats_launch_param {
+    uint32_t function_id;
+    uint32_t iter_idx;
+    uint32_t cluster_idx;
+    uint32_t buffer_idx;
+    uint32_t left_buffer_idx;
+    uint32_t right_buffer_idx;
+    uint32_t num_input_buffers;
+    const uint32_t* input_buffer_idxs;
+    uint32_t is_allocated_to_main_device;
+} lats_launch_param;
+
+/* Enqueues the task's kernels on the context's stream (asynchronous). */
+int lats_vbd_dispatch(lats_vbd* ctx, const lats_launch_param* param);
+/* Energy recording for dispatched tasks (≙ SceneParams::print_xpbd_convergence):
+ * on != 0 clears the history; energies are then appended at the points the
+ * reference's prev/post hooks evaluate them (example3_hybrid_cpu_gpu.cpp:1819-1855).
+ * Read them with lats_vbd_get_energy. */
+int lats_vbd_set_record_energy(lats_vbd* ctx, int on);
+
+/* ---- CPU<->GPU signalling for the heterogeneous launch mode ---------
+ * Replaces the MTLSharedEvent pair (command_list.h:474-480 wait_cpu,
+ * launcher.cpp:1017-1021/1092-1095 signal) and waitUntilCompleted.  The
+ * host flag lives in pinned host memory. */
+/* Enqueue: the stream waits until the host flag >= value. */
+int lats_vbd_stream_wait_host(lats_vbd* ctx, uint32_t value);
+/* Host side: set the host flag (CPU signals the GPU). */
+int lats_vbd_host_signal(lats_vbd* ctx, uint32_t value);
+/* Enqueue a GPU->CPU completion marker; lats_vbd_wait_marker blocks the
+ * host until the marker has passed. */
+int lats_vbd_record_marker(lats_vbd* ctx, uint32_t marker);
+int lats_vbd_wait_marker(lats_vbd* ctx, uint32_t marker);
+/* Elapsed GPU ms between two markers (≙ GPUEndTime - GPUStartTime). */
+int lats_vbd_marker_elapsed_ms(lats_vbd* ctx, uint32_t m0, uint32_t m1, float* ms);
+
+/* ---- device accessors (benchmarks, zero-copy interop) ---------------- */
+/* Device pointer of the colour-ordered position buffer, float[3V]; slot s
+ * holds vertex perm[s] (lats_vbd_copy_permutation). */
+int lats_vbd_device_positions(lats_vbd* ctx, void** dptr);
+int lats_vbd_copy_permutation(lats_vbd* ctx, uint32_t* perm);
+/* Measurement: runs `iterations` colour sweeps with every per-colour VBD
+ * kernel launched eagerly between two HIP events on the context's stream,
+ * then restores the positions.  mean_ms = mean event-to-event duration of
+ * one colour launch; launches = number of launches timed; vertex_updates =
+ * vertex updates they performed. */
+int lats_vbd_time_colour_kernels(lats_vbd* ctx, uint32_t iterations, float* mean_ms, uint32_t* launches,
+                                 uint64_t* vertex_updates);
+/* Bytes of device memory held by the context. */
+int lats_vbd_device_bytes(lats_vbd* ctx, uint64_t* bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LIBATSSIM_HIP_H */

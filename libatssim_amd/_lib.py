@@ -1,0 +1,159 @@
+"""ctypes binding of libatssim_hip.so (the C ABI in include/libatssim_hip.h).
+
+The library is built in-tree (`make -C libatssim_amd/csrc`, or
+`__graft_entry__.build()`).  Loading fails loudly when it is missing: there
+is no CPU fallback for the product path.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libatssim_hip.so")
+
+LATS_OK = 0
+LATS_ERR_INVALID_ARGUMENT = 1
+LATS_ERR_HIP = 2
+LATS_ERR_NO_DEVICE = 3
+LATS_ERR_UNSUPPORTED = 4
+LATS_ERR_OUT_OF_MEMORY = 5
+
+LATS_TOPO_EDGES = 0
+LATS_TOPO_BENDING = 1
+LATS_TOPO_REST_LENGTH = 2
+LATS_TOPO_BENDING_Q = 3
+LATS_TOPO_ADJ_EDGES_CSR = 4
+LATS_TOPO_ADJ_BENDING_CSR = 5
+LATS_TOPO_CLUSTERS_CSR = 6
+LATS_TOPO_COLOUR = 7
+
+LATS_STRATEGY_AUTO = 0
+LATS_STRATEGY_GRAPH = 1
+LATS_STRATEGY_PERSISTENT = 2
+
+LATS_FN_ADDITIONAL_ROOT = 0
+LATS_FN_ADDITIONAL_TERMINAL = 1
+LATS_FN_PREDICT_POSITION = 2
+LATS_FN_VBD_ALL_IN_ONE = 3
+LATS_FN_CONSTRAINT_LAST_NODE = 4
+LATS_FN_UPDATE_VELOCITY = 5
+LATS_FN_RESET_CONSTRAINS = 6
+
+LATS_BUFFER_DEFAULT = 0x0FFFFFFF
+LATS_BUFFER_NONE = 0xFFFFFFFF
+
+
+class LatsError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"libatssim_hip error {code}: {msg}")
+        self.code = code
+
+
+class SceneParams(ctypes.Structure):
+    _fields_ = [
+        ("implicit_dt", ctypes.c_float),
+        ("num_substep", ctypes.c_uint32),
+        ("constraint_iter_count", ctypes.c_uint32),
+        ("stiffness_stretch_spring", ctypes.c_float),
+        ("stiffness_quadratic_bending", ctypes.c_float),
+        ("damping_cloth", ctypes.c_float),
+        ("gravity", ctypes.c_float * 3),
+    ]
+
+
+class MeshDesc(ctypes.Structure):
+    _fields_ = [
+        ("num_verts", ctypes.c_uint32),
+        ("num_edges", ctypes.c_uint32),
+        ("num_bending_edges", ctypes.c_uint32),
+        ("num_clusters", ctypes.c_uint32),
+        ("rest_x", ctypes.c_void_p),
+        ("edges", ctypes.c_void_p),
+        ("bending_edges", ctypes.c_void_p),
+        ("edges_rest_length", ctypes.c_void_p),
+        ("bending_Q", ctypes.c_void_p),
+        ("vert_adj_edges_csr", ctypes.c_void_p),
+        ("vert_adj_bending_csr", ctypes.c_void_p),
+        ("clusters_csr", ctypes.c_void_p),
+        ("vert_mass", ctypes.c_void_p),
+        ("is_fixed", ctypes.c_void_p),
+    ]
+
+
+class LaunchParam(ctypes.Structure):
+    _fields_ = [
+        ("function_id", ctypes.c_uint32),
+        ("iter_idx", ctypes.c_uint32),
+        ("cluster_idx", ctypes.c_uint32),
+        ("buffer_idx", ctypes.c_uint32),
+        ("left_buffer_idx", ctypes.c_uint32),
+        ("right_buffer_idx", ctypes.c_uint32),
+        ("num_input_buffers", ctypes.c_uint32),
+        ("input_buffer_idxs", ctypes.POINTER(ctypes.c_uint32)),
+        ("is_allocated_to_main_device", ctypes.c_uint32),
+    ]
+
+
+# name -> (restype, argtypes); every symbol declared in include/libatssim_hip.h
+_VP = ctypes.c_void_p
+_U32 = ctypes.c_uint32
+_I = ctypes.c_int
+SIGNATURES = {
+    "lats_last_error": (ctypes.c_char_p, []),
+    "lats_topology_build": (_I, [_VP, _U32, _VP, _U32, ctypes.POINTER(_VP)]),
+    "lats_topology_destroy": (None, [_VP]),
+    "lats_topology_counts": (_I, [_VP, _VP]),
+    "lats_topology_size": (_I, [_VP, _I, ctypes.POINTER(ctypes.c_uint64)]),
+    "lats_topology_copy": (_I, [_VP, _I, _VP]),
+    "lats_scene_grid": (_I, [_U32, _U32, ctypes.c_float, ctypes.c_float, ctypes.c_uint64, _VP, _VP]),
+    "lats_scene_wind": (_I, [_U32, _U32, ctypes.c_float, ctypes.c_uint64, _VP]),
+    "lats_vbd_create": (_I, [ctypes.POINTER(MeshDesc), ctypes.POINTER(SceneParams), ctypes.POINTER(_VP)]),
+    "lats_vbd_destroy": (_I, [_VP]),
+    "lats_vbd_set_strategy": (_I, [_VP, _I]),
+    "lats_vbd_set_energy_bending": (_I, [_VP, _I]),
+    "lats_vbd_set_state": (_I, [_VP, _VP, _VP]),
+    "lats_vbd_get_state": (_I, [_VP, _VP, _VP]),
+    "lats_vbd_set_external_force": (_I, [_VP, _VP]),
+    "lats_vbd_physics_step": (_I, [_VP, _I]),
+    "lats_vbd_get_energy": (_I, [_VP, _VP, _U32, ctypes.POINTER(_U32)]),
+    "lats_vbd_run_frames": (_I, [_VP, _U32, _VP]),
+    "lats_vbd_last_elapsed_ms": (_I, [_VP, ctypes.POINTER(ctypes.c_float)]),
+    "lats_vbd_synchronize": (_I, [_VP]),
+    "lats_vbd_dispatch": (_I, [_VP, ctypes.POINTER(LaunchParam)]),
+    "lats_vbd_set_record_energy": (_I, [_VP, _I]),
+    "lats_vbd_stream_wait_host": (_I, [_VP, _U32]),
+    "lats_vbd_host_signal": (_I, [_VP, _U32]),
+    "lats_vbd_record_marker": (_I, [_VP, _U32]),
+    "lats_vbd_wait_marker": (_I, [_VP, _U32]),
+    "lats_vbd_marker_elapsed_ms": (_I, [_VP, _U32, _U32, ctypes.POINTER(ctypes.c_float)]),
+    "lats_vbd_device_positions": (_I, [_VP, ctypes.POINTER(_VP)]),
+    "lats_vbd_copy_permutation": (_I, [_VP, _VP]),
+    "lats_vbd_time_colour_kernels": (_I, [_VP, _U32, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(_U32),
+                                          ctypes.POINTER(ctypes.c_uint64)]),
+    "lats_vbd_device_bytes": (_I, [_VP, ctypes.POINTER(ctypes.c_uint64)]),
+}
+
+_LIB = None
+
+
+def lib():
+    """The loaded C-ABI library (raises if it has not been built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with `make -C libatssim_amd/csrc` "
+                "(the HIP path has no CPU fallback)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def check(rc):
+    if rc != LATS_OK:
+        msg = lib().lats_last_error()
+        raise LatsError(rc, msg.decode() if msg else "")
+    return rc

@@ -1,0 +1,61 @@
+"""Named workloads of the VBD path (synthetic, seeded).
+
+grid_c0      32x32 grid (1,024 vertices), 1 cm spacing, z-jitter U(-1e-4, 1e-4),
+             two top corners pinned, m = 1e-3 kg, k_stretch 1e4, k_bend 1e-2,
+             h = 1/60 s, 100 iterations/frame, 30 frames.
+grid_c1      256x256 grid (65,536 vertices), same material; top row pinned;
+             per-frame wind of total magnitude U(0, 2) N, direction U(0, 2 pi)
+             in the xz plane, split evenly over the vertices; 100 iterations,
+             60 frames.
+square8k     the reference's example-2 run (examples/example2_async_iter.cpp:
+             133-392, 715-749, 1713-1810): square8K.obj, pins where the
+             AABB-normalised z < 0.01 and x < 0.01 or x > 0.99, m = 0.01 kg,
+             k_stretch 1e4, k_bend 5e-3 * 0.5, 100 iterations, 30 frames.
+"""
+import numpy as np
+
+from .vbd import Cloth, Params, make_grid, make_wind
+
+SEED_C0 = 20250101
+SEED_C1 = 20250102
+SEED_WIND = 20250103
+
+
+def grid_cloth(n, pin="corners", seed=SEED_C0, spacing=0.01, jitter=1e-4, mass=1e-3):
+    x, f = make_grid(n, n, spacing, jitter, seed)
+    fixed = np.zeros(n * n, np.uint8)
+    if pin == "corners":
+        fixed[0] = 1
+        fixed[n - 1] = 1
+    elif pin == "top":
+        fixed[:n] = 1
+    return Cloth(x, f, np.float32(mass), fixed)
+
+
+def grid_c0():
+    return grid_cloth(32, "corners", SEED_C0), Params(constraint_iter_count=100), 30, None
+
+
+def grid_c1(frames=60):
+    cloth = grid_cloth(256, "top", SEED_C1)
+    wind = make_wind(frames, cloth.num_verts, 2.0, SEED_WIND)
+    return cloth, Params(constraint_iter_count=100), frames, wind
+
+
+def square8k_pins(x):
+    mn = x.min(0)
+    rng = (x.max(0) - mn).astype(np.float32)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        inv = (np.float32(1) / rng).astype(np.float32)
+        nrm = ((x - mn) * inv).astype(np.float32)
+    return ((nrm[:, 2] < np.float32(0.01)) & ((nrm[:, 0] > np.float32(0.99)) | (nrm[:, 0] < np.float32(0.01)))).astype(np.uint8)
+
+
+def square8k_params():
+    return Params(implicit_dt=np.float32(1) / np.float32(60), constraint_iter_count=100,
+                  stiffness_stretch_spring=1e4,
+                  stiffness_quadratic_bending=float(np.float32(5e-3) * np.float32(0.5)), damping_cloth=2.0)
+
+
+def square8k_cloth(x, faces):
+    return Cloth(x, faces, np.float32(0.01) * np.float32(1.0), square8k_pins(x))

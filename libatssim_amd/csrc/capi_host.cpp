@@ -1,0 +1,99 @@
+// capi_host.cpp -- C ABI of the host-only parts: topology and synthetic scenes.
+#include <cstring>
+#include <string>
+
+#include "../../include/libatssim_hip.h"
+#include "host/errors.h"
+#include "host/scene.h"
+#include "host/topology.h"
+
+struct lats_topology {
+    lats::Topology t;
+};
+
+namespace {
+int set_err(int code, const char* msg) {
+    lats::set_error(msg);
+    return code;
+}
+}  // namespace
+
+extern "C" {
+
+int lats_topology_build(const float* x, uint32_t nv, const int32_t* faces, uint32_t nf, lats_topology** out) {
+    if (!out || (nv && !x) || (nf && !faces)) return set_err(LATS_ERR_INVALID_ARGUMENT, "null argument");
+    auto* t = new lats_topology();
+    std::string err;
+    if (!lats::build_topology(x, nv, faces, nf, t->t, err)) {
+        delete t;
+        *out = nullptr;
+        return set_err(LATS_ERR_INVALID_ARGUMENT, err.c_str());
+    }
+    *out = t;
+    return LATS_OK;
+}
+
+void lats_topology_destroy(lats_topology* t) { delete t; }
+
+int lats_topology_counts(const lats_topology* t, uint32_t counts[4]) {
+    if (!t || !counts) return set_err(LATS_ERR_INVALID_ARGUMENT, "null argument");
+    counts[0] = t->t.num_verts;
+    counts[1] = t->t.num_edges();
+    counts[2] = t->t.num_bending();
+    counts[3] = t->t.num_clusters;
+    return LATS_OK;
+}
+
+int lats_topology_size(const lats_topology* t, int which, uint64_t* n) {
+    if (!t || !n) return set_err(LATS_ERR_INVALID_ARGUMENT, "null argument");
+    const auto& T = t->t;
+    switch (which) {
+        case LATS_TOPO_EDGES: *n = T.edges.size(); break;
+        case LATS_TOPO_BENDING: *n = T.bending.size(); break;
+        case LATS_TOPO_REST_LENGTH: *n = T.rest_length.size(); break;
+        case LATS_TOPO_BENDING_Q: *n = T.bending_Q.size(); break;
+        case LATS_TOPO_ADJ_EDGES_CSR: *n = T.adj_edges_csr.size(); break;
+        case LATS_TOPO_ADJ_BENDING_CSR: *n = T.adj_bending_csr.size(); break;
+        case LATS_TOPO_CLUSTERS_CSR: *n = T.clusters_csr.size(); break;
+        case LATS_TOPO_COLOUR: *n = T.colour.size(); break;
+        default: return set_err(LATS_ERR_INVALID_ARGUMENT, "unknown topology array");
+    }
+    return LATS_OK;
+}
+
+int lats_topology_copy(const lats_topology* t, int which, void* dst) {
+    if (!t || !dst) return set_err(LATS_ERR_INVALID_ARGUMENT, "null argument");
+    const auto& T = t->t;
+    auto cp = [&](const auto& v) { std::memcpy(dst, v.data(), v.size() * sizeof(v[0])); };
+    switch (which) {
+        case LATS_TOPO_EDGES: cp(T.edges); break;
+        case LATS_TOPO_BENDING: cp(T.bending); break;
+        case LATS_TOPO_REST_LENGTH: cp(T.rest_length); break;
+        case LATS_TOPO_BENDING_Q: cp(T.bending_Q); break;
+        case LATS_TOPO_ADJ_EDGES_CSR: cp(T.adj_edges_csr); break;
+        case LATS_TOPO_ADJ_BENDING_CSR: cp(T.adj_bending_csr); break;
+        case LATS_TOPO_CLUSTERS_CSR: cp(T.clusters_csr); break;
+        case LATS_TOPO_COLOUR: cp(T.colour); break;
+        default: return set_err(LATS_ERR_INVALID_ARGUMENT, "unknown topology array");
+    }
+    return LATS_OK;
+}
+
+int lats_scene_grid(uint32_t nx, uint32_t ny, float spacing, float jitter, uint64_t seed, float* x, int32_t* faces) {
+    if (nx < 2 || ny < 2 || !x || !faces) return set_err(LATS_ERR_INVALID_ARGUMENT, "grid needs nx, ny >= 2");
+    std::vector<float> hx;
+    std::vector<int32_t> hf;
+    lats::make_grid(nx, ny, spacing, jitter, seed, hx, hf);
+    std::memcpy(x, hx.data(), hx.size() * sizeof(float));
+    std::memcpy(faces, hf.data(), hf.size() * sizeof(int32_t));
+    return LATS_OK;
+}
+
+int lats_scene_wind(uint32_t nframes, uint32_t nv, float max_total, uint64_t seed, float* out) {
+    if (!out && nframes) return set_err(LATS_ERR_INVALID_ARGUMENT, "null argument");
+    std::vector<float> w = lats::make_wind(nframes, nv, max_total, seed);
+    std::memcpy(out, w.data(), w.size() * sizeof(float));
+    return LATS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,1053 @@
+// vbd_device.hip -- MI355X (gfx950) VBD cloth kernels and the single-cloth
+// context behind the C ABI in include/libatssim_hip.h.
+//
+// Layout in HBM (all FP32 unless noted; "slot" = position in colour order,
+// slots [cstart[c], cstart[c]+n[c]) hold colour c's vertices in ascending
+// vertex id, i.e. the reference's packed cluster CSR flattened):
+//   x, x_start, v      float[3V] AoS per slot (12-byte stride)
+//   mass               float[V]
+//   meta               uint32[V]: edge degree | bending degree << 8 | pinned << 16
+//   ell_e              uint2[sum_c De_c * n_c]: (other slot, rest length bits),
+//                      colour-major ELL, entry (c, j, i) at eoff[c] + j*n_c + i
+//   ell_bq             float4[...]: row o of the pair's Q (Q[o][0..3])
+//   ell_bi             uint4[...]: slots of (e0, e1, o0, o1); o in bits 30-31 of .x
+// so every per-vertex load of one colour pass is one coalesced 64-lane access.
+//
+// Bit-exact parity with the reference's FP32 arithmetic requires this file
+// to be compiled with -ffp-contract=off and IEEE div/sqrt (hipcc default).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/libatssim_hip.h"
+#include "host/errors.h"
+#include "host/topology.h"
+#include "vbd_common.h"
+
+using namespace lats;
+
+// ===========================================================================
+// Device code
+// ===========================================================================
+namespace {
+
+constexpr int kBlock = 256;
+
+struct FrameParams {
+    const float* forces;   // per-frame external force (3 floats / frame) or null
+    uint32_t frame;        // frame counter, advanced by the last kernel of a frame
+    uint32_t pad;
+    float ext[4];          // constant external force when forces == null
+    float ext_cur[4];      // force of the running frame (written by predict)
+};
+
+struct VbdArgs {
+    float* x;                  // positions being iterated (slot order)
+    const float* x0;           // x_start
+    const float* v;            // frame-start velocity
+    const float* mass;
+    const uint32_t* meta;
+    const uint2* ell_e;
+    const float4* ell_bq;
+    const uint4* ell_bi;
+    const FrameParams* fp;
+    float h, h2inv, ks, kb;
+    float g[3];
+};
+
+__device__ __forceinline__ F3 ld3(const float* p, uint32_t s) { return F3{p[3 * s], p[3 * s + 1], p[3 * s + 2]}; }
+__device__ __forceinline__ void st3(float* p, uint32_t s, F3 a) {
+    p[3 * s] = a.x;
+    p[3 * s + 1] = a.y;
+    p[3 * s + 2] = a.z;
+}
+
+// One VBD vertex update: VBD::compute_inertia + compute_stretch_mass_spring +
+// compute_bending_quadratic (xpbd_constraints.h:4027-4205) summed as
+// Hf = inertia; Hf += sum(stretch); Hf += sum(bending)
+// (example2_async_iter.cpp:939-992), then the guarded 3x3 Newton step of
+// vbd_step (:993-1014).  i = index inside the colour, n = colour size.
+__device__ __forceinline__ void vbd_vertex(const VbdArgs& a, uint32_t s, uint32_t i, uint32_t n, uint32_t eoff,
+                                           uint32_t boff) {
+    const F3 xk = ld3(a.x, s);
+    const F3 x0 = ld3(a.x0, s);
+    const F3 v0 = ld3(a.v, s);
+    const float m = a.mass[s];
+    const uint32_t meta = a.meta[s];
+    const uint32_t edeg = meta & 0xffu, bdeg = (meta >> 8) & 0xffu;
+    const float ex = a.fp->ext_cur[0], ey = a.fp->ext_cur[1], ez = a.fp->ext_cur[2];
+
+    // inertia
+    Sym3 H;
+    F3 f;
+    inertia_term(xk, x0, v0, m, (meta >> 16) & 1u, a.g, F3{ex, ey, ez}, a.h, a.h2inv, H, f);
+
+    // stretch: local sum from zero, then added to Hf
+    Sym3 S = sym_zero();
+    F3 fs = F3{0.f, 0.f, 0.f};
+    for (uint32_t j = 0; j < edeg; j++) {
+        const uint2 e = a.ell_e[eoff + j * n + i];
+        const F3 xo = ld3(a.x, e.x);
+        stretch_term(xk, xo, __uint_as_float(e.y), a.ks, S, fs);
+    }
+    sym_add(H, S);
+    f = add3(f, fs);
+
+    // bending: diagonal Hessian only (k * Q[o][o] * I)
+    float hb = 0.f;
+    F3 fb = F3{0.f, 0.f, 0.f};
+    for (uint32_t j = 0; j < bdeg; j++) {
+        const uint32_t k = boff + j * n + i;
+        const float4 q = a.ell_bq[k];
+        const uint4 id = a.ell_bi[k];
+        const uint32_t o = id.x >> 30;
+        const F3 p0 = (o == 0) ? xk : ld3(a.x, id.x & 0x3fffffffu);
+        const F3 p1 = (o == 1) ? xk : ld3(a.x, id.y);
+        const F3 p2 = (o == 2) ? xk : ld3(a.x, id.z);
+        const F3 p3 = (o == 3) ? xk : ld3(a.x, id.w);
+        bending_term(q, o, p0, p1, p2, p3, a.kb, hb, fb);
+    }
+    H.xx = H.xx + hb;
+    H.yy = H.yy + hb;
+    H.zz = H.zz + hb;
+    f = add3(f, fb);
+
+    F3 dx;
+    if (newton_step(H, f, dx)) st3(a.x, s, add3(xk, dx));
+}
+
+__global__ __launch_bounds__(kBlock) void k_vbd_colour(VbdArgs a, uint32_t cs, uint32_t n, uint32_t eoff,
+                                                       uint32_t boff) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    vbd_vertex(a, cs + i, i, n, eoff, boff);
+}
+
+// Core::predict_position (xpbd_constraints.h:321-384); gravity hard-coded
+// (0,-9.8,0) as the reference.  Block 0 / thread 0 also publishes this
+// frame's external force.
+__global__ __launch_bounds__(kBlock) void k_predict(float* x, const float* x0, const float* v, const uint32_t* meta,
+                                                    FrameParams* fp, float h, uint32_t nv) {
+    const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+    if (s == 0) {
+        const float* src = fp->forces ? fp->forces + 3 * (size_t)fp->frame : fp->ext;
+        fp->ext_cur[0] = src[0];
+        fp->ext_cur[1] = src[1];
+        fp->ext_cur[2] = src[2];
+    }
+    if (s >= nv) return;
+    const F3 xp = ld3(x0, s);
+    const F3 vp = ld3(v, s);
+    F3 vpred = add3(vp, scale3(h, F3{0.f, -9.8f, 0.f}));
+    if ((meta[s] >> 16) & 1u) vpred = F3{0.f, 0.f, 0.f};
+    st3(x, s, add3(xp, scale3(h, vpred)));
+}
+
+// Core::update_velocity (xpbd_constraints.h:535-581).  damp = exp(-c h).
+__global__ __launch_bounds__(kBlock) void k_update_velocity(const float* x, float* x0, float* v, FrameParams* fp,
+                                                            float h, float damp, uint32_t nv, int advance_frame) {
+    const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+    if (s == 0 && advance_frame) fp->frame = fp->frame + 1;
+    if (s >= nv) return;
+    const F3 xk = ld3(x, s);
+    const F3 dx = sub3(xk, ld3(x0, s));
+    F3 vel = F3{dx.x / h, dx.y / h, dx.z / h};
+    vel = F3{vel.x * damp, vel.y * damp, vel.z * damp};
+    st3(v, s, vel);
+    st3(x0, s, xk);
+}
+
+// --- energy ---------------------------------------------------------------
+struct EnergyArgs {
+    const float* x;
+    const float* x0;
+    const float* v;
+    const float* mass;
+    const uint32_t* meta;
+    const uint2* edges;      // slots
+    const float* rest;
+    const uint4* pairs;      // slots
+    const float4* Q;         // 4 float4 per pair: column-major Q
+    const FrameParams* fp;
+    uint32_t nv, ne, nb;
+    float h, ks, kb;
+    float g[3];
+};
+
+__device__ __forceinline__ double block_sum(double val) {
+    __shared__ double red[kBlock / 64];
+    for (int off = 32; off > 0; off >>= 1) val += __shfl_xor(val, off, 64);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) red[w] = val;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0)
+        for (int k = 0; k < kBlock / 64; k++) r += red[k];
+    return r;
+}
+
+// Per-element FP32 energies (Energy::compute_energy_inertia / _stretch_mass_spring /
+// _bending, xpbd_constraints.h:2079-2214), FP64 block partial sums.
+__global__ __launch_bounds__(kBlock) void k_energy_partial(EnergyArgs a, double* partial) {
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    float e = 0.f;
+    if (t < a.nv) {
+        const uint32_t s = t;
+        const float m = a.mass[s];
+        F3 aext = F3{a.g[0], a.g[1], a.g[2]};
+        const F3 w = F3{a.fp->ext_cur[0], a.fp->ext_cur[1], a.fp->ext_cur[2]};
+        aext = add3(aext, F3{w.x / m, w.y / m, w.z / m});
+        if ((a.meta[s] >> 16) & 1u) aext = F3{0.f, 0.f, 0.f};
+        const F3 y = add3(add3(ld3(a.x0, s), scale3(a.h, ld3(a.v, s))), scale3(a.h * a.h, aext));
+        const F3 d = sub3(ld3(a.x, s), y);
+        e = (dot3(d, d) * m) / ((2.f * a.h) * a.h);
+    } else if (t < a.nv + a.ne) {
+        const uint32_t k = t - a.nv;
+        const uint2 ed = a.edges[k];
+        const F3 diff = sub3(ld3(a.x, ed.y), ld3(a.x, ed.x));
+        const float l2 = dot3(diff, diff);
+        if (!((double)l2 < 1e-8)) {
+            const float C = sqrtf(l2) - a.rest[k];
+            e = (0.5f * a.ks) * (C * C);
+        }
+    } else if (t < a.nv + a.ne + a.nb) {
+        const uint32_t k = t - a.nv - a.ne;
+        const uint4 id = a.pairs[k];
+        F3 p[4] = {ld3(a.x, id.x), ld3(a.x, id.y), ld3(a.x, id.z), ld3(a.x, id.w)};
+        float sum = 0.f;
+        for (int ii = 0; ii < 4; ii++) {
+            const float4 qc = a.Q[4 * k + ii];
+            const float qq[4] = {qc.x, qc.y, qc.z, qc.w};
+            for (int jj = 0; jj < 4; jj++) sum = sum + qq[jj] * dot3(p[ii], p[jj]);
+        }
+        e = (0.5f * a.kb) * sum;
+    }
+    const double r = block_sum((double)e);
+    if (threadIdx.x == 0) partial[blockIdx.x] = r;
+}
+
+__global__ __launch_bounds__(kBlock) void k_energy_finalize(const double* partial, uint32_t nblk, uint32_t row0,
+                                                            float* out) {
+    const uint32_t row = row0 + blockIdx.x;
+    const double* p = partial + (size_t)row * nblk;
+    double acc = 0.0;
+    for (uint32_t k = threadIdx.x; k < nblk; k += kBlock) acc += p[k];
+    const double r = block_sum(acc);
+    if (threadIdx.x == 0) out[row] = (float)r;
+}
+
+// --- multi-buffer helpers of the heterogeneous schedule ------------------
+__global__ __launch_bounds__(kBlock) void k_copy_to_2(const float* a, float* b, float* c, uint32_t n) {
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= n) return;
+    const float val = a[t];
+    if (b) b[t] = val;
+    if (c) c[t] = val;
+}
+
+// Core::read_and_solve_conflict (xpbd_constraints.h:815-869)
+__global__ __launch_bounds__(kBlock) void k_read_and_solve_conflict(float* begin_self, float* iter_self,
+                                                                    const float* iter_other, float weight,
+                                                                    uint32_t nv) {
+    const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+    if (s >= nv) return;
+    const F3 ps = ld3(iter_self, s), po = ld3(iter_other, s);
+    F3 wp;
+    if (weight > 1.19209290E-7f) {
+        const F3 init = ld3(begin_self, s);
+        const F3 sdx = sub3(ps, init), adx = sub3(po, init);
+        wp = init;
+        const bool aa = dot3(sdx, sdx) != 0.f, ab = dot3(adx, adx) != 0.f;
+        if (aa && ab) wp = add3(init, scale3(weight, add3(sdx, adx)));
+        else if (aa) wp = ps;
+        else if (ab) wp = po;
+    } else {
+        wp = scale3(0.5f, add3(ps, po));
+    }
+    st3(iter_self, s, wp);
+    st3(begin_self, s, wp);
+}
+
+__global__ __launch_bounds__(kBlock) void k_gather_rows(const float* src, const uint32_t* idx, float* dst, uint32_t n) {
+    // dst[t] = src[idx[t]] for float3 rows
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= n) return;
+    const uint32_t k = idx[t];
+    dst[3 * t] = src[3 * k];
+    dst[3 * t + 1] = src[3 * k + 1];
+    dst[3 * t + 2] = src[3 * k + 2];
+}
+
+}  // namespace
+
+// ===========================================================================
+// Host context
+// ===========================================================================
+namespace {
+
+int fail(int code, const std::string& msg) {
+    lats::set_error(msg);
+    return code;
+}
+
+#define HIPCHK(call)                                                                            \
+    do {                                                                                        \
+        hipError_t e_ = (call);                                                                 \
+        if (e_ != hipSuccess)                                                                   \
+            return fail(LATS_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));       \
+    } while (0)
+
+template <typename T> struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    ~DBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    hipError_t alloc(size_t count) {
+        release();
+        n = count;
+        if (count == 0) return hipSuccess;
+        return hipMalloc((void**)&p, count * sizeof(T));
+    }
+    hipError_t upload(const std::vector<T>& h) {
+        hipError_t e = alloc(h.size());
+        if (e != hipSuccess || h.empty()) return e;
+        return hipMemcpy(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice);
+    }
+};
+
+inline uint32_t nblocks(uint64_t n) { return (uint32_t)((n + kBlock - 1) / kBlock); }
+
+}  // namespace
+
+struct lats_vbd {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint32_t V = 0, E = 0, B = 0, C = 0;
+    lats_scene_params prm{};
+    float h = 0.f, h2inv = 0.f, damp = 0.f, h_energy = 0.f;
+    std::vector<uint32_t> cstart, ccount, eoff, boff;
+    std::vector<uint32_t> perm, inv;
+    int strategy = LATS_STRATEGY_AUTO;
+    int energy_bending = 1;
+
+    DBuf<float> x, x0, v, mass;
+    DBuf<uint32_t> meta, perm_d, inv_d;
+    DBuf<uint2> ell_e, e_slots;
+    DBuf<float4> ell_bq, Qd;
+    DBuf<uint4> ell_bi, b_slots;
+    DBuf<float> rest;
+    DBuf<FrameParams> fp;
+    DBuf<float> forces;
+    DBuf<double> partial;
+    DBuf<float> energy;
+    DBuf<float> scratch3;   // 3V staging for permutations
+    uint32_t energy_rows = 0, energy_cap = 0, energy_nblk = 0;
+    bool record_dispatch = false;
+
+    // multi-buffer positions of the async schedule
+    DBuf<float> iterbuf[LATS_NUM_ASYNC_BUFFERS], beginbuf[LATS_NUM_ASYNC_BUFFERS];
+
+    // graph per (record) flag
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    float last_ms = 0.f;
+
+    std::map<uint32_t, hipEvent_t> markers;
+    uint32_t* host_flag = nullptr;    // pinned, mapped
+    uint32_t* host_flag_d = nullptr;  // device alias
+
+    ~lats_vbd() {
+        for (auto& g : graph)
+            if (g) (void)hipGraphExecDestroy(g);
+        for (auto& kv : markers) (void)hipEventDestroy(kv.second);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (host_flag) (void)hipHostFree(host_flag);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    VbdArgs args(float* xbuf) const {
+        VbdArgs a;
+        a.x = xbuf;
+        a.x0 = x0.p;
+        a.v = v.p;
+        a.mass = mass.p;
+        a.meta = meta.p;
+        a.ell_e = ell_e.p;
+        a.ell_bq = ell_bq.p;
+        a.ell_bi = ell_bi.p;
+        a.fp = fp.p;
+        a.h = h;
+        a.h2inv = h2inv;
+        a.ks = prm.stiffness_stretch_spring;
+        a.kb = prm.stiffness_quadratic_bending;
+        for (int c = 0; c < 3; c++) a.g[c] = prm.gravity[c];
+        return a;
+    }
+};
+
+namespace {
+
+int ensure_energy_rows(lats_vbd* c, uint32_t rows) {
+    if (rows <= c->energy_cap) return LATS_OK;
+    const uint32_t cap = std::max(rows, c->energy_cap * 2 + 8);
+    DBuf<double> np;
+    DBuf<float> ne;
+    HIPCHK(np.alloc((size_t)cap * c->energy_nblk));
+    HIPCHK(ne.alloc(cap));
+    if (c->energy_rows) {
+        HIPCHK(hipMemcpyAsync(np.p, c->partial.p, sizeof(double) * (size_t)c->energy_rows * c->energy_nblk,
+                              hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
+    std::swap(c->partial.p, np.p);
+    std::swap(c->partial.n, np.n);
+    std::swap(c->energy.p, ne.p);
+    std::swap(c->energy.n, ne.n);
+    c->energy_cap = cap;
+    // graphs captured against the old buffers are stale
+    for (auto& g : c->graph)
+        if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+    return LATS_OK;
+}
+
+// Enqueue one energy evaluation of buffer xb into history row `row`.
+void enqueue_energy(lats_vbd* c, const float* xb, uint32_t row) {
+    EnergyArgs a;
+    a.x = xb;
+    a.x0 = c->x0.p;
+    a.v = c->v.p;
+    a.mass = c->mass.p;
+    a.meta = c->meta.p;
+    a.edges = c->e_slots.p;
+    a.rest = c->rest.p;
+    a.pairs = c->b_slots.p;
+    a.Q = c->Qd.p;
+    a.fp = c->fp.p;
+    a.nv = c->V;
+    a.ne = c->E;
+    a.nb = c->energy_bending ? c->B : 0;
+    a.h = c->h_energy;
+    a.ks = c->prm.stiffness_stretch_spring;
+    a.kb = c->prm.stiffness_quadratic_bending;
+    for (int k = 0; k < 3; k++) a.g[k] = c->prm.gravity[k];
+    hipLaunchKernelGGL(k_energy_partial, dim3(c->energy_nblk), dim3(kBlock), 0, c->stream, a,
+                       c->partial.p + (size_t)row * c->energy_nblk);
+}
+
+void enqueue_colour(lats_vbd* c, float* xb, uint32_t col) {
+    const uint32_t n = c->ccount[col];
+    hipLaunchKernelGGL(k_vbd_colour, dim3(nblocks(n)), dim3(kBlock), 0, c->stream, c->args(xb), c->cstart[col], n,
+                       c->eoff[col], c->boff[col]);
+}
+
+void enqueue_predict(lats_vbd* c) {
+    hipLaunchKernelGGL(k_predict, dim3(std::max(1u, nblocks(c->V))), dim3(kBlock), 0, c->stream, c->x.p, c->x0.p,
+                       c->v.p, c->meta.p, c->fp.p, c->h, c->V);
+}
+
+void enqueue_update(lats_vbd* c, int advance) {
+    hipLaunchKernelGGL(k_update_velocity, dim3(std::max(1u, nblocks(c->V))), dim3(kBlock), 0, c->stream, c->x.p,
+                       c->x0.p, c->v.p, c->fp.p, c->h, c->damp, c->V, advance);
+}
+
+// The whole frame as a kernel sequence (captured into a graph by the caller).
+void enqueue_frame(lats_vbd* c, bool record) {
+    const uint32_t nsub = record ? 1u : c->prm.num_substep;
+    for (uint32_t sub = 0; sub < nsub; sub++) {
+        enqueue_predict(c);
+        for (uint32_t it = 0; it < c->prm.constraint_iter_count; it++) {
+            if (record && it == 0) enqueue_energy(c, c->x.p, 0);
+            for (uint32_t col = 0; col < c->C; col++) enqueue_colour(c, c->x.p, col);
+            if (record) enqueue_energy(c, c->x.p, it + 1);
+        }
+        enqueue_update(c, sub + 1 == nsub);
+    }
+    if (record)
+        hipLaunchKernelGGL(k_energy_finalize, dim3(c->prm.constraint_iter_count + 1), dim3(kBlock), 0, c->stream,
+                           c->partial.p, c->energy_nblk, 0u, c->energy.p);
+}
+
+int get_graph(lats_vbd* c, bool record, hipGraphExec_t* out) {
+    if (record) {
+        int rc = ensure_energy_rows(c, c->prm.constraint_iter_count + 1);
+        if (rc) return rc;
+    }
+    hipGraphExec_t& g = c->graph[record ? 1 : 0];
+    if (!g) {
+        hipGraph_t graph;
+        HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        enqueue_frame(c, record);
+        HIPCHK(hipStreamEndCapture(c->stream, &graph));
+        hipError_t e = hipGraphInstantiate(&g, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) {
+            g = nullptr;
+            return fail(LATS_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        }
+    }
+    *out = g;
+    return LATS_OK;
+}
+
+int run_frame(lats_vbd* c, bool record) {
+    hipGraphExec_t g;
+    int rc = get_graph(c, record, &g);
+    if (rc) return rc;
+    HIPCHK(hipGraphLaunch(g, c->stream));
+    if (record) c->energy_rows = c->prm.constraint_iter_count + 1;
+    return LATS_OK;
+}
+
+int set_frame_params(lats_vbd* c, const float* forces_dev, uint32_t frame, const float* ext) {
+    FrameParams h;
+    HIPCHK(hipMemcpyAsync(&h, c->fp.p, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    h.forces = forces_dev;
+    h.frame = frame;
+    if (ext) {
+        h.ext[0] = ext[0];
+        h.ext[1] = ext[1];
+        h.ext[2] = ext[2];
+    }
+    HIPCHK(hipMemcpyAsync(c->fp.p, &h, sizeof(h), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return LATS_OK;
+}
+
+int validate_desc(const lats_mesh_desc* m) {
+    if (!m) return fail(LATS_ERR_INVALID_ARGUMENT, "mesh is null");
+    if (m->num_verts == 0) return fail(LATS_ERR_INVALID_ARGUMENT, "mesh has no vertices");
+    if (m->num_verts >= (1u << 30)) return fail(LATS_ERR_UNSUPPORTED, "more than 2^30 vertices");
+    if (!m->rest_x || !m->vert_mass || !m->is_fixed || !m->clusters_csr || !m->vert_adj_edges_csr ||
+        !m->vert_adj_bending_csr)
+        return fail(LATS_ERR_INVALID_ARGUMENT, "mesh array is null");
+    if (m->num_edges && (!m->edges || !m->edges_rest_length))
+        return fail(LATS_ERR_INVALID_ARGUMENT, "edge arrays are null");
+    if (m->num_bending_edges && (!m->bending_edges || !m->bending_Q))
+        return fail(LATS_ERR_INVALID_ARGUMENT, "bending arrays are null");
+    return LATS_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+int lats_vbd_create(const lats_mesh_desc* m, const lats_scene_params* prm, lats_vbd** out) {
+    if (!out || !prm) return fail(LATS_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    int rc = validate_desc(m);
+    if (rc) return rc;
+    if (prm->num_substep == 0 || !(prm->implicit_dt > 0.f))
+        return fail(LATS_ERR_INVALID_ARGUMENT, "implicit_dt must be > 0 and num_substep >= 1");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(LATS_ERR_NO_DEVICE, "no HIP device");
+
+    const uint32_t V = m->num_verts, E = m->num_edges, B = m->num_bending_edges, C = m->num_clusters;
+    const uint32_t* cl = m->clusters_csr;
+    if (cl[0] != C + 1 || cl[C] != C + 1 + V)
+        return fail(LATS_ERR_INVALID_ARGUMENT, "clusters CSR does not partition the vertices");
+    const uint32_t* pe = m->vert_adj_edges_csr;
+    const uint32_t* pb = m->vert_adj_bending_csr;
+    if (pe[0] != V + 1 || pb[0] != V + 1) return fail(LATS_ERR_INVALID_ARGUMENT, "adjacency CSR header mismatch");
+
+    auto c = new lats_vbd();
+    c->V = V;
+    c->E = E;
+    c->B = B;
+    c->C = C;
+    c->prm = *prm;
+    (void)hipGetDevice(&c->device);
+    // scene_params.h:159 get_substep_dt(); the energy uses the same h
+    c->h = prm->implicit_dt / (float)prm->num_substep;
+    c->h_energy = prm->implicit_dt / (float)prm->num_substep;
+    c->h2inv = 1.f / (c->h * c->h);
+    c->damp = std::exp(-prm->damping_cloth * c->h);
+
+    // --- colour order (slots)
+    c->perm.resize(V);
+    c->inv.assign(V, 0xffffffffu);
+    c->cstart.resize(C);
+    c->ccount.resize(C);
+    for (uint32_t k = 0; k < C; k++) {
+        c->cstart[k] = cl[k] - (C + 1);
+        c->ccount[k] = cl[k + 1] - cl[k];
+        if (c->ccount[k] == 0) { delete c; return fail(LATS_ERR_INVALID_ARGUMENT, "empty colour cluster"); }
+    }
+    for (uint32_t s = 0; s < V; s++) {
+        const uint32_t vid = cl[C + 1 + s];
+        if (vid >= V || c->inv[vid] != 0xffffffffu) {
+            delete c;
+            return fail(LATS_ERR_INVALID_ARGUMENT, "clusters CSR is not a permutation of the vertices");
+        }
+        c->perm[s] = vid;
+        c->inv[vid] = s;
+    }
+
+    // --- per-slot arrays and ELL adjacency
+    std::vector<float> hx(3 * (size_t)V), hmass(V);
+    std::vector<uint32_t> hmeta(V);
+    std::vector<uint32_t> e_max(C, 0), b_max(C, 0);
+    for (uint32_t s = 0; s < V; s++) {
+        const uint32_t vid = c->perm[s];
+        for (int k = 0; k < 3; k++) hx[3 * s + k] = m->rest_x[3 * (size_t)vid + k];
+        hmass[s] = m->vert_mass[vid];
+        const uint32_t ed = pe[vid + 1] - pe[vid], bd = pb[vid + 1] - pb[vid];
+        if (ed > 255 || bd > 255) { delete c; return fail(LATS_ERR_UNSUPPORTED, "vertex degree above 255"); }
+        hmeta[s] = ed | (bd << 8) | ((m->is_fixed[vid] ? 1u : 0u) << 16);
+    }
+    for (uint32_t k = 0; k < C; k++)
+        for (uint32_t i = 0; i < c->ccount[k]; i++) {
+            const uint32_t s = c->cstart[k] + i;
+            e_max[k] = std::max(e_max[k], hmeta[s] & 0xffu);
+            b_max[k] = std::max(b_max[k], (hmeta[s] >> 8) & 0xffu);
+        }
+    c->eoff.resize(C);
+    c->boff.resize(C);
+    size_t ne_tot = 0, nb_tot = 0;
+    for (uint32_t k = 0; k < C; k++) {
+        c->eoff[k] = (uint32_t)ne_tot;
+        c->boff[k] = (uint32_t)nb_tot;
+        ne_tot += (size_t)e_max[k] * c->ccount[k];
+        nb_tot += (size_t)b_max[k] * c->ccount[k];
+    }
+    if (ne_tot >= (1ull << 32) || nb_tot >= (1ull << 32)) {
+        delete c;
+        return fail(LATS_ERR_UNSUPPORTED, "adjacency too large");
+    }
+    std::vector<uint2> h_ell_e(std::max<size_t>(ne_tot, 1), uint2{0, 0});
+    std::vector<float4> h_ell_bq(std::max<size_t>(nb_tot, 1), float4{0, 0, 0, 0});
+    std::vector<uint4> h_ell_bi(std::max<size_t>(nb_tot, 1), uint4{0, 0, 0, 0});
+    for (uint32_t k = 0; k < C; k++) {
+        const uint32_t n = c->ccount[k];
+        for (uint32_t i = 0; i < n; i++) {
+            const uint32_t s = c->cstart[k] + i, vid = c->perm[s];
+            for (uint32_t j = 0, r = pe[vid]; r < pe[vid + 1]; r++, j++) {
+                const uint32_t e = pe[r];
+                if (e >= E) { delete c; return fail(LATS_ERR_INVALID_ARGUMENT, "edge id out of range"); }
+                const uint32_t a0 = (uint32_t)m->edges[2 * e], a1 = (uint32_t)m->edges[2 * e + 1];
+                const uint32_t other = (a0 == vid) ? a1 : a0;
+                uint32_t rl;
+                std::memcpy(&rl, &m->edges_rest_length[e], 4);
+                h_ell_e[c->eoff[k] + (size_t)j * n + i] = uint2{c->inv[other], rl};
+            }
+            for (uint32_t j = 0, r = pb[vid]; r < pb[vid + 1]; r++, j++) {
+                const uint32_t b = pb[r];
+                if (b >= B) { delete c; return fail(LATS_ERR_INVALID_ARGUMENT, "bending id out of range"); }
+                const int32_t* q4 = &m->bending_edges[4 * (size_t)b];
+                uint32_t o = 0;
+                while (o < 4 && (uint32_t)q4[o] != vid) o++;
+                if (o == 4) { delete c; return fail(LATS_ERR_INVALID_ARGUMENT, "bending adjacency mismatch"); }
+                const float* Q = &m->bending_Q[16 * (size_t)b + 4 * o];
+                const size_t at = c->boff[k] + (size_t)j * n + i;
+                h_ell_bq[at] = float4{Q[0], Q[1], Q[2], Q[3]};
+                h_ell_bi[at] = uint4{c->inv[q4[0]] | (o << 30), c->inv[q4[1]], c->inv[q4[2]], c->inv[q4[3]]};
+            }
+        }
+    }
+    // element lists in slot ids for the energy
+    std::vector<uint2> h_es(std::max<uint32_t>(E, 1), uint2{0, 0});
+    std::vector<float> h_rest(std::max<uint32_t>(E, 1), 0.f);
+    for (uint32_t e = 0; e < E; e++) {
+        h_es[e] = uint2{c->inv[m->edges[2 * e]], c->inv[m->edges[2 * e + 1]]};
+        h_rest[e] = m->edges_rest_length[e];
+    }
+    std::vector<uint4> h_bs(std::max<uint32_t>(B, 1), uint4{0, 0, 0, 0});
+    std::vector<float4> h_Q(4 * (size_t)std::max<uint32_t>(B, 1), float4{0, 0, 0, 0});
+    for (uint32_t b = 0; b < B; b++) {
+        const int32_t* q4 = &m->bending_edges[4 * (size_t)b];
+        h_bs[b] = uint4{c->inv[q4[0]], c->inv[q4[1]], c->inv[q4[2]], c->inv[q4[3]]};
+        for (int col = 0; col < 4; col++) {
+            const float* Q = &m->bending_Q[16 * (size_t)b + 4 * col];
+            h_Q[4 * (size_t)b + col] = float4{Q[0], Q[1], Q[2], Q[3]};
+        }
+    }
+
+    auto cleanup_fail = [&](int code) { delete c; return code; };
+#define UPCHK(call) do { hipError_t e_ = (call); if (e_ != hipSuccess) return cleanup_fail(fail(e_ == hipErrorOutOfMemory ? LATS_ERR_OUT_OF_MEMORY : LATS_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_))); } while (0)
+    UPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    UPCHK(c->x.upload(hx));
+    UPCHK(c->x0.upload(hx));
+    UPCHK(c->v.upload(std::vector<float>(3 * (size_t)V, 0.f)));
+    UPCHK(c->mass.upload(hmass));
+    UPCHK(c->meta.upload(hmeta));
+    UPCHK(c->perm_d.upload(c->perm));
+    UPCHK(c->inv_d.upload(c->inv));
+    UPCHK(c->ell_e.upload(h_ell_e));
+    UPCHK(c->ell_bq.upload(h_ell_bq));
+    UPCHK(c->ell_bi.upload(h_ell_bi));
+    UPCHK(c->e_slots.upload(h_es));
+    UPCHK(c->rest.upload(h_rest));
+    UPCHK(c->b_slots.upload(h_bs));
+    UPCHK(c->Qd.upload(h_Q));
+    UPCHK(c->scratch3.alloc(3 * (size_t)V));
+    FrameParams fp0;
+    std::memset(&fp0, 0, sizeof(fp0));
+    UPCHK(c->fp.upload(std::vector<FrameParams>{fp0}));
+    UPCHK(hipEventCreate(&c->ev0));
+    UPCHK(hipEventCreate(&c->ev1));
+    UPCHK(hipHostMalloc((void**)&c->host_flag, 64, hipHostMallocMapped));
+    *c->host_flag = 0;
+    UPCHK(hipHostGetDevicePointer((void**)&c->host_flag_d, c->host_flag, 0));
+#undef UPCHK
+    c->energy_nblk = nblocks((uint64_t)V + E + B);
+    *out = c;
+    return LATS_OK;
+}
+
+int lats_vbd_destroy(lats_vbd* c) {
+    if (!c) return LATS_OK;
+    (void)hipStreamSynchronize(c->stream);
+    delete c;
+    return LATS_OK;
+}
+
+int lats_vbd_set_strategy(lats_vbd* c, int strategy) {
+    if (!c) return fail(LATS_ERR_INVALID_ARGUMENT, "null context");
+    if (strategy < LATS_STRATEGY_AUTO || strategy > LATS_STRATEGY_PERSISTENT)
+        return fail(LATS_ERR_INVALID_ARGUMENT, "unknown strategy");
+    if (strategy == LATS_STRATEGY_PERSISTENT) return fail(LATS_ERR_UNSUPPORTED, "persistent strategy not built");
+    c->strategy = strategy;
+    return LATS_OK;
+}
+
+int lats_vbd_set_energy_bending(lats_vbd* c, int on) {
+    if (!c) return fail(LATS_ERR_INVALID_ARGUMENT, "null context");
+    if ((on != 0) != (c->energy_bending != 0)) {
+        c->energy_bending = on != 0;
+        for (auto& g : c->graph)
+            if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+    }
+    return LATS_OK;
+}
+
+int lats_vbd_set_state(lats_vbd* c, const float* x, const float* v) {
+    if (!c || !x || !v) return fail(LATS_ERR_INVALID_ARGUMENT, "null argument");
+    const size_t n = 3 * (size_t)c->V;
+    std::vector<float> hx(n), hv(n);
+    for (uint32_t s = 0; s < c->V; s++)
+        for (int k = 0; k < 3; k++) {
+            hx[3 * s + k] = x[3 * (size_t)c->perm[s] + k];
+            hv[3 * s + k] = v[3 * (size_t)c->perm[s] + k];
+        }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(c->x.p, hx.data(), n * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->x0.p, hx.data(), n * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->v.p, hv.data(), n * 4, hipMemcpyHostToDevice));
+    return LATS_OK;
+}
+
+int lats_vbd_get_state(lats_vbd* c, float* x, float* v) {
+    if (!c) return fail(LATS_ERR_INVALID_ARGUMENT, "null context");
+    const size_t n = 3 * (size_t)c->V;
+    std::vector<float> hx(n), hv(n);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(hx.data(), c->x.p, n * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(hv.data(), c->v.p, n * 4, hipMemcpyDeviceToHost));
+    for (uint32_t s = 0; s < c->V; s++)
+        for (int k = 0; k < 3; k++) {
+            if (x) x[3 * (size_t)c->perm[s] + k] = hx[3 * s + k];
+            if (v) v[3 * (size_t)c->perm[s] + k] = hv[3 * s + k];
+        }
+    return LATS_OK;
+}
+
+int lats_vbd_set_external_force(lats_vbd* c, const float f[3]) {
+    if (!c || !f) return fail(LATS_ERR_INVALID_ARGUMENT, "null argument");
+    return set_frame_params(c, nullptr, 0, f);
+}
+
+int lats_vbd_physics_step(lats_vbd* c, int record) {
+    if (!c) return fail(LATS_ERR_INVALID_ARGUMENT, "null context");
+    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    int rc = run_frame(c, record != 0);
+    if (rc) return rc;
+    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    HIPCHK(hipEventSynchronize(c->ev1));
+    HIPCHK(hipEventElapsedTime(&c->last_ms, c->ev0, c->ev1));
+    return LATS_OK;
+}
+
+int lats_vbd_get_energy(lats_vbd* c, float* out, uint32_t cap, uint32_t* count) {
+    if (!c) return fail(LATS_ERR_INVALID_ARGUMENT, "null context");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const uint32_t n = std::min(cap, c->energy_rows);
+    if (count) *count = c->energy_rows;
+    if (n && out) HIPCHK(hipMemcpy(out, c->energy.p, n * sizeof(float), hipMemcpyDeviceToHost));
+    return LATS_OK;
+}
+
+int lats_vbd_run_frames(lats_vbd* c, uint32_t nframes, const float* forces) {
+    if (!c) return fail(LATS_ERR_INVALID_ARGUMENT, "null context");
+    if (forces) {
+        if (c->forces.n < 3 * (size_t)nframes) HIPCHK(c->forces.alloc(3 * (size_t)nframes));
+        HIPCHK(hipMemcpy(c->forces.p, forces, 3 * (size_t)nframes * sizeof(float), hipMemcpyHostToDevice));
+        int rc = set_frame_params(c, c->forces.p, 0, nullptr);
+        if (rc) return rc;
+    }
+    hipGraphExec_t g;
+    int rc = get_graph(c, false, &g);
+    if (rc) return rc;
+    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    for (uint32_t f = 0; f < nframes; f++) HIPCHK(hipGraphLaunch(g, c->stream));
+    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    HIPCHK(hipEventSynchronize(c->ev1));
+    HIPCHK(hipEventElapsedTime(&c->last_ms, c->ev0, c->ev1));
+    if (forces) {
+        rc = set_frame_params(c, nullptr, 0, forces + 3 * (size_t)(nframes ? nframes - 1 : 0));
+        if (rc) return rc;
+    }
+    return LATS_OK;
+}
+
+int lats_vbd_last_elapsed_ms(lats_vbd* c, float* ms) {
+    if (!c || !ms) return fail(LATS_ERR_INVALID_ARGUMENT, "null argument");
+    *ms = c->last_ms;
+    return LATS_OK;
+}
+
+int lats_vbd_synchronize(lats_vbd* c) {
+    if (!c) return fail(LATS_ERR_INVALID_ARGUMENT, "null context");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return LATS_OK;
+}
+
+// ---- scheduler slot ---------------------------------------------------------
+static int ensure_async(lats_vbd* c, uint32_t idx) {
+    const uint32_t k = idx % LATS_NUM_ASYNC_BUFFERS;
+    if (!c->iterbuf[k].p) {
+        HIPCHK(c->iterbuf[k].alloc(3 * (size_t)c->V));
+        HIPCHK(c->beginbuf[k].alloc(3 * (size_t)c->V));
+    }
+    return LATS_OK;
+}
+static float* iter_buffer(lats_vbd* c, uint32_t idx) {
+    return idx == LATS_BUFFER_DEFAULT ? c->x.p : c->iterbuf[idx % LATS_NUM_ASYNC_BUFFERS].p;
+}
+static float* begin_buffer(lats_vbd* c, uint32_t idx) { return c->beginbuf[idx % LATS_NUM_ASYNC_BUFFERS].p; }
+
+static int dispatch_energy(lats_vbd* c, const float* xb) {
+    int rc = ensure_energy_rows(c, c->energy_rows + 1);
+    if (rc) return rc;
+    enqueue_energy(c, xb, c->energy_rows);
+    hipLaunchKernelGGL(k_energy_finalize, dim3(1), dim3(kBlock), 0, c->stream, c->partial.p, c->energy_nblk,
+                       c->energy_rows, c->energy.p);
+    c->energy_rows++;
+    HIPCHK(hipGetLastError());
+    return LATS_OK;
+}
+
+int lats_vbd_set_record_energy(lats_vbd* c, int on) {
+    if (!c) return fail(LATS_ERR_INVALID_ARGUMENT, "null context");
+    c->record_dispatch = on != 0;
+    if (on) c->energy_rows = 0;
+    return LATS_OK;
+}
+
+int lats_vbd_dispatch(lats_vbd* c, const lats_launch_param* p) {
+    if (!c || !p) return fail(LATS_ERR_INVALID_ARGUMENT, "null argument");
+    const uint32_t nf = 3 * c->V;
+    auto need = [&](uint32_t idx) -> int {
+        return (idx == LATS_BUFFER_DEFAULT || idx == LATS_BUFFER_NONE) ? LATS_OK : ensure_async(c, idx);
+    };
+    auto copy_to_start_and_iter = [&](const float* src, uint32_t out_idx) -> int {
+        int rc = need(out_idx);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_copy_to_2, dim3(nblocks(nf)), dim3(kBlock), 0, c->stream, src,
+                           out_idx == LATS_BUFFER_DEFAULT ? nullptr : begin_buffer(c, out_idx),
+                           iter_buffer(c, out_idx), nf);
+        return LATS_OK;
+    };
+    // GpuSolver::fn_dispatch prev hook (example3_hybrid_cpu_gpu.cpp:1769-1825)
+    auto prev = [&]() -> int {
+        int rc = need(p->buffer_idx);
+        if (rc) return rc;
+        if (p->num_input_buffers && p->left_buffer_idx != LATS_BUFFER_NONE) {
+            rc = need(p->left_buffer_idx);
+            if (rc) return rc;
+            for (uint32_t k = 0; k < p->num_input_buffers; k++) {
+                const uint32_t in = p->input_buffer_idxs[k];
+                rc = need(in);
+                if (rc) return rc;
+                float* bb = p->is_allocated_to_main_device ? begin_buffer(c, in) : begin_buffer(c, p->left_buffer_idx);
+                hipLaunchKernelGGL(k_read_and_solve_conflict, dim3(nblocks(c->V)), dim3(kBlock), 0, c->stream, bb,
+                                   iter_buffer(c, in), iter_buffer(c, p->left_buffer_idx), 0.5f, c->V);
+            }
+        } else if (p->num_input_buffers) {
+            const uint32_t in = p->input_buffer_idxs[p->num_input_buffers - 1];
+            rc = need(in);
+            if (rc) return rc;
+            rc = copy_to_start_and_iter(iter_buffer(c, in), p->buffer_idx);
+            if (rc) return rc;
+        } else if (p->left_buffer_idx == LATS_BUFFER_DEFAULT) {
+            rc = copy_to_start_and_iter(c->x.p, p->buffer_idx);
+            if (rc) return rc;
+        }
+        if (c->record_dispatch && p->iter_idx == 0 && p->cluster_idx == 0) return dispatch_energy(c, iter_buffer(c, p->buffer_idx));
+        return LATS_OK;
+    };
+    // post hook (:1827-1855)
+    auto post = [&]() -> int {
+        if (p->right_buffer_idx != LATS_BUFFER_NONE) {
+            int rc = copy_to_start_and_iter(iter_buffer(c, p->buffer_idx), p->right_buffer_idx);
+            if (rc) return rc;
+        }
+        if (c->record_dispatch &&
+            ((p->function_id == LATS_FN_VBD_ALL_IN_ONE && p->cluster_idx + 1 == c->C) ||
+             p->function_id == LATS_FN_CONSTRAINT_LAST_NODE))
+            return dispatch_energy(c, iter_buffer(c, p->buffer_idx));
+        return LATS_OK;
+    };
+    int rc = LATS_OK;
+    switch (p->function_id) {
+        case LATS_FN_ADDITIONAL_ROOT:
+        case LATS_FN_ADDITIONAL_TERMINAL:
+        case LATS_FN_RESET_CONSTRAINS:
+            break;
+        case LATS_FN_PREDICT_POSITION:
+            enqueue_predict(c);
+            break;
+        case LATS_FN_UPDATE_VELOCITY:
+            enqueue_update(c, 1);
+            break;
+        case LATS_FN_VBD_ALL_IN_ONE: {
+            if (p->cluster_idx >= c->C) return fail(LATS_ERR_INVALID_ARGUMENT, "cluster_idx out of range");
+            rc = prev();
+            if (rc) return rc;
+            enqueue_colour(c, iter_buffer(c, p->buffer_idx), p->cluster_idx);
+            rc = post();
+            break;
+        }
+        case LATS_FN_CONSTRAINT_LAST_NODE: {
+            rc = prev();
+            if (rc) return rc;
+            rc = post();
+            if (rc) return rc;
+            float* src = iter_buffer(c, p->buffer_idx);
+            if (src != c->x.p)
+                hipLaunchKernelGGL(k_copy_to_2, dim3(nblocks(nf)), dim3(kBlock), 0, c->stream, src, c->x.p,
+                                   (float*)nullptr, nf);
+            break;
+        }
+        default:
+            return fail(LATS_ERR_INVALID_ARGUMENT, "unknown function id");
+    }
+    if (rc) return rc;
+    HIPCHK(hipGetLastError());
+    return LATS_OK;
+}
+
+// ---- host <-> device signalling ------------------------------------------
+int lats_vbd_stream_wait_host(lats_vbd* c, uint32_t value) {
+    if (!c) return fail(LATS_ERR_INVALID_ARGUMENT, "null context");
+    HIPCHK(hipStreamWaitValue32(c->stream, c->host_flag_d, value, hipStreamWaitValueGte, 0xffffffffu));
+    return LATS_OK;
+}
+int lats_vbd_host_signal(lats_vbd* c, uint32_t value) {
+    if (!c) return fail(LATS_ERR_INVALID_ARGUMENT, "null context");
+    __atomic_store_n(c->host_flag, value, __ATOMIC_SEQ_CST);
+    return LATS_OK;
+}
+int lats_vbd_record_marker(lats_vbd* c, uint32_t marker) {
+    if (!c) return fail(LATS_ERR_INVALID_ARGUMENT, "null context");
+    auto it = c->markers.find(marker);
+    if (it == c->markers.end()) {
+        hipEvent_t e;
+        HIPCHK(hipEventCreate(&e));
+        it = c->markers.emplace(marker, e).first;
+    }
+    HIPCHK(hipEventRecord(it->second, c->stream));
+    return LATS_OK;
+}
+int lats_vbd_wait_marker(lats_vbd* c, uint32_t marker) {
+    if (!c) return fail(LATS_ERR_INVALID_ARGUMENT, "null context");
+    auto it = c->markers.find(marker);
+    if (it == c->markers.end()) return fail(LATS_ERR_INVALID_ARGUMENT, "marker never recorded");
+    HIPCHK(hipEventSynchronize(it->second));
+    return LATS_OK;
+}
+int lats_vbd_marker_elapsed_ms(lats_vbd* c, uint32_t m0, uint32_t m1, float* ms) {
+    if (!c || !ms) return fail(LATS_ERR_INVALID_ARGUMENT, "null argument");
+    auto a = c->markers.find(m0), b = c->markers.find(m1);
+    if (a == c->markers.end() || b == c->markers.end()) return fail(LATS_ERR_INVALID_ARGUMENT, "marker never recorded");
+    HIPCHK(hipEventElapsedTime(ms, a->second, b->second));
+    return LATS_OK;
+}
+
+int lats_vbd_device_positions(lats_vbd* c, void** d) {
+    if (!c || !d) return fail(LATS_ERR_INVALID_ARGUMENT, "null argument");
+    *d = c->x.p;
+    return LATS_OK;
+}
+int lats_vbd_copy_permutation(lats_vbd* c, uint32_t* perm) {
+    if (!c || !perm) return fail(LATS_ERR_INVALID_ARGUMENT, "null argument");
+    std::copy(c->perm.begin(), c->perm.end(), perm);
+    return LATS_OK;
+}
+int lats_vbd_time_colour_kernels(lats_vbd* c, uint32_t iters, float* mean_ms, uint32_t* launches,
+                                 uint64_t* vertex_updates) {
+    if (!c || !mean_ms) return fail(LATS_ERR_INVALID_ARGUMENT, "null argument");
+    const size_t nf = 3 * (size_t)c->V;
+    HIPCHK(hipMemcpyAsync(c->scratch3.p, c->x.p, nf * 4, hipMemcpyDeviceToDevice, c->stream));
+    const uint32_t nl = iters * c->C;
+    std::vector<hipEvent_t> ev(2 * (size_t)nl, nullptr);
+    int rc = LATS_OK;
+    for (auto& e : ev)
+        if (hipEventCreate(&e) != hipSuccess) { rc = fail(LATS_ERR_HIP, "hipEventCreate"); break; }
+    uint64_t vu = 0;
+    if (!rc) {
+        uint32_t k = 0;
+        for (uint32_t it = 0; it < iters; it++)
+            for (uint32_t col = 0; col < c->C; col++, k++) {
+                (void)hipEventRecord(ev[2 * k], c->stream);
+                enqueue_colour(c, c->x.p, col);
+                (void)hipEventRecord(ev[2 * k + 1], c->stream);
+                vu += c->ccount[col];
+            }
+        hipError_t e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) rc = fail(LATS_ERR_HIP, hipGetErrorString(e));
+    }
+    double tot = 0.0;
+    if (!rc)
+        for (uint32_t k = 0; k < nl; k++) {
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]);
+            tot += ms;
+        }
+    for (auto& e : ev)
+        if (e) (void)hipEventDestroy(e);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(c->x.p, c->scratch3.p, nf * 4, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    *mean_ms = nl ? (float)(tot / nl) : 0.f;
+    if (launches) *launches = nl;
+    if (vertex_updates) *vertex_updates = vu;
+    return LATS_OK;
+}
+
+int lats_vbd_device_bytes(lats_vbd* c, uint64_t* bytes) {
+    if (!c || !bytes) return fail(LATS_ERR_INVALID_ARGUMENT, "null argument");
+    uint64_t b = 0;
+    b += (c->x.n + c->x0.n + c->v.n + c->mass.n + c->rest.n + c->scratch3.n + c->forces.n + c->energy.n) * 4;
+    b += (c->meta.n + c->perm_d.n + c->inv_d.n) * 4;
+    b += (c->ell_e.n + c->e_slots.n) * 8 + (c->ell_bq.n + c->Qd.n + c->ell_bi.n + c->b_slots.n) * 16;
+    b += c->partial.n * 8;
+    for (int k = 0; k < LATS_NUM_ASYNC_BUFFERS; k++) b += (c->iterbuf[k].n + c->beginbuf[k].n) * 4;
+    *bytes = b;
+    return LATS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,294 @@
+"""Host-side Python front-end of the VBD cloth GPU slot.
+
+Mirrors the reference's driver interface for the VBD path:
+`SolverInterface` (examples/example2_async_iter.cpp:1550-1710: physics_step,
+restart_system, save_current_frame_state, load_saved_state,
+save_mesh_to_obj) over the HIP context of include/libatssim_hip.h.  The
+topology, colouring and clusters come from the library's own builder
+(lats_topology_*), in the reference's formats.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+# ---------------------------------------------------------------------------
+# scenes
+# ---------------------------------------------------------------------------
+def make_grid(nx, ny, spacing=0.01, jitter=1e-4, seed=0):
+    """Seeded nx x ny grid cloth in the xy plane (row 0 on top, y = 0)."""
+    x = np.zeros((nx * ny, 3), np.float32)
+    f = np.zeros(((nx - 1) * (ny - 1) * 2, 3), np.int32)
+    check(lib().lats_scene_grid(nx, ny, spacing, jitter, seed, _p(x), _p(f)))
+    return x, f
+
+
+def make_wind(num_frames, num_verts, max_total=2.0, seed=0):
+    w = np.zeros((num_frames, 3), np.float32)
+    check(lib().lats_scene_wind(num_frames, num_verts, max_total, seed, _p(w)))
+    return w
+
+
+# ---------------------------------------------------------------------------
+# topology
+# ---------------------------------------------------------------------------
+class Topology:
+    """Edges, bending pairs, rest state, adjacency CSRs and colour clusters."""
+
+    _ARRAYS = {
+        "edges": (_lib.LATS_TOPO_EDGES, np.int32, 2),
+        "bending": (_lib.LATS_TOPO_BENDING, np.int32, 4),
+        "rest_length": (_lib.LATS_TOPO_REST_LENGTH, np.float32, 1),
+        "bending_Q": (_lib.LATS_TOPO_BENDING_Q, np.float32, 16),
+        "adj_edges_csr": (_lib.LATS_TOPO_ADJ_EDGES_CSR, np.uint32, 1),
+        "adj_bending_csr": (_lib.LATS_TOPO_ADJ_BENDING_CSR, np.uint32, 1),
+        "clusters_csr": (_lib.LATS_TOPO_CLUSTERS_CSR, np.uint32, 1),
+        "colour": (_lib.LATS_TOPO_COLOUR, np.uint8, 1),
+    }
+
+    def __init__(self, x, faces):
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, 3)
+        faces = np.ascontiguousarray(faces, np.int32).reshape(-1, 3)
+        h = ctypes.c_void_p()
+        check(lib().lats_topology_build(_p(x), len(x), _p(faces), len(faces), ctypes.byref(h)))
+        try:
+            c = np.zeros(4, np.uint32)
+            check(lib().lats_topology_counts(h, _p(c)))
+            self.num_verts, self.num_edges, self.num_bending, self.num_clusters = (int(v) for v in c)
+            for name, (which, dt, w) in self._ARRAYS.items():
+                n = ctypes.c_uint64()
+                check(lib().lats_topology_size(h, which, ctypes.byref(n)))
+                a = np.zeros(n.value, dt)
+                if n.value:
+                    check(lib().lats_topology_copy(h, which, _p(a)))
+                setattr(self, name, a.reshape(-1, w) if w > 1 else a)
+        finally:
+            lib().lats_topology_destroy(h)
+        self.bending_Q = self.bending_Q.reshape(-1, 4, 4)
+
+    def clusters(self):
+        p = self.clusters_csr
+        return [p[p[i]:p[i + 1]] for i in range(self.num_clusters)]
+
+
+# ---------------------------------------------------------------------------
+# parameters of a run
+# ---------------------------------------------------------------------------
+class Params:
+    """The SceneParams fields the VBD path reads (scene_params.h:9-163)."""
+
+    def __init__(self, implicit_dt=1.0 / 60.0, num_substep=1, constraint_iter_count=100,
+                 stiffness_stretch_spring=1e4, stiffness_quadratic_bending=1e-2, damping_cloth=2.0,
+                 gravity=(0.0, -9.8, 0.0)):
+        self.implicit_dt = float(np.float32(implicit_dt))
+        self.num_substep = int(num_substep)
+        self.constraint_iter_count = int(constraint_iter_count)
+        self.stiffness_stretch_spring = float(stiffness_stretch_spring)
+        self.stiffness_quadratic_bending = float(stiffness_quadratic_bending)
+        self.damping_cloth = float(damping_cloth)
+        self.gravity = tuple(float(g) for g in gravity)
+
+    def c_struct(self):
+        s = _lib.SceneParams()
+        s.implicit_dt = self.implicit_dt
+        s.num_substep = self.num_substep
+        s.constraint_iter_count = self.constraint_iter_count
+        s.stiffness_stretch_spring = self.stiffness_stretch_spring
+        s.stiffness_quadratic_bending = self.stiffness_quadratic_bending
+        s.damping_cloth = self.damping_cloth
+        s.gravity = (ctypes.c_float * 3)(*self.gravity)
+        return s
+
+
+class Cloth:
+    """One cloth: rest positions, faces, per-vertex mass and pins, topology."""
+
+    def __init__(self, x, faces, mass, fixed, topology=None):
+        self.x = np.ascontiguousarray(x, np.float32).reshape(-1, 3)
+        self.faces = np.ascontiguousarray(faces, np.int32).reshape(-1, 3)
+        nv = len(self.x)
+        self.mass = np.ascontiguousarray(np.broadcast_to(np.asarray(mass, np.float32), (nv,)))
+        self.fixed = np.ascontiguousarray(np.asarray(fixed).astype(np.uint8).reshape(nv))
+        self.topology = topology if topology is not None else Topology(self.x, self.faces)
+
+    @property
+    def num_verts(self):
+        return len(self.x)
+
+
+# ---------------------------------------------------------------------------
+# GPU solver
+# ---------------------------------------------------------------------------
+class GpuVBD:
+    """Single-cloth VBD on the GPU (the reference's GpuSolver VBD path)."""
+
+    def __init__(self, cloth, params, strategy=_lib.LATS_STRATEGY_AUTO):
+        self.cloth = cloth
+        self.params = params
+        t = cloth.topology
+        self._keep = [cloth.x, t.edges, t.bending, t.rest_length, t.bending_Q, t.adj_edges_csr,
+                      t.adj_bending_csr, t.clusters_csr, cloth.mass, cloth.fixed]
+        d = _lib.MeshDesc()
+        d.num_verts = t.num_verts
+        d.num_edges = t.num_edges
+        d.num_bending_edges = t.num_bending
+        d.num_clusters = t.num_clusters
+        d.rest_x = _p(cloth.x).value
+        d.edges = _p(t.edges).value
+        d.bending_edges = _p(t.bending).value
+        d.edges_rest_length = _p(t.rest_length).value
+        d.bending_Q = _p(t.bending_Q).value
+        d.vert_adj_edges_csr = _p(t.adj_edges_csr).value
+        d.vert_adj_bending_csr = _p(t.adj_bending_csr).value
+        d.clusters_csr = _p(t.clusters_csr).value
+        d.vert_mass = _p(cloth.mass).value
+        d.is_fixed = _p(cloth.fixed).value
+        self._params = params.c_struct()
+        h = ctypes.c_void_p()
+        check(lib().lats_vbd_create(ctypes.byref(d), ctypes.byref(self._params), ctypes.byref(h)))
+        self._h = h
+        if strategy != _lib.LATS_STRATEGY_AUTO:
+            check(lib().lats_vbd_set_strategy(self._h, strategy))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().lats_vbd_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    def set_energy_bending(self, on):
+        check(lib().lats_vbd_set_energy_bending(self._h, int(bool(on))))
+
+    def set_state(self, x, v):
+        x = np.ascontiguousarray(x, np.float32)
+        v = np.ascontiguousarray(v, np.float32)
+        check(lib().lats_vbd_set_state(self._h, _p(x), _p(v)))
+
+    def state(self):
+        n = self.cloth.num_verts
+        x = np.zeros((n, 3), np.float32)
+        v = np.zeros((n, 3), np.float32)
+        check(lib().lats_vbd_get_state(self._h, _p(x), _p(v)))
+        return x, v
+
+    def set_external_force(self, f):
+        f = np.ascontiguousarray(f, np.float32)
+        check(lib().lats_vbd_set_external_force(self._h, _p(f)))
+
+    def physics_step(self, record_energy=False):
+        check(lib().lats_vbd_physics_step(self._h, int(bool(record_energy))))
+        return self.energy() if record_energy else None
+
+    def energy(self):
+        cnt = ctypes.c_uint32()
+        check(lib().lats_vbd_get_energy(self._h, None, 0, ctypes.byref(cnt)))
+        e = np.zeros(cnt.value, np.float32)
+        if cnt.value:
+            check(lib().lats_vbd_get_energy(self._h, _p(e), cnt.value, ctypes.byref(cnt)))
+        return e
+
+    def run_frames(self, num_frames, forces=None):
+        fp = None
+        if forces is not None:
+            forces = np.ascontiguousarray(forces, np.float32).reshape(-1, 3)
+            assert len(forces) >= num_frames
+            fp = _p(forces)
+        check(lib().lats_vbd_run_frames(self._h, int(num_frames), fp))
+
+    def last_elapsed_ms(self):
+        ms = ctypes.c_float()
+        check(lib().lats_vbd_last_elapsed_ms(self._h, ctypes.byref(ms)))
+        return ms.value
+
+    def synchronize(self):
+        check(lib().lats_vbd_synchronize(self._h))
+
+    def dispatch(self, function_id, iter_idx=0, cluster_idx=0, buffer_idx=_lib.LATS_BUFFER_DEFAULT,
+                 left_buffer_idx=_lib.LATS_BUFFER_NONE, right_buffer_idx=_lib.LATS_BUFFER_NONE,
+                 input_buffer_idxs=(), is_allocated_to_main_device=True):
+        p = _lib.LaunchParam()
+        p.function_id = function_id
+        p.iter_idx = iter_idx
+        p.cluster_idx = cluster_idx
+        p.buffer_idx = buffer_idx
+        p.left_buffer_idx = left_buffer_idx
+        p.right_buffer_idx = right_buffer_idx
+        arr = (ctypes.c_uint32 * max(1, len(input_buffer_idxs)))(*input_buffer_idxs)
+        p.num_input_buffers = len(input_buffer_idxs)
+        p.input_buffer_idxs = arr
+        p.is_allocated_to_main_device = int(bool(is_allocated_to_main_device))
+        check(lib().lats_vbd_dispatch(self._h, ctypes.byref(p)))
+
+    def set_record_energy(self, on):
+        check(lib().lats_vbd_set_record_energy(self._h, int(bool(on))))
+
+    def permutation(self):
+        p = np.zeros(self.cloth.num_verts, np.uint32)
+        check(lib().lats_vbd_copy_permutation(self._h, _p(p)))
+        return p
+
+    def time_colour_kernels(self, iterations):
+        """(mean ms per colour-kernel launch, launches, vertex updates), HIP events."""
+        ms = ctypes.c_float()
+        n = ctypes.c_uint32()
+        vu = ctypes.c_uint64()
+        check(lib().lats_vbd_time_colour_kernels(self._h, int(iterations), ctypes.byref(ms), ctypes.byref(n),
+                                                 ctypes.byref(vu)))
+        return ms.value, n.value, vu.value
+
+    def device_bytes(self):
+        b = ctypes.c_uint64()
+        check(lib().lats_vbd_device_bytes(self._h, ctypes.byref(b)))
+        return b.value
+
+
+class SolverInterface:
+    """Frame-level driver (example2_async_iter.cpp:1550-1710) over GpuVBD."""
+
+    def __init__(self, cloth, params, **kw):
+        self.cloth = cloth
+        self.solver = GpuVBD(cloth, params, **kw)
+        self.current_frame = 0
+        self._saved = None
+
+    def physics_step(self, record_energy=False):
+        e = self.solver.physics_step(record_energy)
+        self.current_frame += 1
+        return e
+
+    def restart_system(self):
+        self.solver.set_state(self.cloth.x, np.zeros_like(self.cloth.x))
+        self.current_frame = 0
+
+    def save_current_frame_state(self):
+        self._saved = self.solver.state()
+
+    def load_saved_state(self):
+        if self._saved is None:
+            raise RuntimeError("no saved state")
+        self.solver.set_state(*self._saved)
+
+    def save_mesh_to_obj(self, path):
+        """Writes the current positions as an OBJ (example2_async_iter.cpp:1649-1710)."""
+        x, _ = self.solver.state()
+        os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+        with open(path, "w") as f:
+            f.write("o mesh_0\n")
+            for p in x:
+                f.write(f"v {p[0]} {p[1]} {p[2]}\n")
+            for t in self.cloth.faces + 1:
+                f.write(f"f {t[0]} {t[1]} {t[2]}\n")

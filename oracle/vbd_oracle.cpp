@@ -1,0 +1,623 @@
+// vbd_oracle.cpp -- TEST INFRASTRUCTURE ONLY.
+//
+// CPU restatement of the reference's Vertex Block Descent (VBD) cloth path,
+// used as the checker for the HIP product path.  Only tests/, the smoke()
+// entry and bench.py's cpu_baseline leg may load this library.  Nothing in
+// the product links it or falls back to it.
+//
+// Every function cites the reference file:line it follows (paths relative to
+// the reference checkout).  Arithmetic is written operation by operation in
+// the reference's evaluation order (glm semantics on Linux), compiled with
+// -ffp-contract=off, so FP32 results are reproducible bit for bit.
+// The same code is instantiated in double for the FP64 error statement.
+//
+// Third-party arithmetic restated here (glm, version unpinned by the
+// reference's CMake `find_package(glm CONFIG REQUIRED)`):
+//   glm::dot(vec3)         = (a.x*b.x + a.y*b.y) + a.z*b.z
+//   glm::length(vec3)      = sqrt(dot(v, v))
+//   glm::cross(x, y)       = (x.y*y.z - y.y*x.z, x.z*y.x - y.z*x.x, x.x*y.y - y.x*x.y)
+//   glm::determinant(mat3) = + m00*(m11*m22 - m21*m12) - m10*(m01*m22 - m21*m02)
+//                            + m20*(m01*m12 - m11*m02)     (m[col][row])
+//   mat3 * vec3            = row i: (m[0][i]*v.x + m[1][i]*v.y) + m[2][i]*v.z
+//
+// Pinning (how each part is checked) lives in tests/test_oracle_*.py:
+// published energy history of the reference run on square8K, the published
+// colour count / cluster sizes, closed forms, finite differences, FP64 solves
+// and brute force on tiny meshes.
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef uint32_t u32;
+
+namespace orc {
+
+template <typename T> struct V3 {
+    T x, y, z;
+    T& operator[](int i) { return i == 0 ? x : (i == 1 ? y : z); }
+    T operator[](int i) const { return i == 0 ? x : (i == 1 ? y : z); }
+};
+template <typename T> static inline V3<T> mk(T a, T b, T c) { return V3<T>{a, b, c}; }
+template <typename T> static inline V3<T> operator+(V3<T> a, V3<T> b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+template <typename T> static inline V3<T> operator-(V3<T> a, V3<T> b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+template <typename T> static inline V3<T> operator-(V3<T> a) { return mk(-a.x, -a.y, -a.z); }
+template <typename T> static inline V3<T> operator*(T s, V3<T> a) { return mk(s * a.x, s * a.y, s * a.z); }
+template <typename T> static inline V3<T> operator*(V3<T> a, T s) { return mk(a.x * s, a.y * s, a.z * s); }
+template <typename T> static inline V3<T> operator/(V3<T> a, T s) { return mk(a.x / s, a.y / s, a.z / s); }
+// glm compute_dot<vec3>: tmp = a*b; (tmp.x + tmp.y) + tmp.z
+template <typename T> static inline T dot(V3<T> a, V3<T> b) {
+    T px = a.x * b.x, py = a.y * b.y, pz = a.z * b.z;
+    return (px + py) + pz;
+}
+template <typename T> static inline T length(V3<T> a) { return std::sqrt(dot(a, a)); }
+template <typename T> static inline V3<T> cross(V3<T> x, V3<T> y) {
+    return mk(x.y * y.z - y.y * x.z, x.z * y.x - y.z * x.x, x.x * y.y - y.x * x.y);
+}
+
+// Float4x3 of the reference (xpbd_constraints.h:4017-4025): columns 0..2 = H,
+// column 3 = f.  Stored as h[col][row] and f[row].
+template <typename T> struct Hf {
+    T h[3][3];
+    T f[3];
+};
+template <typename T> static inline Hf<T> hf_zero() {
+    Hf<T> r;
+    for (int c = 0; c < 3; c++) {
+        for (int i = 0; i < 3; i++) r.h[c][i] = T(0);
+        r.f[c] = T(0);
+    }
+    return r;
+}
+template <typename T> static inline void hf_add(Hf<T>& a, const Hf<T>& b) {
+    for (int c = 0; c < 3; c++) {
+        for (int i = 0; i < 3; i++) a.h[c][i] = a.h[c][i] + b.h[c][i];
+        a.f[c] = a.f[c] + b.f[c];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Topology.  mesh_reader.h:206-298 (extract_edges_from_surface<true>).
+// ---------------------------------------------------------------------------
+struct Topology {
+    u32 nv = 0;
+    std::vector<int> faces;      // 3F
+    std::vector<int> edges;      // 2E
+    std::vector<int> bending;    // 4B (e0, e1, opposite in lower-fid face, opposite in higher-fid face)
+    std::vector<std::vector<u32>> adj_edges, adj_bending, adj_verts, adj_verts_bend;
+    std::vector<std::vector<u32>> clusters;
+};
+
+static void extract_edges(const std::vector<int>& faces, std::vector<int>& edges, std::vector<int>& bending) {
+    // mesh_reader.h:213-229: per face sort its 3 ids, emit (a,b,fid),(a,c,fid),(b,c,fid)
+    const size_t nf = faces.size() / 3;
+    struct E3 { int a, b, f; };
+    std::vector<E3> tmp(nf * 3);
+    for (size_t fid = 0; fid < nf; fid++) {
+        int t[3] = {faces[3 * fid], faces[3 * fid + 1], faces[3 * fid + 2]};
+        std::sort(t, t + 3);
+        tmp[3 * fid + 0] = {t[0], t[1], (int)fid};
+        tmp[3 * fid + 1] = {t[0], t[2], (int)fid};
+        tmp[3 * fid + 2] = {t[1], t[2], (int)fid};
+    }
+    // mesh_reader.h:230-236: lexicographic sort on (a, b, fid)
+    std::sort(tmp.begin(), tmp.end(), [](const E3& l, const E3& r) {
+        if (l.a != r.a) return l.a < r.a;
+        if (l.b != r.b) return l.b < r.b;
+        return l.f < r.f;
+    });
+    // mesh_reader.h:237-243: type 1 if next shares (a,b); type 2 if previous does (overrides)
+    std::vector<unsigned char> type(tmp.size(), 0);
+    for (size_t i = 0; i < tmp.size(); i++) {
+        if (i + 1 != tmp.size() && tmp[i + 1].a == tmp[i].a && tmp[i + 1].b == tmp[i].b) type[i] = 1;
+        if (i != 0 && tmp[i - 1].a == tmp[i].a && tmp[i - 1].b == tmp[i].b) type[i] = 2;
+    }
+    // mesh_reader.h:245-296: edges = type 0|2 in sorted order; bending = type 1
+    edges.clear();
+    bending.clear();
+    for (size_t i = 0; i < tmp.size(); i++) {
+        if (type[i] == 0 || type[i] == 2) {
+            edges.push_back(tmp[i].a);
+            edges.push_back(tmp[i].b);
+        }
+    }
+    for (size_t i = 0; i < tmp.size(); i++) {
+        if (type[i] != 1) continue;
+        const int* cf = &faces[3 * tmp[i].f];
+        const int* nf2 = &faces[3 * tmp[i + 1].f];
+        const int s = tmp[i].a + tmp[i].b;
+        bending.push_back(tmp[i].a);
+        bending.push_back(tmp[i].b);
+        bending.push_back((cf[0] + cf[1] + cf[2]) - s);
+        bending.push_back((nf2[0] + nf2[1] + nf2[2]) - s);
+    }
+}
+
+// example2_async_iter.cpp:229-304 (adjacency lists) and 572-646 (VBD colouring)
+static void build_topology(Topology& t) {
+    const u32 nv = t.nv;
+    const u32 ne = (u32)t.edges.size() / 2, nb = (u32)t.bending.size() / 4;
+    t.adj_edges.assign(nv, {});
+    t.adj_bending.assign(nv, {});
+    t.adj_verts.assign(nv, {});
+    for (u32 e = 0; e < ne; e++)
+        for (int j = 0; j < 2; j++) t.adj_edges[t.edges[2 * e + j]].push_back(e);          // :245-252
+    for (u32 b = 0; b < nb; b++)
+        for (int j = 0; j < 4; j++) t.adj_bending[t.bending[4 * b + j]].push_back(b);      // :255-261
+    for (u32 e = 0; e < ne; e++)
+        for (int j = 0; j < 2; j++) t.adj_verts[t.edges[2 * e + j]].push_back(t.edges[2 * e + 1 - j]);  // :264-275
+    t.adj_verts_bend = t.adj_verts;                                                          // :278-303
+    for (u32 b = 0; b < nb; b++)
+        for (int i = 0; i < 4; i++)
+            for (int j = 0; j < 4; j++) {
+                if (i == j) continue;
+                auto& l = t.adj_verts_bend[t.bending[4 * b + i]];
+                const u32 v2 = (u32)t.bending[4 * b + j];
+                if (std::find(l.begin(), l.end(), v2) == l.end()) l.push_back(v2);
+            }
+    // :581-625 greedy colouring: repeat { marks = copy(taken); for vid asc: if !marks -> take, mark neighbours }
+    t.clusters.clear();
+    std::vector<bool> taken(nv, false);
+    u32 total = 0;
+    while (total < nv) {
+        std::vector<u32> cur;
+        std::vector<bool> marked(taken);
+        for (u32 v = 0; v < nv; v++) {
+            if (marked[v]) continue;
+            taken[v] = true;
+            cur.push_back(v);
+            marked[v] = true;
+            for (u32 a : t.adj_verts_bend[v]) marked[a] = true;
+        }
+        total += (u32)cur.size();
+        t.clusters.push_back(cur);
+    }
+}
+
+// example2_async_iter.cpp:99-127 (upload_2d_csr_from): offsets start at n+1
+static std::vector<u32> packed_csr(const std::vector<std::vector<u32>>& m) {
+    const u32 n = (u32)m.size();
+    std::vector<u32> out(n + 1);
+    u32 p = n + 1;
+    for (u32 i = 0; i < n; i++) { out[i] = p; p += (u32)m[i].size(); }
+    out[n] = p;
+    for (u32 i = 0; i < n; i++) out.insert(out.end(), m[i].begin(), m[i].end());
+    return out;
+}
+
+// ---------------------------------------------------------------------------
+// Rest state.  example2_async_iter.cpp:308-316 (rest length), 344-374 (Q).
+// ---------------------------------------------------------------------------
+template <typename T> struct Rest {
+    std::vector<T> rest_len;   // E
+    std::vector<T> Q;          // 16B, Q[b*16 + col*4 + row]
+};
+
+template <typename T>
+static void build_rest(const Topology& t, const std::vector<V3<T>>& x, Rest<T>& r) {
+    const u32 ne = (u32)t.edges.size() / 2, nb = (u32)t.bending.size() / 4;
+    r.rest_len.resize(ne);
+    for (u32 e = 0; e < ne; e++) r.rest_len[e] = length(x[t.edges[2 * e]] - x[t.edges[2 * e + 1]]);
+    r.Q.resize((size_t)nb * 16);
+    auto cot = [](V3<T> a, V3<T> b) { return dot(a, b) / length(cross(a, b)); };
+    for (u32 b = 0; b < nb; b++) {
+        const V3<T> p0 = x[t.bending[4 * b + 0]], p1 = x[t.bending[4 * b + 1]];
+        const V3<T> p2 = x[t.bending[4 * b + 2]], p3 = x[t.bending[4 * b + 3]];
+        const V3<T> e0 = p1 - p0, e1 = p2 - p0, e2 = p3 - p0, e3 = p2 - p1, e4 = p3 - p1;
+        const T c01 = cot(e0, -e1), c02 = cot(e0, -e2), c03 = cot(e0, e3), c04 = cot(e0, e4);
+        const T K[4] = {c03 + c04, c01 + c02, -c01 - c03, -c02 - c04};
+        const T A0 = T(0.5f) * length(cross(e0, e1));
+        const T A1 = T(0.5f) * length(cross(e0, e2));
+        const T s = T(3.f) / (A0 + A1);
+        for (int col = 0; col < 4; col++)
+            for (int row = 0; row < 4; row++) r.Q[(size_t)b * 16 + col * 4 + row] = s * (K[row] * K[col]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Simulation state and the VBD frame.  example2_async_iter.cpp:1082-1143.
+// ---------------------------------------------------------------------------
+struct Params {
+    float implicit_dt;     // scene_params.h:94 (1/60)
+    u32 num_substep;       // scene_params.h:68
+    u32 iter_count;        // constraint_iter_count
+    float k_stretch;       // stiffness_stretch_spring
+    float k_bending;       // get_stiffness_quadratic_bending() (already multiplied by bending_ui)
+    float damping;         // damping_cloth
+    float gravity[3];      // scene gravity used by inertia + energy (predict hard-codes (0,-9.8,0))
+};
+
+template <typename T> struct Sim {
+    Topology* topo;
+    Rest<T> rest;
+    Params p;
+    std::vector<T> mass;
+    std::vector<unsigned char> fixed;
+    std::vector<V3<T>> x, v, x_start, v_start, x_frame_start, v_frame_start;
+    std::vector<Hf<T>> hf;
+    V3<T> wind{T(0), T(0), T(0)};
+    bool energy_bending = false;   // CPU reference drops bending energy (example2_async_iter.cpp:832-846)
+    std::vector<float> energy;
+    int nthreads = 1;
+};
+
+template <typename T> static inline T substep_dt(const Params& p) {
+    // scene_params.h:159: implicit_dt / float(num_substep)
+    return T(p.implicit_dt) / T((float)p.num_substep);
+}
+
+// xpbd_constraints.h:321-384
+template <typename T> static void predict_position(Sim<T>& s) {
+    const T h = substep_dt<T>(s.p);
+    const V3<T> g = mk(T(0), T(-9.8f), T(0));
+    const long nv = (long)s.x.size();
+#pragma omp parallel for schedule(static, 256) num_threads(s.nthreads)
+    for (long vid = 0; vid < nv; vid++) {
+        const V3<T> x_prev = s.x_start[vid];
+        const V3<T> v_prev = s.v[vid];
+        V3<T> v_pred = v_prev + h * g;
+        if (s.fixed[vid]) v_pred = mk(T(0), T(0), T(0));
+        s.x[vid] = x_prev + h * v_pred;
+    }
+}
+
+// xpbd_constraints.h:535-581
+template <typename T> static void update_velocity(Sim<T>& s) {
+    const T h = substep_dt<T>(s.p);
+    const T damp = T(std::exp(T(-s.p.damping) * h));
+    const long nv = (long)s.x.size();
+#pragma omp parallel for schedule(static, 256) num_threads(s.nthreads)
+    for (long vid = 0; vid < nv; vid++) {
+        const V3<T> dx = s.x[vid] - s.x_start[vid];
+        V3<T> vel = dx / h;
+        vel = vel * damp;
+        s.v[vid] = vel;
+        s.v_start[vid] = vel;
+        s.x_start[vid] = s.x[vid];
+    }
+}
+
+// VBD::compute_inertia xpbd_constraints.h:4027-4067, extended with a uniform
+// per-vertex external force `wind` added to the outer force after m*g.
+template <typename T> static inline Hf<T> compute_inertia(const Sim<T>& s, u32 vid, T h) {
+    const T h2inv = T(1.f) / (h * h);
+    const V3<T> xk = s.x[vid], x0 = s.x_start[vid], v0 = s.v[vid];
+    const T m = s.mass[vid];
+    const V3<T> g = mk(T(s.p.gravity[0]), T(s.p.gravity[1]), T(s.p.gravity[2]));
+    V3<T> of = mk(T(0), T(0), T(0));
+    of = of + m * g;
+    of = of + s.wind;
+    Hf<T> r = hf_zero<T>();
+    const T md = (T(1) * m) * h2inv;          // Identity3x3 * mass * h_2_inv
+    for (int c = 0; c < 3; c++) r.h[c][c] = md;
+    if (s.fixed[vid])
+        for (int c = 0; c < 3; c++) r.h[c][c] = r.h[c][c] + T(1E9f);
+    const T a = (-m) * h2inv;
+    const V3<T> d = (xk - x0) - (v0 * h);
+    r.f[0] = a * d.x + of.x;
+    r.f[1] = a * d.y + of.y;
+    r.f[2] = a * d.z + of.z;
+    return r;
+}
+
+// VBD::compute_stretch_mass_spring xpbd_constraints.h:4107-4159
+template <typename T> static inline Hf<T> compute_stretch(const Sim<T>& s, u32 vid) {
+    const T k = T(s.p.k_stretch);
+    const T eps = T(1.19209290E-7f);
+    Hf<T> acc = hf_zero<T>();
+    for (u32 eid : s.topo->adj_edges[vid]) {
+        const int e0 = s.topo->edges[2 * eid], e1 = s.topo->edges[2 * eid + 1];
+        const V3<T> diff = s.x[e1] - s.x[e0];
+        const T l = std::max(length(diff), eps);
+        const T L = s.rest.rest_len[eid];
+        const T C = l - L;
+        const V3<T> dir = diff / l;
+        V3<T> f0 = (k * dir) * C;
+        const T xinv = T(1.f) / l;
+        const T xsq = xinv * xinv;
+        const T s1 = k * xsq;
+        const T s2 = k * std::max(T(1) - L * xinv, T(0.0f));
+        Hf<T> he;
+        for (int c = 0; c < 3; c++)
+            for (int r = 0; r < 3; r++) {
+                const T xx = diff[r] * diff[c];             // outer_product: col c = diff * diff[c]
+                const T id = (r == c) ? T(1) : T(0);
+                he.h[c][r] = s1 * xx + s2 * (id - xsq * xx);
+            }
+        const V3<T> f = (vid == (u32)e0) ? f0 : -f0;
+        for (int r = 0; r < 3; r++) he.f[r] = f[r];
+        hf_add(acc, he);
+    }
+    return acc;
+}
+
+// VBD::compute_bending_quadratic xpbd_constraints.h:4161-4205
+template <typename T> static inline Hf<T> compute_bending(const Sim<T>& s, u32 vid) {
+    const T k = T(s.p.k_bending);
+    Hf<T> acc = hf_zero<T>();
+    for (u32 b : s.topo->adj_bending[vid]) {
+        const int* e = &s.topo->bending[4 * b];
+        const T* Q = &s.rest.Q[(size_t)b * 16];
+        int o = 0;
+        while ((u32)e[o] != vid) o++;
+        V3<T> f = mk(T(0), T(0), T(0));
+        for (int jj = 0; jj < 4; jj++) f = f - Q[o * 4 + jj] * s.x[e[jj]];
+        f = k * f;
+        const T hd = k * Q[o * 4 + o];
+        Hf<T> hb = hf_zero<T>();
+        for (int c = 0; c < 3; c++) {
+            for (int r = 0; r < 3; r++) hb.h[c][r] = hd * ((r == c) ? T(1) : T(0));
+            hb.f[c] = f[c];
+        }
+        hf_add(acc, hb);
+    }
+    return acc;
+}
+
+// vbd_step example2_async_iter.cpp:993-1014, determinant (glm), inverse_mat(H, det)
+// float_n_n.h:342-363, Epsilon constant_value.h:9
+template <typename T> static inline void vbd_step(Sim<T>& s, u32 vid) {
+    const Hf<T>& a = s.hf[vid];
+    const T (*m)[3] = a.h;
+    const T det = m[0][0] * (m[1][1] * m[2][2] - m[2][1] * m[1][2])
+                - m[1][0] * (m[0][1] * m[2][2] - m[2][1] * m[0][2])
+                + m[2][0] * (m[0][1] * m[1][2] - m[1][1] * m[0][2]);
+    if (std::fabs(det) > T(1.19209290E-7f)) {
+        auto d2 = [](T x00, T x01, T x10, T x11) { return x00 * x11 - x01 * x10; };
+        T inv[3][3];
+        inv[0][0] = +(d2(m[1][1], m[1][2], m[2][1], m[2][2])) / det;
+        inv[0][1] = -(d2(m[1][0], m[1][2], m[2][0], m[2][2])) / det;
+        inv[0][2] = +(d2(m[1][0], m[1][1], m[2][0], m[2][1])) / det;
+        inv[1][1] = +(d2(m[0][0], m[0][2], m[2][0], m[2][2])) / det;
+        inv[1][2] = -(d2(m[0][0], m[0][1], m[2][0], m[2][1])) / det;
+        inv[2][2] = +(d2(m[0][0], m[0][1], m[1][0], m[1][1])) / det;
+        inv[1][0] = -(d2(m[0][1], m[0][2], m[2][1], m[2][2])) / det;
+        inv[2][0] = +(d2(m[0][1], m[0][2], m[1][1], m[1][2])) / det;
+        inv[2][1] = -(d2(m[0][0], m[0][2], m[1][0], m[1][2])) / det;
+        V3<T> dx;
+        for (int i = 0; i < 3; i++) dx[i] = (inv[0][i] * a.f[0] + inv[1][i] * a.f[1]) + inv[2][i] * a.f[2];
+        s.x[vid] = s.x[vid] + dx;
+    }
+}
+
+// CpuSolver::compute_energy example2_async_iter.cpp:785-880 with
+// Energy::compute_energy_inertia xpbd_constraints.h:2079-2098,
+// compute_energy_stretch_mass_spring :2100-2125, compute_energy_bending :2129-2214
+// (core_bending_quadratic :1062-1083).  Per-element FP32 terms, double sum,
+// stored as float.  Bending is added only when energy_bending is set (the CPU
+// reference discards it; its GPU energy kernels include it).
+template <typename T> static double compute_energy(const Sim<T>& s) {
+    const T h = T(s.p.implicit_dt) / T((float)s.p.num_substep);
+    const long nv = (long)s.x.size();
+    double ei = 0.0, es = 0.0, eb = 0.0;
+#pragma omp parallel for reduction(+ : ei) num_threads(s.nthreads)
+    for (long vid = 0; vid < nv; vid++) {
+        const T m = s.mass[vid];
+        V3<T> aext = mk(T(s.p.gravity[0]), T(s.p.gravity[1]), T(s.p.gravity[2]));
+        aext = aext + s.wind / m;
+        if (s.fixed[vid]) aext = mk(T(0), T(0), T(0));
+        const V3<T> y = (s.x_start[vid] + h * s.v_start[vid]) + (h * h) * aext;
+        const V3<T> d = s.x[vid] - y;
+        const T e = (dot(d, d) * m) / ((T(2) * h) * h);
+        ei += (double)e;
+    }
+    const long ne = (long)s.rest.rest_len.size();
+    const T k = T(s.p.k_stretch);
+#pragma omp parallel for reduction(+ : es) num_threads(s.nthreads)
+    for (long eid = 0; eid < ne; eid++) {
+        const V3<T> diff = s.x[s.topo->edges[2 * eid + 1]] - s.x[s.topo->edges[2 * eid]];
+        const T l2 = dot(diff, diff);
+        if ((double)l2 < 1e-8) continue;    // float promoted against a double literal
+        const T C = std::sqrt(l2) - s.rest.rest_len[eid];
+        es += (double)((T(0.5f) * k) * (C * C));
+    }
+    if (s.energy_bending) {
+        const long nb = (long)s.topo->bending.size() / 4;
+        const T kb = T(s.p.k_bending);
+#pragma omp parallel for reduction(+ : eb) num_threads(s.nthreads)
+        for (long b = 0; b < nb; b++) {
+            const int* e = &s.topo->bending[4 * b];
+            const T* Q = &s.rest.Q[(size_t)b * 16];
+            T sum = T(0);
+            for (int i = 0; i < 4; i++)
+                for (int j = 0; j < 4; j++) sum = sum + Q[i * 4 + j] * dot(s.x[e[i]], s.x[e[j]]);
+            eb += (double)((T(0.5f) * kb) * sum);
+        }
+    }
+    return ei + es + eb;
+}
+
+// solve_constraints_VBD example2_async_iter.cpp:1481-1509; per-colour passes
+// :939-1014 (inertia grain 256, stretch/bending/step grain 32, cpu_parallel.h:83-98)
+template <typename T> static void solve_vbd_iteration(Sim<T>& s, bool record, bool first) {
+    const T h = substep_dt<T>(s.p);
+    if (record && first) s.energy.push_back((float)compute_energy(s));
+    for (const auto& cl : s.topo->clusters) {
+        const long n = (long)cl.size();
+#pragma omp parallel for schedule(static, 256) num_threads(s.nthreads)
+        for (long i = 0; i < n; i++) s.hf[cl[i]] = compute_inertia(s, cl[i], h);
+#pragma omp parallel for schedule(static, 32) num_threads(s.nthreads)
+        for (long i = 0; i < n; i++) { Hf<T> a = compute_stretch(s, cl[i]); hf_add(s.hf[cl[i]], a); }
+#pragma omp parallel for schedule(static, 32) num_threads(s.nthreads)
+        for (long i = 0; i < n; i++) { Hf<T> a = compute_bending(s, cl[i]); hf_add(s.hf[cl[i]], a); }
+#pragma omp parallel for schedule(static, 32) num_threads(s.nthreads)
+        for (long i = 0; i < n; i++) vbd_step(s, cl[i]);
+    }
+    if (record) s.energy.push_back((float)compute_energy(s));
+}
+
+// SolverInterface::physics_step :1615-1648 + CpuSolver::physics_step_vbd :1082-1143
+template <typename T> static void physics_step(Sim<T>& s, bool record) {
+    s.x_frame_start = s.x;   // frame_end -> frame_start (x and v always hold frame end)
+    s.v_frame_start = s.v;
+    s.x_start = s.x_frame_start;
+    s.v_start = s.v_frame_start;
+    s.energy.clear();
+    const u32 nsub = record ? 1u : s.p.num_substep;
+    for (u32 sub = 0; sub < nsub; sub++) {
+        predict_position(s);
+        for (u32 it = 0; it < s.p.iter_count; it++) solve_vbd_iteration(s, record, it == 0);
+        update_velocity(s);
+    }
+}
+
+}  // namespace orc
+
+// ---------------------------------------------------------------------------
+// C ABI for the Python tests and the CPU baseline.
+// ---------------------------------------------------------------------------
+using namespace orc;
+
+struct orc_mesh { Topology t; std::vector<float> x; };
+struct orc_sim {
+    orc_mesh* mesh;
+    bool fp64;
+    Sim<float> s32;
+    Sim<double> s64;
+};
+
+template <typename T>
+static void sim_init(Sim<T>& s, orc_mesh* m, const Params& p, const float* mass, const unsigned char* fixed) {
+    const u32 nv = m->t.nv;
+    s.topo = &m->t;
+    s.p = p;
+    std::vector<V3<T>> x(nv);
+    for (u32 i = 0; i < nv; i++) x[i] = mk(T(m->x[3 * i]), T(m->x[3 * i + 1]), T(m->x[3 * i + 2]));
+    build_rest(m->t, x, s.rest);
+    s.mass.assign(mass, mass + nv);
+    s.fixed.assign(fixed, fixed + nv);
+    s.x = x;
+    s.v.assign(nv, mk(T(0), T(0), T(0)));
+    s.x_start = s.x;
+    s.v_start = s.v;
+    s.hf.assign(nv, hf_zero<T>());
+}
+
+template <typename T> static void get_state(const Sim<T>& s, double* x, double* v) {
+    for (size_t i = 0; i < s.x.size(); i++)
+        for (int c = 0; c < 3; c++) {
+            if (x) x[3 * i + c] = (double)s.x[i][c];
+            if (v) v[3 * i + c] = (double)s.v[i][c];
+        }
+}
+
+template <typename T> static void set_state(Sim<T>& s, const double* x, const double* v) {
+    for (size_t i = 0; i < s.x.size(); i++)
+        for (int c = 0; c < 3; c++) {
+            s.x[i][c] = (T)x[3 * i + c];
+            s.v[i][c] = (T)v[3 * i + c];
+        }
+    s.x_start = s.x;
+    s.v_start = s.v;
+}
+
+extern "C" {
+
+orc_mesh* orc_mesh_create(u32 nv, const float* x, u32 nf, const int* faces) {
+    orc_mesh* m = new orc_mesh();
+    m->t.nv = nv;
+    m->x.assign(x, x + 3 * (size_t)nv);
+    m->t.faces.assign(faces, faces + 3 * (size_t)nf);
+    extract_edges(m->t.faces, m->t.edges, m->t.bending);
+    build_topology(m->t);
+    return m;
+}
+void orc_mesh_destroy(orc_mesh* m) { delete m; }
+
+// counts: nv, ne, nb, nclusters
+void orc_mesh_counts(const orc_mesh* m, u32* out) {
+    out[0] = m->t.nv;
+    out[1] = (u32)m->t.edges.size() / 2;
+    out[2] = (u32)m->t.bending.size() / 4;
+    out[3] = (u32)m->t.clusters.size();
+}
+void orc_mesh_edges(const orc_mesh* m, int* out) { std::copy(m->t.edges.begin(), m->t.edges.end(), out); }
+void orc_mesh_bending(const orc_mesh* m, int* out) { std::copy(m->t.bending.begin(), m->t.bending.end(), out); }
+
+// which: 0 adj_edges, 1 adj_bending, 2 adj_verts, 3 adj_verts_with_bending, 4 clusters
+u32 orc_mesh_csr_size(const orc_mesh* m, int which) {
+    const std::vector<std::vector<u32>>* l[5] = {&m->t.adj_edges, &m->t.adj_bending, &m->t.adj_verts,
+                                                  &m->t.adj_verts_bend, &m->t.clusters};
+    size_t n = l[which]->size() + 1;
+    for (auto& v : *l[which]) n += v.size();
+    return (u32)n;
+}
+void orc_mesh_csr(const orc_mesh* m, int which, u32* out) {
+    const std::vector<std::vector<u32>>* l[5] = {&m->t.adj_edges, &m->t.adj_bending, &m->t.adj_verts,
+                                                  &m->t.adj_verts_bend, &m->t.clusters};
+    std::vector<u32> p = packed_csr(*l[which]);
+    std::copy(p.begin(), p.end(), out);
+}
+void orc_mesh_rest(const orc_mesh* m, float* rest_len, float* Q) {
+    std::vector<V3<float>> x(m->t.nv);
+    for (u32 i = 0; i < m->t.nv; i++) x[i] = mk(m->x[3 * i], m->x[3 * i + 1], m->x[3 * i + 2]);
+    Rest<float> r;
+    build_rest(m->t, x, r);
+    std::copy(r.rest_len.begin(), r.rest_len.end(), rest_len);
+    std::copy(r.Q.begin(), r.Q.end(), Q);
+}
+
+
+orc_sim* orc_sim_create(orc_mesh* m, const float* params9, const u32* iparams2, const float* mass,
+                        const unsigned char* fixed, int fp64) {
+    // params9: implicit_dt, k_stretch, k_bending, damping, g0, g1, g2 (7 used); iparams2: num_substep, iter_count
+    Params p;
+    p.implicit_dt = params9[0];
+    p.k_stretch = params9[1];
+    p.k_bending = params9[2];
+    p.damping = params9[3];
+    p.gravity[0] = params9[4];
+    p.gravity[1] = params9[5];
+    p.gravity[2] = params9[6];
+    p.num_substep = iparams2[0];
+    p.iter_count = iparams2[1];
+    orc_sim* s = new orc_sim();
+    s->mesh = m;
+    s->fp64 = fp64 != 0;
+    if (s->fp64) sim_init(s->s64, m, p, mass, fixed);
+    else sim_init(s->s32, m, p, mass, fixed);
+    return s;
+}
+void orc_sim_destroy(orc_sim* s) { delete s; }
+
+void orc_sim_set_threads(orc_sim* s, int n) { s->s32.nthreads = n; s->s64.nthreads = n; }
+void orc_sim_set_energy_bending(orc_sim* s, int on) { s->s32.energy_bending = on != 0; s->s64.energy_bending = on != 0; }
+void orc_sim_set_wind(orc_sim* s, const float* w) {
+    s->s32.wind = mk(w[0], w[1], w[2]);
+    s->s64.wind = mk((double)w[0], (double)w[1], (double)w[2]);
+}
+
+// One frame.  record != 0 => one substep with the energy history (101 values).
+// Returns the number of energies written (at most cap).
+u32 orc_sim_step(orc_sim* s, int record, float* energy, u32 cap) {
+    std::vector<float>* e;
+    if (s->fp64) { physics_step(s->s64, record != 0); e = &s->s64.energy; }
+    else { physics_step(s->s32, record != 0); e = &s->s32.energy; }
+    u32 n = std::min<u32>(cap, (u32)e->size());
+    if (energy) std::copy(e->begin(), e->begin() + n, energy);
+    return n;
+}
+
+void orc_sim_get_state(const orc_sim* s, double* x, double* v) {
+    if (s->fp64) get_state(s->s64, x, v);
+    else get_state(s->s32, x, v);
+}
+void orc_sim_set_state(orc_sim* s, const double* x, const double* v) {
+    if (s->fp64) set_state(s->s64, x, v);
+    else set_state(s->s32, x, v);
+}
+double orc_sim_energy(orc_sim* s) { return s->fp64 ? compute_energy(s->s64) : compute_energy(s->s32); }
+
+int orc_max_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+}  // extern "C"

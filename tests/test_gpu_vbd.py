@@ -1,0 +1,269 @@
+"""GPU parity of the HIP VBD path against the CPU oracle (FP32, same inputs).
+
+Tolerances: positions 1e-5 relative (max-norm) per frame, energy history
+1e-4 relative -- the HIP kernels follow the reference's FP32 operation order
+(-ffp-contract=off, IEEE div/sqrt), so the positions are expected to be
+bit-identical; the tests check both.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.helpers import GOLDEN, oracle_sim, rel_maxnorm, square8k_mesh
+from libatssim_amd import GpuVBD, Params, configs, _lib
+from libatssim_amd.vbd import Cloth, make_grid
+
+pytestmark = pytest.mark.gpu
+
+NO_ITER = 0xFFFFFFFF
+POS_TOL = 1e-5
+ENERGY_TOL = 1e-4
+
+
+def _print_case(name):
+    print(f"[case] {name}", flush=True)
+
+
+def test_c0_frames_bit_exact(gpu_available):
+    _print_case("grid_c0 30 frames + energy frame")
+    cloth, params, frames, _ = configs.grid_c0()
+    g = GpuVBD(cloth, params)
+    o = oracle_sim(cloth, params)
+    worst = 0.0
+    for fr in range(frames):
+        g.physics_step()
+        o.step()
+        xg, vg = g.state()
+        xo, vo = o.state()
+        worst = max(worst, rel_maxnorm(xg, xo))
+        assert rel_maxnorm(xg, xo) <= POS_TOL, fr
+        assert np.array_equal(xg, xo.astype(np.float32)), f"frame {fr} not bit-exact"
+        assert np.array_equal(vg, vo.astype(np.float32)), f"frame {fr} velocity not bit-exact"
+    g.set_energy_bending(False)
+    eg = g.physics_step(record_energy=True)
+    eo = o.step(record=True)
+    assert len(eg) == len(eo) == params.constraint_iter_count + 1
+    assert np.abs(eg - eo).max() / np.abs(eo).max() <= ENERGY_TOL
+    assert np.array_equal(g.state()[0], o.state()[0].astype(np.float32))
+
+
+def test_energy_with_bending(gpu_available):
+    _print_case("grid_c0 energy with bending")
+    cloth, params, _, _ = configs.grid_c0()
+    g = GpuVBD(cloth, params)
+    o = oracle_sim(cloth, params, energy_bending=True)
+    for _ in range(3):
+        g.physics_step()
+        o.step()
+    eg = g.physics_step(record_energy=True)
+    eo = o.step(record=True)
+    rel = np.abs(eg.astype(np.float64) - eo) / np.abs(eo)
+    assert rel.max() <= ENERGY_TOL, rel.max()
+    # the solver decreases the incremental potential at every iteration
+    assert np.all(np.diff(eg.astype(np.float64)) <= 1e-6 * np.abs(eg).max())
+
+
+def test_square8k_matches_oracle_and_published(gpu_available):
+    _print_case("square8k reference run, 30 frames + energy frame")
+    x, f = square8k_mesh()
+    cloth = configs.square8k_cloth(x, f)
+    params = configs.square8k_params()
+    g = GpuVBD(cloth, params)
+    g.set_energy_bending(False)          # the reference's CPU energy drops bending
+    o = oracle_sim(cloth, params, threads=O.max_threads())
+    for fr in range(30):
+        g.physics_step()
+        o.step()
+        if fr % 10 == 9:
+            assert rel_maxnorm(g.state()[0], o.state()[0]) <= POS_TOL
+    eg = g.physics_step(record_energy=True)
+    o.step(record=True)
+    assert np.array_equal(g.state()[0], o.state()[0].astype(np.float32))
+    pub = np.array(json.load(open(os.path.join(GOLDEN, "square8k_published.json")))["list_sync"])
+    assert (np.abs(eg - pub) / pub).max() < 5e-6
+
+
+def test_c1_first_frames_bit_exact(gpu_available):
+    _print_case("grid_c1 256x256, 2 windy frames vs oracle")
+    cloth, params, _, wind = configs.grid_c1(frames=2)
+    g = GpuVBD(cloth, params)
+    o = oracle_sim(cloth, params, threads=O.max_threads())
+    for fr in range(2):
+        g.set_external_force(wind[fr])
+        o.set_wind(wind[fr])
+        g.physics_step()
+        o.step()
+        xg = g.state()[0]
+        xo = o.state()[0]
+        assert rel_maxnorm(xg, xo) <= POS_TOL
+        assert np.array_equal(xg, xo.astype(np.float32)), f"frame {fr}"
+
+
+def test_run_frames_matches_stepping(gpu_available):
+    _print_case("run_frames with per-frame forces == physics_step loop")
+    cloth, params, _, _ = configs.grid_c0()
+    wind = configs.make_wind(5, cloth.num_verts, 2.0, 99) * 50.0
+    a = GpuVBD(cloth, params)
+    b = GpuVBD(cloth, params)
+    a.run_frames(5, wind)
+    for fr in range(5):
+        b.set_external_force(wind[fr])
+        b.physics_step()
+    assert np.array_equal(a.state()[0], b.state()[0])
+    assert a.last_elapsed_ms() > 0
+
+
+def test_c1_long_run_invariants(gpu_available):
+    """Full-size run (60 windy frames): finite, pins hold, energy decreases."""
+    _print_case("grid_c1 256x256, 60 frames invariants")
+    cloth, params, frames, wind = configs.grid_c1(frames=60)
+    g = GpuVBD(cloth, params)
+    g.run_frames(frames, wind)
+    x, v = g.state()
+    assert np.all(np.isfinite(x)) and np.all(np.isfinite(v))
+    pins = cloth.fixed.astype(bool)
+    assert np.abs(x[pins] - cloth.x[pins]).max() < 1e-5
+    assert x[~pins, 1].min() > -2.56 * 1.2          # stretch stiffness holds the cloth together
+    e = g.physics_step(record_energy=True)
+    assert e[-1] < e[0]
+
+
+def _dispatch_frame(g, ncl, iters, buffer_idx=_lib.LATS_BUFFER_DEFAULT):
+    """The reference's single-device frame DAG, issued task by task."""
+    g.dispatch(_lib.LATS_FN_ADDITIONAL_ROOT, iter_idx=NO_ITER)
+    g.dispatch(_lib.LATS_FN_PREDICT_POSITION, iter_idx=NO_ITER)
+    for it in range(iters):
+        for c in range(ncl):
+            left = _lib.LATS_BUFFER_DEFAULT if (it == 0 and c == 0 and buffer_idx != _lib.LATS_BUFFER_DEFAULT) \
+                else _lib.LATS_BUFFER_NONE
+            g.dispatch(_lib.LATS_FN_VBD_ALL_IN_ONE, iter_idx=it, cluster_idx=c, buffer_idx=buffer_idx,
+                       left_buffer_idx=left)
+    # Task(id_xpbd_constraint_last_node, 0, ...) sets cluster_idx = 0 and
+    # leaves iter_idx = -1u (launcher.h:633-635)
+    g.dispatch(_lib.LATS_FN_CONSTRAINT_LAST_NODE, iter_idx=NO_ITER, buffer_idx=buffer_idx)
+    g.dispatch(_lib.LATS_FN_UPDATE_VELOCITY, iter_idx=NO_ITER)
+    g.dispatch(_lib.LATS_FN_ADDITIONAL_TERMINAL)
+    g.synchronize()
+
+
+@pytest.mark.parametrize("buffer_idx", [_lib.LATS_BUFFER_DEFAULT, 5])
+def test_dispatch_slot_matches_physics_step(gpu_available, buffer_idx):
+    _print_case(f"dispatch slot, buffer {buffer_idx:#x}")
+    cloth, params, _, _ = configs.grid_c0()
+    params = Params(constraint_iter_count=10)
+    a = GpuVBD(cloth, params)
+    b = GpuVBD(cloth, params)
+    ncl = cloth.topology.num_clusters
+    b.set_record_energy(True)
+    for _ in range(3):
+        a.physics_step()
+        _dispatch_frame(b, ncl, 10, buffer_idx)
+        assert np.array_equal(a.state()[0], b.state()[0])
+        assert np.array_equal(a.state()[1], b.state()[1])
+    # energy hooks: before iteration 0 of cluster 0, after the last cluster of
+    # every iteration, and at the last node
+    assert len(b.energy()) == 3 * (1 + 10 + 1)
+
+
+def test_read_and_solve_conflict_merge(gpu_available):
+    """Two buffers iterated independently and merged with weight 0.5
+    (Core::read_and_solve_conflict) == the numpy restatement of the merge."""
+    _print_case("async merge of two buffers")
+    cloth, params, _, _ = configs.grid_c0()
+    params = Params(constraint_iter_count=1)
+    g = GpuVBD(cloth, params)
+    g.physics_step()
+    g.dispatch(_lib.LATS_FN_PREDICT_POSITION)
+    # buffer 1 <- sa_x, one sweep of clusters 0..2 on it
+    for c in range(3):
+        g.dispatch(_lib.LATS_FN_VBD_ALL_IN_ONE, cluster_idx=c, buffer_idx=1,
+                   left_buffer_idx=_lib.LATS_BUFFER_DEFAULT if c == 0 else _lib.LATS_BUFFER_NONE)
+    # buffer 2 <- sa_x, clusters 3..6
+    for c in range(3, cloth.topology.num_clusters):
+        g.dispatch(_lib.LATS_FN_VBD_ALL_IN_ONE, cluster_idx=c, buffer_idx=2,
+                   left_buffer_idx=_lib.LATS_BUFFER_DEFAULT if c == 3 else _lib.LATS_BUFFER_NONE)
+    # read buffers back through last_node copies
+    g.dispatch(_lib.LATS_FN_CONSTRAINT_LAST_NODE, iter_idx=NO_ITER, buffer_idx=1)
+    xa = g.state()[0].copy()
+    g.dispatch(_lib.LATS_FN_CONSTRAINT_LAST_NODE, iter_idx=NO_ITER, buffer_idx=2)
+    xb = g.state()[0].copy()
+    # merge 2 <- (input 2, left 1) as the prev hook of a task on buffer 2
+    g.dispatch(_lib.LATS_FN_CONSTRAINT_LAST_NODE, iter_idx=NO_ITER, buffer_idx=2, left_buffer_idx=1,
+               input_buffer_idxs=(2,),
+               is_allocated_to_main_device=True)
+    xm = g.state()[0]
+    # begin buffer of 2 == predicted positions (copied from sa_x at cluster 3)
+    g2 = GpuVBD(cloth, params)
+    g2.physics_step()
+    g2.dispatch(_lib.LATS_FN_PREDICT_POSITION)
+    xp = g2.state()[0]
+    sd, ad = xb - xp, xa - xp
+    act_a = (sd * sd).sum(1) != 0
+    act_b = (ad * ad).sum(1) != 0
+    want = xp.copy()
+    both = act_a & act_b
+    want[both] = xp[both] + np.float32(0.5) * (sd[both] + ad[both])
+    want[act_a & ~act_b] = xb[act_a & ~act_b]
+    want[~act_a & act_b] = xa[~act_a & act_b]
+    assert np.array_equal(xm, want)
+
+
+def test_det_guard_on_gpu(gpu_available):
+    _print_case("det guard: massless isolated vertex")
+    base = configs.grid_cloth(4, "corners", seed=2)
+    x = np.vstack([base.x, [[0.5, 0.5, 0.0]]]).astype(np.float32)
+    cloth = Cloth(x, base.faces, np.append(base.mass, 0.0), np.append(base.fixed, 0))
+    params = Params(constraint_iter_count=10)
+    g = GpuVBD(cloth, params)
+    o = oracle_sim(cloth, params)
+    for _ in range(3):
+        g.physics_step()
+        o.step()
+    xg = g.state()[0]
+    assert np.all(np.isfinite(xg))
+    assert np.array_equal(xg, o.state()[0].astype(np.float32))
+
+
+@pytest.mark.parametrize("n", [2, 3, 5])
+def test_tiny_cloths(gpu_available, n):
+    _print_case(f"tiny {n}x{n} cloth")
+    cloth = configs.grid_cloth(n, "top", seed=n)
+    params = Params(constraint_iter_count=7)
+    g = GpuVBD(cloth, params)
+    o = oracle_sim(cloth, params)
+    for _ in range(4):
+        g.physics_step()
+        o.step()
+    assert np.array_equal(g.state()[0], o.state()[0].astype(np.float32))
+
+
+def test_substeps(gpu_available):
+    _print_case("num_substep = 3")
+    cloth, _, _, _ = configs.grid_c0()
+    params = Params(num_substep=3, constraint_iter_count=5)
+    g = GpuVBD(cloth, params)
+    o = oracle_sim(cloth, params)
+    for _ in range(3):
+        g.physics_step()
+        o.step()
+    assert np.array_equal(g.state()[0], o.state()[0].astype(np.float32))
+
+
+def test_host_signal_and_markers(gpu_available):
+    _print_case("host signal / markers")
+    cloth, params, _, _ = configs.grid_c0()
+    g = GpuVBD(cloth, Params(constraint_iter_count=3))
+    L = _lib.lib()
+    _lib.check(L.lats_vbd_record_marker(g.handle, 1))
+    _lib.check(L.lats_vbd_host_signal(g.handle, 7))
+    _lib.check(L.lats_vbd_stream_wait_host(g.handle, 7))
+    g.dispatch(_lib.LATS_FN_PREDICT_POSITION)
+    _lib.check(L.lats_vbd_record_marker(g.handle, 2))
+    _lib.check(L.lats_vbd_wait_marker(g.handle, 2))
+    import ctypes
+    ms = ctypes.c_float()
+    _lib.check(L.lats_vbd_marker_elapsed_ms(g.handle, 1, 2, ctypes.byref(ms)))
+    assert ms.value >= 0.0
