@@ -834,8 +834,10 @@ static int ensure_async(lats_vbd* c, uint32_t idx) {
     }
     return LATS_OK;
 }
+// A task the asynchronous pass left without a buffer (-1u) iterates sa_x.
 static float* iter_buffer(lats_vbd* c, uint32_t idx) {
-    return idx == LATS_BUFFER_DEFAULT ? c->x.p : c->iterbuf[idx % LATS_NUM_ASYNC_BUFFERS].p;
+    return (idx == LATS_BUFFER_DEFAULT || idx == LATS_BUFFER_NONE) ? c->x.p
+                                                                   : c->iterbuf[idx % LATS_NUM_ASYNC_BUFFERS].p;
 }
 static float* begin_buffer(lats_vbd* c, uint32_t idx) { return c->beginbuf[idx % LATS_NUM_ASYNC_BUFFERS].p; }
 
