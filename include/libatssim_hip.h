@@ -117,6 +117,14 @@ typedef struct lats_vbd lats_vbd;
 int lats_vbd_create(const lats_mesh_desc* mesh, const lats_scene_params* params, lats_vbd** out);
 int lats_vbd_destroy(lats_vbd* ctx);
 int lats_vbd_set_strategy(lats_vbd* ctx, int strategy);
+/* Grid size of the persistent strategy (0: one thread per vertex of the
+ * largest colour, capped at the co-resident limit). */
+int lats_vbd_set_persistent_blocks(lats_vbd* ctx, uint32_t blocks);
+int lats_vbd_get_strategy(lats_vbd* ctx, int* strategy, uint32_t* persistent_blocks);
+/* Autotuner: times `frames` frames with the graph strategy and with the
+ * persistent strategy at several grid sizes, keeps the fastest for later
+ * lats_vbd_physics_step / lats_vbd_run_frames calls, and restores the state. */
+int lats_vbd_autotune(lats_vbd* ctx, uint32_t frames, int* chosen_strategy, uint32_t* chosen_blocks);
 /* Include the bending term in the recorded energy (the reference's GPU
  * energy does, example3.metal:369-395; its CPU energy drops it,
  * example2_async_iter.cpp:832-846).  Default 1. */
@@ -194,6 +202,32 @@ int lats_vbd_record_marker(lats_vbd* ctx, uint32_t marker);
 int lats_vbd_wait_marker(lats_vbd* ctx, uint32_t marker);
 /* Elapsed GPU ms between two markers (≙ GPUEndTime - GPUStartTime). */
 int lats_vbd_marker_elapsed_ms(lats_vbd* ctx, uint32_t m0, uint32_t m1, float* ms);
+
+/* ======================================================================
+ * Batched cloths: many independent cloths of uneven sizes, one persistent
+ * launch per frame (a block owns a whole cloth; cloths are pulled largest
+ * first from a work counter), the frame captured in a hipGraph.  No
+ * reference counterpart (the reference runs one cloth); every cloth follows
+ * the single-cloth arithmetic above bit for bit.
+ * ====================================================================== */
+typedef struct lats_batch lats_batch;
+/* meshes[k] as for lats_vbd_create; stretch[k] / bending[k] override the
+ * stiffnesses of params per cloth (NULL: use params). */
+int lats_batch_create(uint32_t num_cloths, const lats_mesh_desc* meshes, const float* stretch, const float* bending,
+                      const lats_scene_params* params, lats_batch** out);
+int lats_batch_destroy(lats_batch* b);
+int lats_batch_counts(lats_batch* b, uint32_t* num_cloths, uint64_t* total_verts);
+int lats_batch_set_state(lats_batch* b, uint32_t cloth, const float* x, const float* v);
+int lats_batch_get_state(lats_batch* b, uint32_t cloth, float* x, float* v);
+/* Threads per cloth-block (multiple of 64, <= 1024) and resident blocks
+ * (0: as many as fit). */
+int lats_batch_set_launch(lats_batch* b, uint32_t block_threads, uint32_t blocks);
+int lats_batch_get_launch(lats_batch* b, uint32_t* block_threads, uint32_t* blocks);
+/* Autotuner of the cloth-to-block mapping: times `frames` frames for each
+ * candidate, keeps the fastest, restores the state. */
+int lats_batch_autotune(lats_batch* b, uint32_t frames, uint32_t* block_threads, uint32_t* blocks);
+int lats_batch_run_frames(lats_batch* b, uint32_t num_frames);
+int lats_batch_last_elapsed_ms(lats_batch* b, float* ms);
 
 /* ---- device accessors (benchmarks, zero-copy interop) ---------------- */
 /* Device pointer of the colour-ordered position buffer, float[3V]; slot s

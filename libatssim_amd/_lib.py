@@ -110,6 +110,9 @@ SIGNATURES = {
     "lats_vbd_destroy": (_I, [_VP]),
     "lats_vbd_set_strategy": (_I, [_VP, _I]),
     "lats_vbd_set_energy_bending": (_I, [_VP, _I]),
+    "lats_vbd_set_persistent_blocks": (_I, [_VP, _U32]),
+    "lats_vbd_get_strategy": (_I, [_VP, ctypes.POINTER(_I), ctypes.POINTER(_U32)]),
+    "lats_vbd_autotune": (_I, [_VP, _U32, ctypes.POINTER(_I), ctypes.POINTER(_U32)]),
     "lats_vbd_set_state": (_I, [_VP, _VP, _VP]),
     "lats_vbd_get_state": (_I, [_VP, _VP, _VP]),
     "lats_vbd_set_external_force": (_I, [_VP, _VP]),
@@ -130,6 +133,16 @@ SIGNATURES = {
     "lats_vbd_time_colour_kernels": (_I, [_VP, _U32, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(_U32),
                                           ctypes.POINTER(ctypes.c_uint64)]),
     "lats_vbd_device_bytes": (_I, [_VP, ctypes.POINTER(ctypes.c_uint64)]),
+    "lats_batch_create": (_I, [_U32, _VP, _VP, _VP, ctypes.POINTER(SceneParams), ctypes.POINTER(_VP)]),
+    "lats_batch_destroy": (_I, [_VP]),
+    "lats_batch_counts": (_I, [_VP, ctypes.POINTER(_U32), ctypes.POINTER(ctypes.c_uint64)]),
+    "lats_batch_set_state": (_I, [_VP, _U32, _VP, _VP]),
+    "lats_batch_get_state": (_I, [_VP, _U32, _VP, _VP]),
+    "lats_batch_set_launch": (_I, [_VP, _U32, _U32]),
+    "lats_batch_get_launch": (_I, [_VP, ctypes.POINTER(_U32), ctypes.POINTER(_U32)]),
+    "lats_batch_autotune": (_I, [_VP, _U32, ctypes.POINTER(_U32), ctypes.POINTER(_U32)]),
+    "lats_batch_run_frames": (_I, [_VP, _U32]),
+    "lats_batch_last_elapsed_ms": (_I, [_VP, ctypes.POINTER(ctypes.c_float)]),
 }
 
 _LIB = None

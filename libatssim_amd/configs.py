@@ -7,6 +7,10 @@ grid_c1      256x256 grid (65,536 vertices), same material; top row pinned;
              per-frame wind of total magnitude U(0, 2) N, direction U(0, 2 pi)
              in the xz plane, split evenly over the vertices; 100 iterations,
              60 frames.
+batch_c2     512 independent grid cloths, side n ~ U{16..128} (~3.2 M vertices),
+             k_stretch log-U[1e3, 1e5], k_bend log-U[1e-3, 1e-1] per cloth,
+             1-4 random boundary vertices pinned, m = 1e-3 kg, 100 iterations;
+             all draws from splitmix64(seed).
 square8k     the reference's example-2 run (examples/example2_async_iter.cpp:
              133-392, 715-749, 1713-1810): square8K.obj, pins where the
              AABB-normalised z < 0.01 and x < 0.01 or x > 0.99, m = 0.01 kg,
@@ -19,6 +23,59 @@ from .vbd import Cloth, Params, make_grid, make_wind
 SEED_C0 = 20250101
 SEED_C1 = 20250102
 SEED_WIND = 20250103
+SEED_C2 = 20250104
+
+
+class SplitMix64:
+    """splitmix64 (Steele, Lea, Flood 2014): the same stream as the C++ scenes."""
+
+    MASK = (1 << 64) - 1
+
+    def __init__(self, seed):
+        self.s = seed & self.MASK
+
+    def next(self):
+        self.s = (self.s + 0x9E3779B97F4A7C15) & self.MASK
+        z = self.s
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & self.MASK
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & self.MASK
+        return z ^ (z >> 31)
+
+    def uniform(self):
+        return (self.next() >> 11) * (1.0 / 9007199254740992.0)
+
+
+def batch_c2_spec(num_cloths=512, seed=SEED_C2, nmin=16, nmax=128):
+    """Per cloth: (side n, k_stretch, k_bend, pinned vertex ids, grid seed)."""
+    rng = SplitMix64(seed)
+    spec = []
+    for k in range(num_cloths):
+        n = nmin + min(int(rng.uniform() * (nmax - nmin + 1)), nmax - nmin)
+        ks = 10.0 ** (3.0 + 2.0 * rng.uniform())
+        kb = 10.0 ** (-3.0 + 2.0 * rng.uniform())
+        npins = 1 + min(int(rng.uniform() * 4), 3)
+        boundary = [c for c in range(n)] + [(n - 1) * n + c for c in range(n)] + \
+                   [r * n for r in range(1, n - 1)] + [r * n + n - 1 for r in range(1, n - 1)]
+        pins = []
+        while len(pins) < npins:
+            b = boundary[min(int(rng.uniform() * len(boundary)), len(boundary) - 1)]
+            if b not in pins:
+                pins.append(b)
+        spec.append((n, ks, kb, sorted(pins), seed * 1000003 + k))
+    return spec
+
+
+def batch_c2(num_cloths=512, seed=SEED_C2, nmin=16, nmax=128):
+    """(cloths, per-cloth k_stretch, per-cloth k_bend, params)."""
+    cloths, ks, kb = [], [], []
+    for n, s, b, pins, gseed in batch_c2_spec(num_cloths, seed, nmin, nmax):
+        x, f = make_grid(n, n, 0.01, 1e-4, gseed)
+        fixed = np.zeros(n * n, np.uint8)
+        fixed[pins] = 1
+        cloths.append(Cloth(x, f, np.float32(1e-3), fixed))
+        ks.append(s)
+        kb.append(b)
+    return cloths, np.array(ks, np.float32), np.array(kb, np.float32), Params(constraint_iter_count=100)
 
 
 def grid_cloth(n, pin="corners", seed=SEED_C0, spacing=0.01, jitter=1e-4, mass=1e-3):

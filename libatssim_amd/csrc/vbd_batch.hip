@@ -1,0 +1,433 @@
+// vbd_batch.hip -- many independent cloths of uneven sizes in one persistent
+// launch per frame.
+//
+// All cloths share one set of HBM arrays (vbd_layout.h).  A block owns a
+// whole cloth for a frame: predict, every (iteration, colour) pass with a
+// block barrier between colours, update velocity; it then pulls the next
+// cloth from a work counter.  Cloths are queued largest first (longest
+// processing time first), so the tail of the frame is made of small cloths.
+// Block size and resident blocks are tunable (lats_batch_autotune).  The
+// frame (counter reset + batch kernel) is captured once in a hipGraph.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/libatssim_hip.h"
+#include "host/errors.h"
+#include "vbd_common.h"
+#include "vbd_kernels.h"
+#include "vbd_layout.h"
+
+using namespace lats;
+
+namespace {
+
+struct ClothInfo {
+    uint32_t vbase, nv, cbase, ncol;
+    float ks, kb;
+    uint32_t pad0, pad1;
+};
+
+struct BatchArgs {
+    float* x;
+    float* x0;
+    float* v;
+    const float* mass;
+    const uint32_t* meta;
+    const uint2* ell_e;
+    const float4* ell_bq;
+    const uint4* ell_bi;
+    const ClothInfo* cloths;
+    const uint32_t* order;     // cloth ids, largest first
+    const ColourInfo* colours;
+    uint32_t* work;            // next queue position
+    const FrameParams* fp;
+    uint32_t ncloths, iters;
+    float h, h2inv, damp;
+    float g[3];
+};
+
+__global__ void k_batch_reset(uint32_t* work) { *work = 0u; }
+
+__global__ __launch_bounds__(1024) void k_vbd_batch(BatchArgs p) {
+    __shared__ uint32_t s_item;
+    const F3 ext = F3{p.fp->ext[0], p.fp->ext[1], p.fp->ext[2]};
+    const uint32_t tid = threadIdx.x, nt = blockDim.x;
+    for (;;) {
+        if (tid == 0) s_item = atomicAdd(p.work, 1u);
+        __syncthreads();
+        const uint32_t k = s_item;
+        __syncthreads();
+        if (k >= p.ncloths) break;
+        const ClothInfo ci = p.cloths[p.order[k]];
+        VbdArgs a;
+        a.x = p.x;
+        a.x0 = p.x0;
+        a.v = p.v;
+        a.mass = p.mass;
+        a.meta = p.meta;
+        a.ell_e = p.ell_e;
+        a.ell_bq = p.ell_bq;
+        a.ell_bi = p.ell_bi;
+        a.fp = p.fp;
+        a.h = p.h;
+        a.h2inv = p.h2inv;
+        a.ks = ci.ks;
+        a.kb = ci.kb;
+        a.g[0] = p.g[0];
+        a.g[1] = p.g[1];
+        a.g[2] = p.g[2];
+        const uint32_t s0 = ci.vbase, s1 = ci.vbase + ci.nv;
+        // Core::predict_position
+        for (uint32_t s = s0 + tid; s < s1; s += nt) {
+            F3 vpred = add3(ld3(p.v, s), scale3(p.h, F3{0.f, -9.8f, 0.f}));
+            if ((p.meta[s] >> 16) & 1u) vpred = F3{0.f, 0.f, 0.f};
+            st3(p.x, s, add3(ld3(p.x0, s), scale3(p.h, vpred)));
+        }
+        __syncthreads();
+        for (uint32_t it = 0; it < p.iters; it++) {
+            for (uint32_t c = 0; c < ci.ncol; c++) {
+                const ColourInfo co = p.colours[ci.cbase + c];
+                for (uint32_t i = tid; i < co.count; i += nt)
+                    vbd_vertex_ext(a, co.start + i, i, co.count, co.eoff, co.boff, ext);
+                __syncthreads();
+            }
+        }
+        // Core::update_velocity
+        for (uint32_t s = s0 + tid; s < s1; s += nt) {
+            const F3 xk = ld3(p.x, s);
+            const F3 dx = sub3(xk, ld3(p.x0, s));
+            F3 vel = F3{dx.x / p.h, dx.y / p.h, dx.z / p.h};
+            vel = F3{vel.x * p.damp, vel.y * p.damp, vel.z * p.damp};
+            st3(p.v, s, vel);
+            st3(p.x0, s, xk);
+        }
+        __syncthreads();
+    }
+}
+
+int fail(int code, const std::string& msg) {
+    lats::set_error(msg);
+    return code;
+}
+
+#define HIPCHK(call)                                                                            \
+    do {                                                                                        \
+        hipError_t e_ = (call);                                                                 \
+        if (e_ != hipSuccess)                                                                   \
+            return fail(LATS_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));       \
+    } while (0)
+
+template <typename T> struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    ~DBuf() { if (p) (void)hipFree(p); }
+    hipError_t upload(const std::vector<T>& h) {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = std::max<size_t>(h.size(), 1);
+        hipError_t e = hipMalloc((void**)&p, n * sizeof(T));
+        if (e != hipSuccess || h.empty()) return e;
+        return hipMemcpy(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice);
+    }
+};
+
+}  // namespace
+
+struct lats_batch {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint32_t ncloths = 0;
+    uint64_t total_verts = 0;
+    lats_scene_params prm{};
+    float h = 0.f, h2inv = 0.f, damp = 0.f;
+    std::vector<ClothInfo> cloths;
+    std::vector<std::vector<uint32_t>> perm;   // per cloth: local slot -> vertex id
+    DBuf<float> x, x0, v, mass;
+    DBuf<uint32_t> meta, order, work;
+    DBuf<uint2> ell_e;
+    DBuf<float4> ell_bq;
+    DBuf<uint4> ell_bi;
+    DBuf<ClothInfo> cloths_d;
+    DBuf<ColourInfo> colours_d;
+    DBuf<FrameParams> fp;
+    uint32_t block_threads = 256, blocks = 0, max_blocks_cache = 0, max_blocks_threads = 0;
+    hipGraphExec_t graph = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    float last_ms = 0.f;
+    ~lats_batch() {
+        if (graph) (void)hipGraphExecDestroy(graph);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+int max_resident_blocks(lats_batch* b, uint32_t threads, uint32_t* out) {
+    if (b->max_blocks_threads != threads) {
+        int per_cu = 0, ncu = 0;
+        HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_vbd_batch, (int)threads, 0));
+        HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, b->device));
+        b->max_blocks_cache = (uint32_t)std::max(1, per_cu * ncu);
+        b->max_blocks_threads = threads;
+    }
+    *out = b->max_blocks_cache;
+    return LATS_OK;
+}
+
+int batch_graph(lats_batch* b, hipGraphExec_t* out) {
+    if (!b->graph) {
+        uint32_t mx = 0;
+        int rc = max_resident_blocks(b, b->block_threads, &mx);
+        if (rc) return rc;
+        const uint32_t g = std::min(b->blocks ? b->blocks : mx, std::max(1u, b->ncloths));
+        BatchArgs p;
+        p.x = b->x.p;
+        p.x0 = b->x0.p;
+        p.v = b->v.p;
+        p.mass = b->mass.p;
+        p.meta = b->meta.p;
+        p.ell_e = b->ell_e.p;
+        p.ell_bq = b->ell_bq.p;
+        p.ell_bi = b->ell_bi.p;
+        p.cloths = b->cloths_d.p;
+        p.order = b->order.p;
+        p.colours = b->colours_d.p;
+        p.work = b->work.p;
+        p.fp = b->fp.p;
+        p.ncloths = b->ncloths;
+        p.iters = b->prm.constraint_iter_count;
+        p.h = b->h;
+        p.h2inv = b->h2inv;
+        p.damp = b->damp;
+        for (int k = 0; k < 3; k++) p.g[k] = b->prm.gravity[k];
+        hipGraph_t graph;
+        HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        for (uint32_t sub = 0; sub < b->prm.num_substep; sub++) {
+            hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(1), 0, b->stream, b->work.p);
+            hipLaunchKernelGGL(k_vbd_batch, dim3(g), dim3(b->block_threads), 0, b->stream, p);
+        }
+        HIPCHK(hipStreamEndCapture(b->stream, &graph));
+        hipError_t e = hipGraphInstantiate(&b->graph, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) {
+            b->graph = nullptr;
+            return fail(LATS_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        }
+    }
+    *out = b->graph;
+    return LATS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lats_batch_create(uint32_t num_cloths, const lats_mesh_desc* meshes, const float* stretch, const float* bending,
+                      const lats_scene_params* prm, lats_batch** out) {
+    if (!out || !meshes || !prm || !num_cloths) return fail(LATS_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    if (prm->num_substep == 0 || !(prm->implicit_dt > 0.f))
+        return fail(LATS_ERR_INVALID_ARGUMENT, "implicit_dt must be > 0 and num_substep >= 1");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(LATS_ERR_NO_DEVICE, "no HIP device");
+    auto b = new lats_batch();
+    (void)hipGetDevice(&b->device);
+    b->ncloths = num_cloths;
+    b->prm = *prm;
+    b->h = prm->implicit_dt / (float)prm->num_substep;
+    b->h2inv = 1.f / (b->h * b->h);
+    b->damp = std::exp(-prm->damping_cloth * b->h);
+    ClothLayout L;
+    std::vector<uint32_t> inv;
+    std::string err;
+    for (uint32_t k = 0; k < num_cloths; k++) {
+        const uint32_t vbase = (uint32_t)L.num_slots(), cbase = (uint32_t)L.colours.size();
+        if (!append_cloth(meshes[k], L, inv, err)) {
+            delete b;
+            return fail(LATS_ERR_INVALID_ARGUMENT, "cloth " + std::to_string(k) + ": " + err);
+        }
+        ClothInfo ci;
+        ci.vbase = vbase;
+        ci.nv = meshes[k].num_verts;
+        ci.cbase = cbase;
+        ci.ncol = meshes[k].num_clusters;
+        ci.ks = stretch ? stretch[k] : prm->stiffness_stretch_spring;
+        ci.kb = bending ? bending[k] : prm->stiffness_quadratic_bending;
+        ci.pad0 = ci.pad1 = 0;
+        b->cloths.push_back(ci);
+        b->perm.emplace_back(L.perm.begin() + vbase, L.perm.end());
+    }
+    b->total_verts = L.num_slots();
+    std::vector<uint32_t> order(num_cloths);
+    std::iota(order.begin(), order.end(), 0u);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](uint32_t a, uint32_t c) { return b->cloths[a].nv > b->cloths[c].nv; });
+    FrameParams fp0;
+    std::memset(&fp0, 0, sizeof(fp0));
+    auto bad = [&](hipError_t e, const char* what) {
+        delete b;
+        return fail(e == hipErrorOutOfMemory ? LATS_ERR_OUT_OF_MEMORY : LATS_ERR_HIP,
+                    std::string(what) + ": " + hipGetErrorString(e));
+    };
+    hipError_t e;
+    if ((e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)) != hipSuccess) return bad(e, "stream");
+    if ((e = b->x.upload(L.x)) != hipSuccess) return bad(e, "x");
+    if ((e = b->x0.upload(L.x)) != hipSuccess) return bad(e, "x0");
+    if ((e = b->v.upload(std::vector<float>(L.x.size(), 0.f))) != hipSuccess) return bad(e, "v");
+    if ((e = b->mass.upload(L.mass)) != hipSuccess) return bad(e, "mass");
+    if ((e = b->meta.upload(L.meta)) != hipSuccess) return bad(e, "meta");
+    if ((e = b->order.upload(order)) != hipSuccess) return bad(e, "order");
+    if ((e = b->work.upload(std::vector<uint32_t>{0u})) != hipSuccess) return bad(e, "work");
+    if ((e = b->ell_e.upload(L.ell_e)) != hipSuccess) return bad(e, "ell_e");
+    if ((e = b->ell_bq.upload(L.ell_bq)) != hipSuccess) return bad(e, "ell_bq");
+    if ((e = b->ell_bi.upload(L.ell_bi)) != hipSuccess) return bad(e, "ell_bi");
+    if ((e = b->cloths_d.upload(b->cloths)) != hipSuccess) return bad(e, "cloths");
+    if ((e = b->colours_d.upload(L.colours)) != hipSuccess) return bad(e, "colours");
+    if ((e = b->fp.upload(std::vector<FrameParams>{fp0})) != hipSuccess) return bad(e, "frame params");
+    if ((e = hipEventCreate(&b->ev0)) != hipSuccess) return bad(e, "event");
+    if ((e = hipEventCreate(&b->ev1)) != hipSuccess) return bad(e, "event");
+    *out = b;
+    return LATS_OK;
+}
+
+int lats_batch_destroy(lats_batch* b) {
+    if (!b) return LATS_OK;
+    (void)hipStreamSynchronize(b->stream);
+    delete b;
+    return LATS_OK;
+}
+
+int lats_batch_counts(lats_batch* b, uint32_t* num_cloths, uint64_t* total_verts) {
+    if (!b) return fail(LATS_ERR_INVALID_ARGUMENT, "null batch");
+    if (num_cloths) *num_cloths = b->ncloths;
+    if (total_verts) *total_verts = b->total_verts;
+    return LATS_OK;
+}
+
+int lats_batch_set_state(lats_batch* b, uint32_t k, const float* x, const float* v) {
+    if (!b || !x || !v || k >= b->ncloths) return fail(LATS_ERR_INVALID_ARGUMENT, "bad argument");
+    const ClothInfo& ci = b->cloths[k];
+    std::vector<float> hx(3 * (size_t)ci.nv), hv(3 * (size_t)ci.nv);
+    for (uint32_t s = 0; s < ci.nv; s++)
+        for (int c = 0; c < 3; c++) {
+            hx[3 * s + c] = x[3 * (size_t)b->perm[k][s] + c];
+            hv[3 * s + c] = v[3 * (size_t)b->perm[k][s] + c];
+        }
+    const size_t off = 3 * (size_t)ci.vbase;
+    HIPCHK(hipStreamSynchronize(b->stream));
+    HIPCHK(hipMemcpy(b->x.p + off, hx.data(), hx.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b->x0.p + off, hx.data(), hx.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b->v.p + off, hv.data(), hv.size() * 4, hipMemcpyHostToDevice));
+    return LATS_OK;
+}
+
+int lats_batch_get_state(lats_batch* b, uint32_t k, float* x, float* v) {
+    if (!b || k >= b->ncloths) return fail(LATS_ERR_INVALID_ARGUMENT, "bad argument");
+    const ClothInfo& ci = b->cloths[k];
+    std::vector<float> hx(3 * (size_t)ci.nv), hv(3 * (size_t)ci.nv);
+    const size_t off = 3 * (size_t)ci.vbase;
+    HIPCHK(hipStreamSynchronize(b->stream));
+    HIPCHK(hipMemcpy(hx.data(), b->x.p + off, hx.size() * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(hv.data(), b->v.p + off, hv.size() * 4, hipMemcpyDeviceToHost));
+    for (uint32_t s = 0; s < ci.nv; s++)
+        for (int c = 0; c < 3; c++) {
+            if (x) x[3 * (size_t)b->perm[k][s] + c] = hx[3 * s + c];
+            if (v) v[3 * (size_t)b->perm[k][s] + c] = hv[3 * s + c];
+        }
+    return LATS_OK;
+}
+
+int lats_batch_set_launch(lats_batch* b, uint32_t block_threads, uint32_t blocks) {
+    if (!b) return fail(LATS_ERR_INVALID_ARGUMENT, "null batch");
+    if (block_threads == 0 || block_threads > 1024 || block_threads % 64)
+        return fail(LATS_ERR_INVALID_ARGUMENT, "block_threads must be a multiple of 64 in [64, 1024]");
+    b->block_threads = block_threads;
+    b->blocks = blocks;
+    if (b->graph) { (void)hipGraphExecDestroy(b->graph); b->graph = nullptr; }
+    return LATS_OK;
+}
+
+int lats_batch_get_launch(lats_batch* b, uint32_t* block_threads, uint32_t* blocks) {
+    if (!b) return fail(LATS_ERR_INVALID_ARGUMENT, "null batch");
+    uint32_t mx = 0;
+    int rc = max_resident_blocks(b, b->block_threads, &mx);
+    if (rc) return rc;
+    if (block_threads) *block_threads = b->block_threads;
+    if (blocks) *blocks = std::min(b->blocks ? b->blocks : mx, std::max(1u, b->ncloths));
+    return LATS_OK;
+}
+
+int lats_batch_run_frames(lats_batch* b, uint32_t nframes) {
+    if (!b) return fail(LATS_ERR_INVALID_ARGUMENT, "null batch");
+    hipGraphExec_t g;
+    int rc = batch_graph(b, &g);
+    if (rc) return rc;
+    HIPCHK(hipEventRecord(b->ev0, b->stream));
+    for (uint32_t f = 0; f < nframes; f++) HIPCHK(hipGraphLaunch(g, b->stream));
+    HIPCHK(hipEventRecord(b->ev1, b->stream));
+    HIPCHK(hipEventSynchronize(b->ev1));
+    HIPCHK(hipEventElapsedTime(&b->last_ms, b->ev0, b->ev1));
+    return LATS_OK;
+}
+
+int lats_batch_last_elapsed_ms(lats_batch* b, float* ms) {
+    if (!b || !ms) return fail(LATS_ERR_INVALID_ARGUMENT, "null argument");
+    *ms = b->last_ms;
+    return LATS_OK;
+}
+
+// Times `frames` frames for every (block size, resident blocks) pair of a
+// small grid and keeps the fastest; restores the state afterwards.
+int lats_batch_autotune(lats_batch* b, uint32_t frames, uint32_t* best_threads, uint32_t* best_blocks) {
+    if (!b) return fail(LATS_ERR_INVALID_ARGUMENT, "null batch");
+    if (!frames) frames = 1;
+    const size_t nf = 3 * (size_t)b->total_verts;
+    float *sx = nullptr, *sx0 = nullptr, *sv = nullptr;
+    HIPCHK(hipMalloc((void**)&sx, nf * 4));
+    HIPCHK(hipMalloc((void**)&sx0, nf * 4));
+    HIPCHK(hipMalloc((void**)&sv, nf * 4));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    HIPCHK(hipMemcpy(sx, b->x.p, nf * 4, hipMemcpyDeviceToDevice));
+    HIPCHK(hipMemcpy(sx0, b->x0.p, nf * 4, hipMemcpyDeviceToDevice));
+    HIPCHK(hipMemcpy(sv, b->v.p, nf * 4, hipMemcpyDeviceToDevice));
+    float best = 3.4e38f;
+    uint32_t bt = b->block_threads, bb = b->blocks;
+    int rc = LATS_OK;
+    for (uint32_t threads : {128u, 256u, 512u, 1024u}) {
+        uint32_t mx = 0;
+        rc = max_resident_blocks(b, threads, &mx);
+        if (rc) break;
+        for (uint32_t blocks : {mx / 4, mx / 2, mx}) {
+            if (blocks == 0) continue;
+            rc = lats_batch_set_launch(b, threads, blocks);
+            if (!rc) rc = lats_batch_run_frames(b, 1);
+            if (!rc) rc = lats_batch_run_frames(b, frames);
+            if (rc) break;
+            if (b->last_ms < best) { best = b->last_ms; bt = threads; bb = blocks; }
+        }
+        if (rc) break;
+    }
+    hipError_t e1 = hipMemcpy(b->x.p, sx, nf * 4, hipMemcpyDeviceToDevice);
+    hipError_t e2 = hipMemcpy(b->x0.p, sx0, nf * 4, hipMemcpyDeviceToDevice);
+    hipError_t e3 = hipMemcpy(b->v.p, sv, nf * 4, hipMemcpyDeviceToDevice);
+    (void)hipFree(sx);
+    (void)hipFree(sx0);
+    (void)hipFree(sv);
+    if (rc) return rc;
+    if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) return fail(LATS_ERR_HIP, "state restore failed");
+    rc = lats_batch_set_launch(b, bt, bb);
+    if (rc) return rc;
+    if (best_threads) *best_threads = bt;
+    if (best_blocks) *best_blocks = bb;
+    return LATS_OK;
+}
+
+}  // extern "C"

@@ -29,6 +29,8 @@
 #include "host/errors.h"
 #include "host/topology.h"
 #include "vbd_common.h"
+#include "vbd_kernels.h"
+#include "vbd_layout.h"
 
 using namespace lats;
 
@@ -39,94 +41,89 @@ namespace {
 
 constexpr int kBlock = 256;
 
-struct FrameParams {
-    const float* forces;   // per-frame external force (3 floats / frame) or null
-    uint32_t frame;        // frame counter, advanced by the last kernel of a frame
-    uint32_t pad;
-    float ext[4];          // constant external force when forces == null
-    float ext_cur[4];      // force of the running frame (written by predict)
-};
-
-struct VbdArgs {
-    float* x;                  // positions being iterated (slot order)
-    const float* x0;           // x_start
-    const float* v;            // frame-start velocity
-    const float* mass;
-    const uint32_t* meta;
-    const uint2* ell_e;
-    const float4* ell_bq;
-    const uint4* ell_bi;
-    const FrameParams* fp;
-    float h, h2inv, ks, kb;
-    float g[3];
-};
-
-__device__ __forceinline__ F3 ld3(const float* p, uint32_t s) { return F3{p[3 * s], p[3 * s + 1], p[3 * s + 2]}; }
-__device__ __forceinline__ void st3(float* p, uint32_t s, F3 a) {
-    p[3 * s] = a.x;
-    p[3 * s + 1] = a.y;
-    p[3 * s + 2] = a.z;
-}
-
-// One VBD vertex update: VBD::compute_inertia + compute_stretch_mass_spring +
-// compute_bending_quadratic (xpbd_constraints.h:4027-4205) summed as
-// Hf = inertia; Hf += sum(stretch); Hf += sum(bending)
-// (example2_async_iter.cpp:939-992), then the guarded 3x3 Newton step of
-// vbd_step (:993-1014).  i = index inside the colour, n = colour size.
-__device__ __forceinline__ void vbd_vertex(const VbdArgs& a, uint32_t s, uint32_t i, uint32_t n, uint32_t eoff,
-                                           uint32_t boff) {
-    const F3 xk = ld3(a.x, s);
-    const F3 x0 = ld3(a.x0, s);
-    const F3 v0 = ld3(a.v, s);
-    const float m = a.mass[s];
-    const uint32_t meta = a.meta[s];
-    const uint32_t edeg = meta & 0xffu, bdeg = (meta >> 8) & 0xffu;
-    const float ex = a.fp->ext_cur[0], ey = a.fp->ext_cur[1], ez = a.fp->ext_cur[2];
-
-    // inertia
-    Sym3 H;
-    F3 f;
-    inertia_term(xk, x0, v0, m, (meta >> 16) & 1u, a.g, F3{ex, ey, ez}, a.h, a.h2inv, H, f);
-
-    // stretch: local sum from zero, then added to Hf
-    Sym3 S = sym_zero();
-    F3 fs = F3{0.f, 0.f, 0.f};
-    for (uint32_t j = 0; j < edeg; j++) {
-        const uint2 e = a.ell_e[eoff + j * n + i];
-        const F3 xo = ld3(a.x, e.x);
-        stretch_term(xk, xo, __uint_as_float(e.y), a.ks, S, fs);
-    }
-    sym_add(H, S);
-    f = add3(f, fs);
-
-    // bending: diagonal Hessian only (k * Q[o][o] * I)
-    float hb = 0.f;
-    F3 fb = F3{0.f, 0.f, 0.f};
-    for (uint32_t j = 0; j < bdeg; j++) {
-        const uint32_t k = boff + j * n + i;
-        const float4 q = a.ell_bq[k];
-        const uint4 id = a.ell_bi[k];
-        const uint32_t o = id.x >> 30;
-        const F3 p0 = (o == 0) ? xk : ld3(a.x, id.x & 0x3fffffffu);
-        const F3 p1 = (o == 1) ? xk : ld3(a.x, id.y);
-        const F3 p2 = (o == 2) ? xk : ld3(a.x, id.z);
-        const F3 p3 = (o == 3) ? xk : ld3(a.x, id.w);
-        bending_term(q, o, p0, p1, p2, p3, a.kb, hb, fb);
-    }
-    H.xx = H.xx + hb;
-    H.yy = H.yy + hb;
-    H.zz = H.zz + hb;
-    f = add3(f, fb);
-
-    F3 dx;
-    if (newton_step(H, f, dx)) st3(a.x, s, add3(xk, dx));
-}
-
 __global__ __launch_bounds__(kBlock) void k_vbd_colour(VbdArgs a, uint32_t cs, uint32_t n, uint32_t eoff,
                                                        uint32_t boff) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
-    vbd_vertex(a, cs + i, i, n, eoff, boff);
+    vbd_vertex_ext(a, cs + i, i, n, eoff, boff, F3{a.fp->ext_cur[0], a.fp->ext_cur[1], a.fp->ext_cur[2]});
+}
+
+// --- persistent frame kernel ---------------------------------------------
+// One cooperative launch runs whole frames: predict, every (iteration, colour)
+// phase, update velocity; a grid barrier separates dependent phases.  All
+// blocks are co-resident (hipLaunchCooperativeKernel), so the barrier's wait
+// always ends.
+struct PersistArgs {
+    VbdArgs a;                 // a.x = positions, a.fp = frame parameters
+    float* x0;                 // x_start (written by update velocity)
+    float* v;                  // velocity (written by update velocity)
+    FrameParams* fp;
+    const ColourInfo* colours;
+    uint32_t* barrier;         // monotonically increasing arrival counter
+    uint32_t nv, ncolours, iters, nframes;
+    float damp;
+};
+
+__device__ __forceinline__ void grid_barrier(uint32_t* bar, uint32_t& target) {
+    target += gridDim.x;
+    __threadfence();           // every wave: its own stores reach device scope
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        atomicAdd(bar, 1u);
+        while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target)
+            __builtin_amdgcn_s_sleep(1);
+        __threadfence();
+    }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(kBlock) void k_vbd_persistent(PersistArgs p) {
+    uint32_t target = 0;
+    const uint32_t stride = gridDim.x * kBlock;
+    const uint32_t t0 = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t frame0 = p.fp->frame;
+    for (uint32_t f = 0; f < p.nframes; f++) {
+        // external force of this frame, read by every thread
+        const float* src = p.fp->forces ? p.fp->forces + 3 * (size_t)(frame0 + f) : p.fp->ext;
+        const F3 ext = F3{src[0], src[1], src[2]};
+        // predict (Core::predict_position)
+        for (uint32_t s = t0; s < p.nv; s += stride) {
+            F3 vpred = add3(ld3(p.v, s), scale3(p.a.h, F3{0.f, -9.8f, 0.f}));
+            if ((p.a.meta[s] >> 16) & 1u) vpred = F3{0.f, 0.f, 0.f};
+            st3(p.a.x, s, add3(ld3(p.x0, s), scale3(p.a.h, vpred)));
+        }
+        grid_barrier(p.barrier, target);
+        for (uint32_t it = 0; it < p.iters; it++) {
+            for (uint32_t c = 0; c < p.ncolours; c++) {
+                const ColourInfo ci = p.colours[c];
+                for (uint32_t i = t0; i < ci.count; i += stride)
+                    vbd_vertex_ext(p.a, ci.start + i, i, ci.count, ci.eoff, ci.boff, ext);
+                grid_barrier(p.barrier, target);
+            }
+        }
+        // update velocity (Core::update_velocity)
+        for (uint32_t s = t0; s < p.nv; s += stride) {
+            const F3 xk = ld3(p.a.x, s);
+            const F3 dx = sub3(xk, ld3(p.x0, s));
+            F3 vel = F3{dx.x / p.a.h, dx.y / p.a.h, dx.z / p.a.h};
+            vel = F3{vel.x * p.damp, vel.y * p.damp, vel.z * p.damp};
+            st3(p.v, s, vel);
+            st3(p.x0, s, xk);
+        }
+        grid_barrier(p.barrier, target);
+    }
+    if (t0 == 0) {
+        p.fp->frame = frame0 + p.nframes;
+        const float* src = p.fp->forces ? p.fp->forces + 3 * (size_t)(frame0 + p.nframes - 1) : p.fp->ext;
+        p.fp->ext_cur[0] = src[0];
+        p.fp->ext_cur[1] = src[1];
+        p.fp->ext_cur[2] = src[2];
+    }
+}
+
+__global__ void k_zero_u32(uint32_t* p, uint32_t n) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = 0u;
 }
 
 // Core::predict_position (xpbd_constraints.h:321-384); gravity hard-coded
@@ -357,6 +354,12 @@ struct lats_vbd {
     // multi-buffer positions of the async schedule
     DBuf<float> iterbuf[LATS_NUM_ASYNC_BUFFERS], beginbuf[LATS_NUM_ASYNC_BUFFERS];
 
+    // persistent strategy
+    DBuf<ColourInfo> colours_d;
+    DBuf<uint32_t> barrier;
+    uint32_t persist_blocks = 0;      // 0 = derive from the largest colour
+    uint32_t persist_max_blocks = 0;  // co-residency limit (queried on first use)
+
     // graph per (record) flag
     hipGraphExec_t graph[2] = {nullptr, nullptr};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -478,6 +481,55 @@ void enqueue_frame(lats_vbd* c, bool record) {
                            c->partial.p, c->energy_nblk, 0u, c->energy.p);
 }
 
+int persistent_grid(lats_vbd* c, uint32_t* blocks) {
+    if (!c->persist_max_blocks) {
+        int per_cu = 0, ncu = 0;
+        HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_vbd_persistent, kBlock, 0));
+        HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device));
+        c->persist_max_blocks = (uint32_t)std::max(1, per_cu * ncu);
+    }
+    uint32_t g = c->persist_blocks;
+    if (!g) {
+        uint32_t mx = 0;
+        for (uint32_t k = 0; k < c->C; k++) mx = std::max(mx, c->ccount[k]);
+        g = nblocks(mx);
+    }
+    *blocks = std::max(1u, std::min(g, c->persist_max_blocks));
+    return LATS_OK;
+}
+
+// nframes frames in one cooperative launch (no energy recording).
+int run_persistent(lats_vbd* c, uint32_t nframes) {
+    if (!nframes) return LATS_OK;
+    uint32_t g = 0;
+    int rc = persistent_grid(c, &g);
+    if (rc) return rc;
+    PersistArgs pa;
+    pa.a = c->args(c->x.p);
+    pa.x0 = c->x0.p;
+    pa.v = c->v.p;
+    pa.fp = c->fp.p;
+    pa.colours = c->colours_d.p;
+    pa.barrier = c->barrier.p;
+    pa.nv = c->V;
+    pa.ncolours = c->C;
+    pa.iters = c->prm.constraint_iter_count;
+    pa.nframes = nframes * c->prm.num_substep;
+    pa.damp = c->damp;
+    if (c->prm.num_substep != 1) return fail(LATS_ERR_UNSUPPORTED, "persistent strategy runs one substep per frame");
+    // the barrier counter grows by g per phase: split long runs so it stays below 2^31
+    const uint64_t per_frame = ((uint64_t)pa.iters * c->C + 2) * g;
+    const uint32_t chunk = (uint32_t)std::max<uint64_t>(1, (1ull << 31) / per_frame);
+    for (uint32_t done = 0; done < nframes;) {
+        pa.nframes = std::min(chunk, nframes - done);
+        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c->stream, c->barrier.p, 1u);
+        void* kargs[] = {&pa};
+        HIPCHK(hipLaunchCooperativeKernel((const void*)k_vbd_persistent, dim3(g), dim3(kBlock), kargs, 0, c->stream));
+        done += pa.nframes;
+    }
+    return LATS_OK;
+}
+
 int get_graph(lats_vbd* c, bool record, hipGraphExec_t* out) {
     if (record) {
         int rc = ensure_energy_rows(c, c->prm.constraint_iter_count + 1);
@@ -501,6 +553,7 @@ int get_graph(lats_vbd* c, bool record, hipGraphExec_t* out) {
 }
 
 int run_frame(lats_vbd* c, bool record) {
+    if (!record && c->strategy == LATS_STRATEGY_PERSISTENT) return run_persistent(c, 1);
     hipGraphExec_t g;
     int rc = get_graph(c, record, &g);
     if (rc) return rc;
@@ -525,19 +578,6 @@ int set_frame_params(lats_vbd* c, const float* forces_dev, uint32_t frame, const
     return LATS_OK;
 }
 
-int validate_desc(const lats_mesh_desc* m) {
-    if (!m) return fail(LATS_ERR_INVALID_ARGUMENT, "mesh is null");
-    if (m->num_verts == 0) return fail(LATS_ERR_INVALID_ARGUMENT, "mesh has no vertices");
-    if (m->num_verts >= (1u << 30)) return fail(LATS_ERR_UNSUPPORTED, "more than 2^30 vertices");
-    if (!m->rest_x || !m->vert_mass || !m->is_fixed || !m->clusters_csr || !m->vert_adj_edges_csr ||
-        !m->vert_adj_bending_csr)
-        return fail(LATS_ERR_INVALID_ARGUMENT, "mesh array is null");
-    if (m->num_edges && (!m->edges || !m->edges_rest_length))
-        return fail(LATS_ERR_INVALID_ARGUMENT, "edge arrays are null");
-    if (m->num_bending_edges && (!m->bending_edges || !m->bending_Q))
-        return fail(LATS_ERR_INVALID_ARGUMENT, "bending arrays are null");
-    return LATS_OK;
-}
 
 }  // namespace
 
@@ -549,20 +589,16 @@ extern "C" {
 int lats_vbd_create(const lats_mesh_desc* m, const lats_scene_params* prm, lats_vbd** out) {
     if (!out || !prm) return fail(LATS_ERR_INVALID_ARGUMENT, "null argument");
     *out = nullptr;
-    int rc = validate_desc(m);
-    if (rc) return rc;
     if (prm->num_substep == 0 || !(prm->implicit_dt > 0.f))
         return fail(LATS_ERR_INVALID_ARGUMENT, "implicit_dt must be > 0 and num_substep >= 1");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(LATS_ERR_NO_DEVICE, "no HIP device");
 
     const uint32_t V = m->num_verts, E = m->num_edges, B = m->num_bending_edges, C = m->num_clusters;
-    const uint32_t* cl = m->clusters_csr;
-    if (cl[0] != C + 1 || cl[C] != C + 1 + V)
-        return fail(LATS_ERR_INVALID_ARGUMENT, "clusters CSR does not partition the vertices");
-    const uint32_t* pe = m->vert_adj_edges_csr;
-    const uint32_t* pb = m->vert_adj_bending_csr;
-    if (pe[0] != V + 1 || pb[0] != V + 1) return fail(LATS_ERR_INVALID_ARGUMENT, "adjacency CSR header mismatch");
+    ClothLayout L;
+    std::vector<uint32_t> inv;
+    std::string err;
+    if (!append_cloth(*m, L, inv, err)) return fail(LATS_ERR_INVALID_ARGUMENT, err);
 
     auto c = new lats_vbd();
     c->V = V;
@@ -576,105 +612,23 @@ int lats_vbd_create(const lats_mesh_desc* m, const lats_scene_params* prm, lats_
     c->h_energy = prm->implicit_dt / (float)prm->num_substep;
     c->h2inv = 1.f / (c->h * c->h);
     c->damp = std::exp(-prm->damping_cloth * c->h);
-
-    // --- colour order (slots)
-    c->perm.resize(V);
-    c->inv.assign(V, 0xffffffffu);
-    c->cstart.resize(C);
-    c->ccount.resize(C);
-    for (uint32_t k = 0; k < C; k++) {
-        c->cstart[k] = cl[k] - (C + 1);
-        c->ccount[k] = cl[k + 1] - cl[k];
-        if (c->ccount[k] == 0) { delete c; return fail(LATS_ERR_INVALID_ARGUMENT, "empty colour cluster"); }
+    c->perm = L.perm;
+    c->inv = inv;
+    for (const ColourInfo& ci : L.colours) {
+        c->cstart.push_back(ci.start);
+        c->ccount.push_back(ci.count);
+        c->eoff.push_back(ci.eoff);
+        c->boff.push_back(ci.boff);
     }
-    for (uint32_t s = 0; s < V; s++) {
-        const uint32_t vid = cl[C + 1 + s];
-        if (vid >= V || c->inv[vid] != 0xffffffffu) {
-            delete c;
-            return fail(LATS_ERR_INVALID_ARGUMENT, "clusters CSR is not a permutation of the vertices");
-        }
-        c->perm[s] = vid;
-        c->inv[vid] = s;
-    }
-
-    // --- per-slot arrays and ELL adjacency
-    std::vector<float> hx(3 * (size_t)V), hmass(V);
-    std::vector<uint32_t> hmeta(V);
-    std::vector<uint32_t> e_max(C, 0), b_max(C, 0);
-    for (uint32_t s = 0; s < V; s++) {
-        const uint32_t vid = c->perm[s];
-        for (int k = 0; k < 3; k++) hx[3 * s + k] = m->rest_x[3 * (size_t)vid + k];
-        hmass[s] = m->vert_mass[vid];
-        const uint32_t ed = pe[vid + 1] - pe[vid], bd = pb[vid + 1] - pb[vid];
-        if (ed > 255 || bd > 255) { delete c; return fail(LATS_ERR_UNSUPPORTED, "vertex degree above 255"); }
-        hmeta[s] = ed | (bd << 8) | ((m->is_fixed[vid] ? 1u : 0u) << 16);
-    }
-    for (uint32_t k = 0; k < C; k++)
-        for (uint32_t i = 0; i < c->ccount[k]; i++) {
-            const uint32_t s = c->cstart[k] + i;
-            e_max[k] = std::max(e_max[k], hmeta[s] & 0xffu);
-            b_max[k] = std::max(b_max[k], (hmeta[s] >> 8) & 0xffu);
-        }
-    c->eoff.resize(C);
-    c->boff.resize(C);
-    size_t ne_tot = 0, nb_tot = 0;
-    for (uint32_t k = 0; k < C; k++) {
-        c->eoff[k] = (uint32_t)ne_tot;
-        c->boff[k] = (uint32_t)nb_tot;
-        ne_tot += (size_t)e_max[k] * c->ccount[k];
-        nb_tot += (size_t)b_max[k] * c->ccount[k];
-    }
-    if (ne_tot >= (1ull << 32) || nb_tot >= (1ull << 32)) {
-        delete c;
-        return fail(LATS_ERR_UNSUPPORTED, "adjacency too large");
-    }
-    std::vector<uint2> h_ell_e(std::max<size_t>(ne_tot, 1), uint2{0, 0});
-    std::vector<float4> h_ell_bq(std::max<size_t>(nb_tot, 1), float4{0, 0, 0, 0});
-    std::vector<uint4> h_ell_bi(std::max<size_t>(nb_tot, 1), uint4{0, 0, 0, 0});
-    for (uint32_t k = 0; k < C; k++) {
-        const uint32_t n = c->ccount[k];
-        for (uint32_t i = 0; i < n; i++) {
-            const uint32_t s = c->cstart[k] + i, vid = c->perm[s];
-            for (uint32_t j = 0, r = pe[vid]; r < pe[vid + 1]; r++, j++) {
-                const uint32_t e = pe[r];
-                if (e >= E) { delete c; return fail(LATS_ERR_INVALID_ARGUMENT, "edge id out of range"); }
-                const uint32_t a0 = (uint32_t)m->edges[2 * e], a1 = (uint32_t)m->edges[2 * e + 1];
-                const uint32_t other = (a0 == vid) ? a1 : a0;
-                uint32_t rl;
-                std::memcpy(&rl, &m->edges_rest_length[e], 4);
-                h_ell_e[c->eoff[k] + (size_t)j * n + i] = uint2{c->inv[other], rl};
-            }
-            for (uint32_t j = 0, r = pb[vid]; r < pb[vid + 1]; r++, j++) {
-                const uint32_t b = pb[r];
-                if (b >= B) { delete c; return fail(LATS_ERR_INVALID_ARGUMENT, "bending id out of range"); }
-                const int32_t* q4 = &m->bending_edges[4 * (size_t)b];
-                uint32_t o = 0;
-                while (o < 4 && (uint32_t)q4[o] != vid) o++;
-                if (o == 4) { delete c; return fail(LATS_ERR_INVALID_ARGUMENT, "bending adjacency mismatch"); }
-                const float* Q = &m->bending_Q[16 * (size_t)b + 4 * o];
-                const size_t at = c->boff[k] + (size_t)j * n + i;
-                h_ell_bq[at] = float4{Q[0], Q[1], Q[2], Q[3]};
-                h_ell_bi[at] = uint4{c->inv[q4[0]] | (o << 30), c->inv[q4[1]], c->inv[q4[2]], c->inv[q4[3]]};
-            }
-        }
-    }
-    // element lists in slot ids for the energy
-    std::vector<uint2> h_es(std::max<uint32_t>(E, 1), uint2{0, 0});
-    std::vector<float> h_rest(std::max<uint32_t>(E, 1), 0.f);
-    for (uint32_t e = 0; e < E; e++) {
-        h_es[e] = uint2{c->inv[m->edges[2 * e]], c->inv[m->edges[2 * e + 1]]};
-        h_rest[e] = m->edges_rest_length[e];
-    }
-    std::vector<uint4> h_bs(std::max<uint32_t>(B, 1), uint4{0, 0, 0, 0});
-    std::vector<float4> h_Q(4 * (size_t)std::max<uint32_t>(B, 1), float4{0, 0, 0, 0});
-    for (uint32_t b = 0; b < B; b++) {
-        const int32_t* q4 = &m->bending_edges[4 * (size_t)b];
-        h_bs[b] = uint4{c->inv[q4[0]], c->inv[q4[1]], c->inv[q4[2]], c->inv[q4[3]]};
-        for (int col = 0; col < 4; col++) {
-            const float* Q = &m->bending_Q[16 * (size_t)b + 4 * col];
-            h_Q[4 * (size_t)b + col] = float4{Q[0], Q[1], Q[2], Q[3]};
-        }
-    }
+    auto pad = [](auto& v, auto z) { if (v.empty()) v.push_back(z); };
+    pad(L.ell_e, uint2{0, 0});
+    pad(L.ell_bq, float4{0, 0, 0, 0});
+    pad(L.ell_bi, uint4{0, 0, 0, 0});
+    pad(L.e_slots, uint2{0, 0});
+    pad(L.rest, 0.f);
+    pad(L.b_slots, uint4{0, 0, 0, 0});
+    pad(L.Q, float4{0, 0, 0, 0});
+    const std::vector<float>& hx = L.x;
 
     auto cleanup_fail = [&](int code) { delete c; return code; };
 #define UPCHK(call) do { hipError_t e_ = (call); if (e_ != hipSuccess) return cleanup_fail(fail(e_ == hipErrorOutOfMemory ? LATS_ERR_OUT_OF_MEMORY : LATS_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_))); } while (0)
@@ -682,18 +636,22 @@ int lats_vbd_create(const lats_mesh_desc* m, const lats_scene_params* prm, lats_
     UPCHK(c->x.upload(hx));
     UPCHK(c->x0.upload(hx));
     UPCHK(c->v.upload(std::vector<float>(3 * (size_t)V, 0.f)));
-    UPCHK(c->mass.upload(hmass));
-    UPCHK(c->meta.upload(hmeta));
+    UPCHK(c->mass.upload(L.mass));
+    UPCHK(c->meta.upload(L.meta));
     UPCHK(c->perm_d.upload(c->perm));
     UPCHK(c->inv_d.upload(c->inv));
-    UPCHK(c->ell_e.upload(h_ell_e));
-    UPCHK(c->ell_bq.upload(h_ell_bq));
-    UPCHK(c->ell_bi.upload(h_ell_bi));
-    UPCHK(c->e_slots.upload(h_es));
-    UPCHK(c->rest.upload(h_rest));
-    UPCHK(c->b_slots.upload(h_bs));
-    UPCHK(c->Qd.upload(h_Q));
+    UPCHK(c->ell_e.upload(L.ell_e));
+    UPCHK(c->ell_bq.upload(L.ell_bq));
+    UPCHK(c->ell_bi.upload(L.ell_bi));
+    UPCHK(c->e_slots.upload(L.e_slots));
+    UPCHK(c->rest.upload(L.rest));
+    UPCHK(c->b_slots.upload(L.b_slots));
+    UPCHK(c->Qd.upload(L.Q));
     UPCHK(c->scratch3.alloc(3 * (size_t)V));
+    {
+        UPCHK(c->colours_d.upload(L.colours));
+        UPCHK(c->barrier.alloc(1));
+    }
     FrameParams fp0;
     std::memset(&fp0, 0, sizeof(fp0));
     UPCHK(c->fp.upload(std::vector<FrameParams>{fp0}));
@@ -719,8 +677,83 @@ int lats_vbd_set_strategy(lats_vbd* c, int strategy) {
     if (!c) return fail(LATS_ERR_INVALID_ARGUMENT, "null context");
     if (strategy < LATS_STRATEGY_AUTO || strategy > LATS_STRATEGY_PERSISTENT)
         return fail(LATS_ERR_INVALID_ARGUMENT, "unknown strategy");
-    if (strategy == LATS_STRATEGY_PERSISTENT) return fail(LATS_ERR_UNSUPPORTED, "persistent strategy not built");
+    if (strategy == LATS_STRATEGY_PERSISTENT && c->prm.num_substep != 1)
+        return fail(LATS_ERR_UNSUPPORTED, "persistent strategy runs one substep per frame");
     c->strategy = strategy;
+    return LATS_OK;
+}
+
+int lats_vbd_set_persistent_blocks(lats_vbd* c, uint32_t blocks) {
+    if (!c) return fail(LATS_ERR_INVALID_ARGUMENT, "null context");
+    c->persist_blocks = blocks;
+    return LATS_OK;
+}
+
+int lats_vbd_get_strategy(lats_vbd* c, int* strategy, uint32_t* blocks) {
+    if (!c) return fail(LATS_ERR_INVALID_ARGUMENT, "null context");
+    if (strategy) *strategy = c->strategy;
+    if (blocks) {
+        uint32_t g = 0;
+        int rc = persistent_grid(c, &g);
+        if (rc) return rc;
+        *blocks = g;
+    }
+    return LATS_OK;
+}
+
+// Times `frames` frames with the graph strategy and with the persistent one
+// at several grid sizes, keeps the fastest, restores the state.
+int lats_vbd_autotune(lats_vbd* c, uint32_t frames, int* chosen, uint32_t* chosen_blocks) {
+    if (!c) return fail(LATS_ERR_INVALID_ARGUMENT, "null context");
+    if (!frames) frames = 1;
+    const size_t nf = 3 * (size_t)c->V;
+    DBuf<float> sx, sx0, sv;
+    HIPCHK(sx.alloc(nf));
+    HIPCHK(sx0.alloc(nf));
+    HIPCHK(sv.alloc(nf));
+    FrameParams fps;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(sx.p, c->x.p, nf * 4, hipMemcpyDeviceToDevice));
+    HIPCHK(hipMemcpy(sx0.p, c->x0.p, nf * 4, hipMemcpyDeviceToDevice));
+    HIPCHK(hipMemcpy(sv.p, c->v.p, nf * 4, hipMemcpyDeviceToDevice));
+    HIPCHK(hipMemcpy(&fps, c->fp.p, sizeof(fps), hipMemcpyDeviceToHost));
+    auto restore = [&]() -> int {
+        HIPCHK(hipMemcpy(c->x.p, sx.p, nf * 4, hipMemcpyDeviceToDevice));
+        HIPCHK(hipMemcpy(c->x0.p, sx0.p, nf * 4, hipMemcpyDeviceToDevice));
+        HIPCHK(hipMemcpy(c->v.p, sv.p, nf * 4, hipMemcpyDeviceToDevice));
+        HIPCHK(hipMemcpy(c->fp.p, &fps, sizeof(fps), hipMemcpyHostToDevice));
+        return LATS_OK;
+    };
+    struct Cand { int strategy; uint32_t blocks; };
+    std::vector<Cand> cands = {{LATS_STRATEGY_GRAPH, 0}};
+    if (c->prm.num_substep == 1) {
+        uint32_t g0 = 0;
+        const uint32_t keep = c->persist_blocks;
+        c->persist_blocks = 0;
+        int rc = persistent_grid(c, &g0);
+        if (rc) return rc;
+        for (uint32_t g : {g0 / 4, g0 / 2, g0, 2 * g0, 4 * g0, c->persist_max_blocks})
+            if (g >= 1 && g <= c->persist_max_blocks) cands.push_back({LATS_STRATEGY_PERSISTENT, g});
+        c->persist_blocks = keep;
+    }
+    float best = 3.4e38f;
+    Cand win = cands[0];
+    const int keep_strategy = c->strategy;
+    const uint32_t keep_blocks = c->persist_blocks;
+    for (const Cand& k : cands) {
+        c->strategy = k.strategy;
+        c->persist_blocks = k.blocks;
+        int rc = lats_vbd_run_frames(c, 1, nullptr);   // warm-up / capture
+        if (!rc) rc = lats_vbd_run_frames(c, frames, nullptr);
+        if (rc) { c->strategy = keep_strategy; c->persist_blocks = keep_blocks; restore(); return rc; }
+        if (c->last_ms < best) { best = c->last_ms; win = k; }
+        rc = restore();
+        if (rc) return rc;
+    }
+    c->strategy = win.strategy;
+    c->persist_blocks = win.blocks;
+    if (chosen) *chosen = win.strategy;
+    if (chosen_blocks) *chosen_blocks = win.blocks;
     return LATS_OK;
 }
 
@@ -798,11 +831,18 @@ int lats_vbd_run_frames(lats_vbd* c, uint32_t nframes, const float* forces) {
         int rc = set_frame_params(c, c->forces.p, 0, nullptr);
         if (rc) return rc;
     }
-    hipGraphExec_t g;
-    int rc = get_graph(c, false, &g);
-    if (rc) return rc;
-    HIPCHK(hipEventRecord(c->ev0, c->stream));
-    for (uint32_t f = 0; f < nframes; f++) HIPCHK(hipGraphLaunch(g, c->stream));
+    int rc;
+    if (c->strategy == LATS_STRATEGY_PERSISTENT) {
+        HIPCHK(hipEventRecord(c->ev0, c->stream));
+        rc = run_persistent(c, nframes);
+        if (rc) return rc;
+    } else {
+        hipGraphExec_t g;
+        rc = get_graph(c, false, &g);
+        if (rc) return rc;
+        HIPCHK(hipEventRecord(c->ev0, c->stream));
+        for (uint32_t f = 0; f < nframes; f++) HIPCHK(hipGraphLaunch(g, c->stream));
+    }
     HIPCHK(hipEventRecord(c->ev1, c->stream));
     HIPCHK(hipEventSynchronize(c->ev1));
     HIPCHK(hipEventElapsedTime(&c->last_ms, c->ev0, c->ev1));

@@ -170,6 +170,23 @@ class GpuVBD:
     def handle(self):
         return self._h
 
+    def set_strategy(self, strategy, persistent_blocks=None):
+        check(lib().lats_vbd_set_strategy(self._h, int(strategy)))
+        if persistent_blocks is not None:
+            check(lib().lats_vbd_set_persistent_blocks(self._h, int(persistent_blocks)))
+
+    def strategy(self):
+        st = ctypes.c_int()
+        nb = ctypes.c_uint32()
+        check(lib().lats_vbd_get_strategy(self._h, ctypes.byref(st), ctypes.byref(nb)))
+        return st.value, nb.value
+
+    def autotune(self, frames=2):
+        st = ctypes.c_int()
+        nb = ctypes.c_uint32()
+        check(lib().lats_vbd_autotune(self._h, int(frames), ctypes.byref(st), ctypes.byref(nb)))
+        return st.value, nb.value
+
     def set_energy_bending(self, on):
         check(lib().lats_vbd_set_energy_bending(self._h, int(bool(on))))
 
@@ -292,3 +309,88 @@ class SolverInterface:
                 f.write(f"v {p[0]} {p[1]} {p[2]}\n")
             for t in self.cloth.faces + 1:
                 f.write(f"f {t[0]} {t[1]} {t[2]}\n")
+
+
+class GpuBatch:
+    """Many independent cloths stepped together (one block per cloth)."""
+
+    def __init__(self, cloths, params, stretch=None, bending=None):
+        self.cloths = list(cloths)
+        self.params = params
+        n = len(self.cloths)
+        descs = (_lib.MeshDesc * n)()
+        self._keep = []
+        for k, cl in enumerate(self.cloths):
+            t = cl.topology
+            self._keep += [cl.x, t.edges, t.bending, t.rest_length, t.bending_Q, t.adj_edges_csr,
+                           t.adj_bending_csr, t.clusters_csr, cl.mass, cl.fixed]
+            d = descs[k]
+            d.num_verts, d.num_edges = t.num_verts, t.num_edges
+            d.num_bending_edges, d.num_clusters = t.num_bending, t.num_clusters
+            d.rest_x = _p(cl.x).value
+            d.edges = _p(t.edges).value
+            d.bending_edges = _p(t.bending).value
+            d.edges_rest_length = _p(t.rest_length).value
+            d.bending_Q = _p(t.bending_Q).value
+            d.vert_adj_edges_csr = _p(t.adj_edges_csr).value
+            d.vert_adj_bending_csr = _p(t.adj_bending_csr).value
+            d.clusters_csr = _p(t.clusters_csr).value
+            d.vert_mass = _p(cl.mass).value
+            d.is_fixed = _p(cl.fixed).value
+        ks = None if stretch is None else np.ascontiguousarray(stretch, np.float32)
+        kb = None if bending is None else np.ascontiguousarray(bending, np.float32)
+        self._params = params.c_struct()
+        h = ctypes.c_void_p()
+        check(lib().lats_batch_create(n, descs, None if ks is None else _p(ks), None if kb is None else _p(kb),
+                                      ctypes.byref(self._params), ctypes.byref(h)))
+        self._h = h
+        del descs
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().lats_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def counts(self):
+        n = ctypes.c_uint32()
+        v = ctypes.c_uint64()
+        check(lib().lats_batch_counts(self._h, ctypes.byref(n), ctypes.byref(v)))
+        return n.value, v.value
+
+    def state(self, k):
+        nv = self.cloths[k].num_verts
+        x = np.zeros((nv, 3), np.float32)
+        v = np.zeros((nv, 3), np.float32)
+        check(lib().lats_batch_get_state(self._h, k, _p(x), _p(v)))
+        return x, v
+
+    def set_state(self, k, x, v):
+        x = np.ascontiguousarray(x, np.float32)
+        v = np.ascontiguousarray(v, np.float32)
+        check(lib().lats_batch_set_state(self._h, k, _p(x), _p(v)))
+
+    def set_launch(self, block_threads, blocks=0):
+        check(lib().lats_batch_set_launch(self._h, int(block_threads), int(blocks)))
+
+    def launch(self):
+        t = ctypes.c_uint32()
+        b = ctypes.c_uint32()
+        check(lib().lats_batch_get_launch(self._h, ctypes.byref(t), ctypes.byref(b)))
+        return t.value, b.value
+
+    def autotune(self, frames=1):
+        t = ctypes.c_uint32()
+        b = ctypes.c_uint32()
+        check(lib().lats_batch_autotune(self._h, int(frames), ctypes.byref(t), ctypes.byref(b)))
+        return t.value, b.value
+
+    def run_frames(self, num_frames):
+        check(lib().lats_batch_run_frames(self._h, int(num_frames)))
+
+    def last_elapsed_ms(self):
+        ms = ctypes.c_float()
+        check(lib().lats_batch_last_elapsed_ms(self._h, ctypes.byref(ms)))
+        return ms.value

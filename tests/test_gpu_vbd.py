@@ -125,7 +125,10 @@ def test_c1_long_run_invariants(gpu_available):
     x, v = g.state()
     assert np.all(np.isfinite(x)) and np.all(np.isfinite(v))
     pins = cloth.fixed.astype(bool)
-    assert np.abs(x[pins] - cloth.x[pins]).max() < 1e-5
+    # the reference's pin is a 1e9 Hessian term with no restoring force
+    # (VBD::compute_inertia): the 65 kg cloth hanging on 256 pins drags them
+    # by ~1e-3 m over 60 frames, identically in the oracle
+    assert np.abs(x[pins] - cloth.x[pins]).max() < 5e-3
     assert x[~pins, 1].min() > -2.56 * 1.2          # stretch stiffness holds the cloth together
     e = g.physics_step(record_energy=True)
     assert e[-1] < e[0]
@@ -267,3 +270,41 @@ def test_host_signal_and_markers(gpu_available):
     ms = ctypes.c_float()
     _lib.check(L.lats_vbd_marker_elapsed_ms(g.handle, 1, 2, ctypes.byref(ms)))
     assert ms.value >= 0.0
+
+
+@pytest.mark.parametrize("blocks", [0, 7, 200])
+def test_persistent_strategy_bit_exact(gpu_available, blocks):
+    """One cooperative launch per run (grid barrier per colour) == graph path."""
+    _print_case(f"persistent strategy, {blocks} blocks, grid_c0 + wind")
+    cloth, params, _, _ = configs.grid_c0()
+    wind = configs.make_wind(6, cloth.num_verts, 2.0, 5) * 30.0
+    a = GpuVBD(cloth, params)
+    a.set_strategy(_lib.LATS_STRATEGY_GRAPH)
+    b = GpuVBD(cloth, params)
+    b.set_strategy(_lib.LATS_STRATEGY_PERSISTENT, persistent_blocks=blocks)
+    a.run_frames(6, wind)
+    b.run_frames(6, wind)
+    assert np.array_equal(a.state()[0], b.state()[0])
+    assert np.array_equal(a.state()[1], b.state()[1])
+    # physics_step after a persistent run continues the same frame sequence
+    a.set_external_force(wind[0])
+    b.set_external_force(wind[0])
+    a.physics_step()
+    b.physics_step()
+    assert np.array_equal(a.state()[0], b.state()[0])
+
+
+def test_persistent_c1_and_autotune(gpu_available):
+    _print_case("persistent strategy on grid_c1 + autotune")
+    cloth, params, _, wind = configs.grid_c1(frames=3)
+    a = GpuVBD(cloth, params)
+    a.set_strategy(_lib.LATS_STRATEGY_GRAPH)
+    b = GpuVBD(cloth, params)
+    x_before = b.state()[0]
+    st, nb = b.autotune(frames=1)
+    assert st in (_lib.LATS_STRATEGY_GRAPH, _lib.LATS_STRATEGY_PERSISTENT)
+    assert np.array_equal(b.state()[0], x_before)          # autotune restores the state
+    b.set_strategy(_lib.LATS_STRATEGY_PERSISTENT)
+    a.run_frames(3, wind)
+    b.run_frames(3, wind)
+    assert np.array_equal(a.state()[0], b.state()[0])
