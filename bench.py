@@ -70,18 +70,30 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", default="grid_c1", choices=["grid_c1", "batch_c2"],
+                    help="grid_c1 (default, the metric's workload) or batch_c2 (512 cloths)")
+    ap.add_argument("--strategy", default="auto", choices=["auto", "graph", "persistent"])
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the VBD hot path on a single MI355X "
               "(multi-GPU sharding is a design note only); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
-    from libatssim_amd import GpuVBD, configs
+    from libatssim_amd import GpuVBD, configs, _lib
+
+    if args.workload == "batch_c2":
+        return bench_batch(args)
 
     K, W = args.steps, args.warmup
     cloth, params, _, wind = configs.grid_c1(frames=W + K)
     t = cloth.topology
     g = GpuVBD(cloth, params)
+    if args.strategy == "auto":
+        strategy, blocks = g.autotune(frames=2)
+    else:
+        strategy = _lib.LATS_STRATEGY_GRAPH if args.strategy == "graph" else _lib.LATS_STRATEGY_PERSISTENT
+        g.set_strategy(strategy)
+        blocks = g.strategy()[1]
     if W:
         g.run_frames(W, wind[:W])
     g.synchronize()
@@ -94,10 +106,18 @@ def main():
     vu = cloth.num_verts * iters * K
     value = vu / wall
 
-    # dominant kernel: per-colour VBD launches timed with HIP events
-    mean_ms, launches, kvu = g.time_colour_kernels(10)
     b_sweep = algorithmic_bytes_per_sweep(t.num_verts, t.num_edges, t.num_bending)
-    bytes_per_launch = b_sweep / t.num_clusters
+    if strategy == _lib.LATS_STRATEGY_PERSISTENT:
+        # one cooperative launch runs the K timed frames: its duration is the
+        # HIP-event time of the timed region on the solver's stream
+        kernel = "k_vbd_persistent"
+        mean_ms = gpu_ms
+        bytes_per_launch = b_sweep * iters * K
+    else:
+        # per-colour VBD launches timed one by one with HIP events
+        kernel = "k_vbd_colour"
+        mean_ms, launches, kvu = g.time_colour_kernels(10)
+        bytes_per_launch = b_sweep / t.num_clusters
     achieved = bytes_per_launch / (mean_ms * 1e-3) / 1e9
     traffic = None
     if os.path.exists(TRAFFIC_FILE):
@@ -109,7 +129,7 @@ def main():
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 5),
         "traffic": traffic,
-        "kernel": "k_vbd_colour",
+        "kernel": kernel,
         "mean_launch_us": round(mean_ms * 1e3, 3),
         "algorithmic_bytes_per_launch": round(bytes_per_launch, 1),
         "bytes_per_vertex_update": round(b_sweep / t.num_verts, 2),
@@ -132,13 +152,48 @@ def main():
                         "top row pinned, seeded wind, 100 VBD iterations/frame, one frame per step",
             "colours": t.num_clusters,
             "iterations": iters,
-            "strategy": "graph",
+            "strategy": "persistent" if strategy == _lib.LATS_STRATEGY_PERSISTENT else "graph",
+            "persistent_blocks": blocks,
             "gpu_ms_per_step": gpu_ms / K,
         },
         "roofline": roofline,
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(cloth, params, wind)
+    print(json.dumps(out), flush=True)
+
+
+def bench_batch(args):
+    """512 uneven cloths (configs batch_c2), one persistent launch per frame."""
+    from libatssim_amd import configs
+    from libatssim_amd.vbd import GpuBatch
+    cloths, ks, kb, params = configs.batch_c2()
+    b = GpuBatch(cloths, params, ks, kb)
+    n, nv = b.counts()
+    threads, blocks = b.autotune(frames=1)
+    K, W = args.steps, args.warmup
+    if W:
+        b.run_frames(W)
+    t0 = time.perf_counter()
+    b.run_frames(K)
+    wall = time.perf_counter() - t0
+    gpu_ms = b.last_elapsed_ms()
+    iters = params.constraint_iter_count
+    vu = nv * iters * K
+    nb_sweep = sum(algorithmic_bytes_per_sweep(c.topology.num_verts, c.topology.num_edges, c.topology.num_bending)
+                   for c in cloths)
+    achieved = nb_sweep * iters * K / (gpu_ms * 1e-3) / 1e9
+    out = {
+        "metric": "batched VBD vertex-updates/s (512 cloths)",
+        "value": vu / wall,
+        "unit": "vertex-updates/s",
+        "n_gpus": 1, "steps": K, "warmup": W, "ms_per_step": wall * 1e3 / K,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"batch_c2: {n} cloths, {nv} verts, 100 iterations/frame", "block_threads": threads,
+                   "blocks": blocks, "gpu_ms_per_step": gpu_ms / K},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": "k_vbd_batch"},
+    }
     print(json.dumps(out), flush=True)
 
 

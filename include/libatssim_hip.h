@@ -69,6 +69,10 @@ int lats_topology_copy(const lats_topology* t, int which, void* dst);
 /* x: float[3*nx*ny]; faces: int32[6*(nx-1)*(ny-1)] */
 int lats_scene_grid(uint32_t nx, uint32_t ny, float spacing, float jitter, uint64_t seed, float* x,
                     int32_t* faces);
+/* Icosphere (icosahedron subdivided `level` times): call with x == NULL to
+ * get the sizes, then with arrays of 3*num_verts floats and 3*num_faces ints. */
+int lats_scene_icosphere(uint32_t level, float radius, const float centre[3], float* x, int32_t* faces,
+                         uint32_t* num_verts, uint32_t* num_faces);
 /* out: float[3*num_frames], force per vertex for each frame */
 int lats_scene_wind(uint32_t num_frames, uint32_t num_verts, float max_total_force, uint64_t seed, float* out);
 
@@ -117,12 +121,18 @@ typedef struct lats_vbd lats_vbd;
 int lats_vbd_create(const lats_mesh_desc* mesh, const lats_scene_params* params, lats_vbd** out);
 int lats_vbd_destroy(lats_vbd* ctx);
 int lats_vbd_set_strategy(lats_vbd* ctx, int strategy);
+/* Lanes per vertex of the colour passes (1, 4, 8, 16): with G > 1 the
+ * incident springs and bending pairs of a vertex are gathered by G lanes in
+ * parallel and summed in the reference's order (results are identical). */
+int lats_vbd_set_lanes_per_vertex(lats_vbd* ctx, uint32_t lanes);
+int lats_vbd_get_lanes_per_vertex(lats_vbd* ctx, uint32_t* lanes);
 /* Grid size of the persistent strategy (0: one thread per vertex of the
  * largest colour, capped at the co-resident limit). */
 int lats_vbd_set_persistent_blocks(lats_vbd* ctx, uint32_t blocks);
 int lats_vbd_get_strategy(lats_vbd* ctx, int* strategy, uint32_t* persistent_blocks);
 /* Autotuner: times `frames` frames with the graph strategy and with the
- * persistent strategy at several grid sizes, keeps the fastest for later
+ * persistent strategy at several grid sizes, for 1/4/8/16 lanes per vertex,
+ * keeps the fastest for later
  * lats_vbd_physics_step / lats_vbd_run_frames calls, and restores the state. */
 int lats_vbd_autotune(lats_vbd* ctx, uint32_t frames, int* chosen_strategy, uint32_t* chosen_blocks);
 /* Include the bending term in the recorded energy (the reference's GPU
@@ -151,6 +161,22 @@ int lats_vbd_run_frames(lats_vbd* ctx, uint32_t num_frames, const float* forces)
  * lats_vbd_physics_step / lats_vbd_run_frames call, in ms. */
 int lats_vbd_last_elapsed_ms(lats_vbd* ctx, float* ms);
 int lats_vbd_synchronize(lats_vbd* ctx);
+
+/* ---- obstacle contact (VBD::compute_obstacle_collision,
+ *      xpbd_constraints.h:4238-4284; narrow phase NarrowPhase::template_vf
+ *      :2742-2820; hash SpatialHashing :2333-2393) ------------------------
+ * A static triangle obstacle (face normals = normalize(cross(b-a, c-a)),
+ * i.e. counter-clockwise faces point outwards).  After predict, every
+ * vertex takes the (up to 4) nearest faces within `radius` from the hash
+ * cell that contains it (cells of size `cell` >= 2*radius hold every face
+ * whose box grown by `radius` overlaps them); each contact adds the
+ * penalty k*(thickness - (x - xs).n)^2/2 while (x - xs).n < thickness.
+ * nf == 0 removes the obstacle. */
+int lats_vbd_set_obstacle(lats_vbd* ctx, const float* x, uint32_t num_verts, const int32_t* faces, uint32_t num_faces,
+                          float thickness, float stiffness, float radius, float cell);
+/* Contacts of the last detection, vertex-id order: counts[V], face ids
+ * fids[4*V] (ascending, -1 padded). */
+int lats_vbd_get_contacts(lats_vbd* ctx, uint32_t* counts, int32_t* fids);
 
 /* ---- scheduler slot: GpuSolver::fn_dispatch ------------------------- */
 /* Function ids of the tasks the VBD DAG registers

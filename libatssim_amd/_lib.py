@@ -111,6 +111,12 @@ SIGNATURES = {
     "lats_vbd_set_strategy": (_I, [_VP, _I]),
     "lats_vbd_set_energy_bending": (_I, [_VP, _I]),
     "lats_vbd_set_persistent_blocks": (_I, [_VP, _U32]),
+    "lats_vbd_set_lanes_per_vertex": (_I, [_VP, _U32]),
+    "lats_vbd_set_obstacle": (_I, [_VP, _VP, _U32, _VP, _U32, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                   ctypes.c_float]),
+    "lats_vbd_get_contacts": (_I, [_VP, _VP, _VP]),
+    "lats_scene_icosphere": (_I, [_U32, ctypes.c_float, _VP, _VP, _VP, ctypes.POINTER(_U32), ctypes.POINTER(_U32)]),
+    "lats_vbd_get_lanes_per_vertex": (_I, [_VP, ctypes.POINTER(_U32)]),
     "lats_vbd_get_strategy": (_I, [_VP, ctypes.POINTER(_I), ctypes.POINTER(_U32)]),
     "lats_vbd_autotune": (_I, [_VP, _U32, ctypes.POINTER(_I), ctypes.POINTER(_U32)]),
     "lats_vbd_set_state": (_I, [_VP, _VP, _VP]),

@@ -11,6 +11,11 @@ batch_c2     512 independent grid cloths, side n ~ U{16..128} (~3.2 M vertices),
              k_stretch log-U[1e3, 1e5], k_bend log-U[1e-3, 1e-1] per cloth,
              1-4 random boundary vertices pinned, m = 1e-3 kg, 100 iterations;
              all draws from splitmix64(seed).
+sphere_c4    1024x1024 horizontal grid (2 mm spacing, 2.05 m wide, no pins)
+             10 cm above a fixed sphere of radius 0.3 m (icosphere, 20,480
+             faces); contact thickness 2 mm, k_contact 1e5, detection radius
+             2.5 cm (one frame of fall at impact speed), hash cell 5 cm;
+             50 iterations/frame, 120 frames.
 square8k     the reference's example-2 run (examples/example2_async_iter.cpp:
              133-392, 715-749, 1713-1810): square8K.obj, pins where the
              AABB-normalised z < 0.01 and x < 0.01 or x > 0.99, m = 0.01 kg,
@@ -18,7 +23,7 @@ square8k     the reference's example-2 run (examples/example2_async_iter.cpp:
 """
 import numpy as np
 
-from .vbd import Cloth, Params, make_grid, make_wind
+from .vbd import Cloth, Params, make_grid, make_icosphere, make_wind
 
 SEED_C0 = 20250101
 SEED_C1 = 20250102
@@ -97,6 +102,25 @@ def grid_c1(frames=60):
     cloth = grid_cloth(256, "top", SEED_C1)
     wind = make_wind(frames, cloth.num_verts, 2.0, SEED_WIND)
     return cloth, Params(constraint_iter_count=100), frames, wind
+
+
+def horizontal_grid(n, spacing, height, seed, jitter=1e-4):
+    """n x n grid in the xz plane at y = height (grid rows along -z)."""
+    x, f = make_grid(n, n, spacing, jitter, seed)
+    xh = np.empty_like(x)
+    xh[:, 0] = x[:, 0]
+    xh[:, 1] = np.float32(height) + x[:, 2]
+    xh[:, 2] = -x[:, 1] - np.float32(0.5 * (n - 1) * spacing)
+    return xh.astype(np.float32), f
+
+
+def sphere_scene(n=1024, spacing=0.002, radius=0.3, drop=0.1, level=5, seed=20250105, iters=50):
+    """(cloth, params, obstacle (x, faces), contact parameters dict)."""
+    x, f = horizontal_grid(n, spacing, radius + drop, seed)
+    cloth = Cloth(x, f, np.float32(1e-3), np.zeros(n * n, np.uint8))
+    ox, of = make_icosphere(level, radius)
+    contact = dict(thickness=0.002, stiffness=1e5, radius=0.025, cell=0.05)
+    return cloth, Params(constraint_iter_count=iters), (ox, of), contact
 
 
 def square8k_pins(x):

@@ -4,6 +4,7 @@
 
 #include "../../include/libatssim_hip.h"
 #include "host/errors.h"
+#include "host/obstacle.h"
 #include "host/scene.h"
 #include "host/topology.h"
 
@@ -86,6 +87,19 @@ int lats_scene_grid(uint32_t nx, uint32_t ny, float spacing, float jitter, uint6
     lats::make_grid(nx, ny, spacing, jitter, seed, hx, hf);
     std::memcpy(x, hx.data(), hx.size() * sizeof(float));
     std::memcpy(faces, hf.data(), hf.size() * sizeof(int32_t));
+    return LATS_OK;
+}
+
+int lats_scene_icosphere(uint32_t level, float radius, const float centre[3], float* x, int32_t* faces,
+                         uint32_t* num_verts, uint32_t* num_faces) {
+    if (!centre || level > 8) return set_err(LATS_ERR_INVALID_ARGUMENT, "bad icosphere arguments");
+    std::vector<float> hx;
+    std::vector<int32_t> hf;
+    lats::make_icosphere(level, radius, centre, hx, hf);
+    if (num_verts) *num_verts = (uint32_t)(hx.size() / 3);
+    if (num_faces) *num_faces = (uint32_t)(hf.size() / 3);
+    if (x) std::memcpy(x, hx.data(), hx.size() * sizeof(float));
+    if (faces) std::memcpy(faces, hf.data(), hf.size() * sizeof(int32_t));
     return LATS_OK;
 }
 

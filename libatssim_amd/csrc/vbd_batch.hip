@@ -82,6 +82,11 @@ __global__ __launch_bounds__(1024) void k_vbd_batch(BatchArgs p) {
         a.g[0] = p.g[0];
         a.g[1] = p.g[1];
         a.g[2] = p.g[2];
+        a.ccount = nullptr;
+        a.ct_xs = nullptr;
+        a.ct_n = nullptr;
+        a.nslots = 0;
+        a.thickness = 0.f;
         const uint32_t s0 = ci.vbase, s1 = ci.vbase + ci.nv;
         // Core::predict_position
         for (uint32_t s = s0 + tid; s < s1; s += nt) {

@@ -86,6 +86,13 @@ LATS_HD void stretch_term(F3 xk, F3 xo, float L, float k, Sym3& S, F3& fs) {
     fs = add3(fs, f);
 }
 
+// The same spring term on its own (equal to stretch_term from a zero sum).
+LATS_HD void stretch_values(F3 xk, F3 xo, float L, float k, Sym3& he, F3& f) {
+    he = sym_zero();
+    f = F3{0.f, 0.f, 0.f};
+    stretch_term(xk, xo, L, k, he, f);
+}
+
 // One incident bending pair: f = k * (-(sum_j Q[o][j] x_j)), H += k Q[o][o] I.
 // q = row o of Q in pair-local order (e0, e1, opp0, opp1).
 LATS_HD void bending_term(float4 q, uint32_t o, F3 p0, F3 p1, F3 p2, F3 p3, float k, float& hb, F3& fb) {

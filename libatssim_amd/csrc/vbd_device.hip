@@ -27,6 +27,7 @@
 
 #include "../../include/libatssim_hip.h"
 #include "host/errors.h"
+#include "host/obstacle.h"
 #include "host/topology.h"
 #include "vbd_common.h"
 #include "vbd_kernels.h"
@@ -41,11 +42,128 @@ namespace {
 
 constexpr int kBlock = 256;
 
+// One colour pass, G lanes per vertex (G = 1: one thread per vertex).
+template <int G>
 __global__ __launch_bounds__(kBlock) void k_vbd_colour(VbdArgs a, uint32_t cs, uint32_t n, uint32_t eoff,
                                                        uint32_t boff) {
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    vbd_vertex_ext(a, cs + i, i, n, eoff, boff, F3{a.fp->ext_cur[0], a.fp->ext_cur[1], a.fp->ext_cur[2]});
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    const F3 ext = F3{a.fp->ext_cur[0], a.fp->ext_cur[1], a.fp->ext_cur[2]};
+    if (G == 1) {
+        if (t >= n) return;
+        vbd_vertex_ext(a, cs + t, t, n, eoff, boff, ext);
+    } else {
+        const uint32_t i = t / G, lane = t % G;
+        if (i - (i % (64 / G)) >= n) return;      // the whole wave is past the colour
+        const bool valid = i < n;
+        vbd_vertex_group<G>(a, cs + (valid ? i : 0u), valid ? i : 0u, n, eoff, boff, ext, lane, valid);
+    }
+}
+
+// --- obstacle contact detection -------------------------------------------
+struct DetectArgs {
+    const float* x;
+    uint32_t nv;
+    const float* ob_x;
+    const int32_t* ob_f;
+    const float* ob_n;
+    const uint32_t* belongs;
+    const uint32_t* start;
+    const uint32_t* count;
+    const uint32_t* faces;
+    float ox, oy, oz, inv_cell;
+    int dimx, dimy, dimz;
+    uint32_t table_size;
+    float radius, stiffness;
+    uint32_t* ccount;
+    float4* ct_xs;
+    float4* ct_n;
+    uint32_t* ct_fid;
+};
+
+__device__ __forceinline__ uint32_t hash_coord_d(int x, int y, int z, uint32_t T) {
+    const int32_t hx = (int32_t)((uint32_t)x * 92837111u);
+    const int32_t hy = (int32_t)((uint32_t)y * 689287499u);
+    const int32_t hz = (int32_t)((uint32_t)z * 283923481u);
+    long long v = (long long)hx ^ (long long)hy ^ (long long)hz;
+    if (v < 0) v = -v;
+    return (uint32_t)(v % (long long)T);
+}
+
+// Contacts of slot s at its current (predicted) position: faces of the hash
+// cell containing it that pass the narrow phase within `radius`; the
+// kMaxContacts nearest by (distance, face id), stored in ascending face id.
+__device__ __forceinline__ void detect_vertex(const DetectArgs& d, uint32_t s) {
+    const F3 p = ld3(d.x, s);
+    const int ix = (int)floorf((p.x - d.ox) * d.inv_cell);
+    const int iy = (int)floorf((p.y - d.oy) * d.inv_cell);
+    const int iz = (int)floorf((p.z - d.oz) * d.inv_cell);
+    uint32_t n = 0;
+    float kd[kMaxContacts];
+    uint32_t kf[kMaxContacts];
+    float kb[kMaxContacts][3];
+    if (ix >= 0 && iy >= 0 && iz >= 0 && ix < d.dimx && iy < d.dimy && iz < d.dimz) {
+        const uint32_t key = ((uint32_t)ix << 20) | ((uint32_t)iy << 10) | (uint32_t)iz;
+        uint32_t h = hash_coord_d(ix, iy, iz, d.table_size);
+        bool found = false;
+        for (int j = 0; j < 32; j++) {
+            const uint32_t b = d.belongs[h];
+            if (b == key) { found = true; break; }
+            if (b == 0xffffffffu) break;
+            h = (h + 1) % d.table_size;
+        }
+        if (found) {
+            const uint32_t f0 = d.start[h], f1 = f0 + d.count[h];
+            for (uint32_t j = f0; j < f1; j++) {
+                const uint32_t fid = d.faces[j];
+                const F3 A = ld3(d.ob_x, (uint32_t)d.ob_f[3 * fid]);
+                const F3 B = ld3(d.ob_x, (uint32_t)d.ob_f[3 * fid + 1]);
+                const F3 C = ld3(d.ob_x, (uint32_t)d.ob_f[3 * fid + 2]);
+                float dist, bary[3];
+                if (!template_vf(p, A, B, C, d.radius, dist, bary)) continue;
+                // insert into the nearest-k list ordered by (distance, face id)
+                uint32_t pos = n;
+                while (pos > 0 && (kd[pos - 1] > dist || (kd[pos - 1] == dist && kf[pos - 1] > fid))) pos--;
+                if (pos >= (uint32_t)kMaxContacts) continue;
+                const uint32_t last = (n < (uint32_t)kMaxContacts) ? n : (uint32_t)kMaxContacts - 1;
+                for (uint32_t q = last; q > pos; q--) {
+                    kd[q] = kd[q - 1];
+                    kf[q] = kf[q - 1];
+                    kb[q][0] = kb[q - 1][0];
+                    kb[q][1] = kb[q - 1][1];
+                    kb[q][2] = kb[q - 1][2];
+                }
+                kd[pos] = dist;
+                kf[pos] = fid;
+                kb[pos][0] = bary[0];
+                kb[pos][1] = bary[1];
+                kb[pos][2] = bary[2];
+                if (n < (uint32_t)kMaxContacts) n++;
+            }
+        }
+    }
+    // ascending face id
+    for (uint32_t i = 1; i < n; i++)
+        for (uint32_t q = i; q > 0 && kf[q - 1] > kf[q]; q--) {
+            const float td = kd[q]; kd[q] = kd[q - 1]; kd[q - 1] = td;
+            const uint32_t tf = kf[q]; kf[q] = kf[q - 1]; kf[q - 1] = tf;
+            for (int c = 0; c < 3; c++) { const float tb = kb[q][c]; kb[q][c] = kb[q - 1][c]; kb[q - 1][c] = tb; }
+        }
+    for (uint32_t k = 0; k < n; k++) {
+        const uint32_t fid = kf[k];
+        const F3 A = ld3(d.ob_x, (uint32_t)d.ob_f[3 * fid]);
+        const F3 B = ld3(d.ob_x, (uint32_t)d.ob_f[3 * fid + 1]);
+        const F3 C = ld3(d.ob_x, (uint32_t)d.ob_f[3 * fid + 2]);
+        const F3 xs = add3(add3(scale3(kb[k][0], A), scale3(kb[k][1], B)), scale3(kb[k][2], C));
+        d.ct_xs[k * d.nv + s] = float4{xs.x, xs.y, xs.z, d.stiffness * 1.0f};
+        d.ct_n[k * d.nv + s] = float4{d.ob_n[3 * fid], d.ob_n[3 * fid + 1], d.ob_n[3 * fid + 2], 0.f};
+        d.ct_fid[k * d.nv + s] = fid;
+    }
+    d.ccount[s] = n;
+}
+
+__global__ __launch_bounds__(kBlock) void k_detect(DetectArgs d) {
+    const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+    if (s < d.nv) detect_vertex(d, s);
 }
 
 // --- persistent frame kernel ---------------------------------------------
@@ -62,6 +180,8 @@ struct PersistArgs {
     uint32_t* barrier;         // monotonically increasing arrival counter
     uint32_t nv, ncolours, iters, nframes;
     float damp;
+    int detect;                // run contact detection after predict
+    DetectArgs det;
 };
 
 __device__ __forceinline__ void grid_barrier(uint32_t* bar, uint32_t& target) {
@@ -77,6 +197,7 @@ __device__ __forceinline__ void grid_barrier(uint32_t* bar, uint32_t& target) {
     __syncthreads();
 }
 
+template <int G>
 __global__ __launch_bounds__(kBlock) void k_vbd_persistent(PersistArgs p) {
     uint32_t target = 0;
     const uint32_t stride = gridDim.x * kBlock;
@@ -93,11 +214,24 @@ __global__ __launch_bounds__(kBlock) void k_vbd_persistent(PersistArgs p) {
             st3(p.a.x, s, add3(ld3(p.x0, s), scale3(p.a.h, vpred)));
         }
         grid_barrier(p.barrier, target);
+        if (p.detect) {
+            for (uint32_t s = t0; s < p.nv; s += stride) detect_vertex(p.det, s);
+            grid_barrier(p.barrier, target);
+        }
         for (uint32_t it = 0; it < p.iters; it++) {
             for (uint32_t c = 0; c < p.ncolours; c++) {
                 const ColourInfo ci = p.colours[c];
-                for (uint32_t i = t0; i < ci.count; i += stride)
-                    vbd_vertex_ext(p.a, ci.start + i, i, ci.count, ci.eoff, ci.boff, ext);
+                if (G == 1) {
+                    for (uint32_t i = t0; i < ci.count; i += stride)
+                        vbd_vertex_ext(p.a, ci.start + i, i, ci.count, ci.eoff, ci.boff, ext);
+                } else {
+                    // a whole wave moves through the colour together
+                    for (uint32_t i = t0 / G; i - (i % (64 / G)) < ci.count; i += stride / G) {
+                        const bool valid = i < ci.count;
+                        vbd_vertex_group<G>(p.a, ci.start + (valid ? i : 0u), valid ? i : 0u, ci.count, ci.eoff,
+                                            ci.boff, ext, t0 % G, valid);
+                    }
+                }
                 grid_barrier(p.barrier, target);
             }
         }
@@ -175,6 +309,10 @@ struct EnergyArgs {
     uint32_t nv, ne, nb;
     float h, ks, kb;
     float g[3];
+    const uint32_t* ccount;    // obstacle contacts (null: none)
+    const float4* ct_xs;
+    const float4* ct_n;
+    float thickness;
 };
 
 __device__ __forceinline__ double block_sum(double val) {
@@ -193,7 +331,7 @@ __device__ __forceinline__ double block_sum(double val) {
 // _bending, xpbd_constraints.h:2079-2214), FP64 block partial sums.
 __global__ __launch_bounds__(kBlock) void k_energy_partial(EnergyArgs a, double* partial) {
     const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
-    float e = 0.f;
+    double val = 0.0;
     if (t < a.nv) {
         const uint32_t s = t;
         const float m = a.mass[s];
@@ -202,8 +340,17 @@ __global__ __launch_bounds__(kBlock) void k_energy_partial(EnergyArgs a, double*
         aext = add3(aext, F3{w.x / m, w.y / m, w.z / m});
         if ((a.meta[s] >> 16) & 1u) aext = F3{0.f, 0.f, 0.f};
         const F3 y = add3(add3(ld3(a.x0, s), scale3(a.h, ld3(a.v, s))), scale3(a.h * a.h, aext));
-        const F3 d = sub3(ld3(a.x, s), y);
-        e = (dot3(d, d) * m) / ((2.f * a.h) * a.h);
+        const F3 xv = ld3(a.x, s);
+        const F3 d = sub3(xv, y);
+        val = (double)((dot3(d, d) * m) / ((2.f * a.h) * a.h));
+        if (a.ccount) {
+            // Energy::compute_energy_collision_vf per contact
+            for (uint32_t k = 0; k < a.ccount[s]; k++) {
+                const float4 q = a.ct_xs[k * a.nv + s];
+                const float4 nn = a.ct_n[k * a.nv + s];
+                val += (double)contact_energy(xv, F3{q.x, q.y, q.z}, F3{nn.x, nn.y, nn.z}, q.w, a.thickness);
+            }
+        }
     } else if (t < a.nv + a.ne) {
         const uint32_t k = t - a.nv;
         const uint2 ed = a.edges[k];
@@ -211,7 +358,7 @@ __global__ __launch_bounds__(kBlock) void k_energy_partial(EnergyArgs a, double*
         const float l2 = dot3(diff, diff);
         if (!((double)l2 < 1e-8)) {
             const float C = sqrtf(l2) - a.rest[k];
-            e = (0.5f * a.ks) * (C * C);
+            val = (double)((0.5f * a.ks) * (C * C));
         }
     } else if (t < a.nv + a.ne + a.nb) {
         const uint32_t k = t - a.nv - a.ne;
@@ -223,9 +370,9 @@ __global__ __launch_bounds__(kBlock) void k_energy_partial(EnergyArgs a, double*
             const float qq[4] = {qc.x, qc.y, qc.z, qc.w};
             for (int jj = 0; jj < 4; jj++) sum = sum + qq[jj] * dot3(p[ii], p[jj]);
         }
-        e = (0.5f * a.kb) * sum;
+        val = (double)((0.5f * a.kb) * sum);
     }
-    const double r = block_sum((double)e);
+    const double r = block_sum(val);
     if (threadIdx.x == 0) partial[blockIdx.x] = r;
 }
 
@@ -354,10 +501,22 @@ struct lats_vbd {
     // multi-buffer positions of the async schedule
     DBuf<float> iterbuf[LATS_NUM_ASYNC_BUFFERS], beginbuf[LATS_NUM_ASYNC_BUFFERS];
 
+    // obstacle contact (row a10)
+    bool has_obstacle = false;
+    float thickness = 0.f, ob_stiffness = 0.f, ob_radius = 0.f;
+    ObstacleHash ob_hash;
+    DBuf<float> ob_x, ob_n;
+    DBuf<int32_t> ob_f;
+    DBuf<uint32_t> hb_belongs, hb_start, hb_count, hb_faces;
+    DBuf<uint32_t> ct_count, ct_fid;
+    DBuf<float4> ct_xs, ct_n;
+
     // persistent strategy
     DBuf<ColourInfo> colours_d;
     DBuf<uint32_t> barrier;
     uint32_t persist_blocks = 0;      // 0 = derive from the largest colour
+    uint32_t lanes = 1;               // lanes per vertex (1, 4, 8, 16)
+    uint32_t persist_max_lanes = 0;
     uint32_t persist_max_blocks = 0;  // co-residency limit (queried on first use)
 
     // graph per (record) flag
@@ -395,6 +554,11 @@ struct lats_vbd {
         a.ks = prm.stiffness_stretch_spring;
         a.kb = prm.stiffness_quadratic_bending;
         for (int c = 0; c < 3; c++) a.g[c] = prm.gravity[c];
+        a.ccount = has_obstacle ? ct_count.p : nullptr;
+        a.ct_xs = ct_xs.p;
+        a.ct_n = ct_n.p;
+        a.nslots = V;
+        a.thickness = thickness;
         return a;
     }
 };
@@ -444,19 +608,63 @@ void enqueue_energy(lats_vbd* c, const float* xb, uint32_t row) {
     a.ks = c->prm.stiffness_stretch_spring;
     a.kb = c->prm.stiffness_quadratic_bending;
     for (int k = 0; k < 3; k++) a.g[k] = c->prm.gravity[k];
+    a.ccount = c->has_obstacle ? c->ct_count.p : nullptr;
+    a.ct_xs = c->ct_xs.p;
+    a.ct_n = c->ct_n.p;
+    a.thickness = c->thickness;
     hipLaunchKernelGGL(k_energy_partial, dim3(c->energy_nblk), dim3(kBlock), 0, c->stream, a,
                        c->partial.p + (size_t)row * c->energy_nblk);
 }
 
 void enqueue_colour(lats_vbd* c, float* xb, uint32_t col) {
     const uint32_t n = c->ccount[col];
-    hipLaunchKernelGGL(k_vbd_colour, dim3(nblocks(n)), dim3(kBlock), 0, c->stream, c->args(xb), c->cstart[col], n,
-                       c->eoff[col], c->boff[col]);
+    const uint64_t threads = (uint64_t)n * c->lanes;
+    const dim3 grid(nblocks(threads)), block(kBlock);
+    const VbdArgs a = c->args(xb);
+    switch (c->lanes) {
+        case 4: hipLaunchKernelGGL(k_vbd_colour<4>, grid, block, 0, c->stream, a, c->cstart[col], n, c->eoff[col], c->boff[col]); break;
+        case 8: hipLaunchKernelGGL(k_vbd_colour<8>, grid, block, 0, c->stream, a, c->cstart[col], n, c->eoff[col], c->boff[col]); break;
+        case 16: hipLaunchKernelGGL(k_vbd_colour<16>, grid, block, 0, c->stream, a, c->cstart[col], n, c->eoff[col], c->boff[col]); break;
+        default: hipLaunchKernelGGL(k_vbd_colour<1>, grid, block, 0, c->stream, a, c->cstart[col], n, c->eoff[col], c->boff[col]); break;
+    }
 }
 
 void enqueue_predict(lats_vbd* c) {
     hipLaunchKernelGGL(k_predict, dim3(std::max(1u, nblocks(c->V))), dim3(kBlock), 0, c->stream, c->x.p, c->x0.p,
                        c->v.p, c->meta.p, c->fp.p, c->h, c->V);
+}
+
+DetectArgs detect_args(lats_vbd* c, const float* xb) {
+    DetectArgs d;
+    d.x = xb;
+    d.nv = c->V;
+    d.ob_x = c->ob_x.p;
+    d.ob_f = c->ob_f.p;
+    d.ob_n = c->ob_n.p;
+    d.belongs = c->hb_belongs.p;
+    d.start = c->hb_start.p;
+    d.count = c->hb_count.p;
+    d.faces = c->hb_faces.p;
+    d.ox = c->ob_hash.origin[0];
+    d.oy = c->ob_hash.origin[1];
+    d.oz = c->ob_hash.origin[2];
+    d.inv_cell = c->ob_hash.inv_cell;
+    d.dimx = c->ob_hash.dims[0];
+    d.dimy = c->ob_hash.dims[1];
+    d.dimz = c->ob_hash.dims[2];
+    d.table_size = c->ob_hash.table_size;
+    d.radius = c->ob_radius;
+    d.stiffness = c->ob_stiffness;
+    d.ccount = c->ct_count.p;
+    d.ct_xs = c->ct_xs.p;
+    d.ct_n = c->ct_n.p;
+    d.ct_fid = c->ct_fid.p;
+    return d;
+}
+
+void enqueue_detect(lats_vbd* c) {
+    if (!c->has_obstacle) return;
+    hipLaunchKernelGGL(k_detect, dim3(nblocks(c->V)), dim3(kBlock), 0, c->stream, detect_args(c, c->x.p));
 }
 
 void enqueue_update(lats_vbd* c, int advance) {
@@ -469,6 +677,7 @@ void enqueue_frame(lats_vbd* c, bool record) {
     const uint32_t nsub = record ? 1u : c->prm.num_substep;
     for (uint32_t sub = 0; sub < nsub; sub++) {
         enqueue_predict(c);
+        enqueue_detect(c);
         for (uint32_t it = 0; it < c->prm.constraint_iter_count; it++) {
             if (record && it == 0) enqueue_energy(c, c->x.p, 0);
             for (uint32_t col = 0; col < c->C; col++) enqueue_colour(c, c->x.p, col);
@@ -481,18 +690,28 @@ void enqueue_frame(lats_vbd* c, bool record) {
                            c->partial.p, c->energy_nblk, 0u, c->energy.p);
 }
 
+const void* persistent_kernel(uint32_t lanes) {
+    switch (lanes) {
+        case 4: return (const void*)k_vbd_persistent<4>;
+        case 8: return (const void*)k_vbd_persistent<8>;
+        case 16: return (const void*)k_vbd_persistent<16>;
+        default: return (const void*)k_vbd_persistent<1>;
+    }
+}
+
 int persistent_grid(lats_vbd* c, uint32_t* blocks) {
-    if (!c->persist_max_blocks) {
+    if (c->persist_max_lanes != c->lanes || !c->persist_max_blocks) {
         int per_cu = 0, ncu = 0;
-        HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_vbd_persistent, kBlock, 0));
+        HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, persistent_kernel(c->lanes), kBlock, 0));
         HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device));
         c->persist_max_blocks = (uint32_t)std::max(1, per_cu * ncu);
+        c->persist_max_lanes = c->lanes;
     }
     uint32_t g = c->persist_blocks;
     if (!g) {
         uint32_t mx = 0;
         for (uint32_t k = 0; k < c->C; k++) mx = std::max(mx, c->ccount[k]);
-        g = nblocks(mx);
+        g = nblocks((uint64_t)mx * c->lanes);
     }
     *blocks = std::max(1u, std::min(g, c->persist_max_blocks));
     return LATS_OK;
@@ -516,6 +735,8 @@ int run_persistent(lats_vbd* c, uint32_t nframes) {
     pa.iters = c->prm.constraint_iter_count;
     pa.nframes = nframes * c->prm.num_substep;
     pa.damp = c->damp;
+    pa.detect = c->has_obstacle ? 1 : 0;
+    pa.det = detect_args(c, c->x.p);
     if (c->prm.num_substep != 1) return fail(LATS_ERR_UNSUPPORTED, "persistent strategy runs one substep per frame");
     // the barrier counter grows by g per phase: split long runs so it stays below 2^31
     const uint64_t per_frame = ((uint64_t)pa.iters * c->C + 2) * g;
@@ -524,7 +745,7 @@ int run_persistent(lats_vbd* c, uint32_t nframes) {
         pa.nframes = std::min(chunk, nframes - done);
         hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c->stream, c->barrier.p, 1u);
         void* kargs[] = {&pa};
-        HIPCHK(hipLaunchCooperativeKernel((const void*)k_vbd_persistent, dim3(g), dim3(kBlock), kargs, 0, c->stream));
+        HIPCHK(hipLaunchCooperativeKernel(persistent_kernel(c->lanes), dim3(g), dim3(kBlock), kargs, 0, c->stream));
         done += pa.nframes;
     }
     return LATS_OK;
@@ -683,6 +904,77 @@ int lats_vbd_set_strategy(lats_vbd* c, int strategy) {
     return LATS_OK;
 }
 
+int lats_vbd_set_obstacle(lats_vbd* c, const float* x, uint32_t nv, const int32_t* faces, uint32_t nf,
+                          float thickness, float stiffness, float radius, float cell) {
+    if (!c) return fail(LATS_ERR_INVALID_ARGUMENT, "null context");
+    if (nf == 0) {
+        c->has_obstacle = false;
+    } else {
+        if (!x || !faces) return fail(LATS_ERR_INVALID_ARGUMENT, "null obstacle arrays");
+        if (!(thickness > 0.f) || !(stiffness > 0.f) || !(radius > 0.f))
+            return fail(LATS_ERR_INVALID_ARGUMENT, "thickness, stiffness and radius must be > 0");
+        ObstacleHash h;
+        std::string err;
+        if (!build_obstacle_hash(x, nv, faces, nf, radius, cell, h, err)) return fail(LATS_ERR_INVALID_ARGUMENT, err);
+        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(c->ob_x.upload(std::vector<float>(x, x + 3 * (size_t)nv)));
+        HIPCHK(c->ob_f.upload(std::vector<int32_t>(faces, faces + 3 * (size_t)nf)));
+        HIPCHK(c->ob_n.upload(face_normals(x, faces, nf)));
+        HIPCHK(c->hb_belongs.upload(h.belongs));
+        HIPCHK(c->hb_start.upload(h.start));
+        HIPCHK(c->hb_count.upload(h.count));
+        HIPCHK(c->hb_faces.upload(h.faces.empty() ? std::vector<uint32_t>{0u} : h.faces));
+        HIPCHK(c->ct_count.upload(std::vector<uint32_t>(c->V, 0u)));
+        HIPCHK(c->ct_fid.alloc((size_t)kMaxContacts * c->V));
+        HIPCHK(c->ct_xs.alloc((size_t)kMaxContacts * c->V));
+        HIPCHK(c->ct_n.alloc((size_t)kMaxContacts * c->V));
+        c->ob_hash = std::move(h);
+        c->thickness = thickness;
+        c->ob_stiffness = stiffness;
+        c->ob_radius = radius;
+        c->has_obstacle = true;
+    }
+    for (auto& g : c->graph)
+        if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+    return LATS_OK;
+}
+
+int lats_vbd_get_contacts(lats_vbd* c, uint32_t* counts, int32_t* fids) {
+    if (!c) return fail(LATS_ERR_INVALID_ARGUMENT, "null context");
+    std::vector<uint32_t> hc(c->V, 0u), hf((size_t)kMaxContacts * c->V, 0u);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (c->has_obstacle) {
+        HIPCHK(hipMemcpy(hc.data(), c->ct_count.p, hc.size() * 4, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(hf.data(), c->ct_fid.p, hf.size() * 4, hipMemcpyDeviceToHost));
+    }
+    for (uint32_t s = 0; s < c->V; s++) {
+        const uint32_t vid = c->perm[s];
+        if (counts) counts[vid] = hc[s];
+        if (fids)
+            for (int k = 0; k < kMaxContacts; k++)
+                fids[(size_t)kMaxContacts * vid + k] = (uint32_t)k < hc[s] ? (int32_t)hf[(size_t)k * c->V + s] : -1;
+    }
+    return LATS_OK;
+}
+
+int lats_vbd_set_lanes_per_vertex(lats_vbd* c, uint32_t lanes) {
+    if (!c) return fail(LATS_ERR_INVALID_ARGUMENT, "null context");
+    if (lanes != 1 && lanes != 4 && lanes != 8 && lanes != 16)
+        return fail(LATS_ERR_INVALID_ARGUMENT, "lanes per vertex must be 1, 4, 8 or 16");
+    if (lanes != c->lanes) {
+        c->lanes = lanes;
+        for (auto& g : c->graph)
+            if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+    }
+    return LATS_OK;
+}
+
+int lats_vbd_get_lanes_per_vertex(lats_vbd* c, uint32_t* lanes) {
+    if (!c || !lanes) return fail(LATS_ERR_INVALID_ARGUMENT, "null argument");
+    *lanes = c->lanes;
+    return LATS_OK;
+}
+
 int lats_vbd_set_persistent_blocks(lats_vbd* c, uint32_t blocks) {
     if (!c) return fail(LATS_ERR_INVALID_ARGUMENT, "null context");
     c->persist_blocks = blocks;
@@ -724,18 +1016,23 @@ int lats_vbd_autotune(lats_vbd* c, uint32_t frames, int* chosen, uint32_t* chose
         HIPCHK(hipMemcpy(c->fp.p, &fps, sizeof(fps), hipMemcpyHostToDevice));
         return LATS_OK;
     };
-    struct Cand { int strategy; uint32_t blocks; };
-    std::vector<Cand> cands = {{LATS_STRATEGY_GRAPH, 0}};
-    if (c->prm.num_substep == 1) {
+    struct Cand { int strategy; uint32_t blocks; uint32_t lanes; };
+    std::vector<Cand> cands;
+    const uint32_t keep_lanes = c->lanes;
+    for (uint32_t lanes : {1u, 4u, 8u, 16u}) {
+        cands.push_back({LATS_STRATEGY_GRAPH, 0, lanes});
+        if (c->prm.num_substep != 1) continue;
         uint32_t g0 = 0;
         const uint32_t keep = c->persist_blocks;
         c->persist_blocks = 0;
+        c->lanes = lanes;
         int rc = persistent_grid(c, &g0);
-        if (rc) return rc;
-        for (uint32_t g : {g0 / 4, g0 / 2, g0, 2 * g0, 4 * g0, c->persist_max_blocks})
-            if (g >= 1 && g <= c->persist_max_blocks) cands.push_back({LATS_STRATEGY_PERSISTENT, g});
         c->persist_blocks = keep;
+        if (rc) { c->lanes = keep_lanes; return rc; }
+        for (uint32_t g : {g0 / 2, g0, 2 * g0, c->persist_max_blocks})
+            if (g >= 1 && g <= c->persist_max_blocks) cands.push_back({LATS_STRATEGY_PERSISTENT, g, lanes});
     }
+    c->lanes = keep_lanes;
     float best = 3.4e38f;
     Cand win = cands[0];
     const int keep_strategy = c->strategy;
@@ -743,15 +1040,33 @@ int lats_vbd_autotune(lats_vbd* c, uint32_t frames, int* chosen, uint32_t* chose
     for (const Cand& k : cands) {
         c->strategy = k.strategy;
         c->persist_blocks = k.blocks;
+        if (c->lanes != k.lanes) {
+            c->lanes = k.lanes;
+            for (auto& gr : c->graph)
+                if (gr) { (void)hipGraphExecDestroy(gr); gr = nullptr; }
+        }
         int rc = lats_vbd_run_frames(c, 1, nullptr);   // warm-up / capture
         if (!rc) rc = lats_vbd_run_frames(c, frames, nullptr);
-        if (rc) { c->strategy = keep_strategy; c->persist_blocks = keep_blocks; restore(); return rc; }
+        if (rc) {
+            c->strategy = keep_strategy;
+            c->persist_blocks = keep_blocks;
+            c->lanes = keep_lanes;
+            for (auto& gr : c->graph)
+                if (gr) { (void)hipGraphExecDestroy(gr); gr = nullptr; }
+            restore();
+            return rc;
+        }
         if (c->last_ms < best) { best = c->last_ms; win = k; }
         rc = restore();
         if (rc) return rc;
     }
     c->strategy = win.strategy;
     c->persist_blocks = win.blocks;
+    if (c->lanes != win.lanes) {
+        c->lanes = win.lanes;
+        for (auto& gr : c->graph)
+            if (gr) { (void)hipGraphExecDestroy(gr); gr = nullptr; }
+    }
     if (chosen) *chosen = win.strategy;
     if (chosen_blocks) *chosen_blocks = win.blocks;
     return LATS_OK;
@@ -961,6 +1276,7 @@ int lats_vbd_dispatch(lats_vbd* c, const lats_launch_param* p) {
             break;
         case LATS_FN_PREDICT_POSITION:
             enqueue_predict(c);
+            enqueue_detect(c);     // collision_detection() follows predict (example2_async_iter.cpp:1104-1106)
             break;
         case LATS_FN_UPDATE_VELOCITY:
             enqueue_update(c, 1);

@@ -6,6 +6,7 @@
 
 #include <cstdint>
 
+#include "vbd_collide.h"
 #include "vbd_common.h"
 
 namespace lats {
@@ -35,6 +36,12 @@ struct VbdArgs {
     const FrameParams* fp;
     float h, h2inv, ks, kb;
     float g[3];
+    // obstacle contacts of the running frame (ccount == null: none)
+    const uint32_t* ccount;    // contacts per slot
+    const float4* ct_xs;       // (closest point, stiffness), entry k of slot s at k * nslots + s
+    const float4* ct_n;        // face normal
+    uint32_t nslots;
+    float thickness;
 };
 
 __device__ __forceinline__ F3 ld3(const float* p, uint32_t s) { return F3{p[3 * s], p[3 * s + 1], p[3 * s + 2]}; }
@@ -93,9 +100,132 @@ __device__ __forceinline__ void vbd_vertex_ext(const VbdArgs& a, uint32_t s, uin
     H.zz = H.zz + hb;
     f = add3(f, fb);
 
+    // obstacle contacts: local sum from zero, then added (the contact
+    // Hessian is not exactly symmetric, so the full 3x3 is solved)
+    if (a.ccount) {
+        const uint32_t nc = a.ccount[s];
+        if (nc) {
+            float Hc[3][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
+            F3 fc = F3{0.f, 0.f, 0.f};
+            for (uint32_t k = 0; k < nc; k++) {
+                const float4 q = a.ct_xs[k * a.nslots + s];
+                const float4 nn = a.ct_n[k * a.nslots + s];
+                contact_term(xk, F3{q.x, q.y, q.z}, F3{nn.x, nn.y, nn.z}, q.w, a.thickness, Hc, fc);
+            }
+            const float M[3][3] = {{H.xx + Hc[0][0], H.xy + Hc[0][1], H.xz + Hc[0][2]},
+                                   {H.xy + Hc[1][0], H.yy + Hc[1][1], H.yz + Hc[1][2]},
+                                   {H.xz + Hc[2][0], H.yz + Hc[2][1], H.zz + Hc[2][2]}};
+            f = add3(f, fc);
+            F3 dx;
+            if (newton_step_full(M, f, dx)) st3(a.x, s, add3(xk, dx));
+            return;
+        }
+    }
+
     F3 dx;
     if (newton_step(H, f, dx)) st3(a.x, s, add3(xk, dx));
 }
 
+
+// The same update with G lanes per vertex (lane = 0..G-1 inside an aligned
+// group of G lanes).  Lane j computes the terms of incident springs / pairs
+// j, j+G, ...; the terms are then added in incidence order through in-group
+// shuffles, so every sum is formed in the reference's order and the result
+// is bit-identical to vbd_vertex_ext.  All G lanes of a group must call this
+// together (valid == false: the group has no vertex; it still takes part in
+// the shuffles and stores nothing).
+template <int G>
+__device__ __forceinline__ void vbd_vertex_group(const VbdArgs& a, uint32_t s, uint32_t i, uint32_t n, uint32_t eoff,
+                                                 uint32_t boff, F3 ext, uint32_t lane, bool valid) {
+    const F3 xk = ld3(a.x, s);
+    const F3 x0 = ld3(a.x0, s);
+    const F3 v0 = ld3(a.v, s);
+    const float m = a.mass[s];
+    const uint32_t meta = valid ? a.meta[s] : 0u;
+    const uint32_t edeg = meta & 0xffu, bdeg = (meta >> 8) & 0xffu;
+
+    Sym3 H;
+    F3 f;
+    inertia_term(xk, x0, v0, m, (meta >> 16) & 1u, a.g, ext, a.h, a.h2inv, H, f);
+
+    Sym3 S = sym_zero();
+    F3 fs = F3{0.f, 0.f, 0.f};
+    for (uint32_t r = 0; r < edeg; r += G) {
+        const uint32_t j = r + lane;
+        Sym3 t = sym_zero();
+        F3 tf = F3{0.f, 0.f, 0.f};
+        if (j < edeg) {
+            const uint2 e = a.ell_e[eoff + j * n + i];
+            stretch_values(xk, ld3(a.x, e.x), __uint_as_float(e.y), a.ks, t, tf);
+        }
+        const uint32_t cnt = (edeg - r < (uint32_t)G) ? edeg - r : (uint32_t)G;
+        for (uint32_t q = 0; q < cnt; q++) {
+            S.xx = S.xx + __shfl(t.xx, (int)q, G);
+            S.xy = S.xy + __shfl(t.xy, (int)q, G);
+            S.xz = S.xz + __shfl(t.xz, (int)q, G);
+            S.yy = S.yy + __shfl(t.yy, (int)q, G);
+            S.yz = S.yz + __shfl(t.yz, (int)q, G);
+            S.zz = S.zz + __shfl(t.zz, (int)q, G);
+            fs.x = fs.x + __shfl(tf.x, (int)q, G);
+            fs.y = fs.y + __shfl(tf.y, (int)q, G);
+            fs.z = fs.z + __shfl(tf.z, (int)q, G);
+        }
+    }
+    sym_add(H, S);
+    f = add3(f, fs);
+
+    float hb = 0.f;
+    F3 fb = F3{0.f, 0.f, 0.f};
+    for (uint32_t r = 0; r < bdeg; r += G) {
+        const uint32_t j = r + lane;
+        float th = 0.f;
+        F3 tf = F3{0.f, 0.f, 0.f};
+        if (j < bdeg) {
+            const uint32_t k = boff + j * n + i;
+            const float4 q = a.ell_bq[k];
+            const uint4 id = a.ell_bi[k];
+            const uint32_t o = id.x >> 30;
+            const F3 p0 = (o == 0) ? xk : ld3(a.x, id.x & 0x3fffffffu);
+            const F3 p1 = (o == 1) ? xk : ld3(a.x, id.y);
+            const F3 p2 = (o == 2) ? xk : ld3(a.x, id.z);
+            const F3 p3 = (o == 3) ? xk : ld3(a.x, id.w);
+            bending_term(q, o, p0, p1, p2, p3, a.kb, th, tf);
+        }
+        const uint32_t cnt = (bdeg - r < (uint32_t)G) ? bdeg - r : (uint32_t)G;
+        for (uint32_t q = 0; q < cnt; q++) {
+            hb = hb + __shfl(th, (int)q, G);
+            fb.x = fb.x + __shfl(tf.x, (int)q, G);
+            fb.y = fb.y + __shfl(tf.y, (int)q, G);
+            fb.z = fb.z + __shfl(tf.z, (int)q, G);
+        }
+    }
+    H.xx = H.xx + hb;
+    H.yy = H.yy + hb;
+    H.zz = H.zz + hb;
+    f = add3(f, fb);
+    if (!valid || lane != 0) return;
+
+    if (a.ccount) {
+        const uint32_t nc = a.ccount[s];
+        if (nc) {
+            float Hc[3][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
+            F3 fc = F3{0.f, 0.f, 0.f};
+            for (uint32_t k = 0; k < nc; k++) {
+                const float4 q = a.ct_xs[k * a.nslots + s];
+                const float4 nn = a.ct_n[k * a.nslots + s];
+                contact_term(xk, F3{q.x, q.y, q.z}, F3{nn.x, nn.y, nn.z}, q.w, a.thickness, Hc, fc);
+            }
+            const float M[3][3] = {{H.xx + Hc[0][0], H.xy + Hc[0][1], H.xz + Hc[0][2]},
+                                   {H.xy + Hc[1][0], H.yy + Hc[1][1], H.yz + Hc[1][2]},
+                                   {H.xz + Hc[2][0], H.yz + Hc[2][1], H.zz + Hc[2][2]}};
+            f = add3(f, fc);
+            F3 dx;
+            if (newton_step_full(M, f, dx)) st3(a.x, s, add3(xk, dx));
+            return;
+        }
+    }
+    F3 dx;
+    if (newton_step(H, f, dx)) st3(a.x, s, add3(xk, dx));
+}
 
 }  // namespace lats

@@ -31,6 +31,18 @@ def make_grid(nx, ny, spacing=0.01, jitter=1e-4, seed=0):
     return x, f
 
 
+def make_icosphere(level, radius, centre=(0.0, 0.0, 0.0)):
+    """Icosahedron subdivided `level` times on a sphere; faces point outwards."""
+    c = np.ascontiguousarray(centre, np.float32)
+    nv = ctypes.c_uint32()
+    nf = ctypes.c_uint32()
+    check(lib().lats_scene_icosphere(level, radius, _p(c), None, None, ctypes.byref(nv), ctypes.byref(nf)))
+    x = np.zeros((nv.value, 3), np.float32)
+    f = np.zeros((nf.value, 3), np.int32)
+    check(lib().lats_scene_icosphere(level, radius, _p(c), _p(x), _p(f), ctypes.byref(nv), ctypes.byref(nf)))
+    return x, f
+
+
 def make_wind(num_frames, num_verts, max_total=2.0, seed=0):
     w = np.zeros((num_frames, 3), np.float32)
     check(lib().lats_scene_wind(num_frames, num_verts, max_total, seed, _p(w)))
@@ -175,6 +187,14 @@ class GpuVBD:
         if persistent_blocks is not None:
             check(lib().lats_vbd_set_persistent_blocks(self._h, int(persistent_blocks)))
 
+    def set_lanes_per_vertex(self, lanes):
+        check(lib().lats_vbd_set_lanes_per_vertex(self._h, int(lanes)))
+
+    def lanes_per_vertex(self):
+        n = ctypes.c_uint32()
+        check(lib().lats_vbd_get_lanes_per_vertex(self._h, ctypes.byref(n)))
+        return n.value
+
     def strategy(self):
         st = ctypes.c_int()
         nb = ctypes.c_uint32()
@@ -186,6 +206,21 @@ class GpuVBD:
         nb = ctypes.c_uint32()
         check(lib().lats_vbd_autotune(self._h, int(frames), ctypes.byref(st), ctypes.byref(nb)))
         return st.value, nb.value
+
+    def set_obstacle(self, x, faces, thickness, stiffness, radius, cell):
+        """Static triangle obstacle with penalty contact (see lats_vbd_set_obstacle)."""
+        self._obs = (np.ascontiguousarray(x, np.float32), np.ascontiguousarray(faces, np.int32))
+        x, f = self._obs
+        check(lib().lats_vbd_set_obstacle(self._h, _p(x), len(x), _p(f), len(f), float(thickness),
+                                          float(stiffness), float(radius), float(cell)))
+
+    def contacts(self):
+        """(counts[V], face ids[V, 4] ascending, -1 padded) of the last detection."""
+        n = self.cloth.num_verts
+        c = np.zeros(n, np.uint32)
+        f = np.zeros((n, 4), np.int32)
+        check(lib().lats_vbd_get_contacts(self._h, _p(c), _p(f)))
+        return c, f
 
     def set_energy_bending(self, on):
         check(lib().lats_vbd_set_energy_bending(self._h, int(bool(on))))

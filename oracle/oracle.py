@@ -56,6 +56,12 @@ def lib():
         L.orc_sim_energy.restype = ctypes.c_double
         L.orc_sim_energy.argtypes = [ctypes.c_void_p]
         L.orc_max_threads.restype = ctypes.c_int
+        L.orc_sim_set_obstacle.argtypes = [ctypes.c_void_p, ctypes.c_uint32, _f32p, ctypes.c_uint32, _i32p,
+                                           ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_int]
+        L.orc_sim_contacts.restype = ctypes.c_uint32
+        L.orc_sim_contacts.argtypes = [ctypes.c_void_p, _u32p, _i32p, ctypes.c_uint32]
+        L.orc_template_vf.restype = ctypes.c_int
+        L.orc_template_vf.argtypes = [_f32p, ctypes.c_float, _f32p, _f32p]
         _LIB = L
     return _LIB
 
@@ -156,6 +162,36 @@ class Sim:
 
     def energy(self):
         return lib().orc_sim_energy(self._h)
+
+    def set_obstacle(self, x, faces, thickness, stiffness, radius, max_contacts=4):
+        self._obs = (np.ascontiguousarray(x, np.float32), np.ascontiguousarray(faces, np.int32))
+        x, f = self._obs
+        lib().orc_sim_set_obstacle(self._h, len(x), _ptr(x, _f32p), len(f), _ptr(f, _i32p), float(thickness),
+                                   float(stiffness), float(radius), int(max_contacts))
+
+    def contacts(self, max_contacts=4):
+        """(counts[V], face ids[V, max_contacts] ascending, -1 padded)."""
+        n = self.mesh.nv
+        counts = np.zeros(n, np.uint32)
+        tot = lib().orc_sim_contacts(self._h, _ptr(counts, _u32p), None, 0)
+        flat = np.zeros(max(tot, 1), np.int32)
+        lib().orc_sim_contacts(self._h, _ptr(counts, _u32p), _ptr(flat, _i32p), tot)
+        out = np.full((n, max_contacts), -1, np.int32)
+        pos = 0
+        for v in range(n):
+            k = int(counts[v])
+            out[v, :k] = flat[pos:pos + k]
+            pos += k
+        return counts, out
+
+
+def template_vf(p, a, b, c, min_distance):
+    """The reference's vertex-face narrow phase: (hit, distance, barycentric)."""
+    pts = np.ascontiguousarray(np.stack([p, a, b, c]), np.float32)
+    d = np.zeros(1, np.float32)
+    bary = np.zeros(3, np.float32)
+    hit = lib().orc_template_vf(_ptr(pts, _f32p), float(min_distance), _ptr(d, _f32p), _ptr(bary, _f32p))
+    return bool(hit), float(d[0]), bary
 
 
 def max_threads():

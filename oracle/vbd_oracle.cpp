@@ -243,11 +243,118 @@ template <typename T> struct Sim {
     bool energy_bending = false;   // CPU reference drops bending energy (example2_async_iter.cpp:832-846)
     std::vector<float> energy;
     int nthreads = 1;
+    // obstacle contact (row a10): static triangle mesh, per-frame detection
+    std::vector<V3<T>> obs_x;
+    std::vector<int> obs_f;
+    std::vector<V3<T>> obs_n;              // face normals
+    T thickness = T(0), stiffness = T(0), radius = T(0);
+    int max_contacts = 0;
+    struct Contact { int fid; V3<T> xs, n; };
+    std::vector<std::vector<Contact>> contacts;  // per vertex, ascending fid
 };
 
 template <typename T> static inline T substep_dt(const Params& p) {
     // scene_params.h:159: implicit_dt / float(num_substep)
     return T(p.implicit_dt) / T((float)p.num_substep);
+}
+
+// glm::normalize(v) = v * inversesqrt(dot(v, v)), inversesqrt(x) = 1 / sqrt(x)
+template <typename T> static inline V3<T> normalize(V3<T> v) { return v * (T(1) / std::sqrt(dot(v, v))); }
+
+// xpbd_constraints.h:2698-2718
+template <typename T> static inline T point_line_distance(V3<T> v0, V3<T> v1, V3<T> v2, V3<T> e1hat, T projection) {
+    const V3<T> e0 = v0 - v1;
+    const V3<T> e1 = v2 - v1;
+    if ((double)projection > 0.0 && projection < length(e1)) return length(e0 - projection * e1hat);
+    const T d01 = length(v0 - v1);
+    const T d02 = length(v0 - v2);
+    return (d01 < d02) ? d01 : d02;
+}
+// xpbd_constraints.h:2724-2740
+template <typename T> static inline void get_lerp(V3<T> v1, V3<T> v2, T projection, T out[2]) {
+    const V3<T> e1 = v2 - v1;
+    if ((double)projection < 0.0) { out[0] = T(1); out[1] = T(0); return; }
+    if (projection >= length(e1)) { out[0] = T(0); out[1] = T(1); return; }
+    const T ratio = projection / length(e1);
+    out[0] = T(1.0f) - ratio;
+    out[1] = ratio;
+}
+// NarrowPhase::template_vf<false> xpbd_constraints.h:2742-2820: closest point
+// of v[0] on triangle (v[1], v[2], v[3]) as barycentric weights, with the
+// reference's edge fallback when the projection leaves the triangle.
+template <typename T> static bool template_vf(const V3<T> v[4], T min_distance, T& distance, T bary[3]) {
+    const V3<T> e1 = v[2] - v[1], e2 = v[3] - v[2], e3 = v[1] - v[3];
+    const V3<T> n = cross(e1, e2);
+    const V3<T> na = cross(v[3] - v[2], v[0] - v[2]);
+    const V3<T> nb = cross(v[1] - v[3], v[0] - v[3]);
+    const V3<T> nc = cross(v[2] - v[1], v[0] - v[1]);
+    bary[0] = dot(n, na) / dot(n, n);
+    bary[1] = dot(n, nb) / dot(n, n);
+    bary[2] = dot(n, nc) / dot(n, n);
+    const T bsum = (std::fabs(bary[0]) + std::fabs(bary[1])) + std::fabs(bary[2]);
+    if ((double)bsum - 1.0 > 1e-8) {
+        const V3<T> h12 = normalize(e1), h23 = normalize(e2), h31 = normalize(e3);
+        const T p01 = dot(v[0] - v[1], h12), p02 = dot(v[0] - v[2], h23), p03 = dot(v[0] - v[3], h31);
+        const T d12 = point_line_distance(v[0], v[1], v[2], h12, p01);
+        const T d23 = point_line_distance(v[0], v[2], v[3], h23, p02);
+        const T d31 = point_line_distance(v[0], v[3], v[1], h31, p03);
+        T l[2];
+        if (d12 <= d23 && d12 <= d31) { get_lerp(v[1], v[2], p01, l); bary[0] = l[0]; bary[1] = l[1]; bary[2] = T(0.0f); }
+        else if (d23 <= d12 && d23 <= d31) { get_lerp(v[2], v[3], p02, l); bary[0] = T(0.0f); bary[1] = l[0]; bary[2] = l[1]; }
+        else { get_lerp(v[3], v[1], p03, l); bary[0] = l[1]; bary[1] = T(0.0f); bary[2] = l[0]; }
+    }
+    const V3<T> xs = (bary[0] * v[1] + bary[1] * v[2]) + bary[2] * v[3];
+    distance = length(v[0] - xs);
+    return (double)distance < (double)min_distance && (double)distance > 1e-8;
+}
+
+// Contact detection after predict: every obstacle face within `radius` of
+// the predicted position (template_vf), the max_contacts nearest by
+// (distance, face id), kept in ascending face id.  Brute force over faces.
+template <typename T> static void detect_contacts(Sim<T>& s) {
+    const long nv = (long)s.x.size();
+    const int nf = (int)s.obs_f.size() / 3;
+    s.contacts.assign(nv, {});
+#pragma omp parallel for schedule(dynamic, 64) num_threads(s.nthreads)
+    for (long vid = 0; vid < nv; vid++) {
+        struct Cand { T d; int fid; T b[3]; };
+        std::vector<Cand> keep;
+        for (int f = 0; f < nf; f++) {
+            const V3<T> v[4] = {s.x[vid], s.obs_x[s.obs_f[3 * f]], s.obs_x[s.obs_f[3 * f + 1]], s.obs_x[s.obs_f[3 * f + 2]]};
+            T d, b[3];
+            if (!template_vf(v, s.radius, d, b)) continue;
+            keep.push_back(Cand{d, f, {b[0], b[1], b[2]}});
+        }
+        std::sort(keep.begin(), keep.end(), [](const Cand& a, const Cand& c) { return a.d != c.d ? a.d < c.d : a.fid < c.fid; });
+        if ((int)keep.size() > s.max_contacts) keep.resize(s.max_contacts);
+        std::sort(keep.begin(), keep.end(), [](const Cand& a, const Cand& c) { return a.fid < c.fid; });
+        for (const Cand& c : keep) {
+            const V3<T> p1 = s.obs_x[s.obs_f[3 * c.fid]], p2 = s.obs_x[s.obs_f[3 * c.fid + 1]], p3 = s.obs_x[s.obs_f[3 * c.fid + 2]];
+            const V3<T> xs = (c.b[0] * p1 + c.b[1] * p2) + c.b[2] * p3;
+            s.contacts[vid].push_back(typename Sim<T>::Contact{c.fid, xs, s.obs_n[c.fid]});
+        }
+    }
+}
+
+// VBD::compute_obstacle_collision xpbd_constraints.h:4238-4284 (stiff =
+// stiffness * area, area = 1): f = (stiff C) n, H[col j][row i] = n_i (stiff n_j)
+template <typename T> static inline Hf<T> compute_contact(const Sim<T>& s, u32 vid) {
+    Hf<T> acc = hf_zero<T>();
+    if (s.contacts.empty()) return acc;
+    for (const auto& c : s.contacts[vid]) {
+        const V3<T> diff = s.x[vid] - c.xs;
+        const T proj = dot(diff, c.n);
+        if (!(proj < s.thickness)) continue;
+        const T C = s.thickness - proj;
+        const V3<T> f = (s.stiffness * C) * c.n;
+        const V3<T> sn = s.stiffness * c.n;
+        Hf<T> hc;
+        for (int col = 0; col < 3; col++)
+            for (int row = 0; row < 3; row++) hc.h[col][row] = c.n[row] * sn[col];
+        for (int r = 0; r < 3; r++) hc.f[r] = f[r];
+        hf_add(acc, hc);
+    }
+    return acc;
 }
 
 // xpbd_constraints.h:321-384
@@ -428,7 +535,18 @@ template <typename T> static double compute_energy(const Sim<T>& s) {
             eb += (double)((T(0.5f) * kb) * sum);
         }
     }
-    return ei + es + eb;
+    double ec = 0.0;
+    if (!s.contacts.empty()) {
+        for (long vid = 0; vid < nv; vid++)
+            for (const auto& c : s.contacts[vid]) {
+                const T proj = dot(s.x[vid] - c.xs, c.n);
+                if (proj < s.thickness) {
+                    const T C = s.thickness - proj;
+                    ec += (double)((T(0.5f) * s.stiffness) * (C * C));
+                }
+            }
+    }
+    return ei + es + eb + ec;
 }
 
 // solve_constraints_VBD example2_async_iter.cpp:1481-1509; per-colour passes
@@ -444,6 +562,10 @@ template <typename T> static void solve_vbd_iteration(Sim<T>& s, bool record, bo
         for (long i = 0; i < n; i++) { Hf<T> a = compute_stretch(s, cl[i]); hf_add(s.hf[cl[i]], a); }
 #pragma omp parallel for schedule(static, 32) num_threads(s.nthreads)
         for (long i = 0; i < n; i++) { Hf<T> a = compute_bending(s, cl[i]); hf_add(s.hf[cl[i]], a); }
+        if (!s.contacts.empty()) {
+#pragma omp parallel for schedule(static, 32) num_threads(s.nthreads)
+            for (long i = 0; i < n; i++) { Hf<T> a = compute_contact(s, cl[i]); hf_add(s.hf[cl[i]], a); }
+        }
 #pragma omp parallel for schedule(static, 32) num_threads(s.nthreads)
         for (long i = 0; i < n; i++) vbd_step(s, cl[i]);
     }
@@ -460,6 +582,7 @@ template <typename T> static void physics_step(Sim<T>& s, bool record) {
     const u32 nsub = record ? 1u : s.p.num_substep;
     for (u32 sub = 0; sub < nsub; sub++) {
         predict_position(s);
+        if (!s.obs_f.empty()) detect_contacts(s);
         for (u32 it = 0; it < s.p.iter_count; it++) solve_vbd_iteration(s, record, it == 0);
         update_velocity(s);
     }
@@ -513,6 +636,35 @@ template <typename T> static void set_state(Sim<T>& s, const double* x, const do
         }
     s.x_start = s.x;
     s.v_start = s.v;
+}
+
+template <typename T>
+static void set_obstacle(Sim<T>& s, u32 nv, const float* x, u32 nf, const int* f, float thickness, float stiffness,
+                         float radius, int max_contacts) {
+    s.obs_x.resize(nv);
+    for (u32 i = 0; i < nv; i++) s.obs_x[i] = mk(T(x[3 * i]), T(x[3 * i + 1]), T(x[3 * i + 2]));
+    s.obs_f.assign(f, f + 3 * (size_t)nf);
+    s.obs_n.resize(nf);
+    for (u32 k = 0; k < nf; k++) {
+        const V3<T> a = s.obs_x[f[3 * k]], b = s.obs_x[f[3 * k + 1]], c = s.obs_x[f[3 * k + 2]];
+        s.obs_n[k] = normalize(cross(b - a, c - a));
+    }
+    s.thickness = T(thickness);
+    s.stiffness = T(stiffness);
+    s.radius = T(radius);
+    s.max_contacts = max_contacts;
+}
+
+template <typename T> static u32 contact_list(const Sim<T>& s, u32* counts, int* fids, u32 cap) {
+    u32 tot = 0;
+    for (size_t v = 0; v < s.contacts.size(); v++) {
+        if (counts) counts[v] = (u32)s.contacts[v].size();
+        for (const auto& c : s.contacts[v]) {
+            if (fids && tot < cap) fids[tot] = c.fid;
+            tot++;
+        }
+    }
+    return tot;
 }
 
 extern "C" {
@@ -611,6 +763,27 @@ void orc_sim_set_state(orc_sim* s, const double* x, const double* v) {
     else set_state(s->s32, x, v);
 }
 double orc_sim_energy(orc_sim* s) { return s->fp64 ? compute_energy(s->s64) : compute_energy(s->s32); }
+
+// Static triangle obstacle (face normals = normalize(cross(b - a, c - a))).
+void orc_sim_set_obstacle(orc_sim* s, u32 nv, const float* x, u32 nf, const int* f, float thickness, float stiffness,
+                          float radius, int max_contacts) {
+    if (s->fp64) set_obstacle(s->s64, nv, x, nf, f, thickness, stiffness, radius, max_contacts);
+    else set_obstacle(s->s32, nv, x, nf, f, thickness, stiffness, radius, max_contacts);
+}
+// Contacts of the last frame: per vertex count, then (fid) lists; returns total.
+u32 orc_sim_contacts(const orc_sim* s, u32* counts, int* fids, u32 cap) {
+    return s->fp64 ? contact_list(s->s64, counts, fids, cap) : contact_list(s->s32, counts, fids, cap);
+}
+// Narrow phase alone (for tests): returns hit, writes distance and barycentric weights.
+int orc_template_vf(const float* p4, float min_distance, float* distance, float* bary) {
+    V3<float> v[4];
+    for (int k = 0; k < 4; k++) v[k] = mk(p4[3 * k], p4[3 * k + 1], p4[3 * k + 2]);
+    float d = 0.f, b[3] = {0.f, 0.f, 0.f};
+    const bool hit = template_vf(v, min_distance, d, b);
+    *distance = d;
+    for (int k = 0; k < 3; k++) bary[k] = b[k];
+    return hit ? 1 : 0;
+}
 
 int orc_max_threads(void) {
 #ifdef _OPENMP
