@@ -178,6 +178,14 @@ int lats_vbd_set_obstacle(lats_vbd* ctx, const float* x, uint32_t num_verts, con
  * fids[4*V] (ascending, -1 padded). */
 int lats_vbd_get_contacts(lats_vbd* ctx, uint32_t* counts, int32_t* fids);
 
+/* ---- frame pieces (physics_step_vbd, example2_async_iter.cpp:1082-1143,
+ *      in stages; all enqueue on the context's stream) ------------------- */
+int lats_vbd_frame_begin(lats_vbd* ctx);              /* predict (+ contact detection)      */
+int lats_vbd_iterate(lats_vbd* ctx, uint32_t sweeps); /* colour sweeps on the positions     */
+int lats_vbd_frame_end(lats_vbd* ctx);                /* update velocity, advance the frame */
+/* Energy (incremental potential) of the current positions; synchronises. */
+int lats_vbd_energy_now(lats_vbd* ctx, float* energy);
+
 /* ---- scheduler slot: GpuSolver::fn_dispatch ------------------------- */
 /* Function ids of the tasks the VBD DAG registers
  * (example3_hybrid_cpu_gpu.cpp:1921-1988; FunctionID launcher.h:15-221). */

@@ -1,0 +1,132 @@
+// vbd_context.h -- the single-cloth device context behind lats_vbd (shared
+// by the solver and the learned warm start translation units).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <vector>
+
+#include "../../include/libatssim_hip.h"
+#include "host/obstacle.h"
+#include "vbd_kernels.h"
+
+namespace lats {
+
+constexpr int kThreads = 256;
+
+template <typename T> struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    ~DBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    hipError_t alloc(size_t count) {
+        release();
+        n = count;
+        if (count == 0) return hipSuccess;
+        return hipMalloc((void**)&p, count * sizeof(T));
+    }
+    hipError_t upload(const std::vector<T>& h) {
+        hipError_t e = alloc(h.size());
+        if (e != hipSuccess || h.empty()) return e;
+        return hipMemcpy(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice);
+    }
+};
+
+inline uint32_t nblocks(uint64_t n) { return (uint32_t)((n + kThreads - 1) / kThreads); }
+
+}  // namespace lats
+
+struct lats_vbd {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint32_t V = 0, E = 0, B = 0, C = 0;
+    lats_scene_params prm{};
+    float h = 0.f, h2inv = 0.f, damp = 0.f, h_energy = 0.f;
+    std::vector<uint32_t> cstart, ccount, eoff, boff;
+    std::vector<uint32_t> perm, inv;
+    int strategy = LATS_STRATEGY_AUTO;
+    int energy_bending = 1;
+
+    lats::DBuf<float> x, x0, v, mass;
+    lats::DBuf<uint32_t> meta, perm_d, inv_d;
+    lats::DBuf<uint2> ell_e, e_slots;
+    lats::DBuf<float4> ell_bq, Qd;
+    lats::DBuf<uint4> ell_bi, b_slots;
+    lats::DBuf<float> rest;
+    lats::DBuf<lats::FrameParams> fp;
+    lats::DBuf<float> forces;
+    lats::DBuf<double> partial;
+    lats::DBuf<float> energy;
+    lats::DBuf<float> scratch3;   // 3V staging for permutations
+    uint32_t energy_rows = 0, energy_cap = 0, energy_nblk = 0;
+    bool record_dispatch = false;
+
+    // multi-buffer positions of the async schedule
+    lats::DBuf<float> iterbuf[LATS_NUM_ASYNC_BUFFERS], beginbuf[LATS_NUM_ASYNC_BUFFERS];
+
+    // obstacle contact (row a10)
+    bool has_obstacle = false;
+    float thickness = 0.f, ob_stiffness = 0.f, ob_radius = 0.f;
+    lats::ObstacleHash ob_hash;
+    lats::DBuf<float> ob_x, ob_n;
+    lats::DBuf<int32_t> ob_f;
+    lats::DBuf<uint32_t> hb_belongs, hb_start, hb_count, hb_faces;
+    lats::DBuf<uint32_t> ct_count, ct_fid;
+    lats::DBuf<float4> ct_xs, ct_n;
+
+    // persistent strategy
+    lats::DBuf<lats::ColourInfo> colours_d;
+    lats::DBuf<uint32_t> barrier;
+    uint32_t persist_blocks = 0;      // 0 = derive from the largest colour
+    uint32_t lanes = 1;               // lanes per vertex (1, 4, 8, 16)
+    uint32_t persist_max_lanes = 0;
+    uint32_t persist_max_blocks = 0;  // co-residency limit (queried on first use)
+
+    // graph per (record) flag
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    float last_ms = 0.f;
+
+    std::map<uint32_t, hipEvent_t> markers;
+    uint32_t* host_flag = nullptr;    // pinned, mapped
+    uint32_t* host_flag_d = nullptr;  // device alias
+
+    ~lats_vbd() {
+        for (auto& g : graph)
+            if (g) (void)hipGraphExecDestroy(g);
+        for (auto& kv : markers) (void)hipEventDestroy(kv.second);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (host_flag) (void)hipHostFree(host_flag);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    lats::VbdArgs args(float* xbuf) const {
+        lats::VbdArgs a;
+        a.x = xbuf;
+        a.x0 = x0.p;
+        a.v = v.p;
+        a.mass = mass.p;
+        a.meta = meta.p;
+        a.ell_e = ell_e.p;
+        a.ell_bq = ell_bq.p;
+        a.ell_bi = ell_bi.p;
+        a.fp = fp.p;
+        a.h = h;
+        a.h2inv = h2inv;
+        a.ks = prm.stiffness_stretch_spring;
+        a.kb = prm.stiffness_quadratic_bending;
+        for (int c = 0; c < 3; c++) a.g[c] = prm.gravity[c];
+        a.ccount = has_obstacle ? ct_count.p : nullptr;
+        a.ct_xs = ct_xs.p;
+        a.ct_n = ct_n.p;
+        a.nslots = V;
+        a.thickness = thickness;
+        return a;
+    }
+};
+

@@ -32,6 +32,7 @@
 #include "vbd_common.h"
 #include "vbd_kernels.h"
 #include "vbd_layout.h"
+#include "vbd_context.h"
 
 using namespace lats;
 
@@ -447,121 +448,9 @@ int fail(int code, const std::string& msg) {
             return fail(LATS_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));       \
     } while (0)
 
-template <typename T> struct DBuf {
-    T* p = nullptr;
-    size_t n = 0;
-    ~DBuf() { release(); }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-    }
-    hipError_t alloc(size_t count) {
-        release();
-        n = count;
-        if (count == 0) return hipSuccess;
-        return hipMalloc((void**)&p, count * sizeof(T));
-    }
-    hipError_t upload(const std::vector<T>& h) {
-        hipError_t e = alloc(h.size());
-        if (e != hipSuccess || h.empty()) return e;
-        return hipMemcpy(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice);
-    }
-};
-
-inline uint32_t nblocks(uint64_t n) { return (uint32_t)((n + kBlock - 1) / kBlock); }
 
 }  // namespace
 
-struct lats_vbd {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    uint32_t V = 0, E = 0, B = 0, C = 0;
-    lats_scene_params prm{};
-    float h = 0.f, h2inv = 0.f, damp = 0.f, h_energy = 0.f;
-    std::vector<uint32_t> cstart, ccount, eoff, boff;
-    std::vector<uint32_t> perm, inv;
-    int strategy = LATS_STRATEGY_AUTO;
-    int energy_bending = 1;
-
-    DBuf<float> x, x0, v, mass;
-    DBuf<uint32_t> meta, perm_d, inv_d;
-    DBuf<uint2> ell_e, e_slots;
-    DBuf<float4> ell_bq, Qd;
-    DBuf<uint4> ell_bi, b_slots;
-    DBuf<float> rest;
-    DBuf<FrameParams> fp;
-    DBuf<float> forces;
-    DBuf<double> partial;
-    DBuf<float> energy;
-    DBuf<float> scratch3;   // 3V staging for permutations
-    uint32_t energy_rows = 0, energy_cap = 0, energy_nblk = 0;
-    bool record_dispatch = false;
-
-    // multi-buffer positions of the async schedule
-    DBuf<float> iterbuf[LATS_NUM_ASYNC_BUFFERS], beginbuf[LATS_NUM_ASYNC_BUFFERS];
-
-    // obstacle contact (row a10)
-    bool has_obstacle = false;
-    float thickness = 0.f, ob_stiffness = 0.f, ob_radius = 0.f;
-    ObstacleHash ob_hash;
-    DBuf<float> ob_x, ob_n;
-    DBuf<int32_t> ob_f;
-    DBuf<uint32_t> hb_belongs, hb_start, hb_count, hb_faces;
-    DBuf<uint32_t> ct_count, ct_fid;
-    DBuf<float4> ct_xs, ct_n;
-
-    // persistent strategy
-    DBuf<ColourInfo> colours_d;
-    DBuf<uint32_t> barrier;
-    uint32_t persist_blocks = 0;      // 0 = derive from the largest colour
-    uint32_t lanes = 1;               // lanes per vertex (1, 4, 8, 16)
-    uint32_t persist_max_lanes = 0;
-    uint32_t persist_max_blocks = 0;  // co-residency limit (queried on first use)
-
-    // graph per (record) flag
-    hipGraphExec_t graph[2] = {nullptr, nullptr};
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    float last_ms = 0.f;
-
-    std::map<uint32_t, hipEvent_t> markers;
-    uint32_t* host_flag = nullptr;    // pinned, mapped
-    uint32_t* host_flag_d = nullptr;  // device alias
-
-    ~lats_vbd() {
-        for (auto& g : graph)
-            if (g) (void)hipGraphExecDestroy(g);
-        for (auto& kv : markers) (void)hipEventDestroy(kv.second);
-        if (ev0) (void)hipEventDestroy(ev0);
-        if (ev1) (void)hipEventDestroy(ev1);
-        if (host_flag) (void)hipHostFree(host_flag);
-        if (stream) (void)hipStreamDestroy(stream);
-    }
-
-    VbdArgs args(float* xbuf) const {
-        VbdArgs a;
-        a.x = xbuf;
-        a.x0 = x0.p;
-        a.v = v.p;
-        a.mass = mass.p;
-        a.meta = meta.p;
-        a.ell_e = ell_e.p;
-        a.ell_bq = ell_bq.p;
-        a.ell_bi = ell_bi.p;
-        a.fp = fp.p;
-        a.h = h;
-        a.h2inv = h2inv;
-        a.ks = prm.stiffness_stretch_spring;
-        a.kb = prm.stiffness_quadratic_bending;
-        for (int c = 0; c < 3; c++) a.g[c] = prm.gravity[c];
-        a.ccount = has_obstacle ? ct_count.p : nullptr;
-        a.ct_xs = ct_xs.p;
-        a.ct_n = ct_n.p;
-        a.nslots = V;
-        a.thickness = thickness;
-        return a;
-    }
-};
 
 namespace {
 
@@ -1355,6 +1244,43 @@ int lats_vbd_copy_permutation(lats_vbd* c, uint32_t* perm) {
     std::copy(c->perm.begin(), c->perm.end(), perm);
     return LATS_OK;
 }
+// ---- frame pieces (the reference's physics_step_vbd split in its stages)
+int lats_vbd_frame_begin(lats_vbd* c) {
+    if (!c) return fail(LATS_ERR_INVALID_ARGUMENT, "null context");
+    enqueue_predict(c);
+    enqueue_detect(c);
+    HIPCHK(hipGetLastError());
+    return LATS_OK;
+}
+
+int lats_vbd_iterate(lats_vbd* c, uint32_t sweeps) {
+    if (!c) return fail(LATS_ERR_INVALID_ARGUMENT, "null context");
+    for (uint32_t it = 0; it < sweeps; it++)
+        for (uint32_t col = 0; col < c->C; col++) enqueue_colour(c, c->x.p, col);
+    HIPCHK(hipGetLastError());
+    return LATS_OK;
+}
+
+int lats_vbd_frame_end(lats_vbd* c) {
+    if (!c) return fail(LATS_ERR_INVALID_ARGUMENT, "null context");
+    enqueue_update(c, 1);
+    HIPCHK(hipGetLastError());
+    return LATS_OK;
+}
+
+int lats_vbd_energy_now(lats_vbd* c, float* e) {
+    if (!c || !e) return fail(LATS_ERR_INVALID_ARGUMENT, "null argument");
+    int rc = ensure_energy_rows(c, 1);
+    if (rc) return rc;
+    enqueue_energy(c, c->x.p, 0);
+    hipLaunchKernelGGL(k_energy_finalize, dim3(1), dim3(kBlock), 0, c->stream, c->partial.p, c->energy_nblk, 0u,
+                       c->energy.p);
+    HIPCHK(hipMemcpyAsync(e, c->energy.p, sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->energy_rows = 0;
+    return LATS_OK;
+}
+
 int lats_vbd_time_colour_kernels(lats_vbd* c, uint32_t iters, float* mean_ms, uint32_t* launches,
                                  uint64_t* vertex_updates) {
     if (!c || !mean_ms) return fail(LATS_ERR_INVALID_ARGUMENT, "null argument");

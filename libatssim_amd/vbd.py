@@ -214,6 +214,20 @@ class GpuVBD:
         check(lib().lats_vbd_set_obstacle(self._h, _p(x), len(x), _p(f), len(f), float(thickness),
                                           float(stiffness), float(radius), float(cell)))
 
+    def frame_begin(self):
+        check(lib().lats_vbd_frame_begin(self._h))
+
+    def iterate(self, sweeps):
+        check(lib().lats_vbd_iterate(self._h, int(sweeps)))
+
+    def frame_end(self):
+        check(lib().lats_vbd_frame_end(self._h))
+
+    def energy_now(self):
+        e = ctypes.c_float()
+        check(lib().lats_vbd_energy_now(self._h, ctypes.byref(e)))
+        return e.value
+
     def contacts(self):
         """(counts[V], face ids[V, 4] ascending, -1 padded) of the last detection."""
         n = self.cloth.num_verts
