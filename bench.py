@@ -11,7 +11,8 @@ Also reported:
   roofline      the per-colour VBD kernel (k_vbd_colour): algorithmic bytes
                 per launch (53 V + 20 E + 72 B + 8 bytes per full sweep, spread
                 over the colour launches of one sweep) / mean launch duration
-                from HIP events on the solver's stream, vs 8 TB/s HBM.
+                (HIP events around a graph of 20 sweeps on the solver's
+                stream), vs 8 TB/s HBM.
   cpu_baseline  the CPU oracle (C++ restatement of the reference's CPU VBD,
                 per-colour parallel passes) on this host's cores, over a bounded
                 sample of the same workload (whole frames, >= 10 s of work).
@@ -29,7 +30,17 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "VBD vertex-updates/s (256² cloth); ms/frame to fixed residual, learned vs plain"
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "vbd_colour_grid256_traffic.json")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_vbd_colour_grid256_traffic.txt")
+
+
+def read_traffic(path):
+    """HBM bytes per launch from the committed PMC summary (`hbm_bytes_per_launch: N` line)."""
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("hbm_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
 
 
 def algorithmic_bytes_per_sweep(nv, ne, nb):
@@ -70,9 +81,11 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", default="grid_c1", choices=["grid_c1", "batch_c2"],
-                    help="grid_c1 (default, the metric's workload) or batch_c2 (512 cloths)")
+    ap.add_argument("--workload", default="grid_c1", choices=["grid_c1", "batch_c2", "copy_calibration"],
+                    help="grid_c1 (default, the metric's workload), batch_c2 (512 cloths) or "
+                         "copy_calibration (a plain 256 MiB device copy, the reference for HBM counters)")
     ap.add_argument("--strategy", default="auto", choices=["auto", "graph", "persistent"])
+    ap.add_argument("--lanes", type=int, default=0, help="lanes per vertex (1, 4, 8, 16); 0 = autotuned")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the VBD hot path on a single MI355X "
@@ -83,6 +96,8 @@ def main():
 
     if args.workload == "batch_c2":
         return bench_batch(args)
+    if args.workload == "copy_calibration":
+        return copy_calibration(args)
 
     K, W = args.steps, args.warmup
     cloth, params, _, wind = configs.grid_c1(frames=W + K)
@@ -94,6 +109,9 @@ def main():
         strategy = _lib.LATS_STRATEGY_GRAPH if args.strategy == "graph" else _lib.LATS_STRATEGY_PERSISTENT
         g.set_strategy(strategy)
         blocks = g.strategy()[1]
+    if args.lanes:
+        g.set_lanes_per_vertex(args.lanes)
+        strategy, blocks = g.strategy()
     if W:
         g.run_frames(W, wind[:W])
     g.synchronize()
@@ -114,14 +132,13 @@ def main():
         mean_ms = gpu_ms
         bytes_per_launch = b_sweep * iters * K
     else:
-        # per-colour VBD launches timed one by one with HIP events
+        # 20 sweeps of per-colour VBD launches captured in a graph, timed
+        # with HIP events on the solver's stream: mean launch duration
         kernel = "k_vbd_colour"
-        mean_ms, launches, kvu = g.time_colour_kernels(10)
+        mean_ms, launches, kvu = g.time_colour_kernels(20)
         bytes_per_launch = b_sweep / t.num_clusters
     achieved = bytes_per_launch / (mean_ms * 1e-3) / 1e9
-    traffic = None
-    if os.path.exists(TRAFFIC_FILE):
-        traffic = json.load(open(TRAFFIC_FILE)).get("hbm_bytes_per_launch")
+    traffic = read_traffic(TRAFFIC_FILE) if kernel == "k_vbd_colour" else None
     roofline = {
         "bound": "hbm",
         "achieved": round(achieved, 2),
@@ -154,6 +171,7 @@ def main():
             "iterations": iters,
             "strategy": "persistent" if strategy == _lib.LATS_STRATEGY_PERSISTENT else "graph",
             "persistent_blocks": blocks,
+            "lanes_per_vertex": g.lanes_per_vertex(),
             "gpu_ms_per_step": gpu_ms / K,
         },
         "roofline": roofline,
@@ -195,6 +213,20 @@ def bench_batch(args):
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": "k_vbd_batch"},
     }
     print(json.dumps(out), flush=True)
+
+
+def copy_calibration(args):
+    """A plain device-to-device copy of 256 MiB: the HBM-counter calibration
+    (FETCH_SIZE / WRITE_SIZE per dispatch of this copy vs its 256 MiB)."""
+    import torch
+    n = 64 << 20
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(args.warmup + args.steps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"workload": "copy_calibration", "bytes_read_per_copy": 4 * n, "bytes_written_per_copy": 4 * n,
+                      "copies": args.warmup + args.steps}), flush=True)
 
 
 if __name__ == "__main__":

@@ -268,15 +268,75 @@ int lats_batch_last_elapsed_ms(lats_batch* b, float* ms);
  * holds vertex perm[s] (lats_vbd_copy_permutation). */
 int lats_vbd_device_positions(lats_vbd* ctx, void** dptr);
 int lats_vbd_copy_permutation(lats_vbd* ctx, uint32_t* perm);
-/* Measurement: runs `iterations` colour sweeps with every per-colour VBD
- * kernel launched eagerly between two HIP events on the context's stream,
- * then restores the positions.  mean_ms = mean event-to-event duration of
- * one colour launch; launches = number of launches timed; vertex_updates =
- * vertex updates they performed. */
+/* Measurement: captures `iterations` colour sweeps (one VBD kernel launch
+ * per colour) in a graph, replays it once to warm up and once between two
+ * HIP events on the context's stream, then restores the positions.
+ * mean_ms = event time / launches (kernels run back to back, as in a frame
+ * graph); launches = number of launches timed; vertex_updates = vertex
+ * updates they performed. */
 int lats_vbd_time_colour_kernels(lats_vbd* ctx, uint32_t iterations, float* mean_ms, uint32_t* launches,
                                  uint64_t* vertex_updates);
 /* Bytes of device memory held by the context. */
 int lats_vbd_device_bytes(lats_vbd* ctx, uint64_t* bytes);
+
+/* ---- learned warm start (no reference counterpart) -------------------
+ * The reference starts every VBD frame from the inertial guess
+ * (Core::predict_position, src/Solver/xpbd_constraints.h:321-384).  This
+ * optional module predicts the end-of-frame correction x_end - x_pred of
+ * every vertex with a per-vertex MLP (48 features -> 128 -> 128 -> 128 -> 3,
+ * SiLU; BF16 weights on the matrix cores, FP32 accumulation and FP32 master
+ * weights), trained with Adam on frames the plain solver produces, and
+ * keeps the prediction only when it lowers the frame's VBD energy.
+ * Parameters are float[43408]: W1 [64][128] (rows 48..63 zero), b1[128],
+ * W2 [128][128], b2, W3 [128][128], b3, W4 [128][16] (columns 3..15 zero),
+ * b4[16]; W row-major [in][out]. */
+typedef struct lats_mlp lats_mlp;
+int lats_mlp_create(uint64_t seed, lats_mlp** out);
+int lats_mlp_destroy(lats_mlp* m);
+int lats_mlp_num_params(uint32_t* n);
+int lats_mlp_get_params(lats_mlp* m, float* params);
+int lats_mlp_set_params(lats_mlp* m, const float* params);
+/* Features are divided by these 48 scales before the network sees them. */
+int lats_mlp_get_feature_scale(lats_mlp* m, float* scale48);
+int lats_mlp_set_feature_scale(lats_mlp* m, const float* scale48);
+/* Checks on caller rows (float[M][48] -> float[M][3]; inputs rounded to BF16). */
+int lats_mlp_forward_host(lats_mlp* m, const float* X, uint32_t M, float* Y);
+/* Gradient of mean((Y - T)^2) over the M x 3 outputs (no update). */
+int lats_mlp_gradient_host(lats_mlp* m, const float* X, const float* T, uint32_t M, float* grad, double* loss);
+/* Binds a solver context (allocates per-vertex buffers, clears the
+ * correction history).  The frame calls below need the bound context and
+ * fail on any other one (also on a new context created where a destroyed
+ * bound one was).  Every frame call returns with the context's stream idle. */
+int lats_mlp_bind(lats_mlp* m, lats_vbd* ctx);
+int lats_mlp_reset_history(lats_mlp* m);
+/* Training rows: one plain frame (predict, all iterations, update) of the
+ * bound context, recording its features and its correction as rows. */
+int lats_mlp_reserve_rows(lats_mlp* m, uint64_t rows);
+int lats_mlp_record_frame(lats_mlp* m, lats_vbd* ctx);
+/* Appends caller rows (float[M][48] features, already scaled; float[M][3] targets). */
+int lats_mlp_dataset_append(lats_mlp* m, const float* X, const float* T, uint32_t M);
+int lats_mlp_dataset_rows(lats_mlp* m, uint64_t* rows);
+int lats_mlp_dataset_read(lats_mlp* m, uint64_t first, uint32_t count, float* X, float* T);
+int lats_mlp_dataset_clear(lats_mlp* m);
+/* Sets the scales to the recorded features' RMS and rescales the rows. */
+int lats_mlp_fit_feature_scale(lats_mlp* m, float* scale48);
+/* `steps` Adam steps (lr, beta 0.9 / 0.999) on batches of batch_rows rows
+ * drawn with replacement from splitmix64(seed + step). */
+int lats_mlp_train(lats_mlp* m, uint32_t steps, uint32_t batch_rows, float lr, uint64_t seed, double* mean_loss);
+/* One frame of the bound context: predict, learned guess (use_model != 0)
+ * kept only if its energy is <= the inertial guess's, then colour sweeps
+ * until the energy is <= target (target <= 0: run max_sweeps); the first
+ * check comes after about as many sweeps as the previous frame needed, the
+ * next ones every check_every sweeps; update.  out[6] = {sweeps, kept,
+ * E(inertial guess), E(learned guess), E(final), target}. */
+int lats_mlp_warm_start_frame(lats_mlp* m, lats_vbd* ctx, float target, uint32_t max_sweeps, uint32_t check_every,
+                              int use_model, float* out);
+/* The same frame, with the target set to the energy that `ref` (a second
+ * context of the same mesh) reaches after its full iteration count from
+ * ctx's frame start, recording ctx's features and ref's correction as
+ * training rows (rows from the states the warm-started solver visits). */
+int lats_mlp_record_guided_frame(lats_mlp* m, lats_vbd* ctx, lats_vbd* ref, uint32_t max_sweeps, uint32_t check_every,
+                                 int use_model, float* out);
 
 #ifdef __cplusplus
 }

@@ -5,3 +5,4 @@ include/libatssim_hip.h); this package is its Python front-end.
 """
 from . import _lib  # noqa: F401
 from .vbd import Cloth, GpuVBD, Params, SolverInterface, Topology, make_grid, make_wind  # noqa: F401
+from .warm_start import WarmStartMLP  # noqa: F401

@@ -153,6 +153,27 @@ SIGNATURES = {
     "lats_batch_autotune": (_I, [_VP, _U32, ctypes.POINTER(_U32), ctypes.POINTER(_U32)]),
     "lats_batch_run_frames": (_I, [_VP, _U32]),
     "lats_batch_last_elapsed_ms": (_I, [_VP, ctypes.POINTER(ctypes.c_float)]),
+    "lats_mlp_create": (_I, [ctypes.c_uint64, ctypes.POINTER(_VP)]),
+    "lats_mlp_destroy": (_I, [_VP]),
+    "lats_mlp_num_params": (_I, [ctypes.POINTER(_U32)]),
+    "lats_mlp_get_params": (_I, [_VP, _VP]),
+    "lats_mlp_set_params": (_I, [_VP, _VP]),
+    "lats_mlp_get_feature_scale": (_I, [_VP, _VP]),
+    "lats_mlp_set_feature_scale": (_I, [_VP, _VP]),
+    "lats_mlp_forward_host": (_I, [_VP, _VP, _U32, _VP]),
+    "lats_mlp_gradient_host": (_I, [_VP, _VP, _VP, _U32, _VP, ctypes.POINTER(ctypes.c_double)]),
+    "lats_mlp_bind": (_I, [_VP, _VP]),
+    "lats_mlp_reset_history": (_I, [_VP]),
+    "lats_mlp_reserve_rows": (_I, [_VP, ctypes.c_uint64]),
+    "lats_mlp_record_frame": (_I, [_VP, _VP]),
+    "lats_mlp_dataset_append": (_I, [_VP, _VP, _VP, _U32]),
+    "lats_mlp_dataset_rows": (_I, [_VP, ctypes.POINTER(ctypes.c_uint64)]),
+    "lats_mlp_dataset_read": (_I, [_VP, ctypes.c_uint64, _U32, _VP, _VP]),
+    "lats_mlp_dataset_clear": (_I, [_VP]),
+    "lats_mlp_fit_feature_scale": (_I, [_VP, _VP]),
+    "lats_mlp_train": (_I, [_VP, _U32, _U32, ctypes.c_float, ctypes.c_uint64, ctypes.POINTER(ctypes.c_double)]),
+    "lats_mlp_warm_start_frame": (_I, [_VP, _VP, ctypes.c_float, _U32, _U32, _I, _VP]),
+    "lats_mlp_record_guided_frame": (_I, [_VP, _VP, _VP, _U32, _U32, _I, _VP]),
 }
 
 _LIB = None

@@ -16,6 +16,11 @@ sphere_c4    1024x1024 horizontal grid (2 mm spacing, 2.05 m wide, no pins)
              faces); contact thickness 2 mm, k_contact 1e5, detection radius
              2.5 cm (one frame of fall at impact speed), hash cell 5 cm;
              50 iterations/frame, 120 frames.
+warm_c3      learned warm start on the grid_c1 cloth: training scenarios k =
+             0..31 of 200 frames each, per-scenario wind seed and pin set
+             (k % 4: 0, 2 top row; 1 two top corners; 3 one to four random
+             top-boundary vertices plus both top corners); evaluation on the
+             grid_c1 scene itself (top row pinned, its wind).
 square8k     the reference's example-2 run (examples/example2_async_iter.cpp:
              133-392, 715-749, 1713-1810): square8K.obj, pins where the
              AABB-normalised z < 0.01 and x < 0.01 or x > 0.99, m = 0.01 kg,
@@ -29,6 +34,7 @@ SEED_C0 = 20250101
 SEED_C1 = 20250102
 SEED_WIND = 20250103
 SEED_C2 = 20250104
+SEED_C3 = 20250106
 
 
 class SplitMix64:
@@ -102,6 +108,23 @@ def grid_c1(frames=60):
     cloth = grid_cloth(256, "top", SEED_C1)
     wind = make_wind(frames, cloth.num_verts, 2.0, SEED_WIND)
     return cloth, Params(constraint_iter_count=100), frames, wind
+
+
+def warm_c3_scenario(k, n=256, frames=200, seed=SEED_C3):
+    """Training scenario k of the learned warm start: (cloth, wind[frames])."""
+    rng = SplitMix64(seed * 1000003 + k)
+    fixed = np.zeros(n * n, np.uint8)
+    kind = k % 4
+    if kind in (0, 2):
+        fixed[:n] = 1
+    else:
+        fixed[0] = fixed[n - 1] = 1
+        if kind == 3:
+            for _ in range(1 + min(int(rng.uniform() * 4), 3)):
+                fixed[min(int(rng.uniform() * n), n - 1)] = 1
+    x, f = make_grid(n, n, 0.01, 1e-4, SEED_C1)
+    wind = make_wind(frames, n * n, 2.0, rng.next() & ((1 << 63) - 1))
+    return Cloth(x, f, np.float32(1e-3), fixed), wind
 
 
 def horizontal_grid(n, spacing, height, seed, jitter=1e-4):

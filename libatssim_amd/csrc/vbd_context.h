@@ -41,6 +41,7 @@ inline uint32_t nblocks(uint64_t n) { return (uint32_t)((n + kThreads - 1) / kTh
 }  // namespace lats
 
 struct lats_vbd {
+    uint64_t uid = 0;          // unique per created context (never reused)
     int device = 0;
     hipStream_t stream = nullptr;
     uint32_t V = 0, E = 0, B = 0, C = 0;
@@ -86,8 +87,9 @@ struct lats_vbd {
     uint32_t persist_max_lanes = 0;
     uint32_t persist_max_blocks = 0;  // co-residency limit (queried on first use)
 
-    // graph per (record) flag
+    // graph per (record) flag; graphs of k colour sweeps (lats_vbd_iterate)
     hipGraphExec_t graph[2] = {nullptr, nullptr};
+    std::map<uint32_t, hipGraphExec_t> sweep_graphs;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     float last_ms = 0.f;
 
@@ -95,9 +97,20 @@ struct lats_vbd {
     uint32_t* host_flag = nullptr;    // pinned, mapped
     uint32_t* host_flag_d = nullptr;  // device alias
 
-    ~lats_vbd() {
+    // every captured graph holds kernel choices and buffer pointers of the
+    // moment it was captured: drop them whenever one of those changes
+    void drop_graphs() {
         for (auto& g : graph)
-            if (g) (void)hipGraphExecDestroy(g);
+            if (g) {
+                (void)hipGraphExecDestroy(g);
+                g = nullptr;
+            }
+        for (auto& kv : sweep_graphs) (void)hipGraphExecDestroy(kv.second);
+        sweep_graphs.clear();
+    }
+
+    ~lats_vbd() {
+        drop_graphs();
         for (auto& kv : markers) (void)hipEventDestroy(kv.second);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);

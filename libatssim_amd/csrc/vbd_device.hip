@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -472,8 +473,7 @@ int ensure_energy_rows(lats_vbd* c, uint32_t rows) {
     std::swap(c->energy.n, ne.n);
     c->energy_cap = cap;
     // graphs captured against the old buffers are stale
-    for (auto& g : c->graph)
-        if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+    c->drop_graphs();
     return LATS_OK;
 }
 
@@ -742,6 +742,8 @@ int lats_vbd_create(const lats_mesh_desc* m, const lats_scene_params* prm, lats_
 
     auto cleanup_fail = [&](int code) { delete c; return code; };
 #define UPCHK(call) do { hipError_t e_ = (call); if (e_ != hipSuccess) return cleanup_fail(fail(e_ == hipErrorOutOfMemory ? LATS_ERR_OUT_OF_MEMORY : LATS_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_))); } while (0)
+    static std::atomic<uint64_t> next_uid{1};
+    c->uid = next_uid.fetch_add(1);
     UPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     UPCHK(c->x.upload(hx));
     UPCHK(c->x0.upload(hx));
@@ -823,8 +825,7 @@ int lats_vbd_set_obstacle(lats_vbd* c, const float* x, uint32_t nv, const int32_
         c->ob_radius = radius;
         c->has_obstacle = true;
     }
-    for (auto& g : c->graph)
-        if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+    c->drop_graphs();
     return LATS_OK;
 }
 
@@ -852,8 +853,7 @@ int lats_vbd_set_lanes_per_vertex(lats_vbd* c, uint32_t lanes) {
         return fail(LATS_ERR_INVALID_ARGUMENT, "lanes per vertex must be 1, 4, 8 or 16");
     if (lanes != c->lanes) {
         c->lanes = lanes;
-        for (auto& g : c->graph)
-            if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+        c->drop_graphs();
     }
     return LATS_OK;
 }
@@ -931,8 +931,7 @@ int lats_vbd_autotune(lats_vbd* c, uint32_t frames, int* chosen, uint32_t* chose
         c->persist_blocks = k.blocks;
         if (c->lanes != k.lanes) {
             c->lanes = k.lanes;
-            for (auto& gr : c->graph)
-                if (gr) { (void)hipGraphExecDestroy(gr); gr = nullptr; }
+            c->drop_graphs();
         }
         int rc = lats_vbd_run_frames(c, 1, nullptr);   // warm-up / capture
         if (!rc) rc = lats_vbd_run_frames(c, frames, nullptr);
@@ -940,8 +939,7 @@ int lats_vbd_autotune(lats_vbd* c, uint32_t frames, int* chosen, uint32_t* chose
             c->strategy = keep_strategy;
             c->persist_blocks = keep_blocks;
             c->lanes = keep_lanes;
-            for (auto& gr : c->graph)
-                if (gr) { (void)hipGraphExecDestroy(gr); gr = nullptr; }
+            c->drop_graphs();
             restore();
             return rc;
         }
@@ -953,8 +951,7 @@ int lats_vbd_autotune(lats_vbd* c, uint32_t frames, int* chosen, uint32_t* chose
     c->persist_blocks = win.blocks;
     if (c->lanes != win.lanes) {
         c->lanes = win.lanes;
-        for (auto& gr : c->graph)
-            if (gr) { (void)hipGraphExecDestroy(gr); gr = nullptr; }
+        c->drop_graphs();
     }
     if (chosen) *chosen = win.strategy;
     if (chosen_blocks) *chosen_blocks = win.blocks;
@@ -965,8 +962,7 @@ int lats_vbd_set_energy_bending(lats_vbd* c, int on) {
     if (!c) return fail(LATS_ERR_INVALID_ARGUMENT, "null context");
     if ((on != 0) != (c->energy_bending != 0)) {
         c->energy_bending = on != 0;
-        for (auto& g : c->graph)
-            if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+        c->drop_graphs();
     }
     return LATS_OK;
 }
@@ -1255,9 +1251,23 @@ int lats_vbd_frame_begin(lats_vbd* c) {
 
 int lats_vbd_iterate(lats_vbd* c, uint32_t sweeps) {
     if (!c) return fail(LATS_ERR_INVALID_ARGUMENT, "null context");
-    for (uint32_t it = 0; it < sweeps; it++)
-        for (uint32_t col = 0; col < c->C; col++) enqueue_colour(c, c->x.p, col);
-    HIPCHK(hipGetLastError());
+    if (sweeps == 0) return LATS_OK;
+    // one graph per distinct sweep count (a handful in practice)
+    auto it = c->sweep_graphs.find(sweeps);
+    if (it == c->sweep_graphs.end()) {
+        if (c->sweep_graphs.size() >= 16) c->drop_graphs();
+        hipGraph_t graph;
+        HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        for (uint32_t k = 0; k < sweeps; k++)
+            for (uint32_t col = 0; col < c->C; col++) enqueue_colour(c, c->x.p, col);
+        HIPCHK(hipStreamEndCapture(c->stream, &graph));
+        hipGraphExec_t exec = nullptr;
+        hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) return fail(LATS_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        it = c->sweep_graphs.emplace(sweeps, exec).first;
+    }
+    HIPCHK(hipGraphLaunch(it->second, c->stream));
     return LATS_OK;
 }
 
@@ -1284,39 +1294,35 @@ int lats_vbd_energy_now(lats_vbd* c, float* e) {
 int lats_vbd_time_colour_kernels(lats_vbd* c, uint32_t iters, float* mean_ms, uint32_t* launches,
                                  uint64_t* vertex_updates) {
     if (!c || !mean_ms) return fail(LATS_ERR_INVALID_ARGUMENT, "null argument");
+    if (iters == 0) return fail(LATS_ERR_INVALID_ARGUMENT, "iterations must be positive");
     const size_t nf = 3 * (size_t)c->V;
-    HIPCHK(hipMemcpyAsync(c->scratch3.p, c->x.p, nf * 4, hipMemcpyDeviceToDevice, c->stream));
     const uint32_t nl = iters * c->C;
-    std::vector<hipEvent_t> ev(2 * (size_t)nl, nullptr);
-    int rc = LATS_OK;
-    for (auto& e : ev)
-        if (hipEventCreate(&e) != hipSuccess) { rc = fail(LATS_ERR_HIP, "hipEventCreate"); break; }
     uint64_t vu = 0;
-    if (!rc) {
-        uint32_t k = 0;
-        for (uint32_t it = 0; it < iters; it++)
-            for (uint32_t col = 0; col < c->C; col++, k++) {
-                (void)hipEventRecord(ev[2 * k], c->stream);
-                enqueue_colour(c, c->x.p, col);
-                (void)hipEventRecord(ev[2 * k + 1], c->stream);
-                vu += c->ccount[col];
-            }
-        hipError_t e = hipStreamSynchronize(c->stream);
-        if (e != hipSuccess) rc = fail(LATS_ERR_HIP, hipGetErrorString(e));
-    }
-    double tot = 0.0;
-    if (!rc)
-        for (uint32_t k = 0; k < nl; k++) {
-            float ms = 0.f;
-            (void)hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]);
-            tot += ms;
-        }
-    for (auto& e : ev)
-        if (e) (void)hipEventDestroy(e);
-    if (rc) return rc;
+    for (uint32_t col = 0; col < c->C; col++) vu += (uint64_t)c->ccount[col] * iters;
+    // the sweeps as one graph: launches run back to back as in a frame graph
+    hipGraph_t graph;
+    hipGraphExec_t exec = nullptr;
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    for (uint32_t it = 0; it < iters; it++)
+        for (uint32_t col = 0; col < c->C; col++) enqueue_colour(c, c->x.p, col);
+    HIPCHK(hipStreamEndCapture(c->stream, &graph));
+    hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return fail(LATS_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    float ms = 0.f;
+    HIPCHK(hipMemcpyAsync(c->scratch3.p, c->x.p, nf * 4, hipMemcpyDeviceToDevice, c->stream));
+    e = hipGraphLaunch(exec, c->stream);   // warm-up replay
+    if (e == hipSuccess) e = hipMemcpyAsync(c->x.p, c->scratch3.p, nf * 4, hipMemcpyDeviceToDevice, c->stream);
+    if (e == hipSuccess) e = hipEventRecord(c->ev0, c->stream);
+    if (e == hipSuccess) e = hipGraphLaunch(exec, c->stream);
+    if (e == hipSuccess) e = hipEventRecord(c->ev1, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    (void)hipGraphExecDestroy(exec);
+    if (e != hipSuccess) return fail(LATS_ERR_HIP, hipGetErrorString(e));
     HIPCHK(hipMemcpyAsync(c->x.p, c->scratch3.p, nf * 4, hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    *mean_ms = nl ? (float)(tot / nl) : 0.f;
+    *mean_ms = ms / (float)nl;
     if (launches) *launches = nl;
     if (vertex_updates) *vertex_updates = vu;
     return LATS_OK;
