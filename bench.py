@@ -86,6 +86,10 @@ def main():
                          "copy_calibration (a plain 256 MiB device copy, the reference for HBM counters)")
     ap.add_argument("--strategy", default="auto", choices=["auto", "graph", "persistent"])
     ap.add_argument("--lanes", type=int, default=0, help="lanes per vertex (1, 4, 8, 16); 0 = autotuned")
+    ap.add_argument("--no-warm-start", action="store_true", help="skip the learned warm-start measurement")
+    ap.add_argument("--warm-scenarios", type=int, default=8, help="training scenarios of the warm start")
+    ap.add_argument("--warm-frames", type=int, default=60, help="frames per training scenario")
+    ap.add_argument("--warm-record-every", type=int, default=1, help="record every k-th training frame")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the VBD hot path on a single MI355X "
@@ -178,7 +182,43 @@ def main():
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(cloth, params, wind)
+    if not args.no_warm_start:
+        out["warm_start"] = bench_warm_start(args, cloth, params, wind, plain_ms=gpu_ms / K)
     print(json.dumps(out), flush=True)
+
+
+def bench_warm_start(args, cloth, params, wind, plain_ms):
+    """Learned warm start on the same cloth and wind: per frame, wall time of
+    the warm-started solve until the energy reaches the plain solver's
+    100-iteration energy from the same frame start (computed on a second
+    context, outside the timed call), against the plain frame graph."""
+    import numpy as np
+    from libatssim_amd import configs
+    from libatssim_amd.warm_start import WarmStartMLP, evaluate, train_recipe
+    t0 = time.perf_counter()
+    scen = [configs.warm_c3_scenario(k, n=256, frames=args.warm_frames) for k in range(args.warm_scenarios)]
+    m = WarmStartMLP(2024)
+    steps = 2000
+    train_recipe(m, scen, params, args.warm_frames, plain_steps=steps, guided_rounds=3, guided_steps=steps,
+                 record_every=args.warm_record_every)
+    train_s = time.perf_counter() - t0
+    frames = len(wind)
+    res = evaluate(m, cloth, params, wind, frames)
+    wall = np.array([r["wall_ms"] for r in res])
+    sweeps = np.array([r["sweeps"] for r in res])
+    return {
+        "ms_per_frame_learned": round(float(wall.mean()), 4),
+        "ms_per_frame_plain": round(plain_ms, 4),
+        "speedup": round(plain_ms / float(wall.mean()), 3),
+        "frames": frames,
+        "mean_sweeps": round(float(sweeps.mean()), 2),
+        "max_sweeps": int(sweeps.max()),
+        "kept_fraction": round(float(np.mean([r["kept"] for r in res])), 3),
+        "target": "per-frame energy of the plain 100-iteration solve from the same frame start",
+        "training": {"scenarios": args.warm_scenarios, "frames": args.warm_frames,
+                     "record_every": args.warm_record_every, "guided_rounds": 3, "adam_steps": 4 * steps,
+                     "batch_rows": 16384, "rows": m.dataset_rows(), "seconds": round(train_s, 1)},
+    }
 
 
 def bench_batch(args):

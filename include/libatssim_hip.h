@@ -309,6 +309,10 @@ int lats_mlp_gradient_host(lats_mlp* m, const float* X, const float* T, uint32_t
  * bound one was).  Every frame call returns with the context's stream idle. */
 int lats_mlp_bind(lats_mlp* m, lats_vbd* ctx);
 int lats_mlp_reset_history(lats_mlp* m);
+/* The learned guess: the prediction smoothed `smooth_passes` times over the
+ * spring 1-ring ((own + ring mean) / 2, pinned neighbours as zero), and, with
+ * half_step != 0, retried at half length before the inertial fallback. */
+int lats_mlp_set_guess_options(lats_mlp* m, uint32_t smooth_passes, int half_step);
 /* Training rows: one plain frame (predict, all iterations, update) of the
  * bound context, recording its features and its correction as rows. */
 int lats_mlp_reserve_rows(lats_mlp* m, uint64_t rows);
@@ -325,18 +329,19 @@ int lats_mlp_fit_feature_scale(lats_mlp* m, float* scale48);
 int lats_mlp_train(lats_mlp* m, uint32_t steps, uint32_t batch_rows, float lr, uint64_t seed, double* mean_loss);
 /* One frame of the bound context: predict, learned guess (use_model != 0)
  * kept only if its energy is <= the inertial guess's, then colour sweeps
- * until the energy is <= target (target <= 0: run max_sweeps); the first
- * check comes after about as many sweeps as the previous frame needed, the
- * next ones every check_every sweeps; update.  out[6] = {sweeps, kept,
+ * until the energy is <= target (target <= 0: run max_sweeps), at most
+ * max_sweeps; a kept guess gets at least first_check sweeps, then a check
+ * every check_every; from the inertial guess the first check comes after
+ * the plain iteration count; update.  out[6] = {sweeps, kept,
  * E(inertial guess), E(learned guess), E(final), target}. */
-int lats_mlp_warm_start_frame(lats_mlp* m, lats_vbd* ctx, float target, uint32_t max_sweeps, uint32_t check_every,
-                              int use_model, float* out);
+int lats_mlp_warm_start_frame(lats_mlp* m, lats_vbd* ctx, float target, uint32_t max_sweeps, uint32_t first_check,
+                              uint32_t check_every, int use_model, float* out);
 /* The same frame, with the target set to the energy that `ref` (a second
  * context of the same mesh) reaches after its full iteration count from
  * ctx's frame start, recording ctx's features and ref's correction as
  * training rows (rows from the states the warm-started solver visits). */
-int lats_mlp_record_guided_frame(lats_mlp* m, lats_vbd* ctx, lats_vbd* ref, uint32_t max_sweeps, uint32_t check_every,
-                                 int use_model, float* out);
+int lats_mlp_record_guided_frame(lats_mlp* m, lats_vbd* ctx, lats_vbd* ref, uint32_t max_sweeps, uint32_t first_check,
+                                 uint32_t check_every, int use_model, float* out);
 
 #ifdef __cplusplus
 }

@@ -164,6 +164,7 @@ SIGNATURES = {
     "lats_mlp_gradient_host": (_I, [_VP, _VP, _VP, _U32, _VP, ctypes.POINTER(ctypes.c_double)]),
     "lats_mlp_bind": (_I, [_VP, _VP]),
     "lats_mlp_reset_history": (_I, [_VP]),
+    "lats_mlp_set_guess_options": (_I, [_VP, _U32, _I]),
     "lats_mlp_reserve_rows": (_I, [_VP, ctypes.c_uint64]),
     "lats_mlp_record_frame": (_I, [_VP, _VP]),
     "lats_mlp_dataset_append": (_I, [_VP, _VP, _VP, _U32]),
@@ -172,8 +173,8 @@ SIGNATURES = {
     "lats_mlp_dataset_clear": (_I, [_VP]),
     "lats_mlp_fit_feature_scale": (_I, [_VP, _VP]),
     "lats_mlp_train": (_I, [_VP, _U32, _U32, ctypes.c_float, ctypes.c_uint64, ctypes.POINTER(ctypes.c_double)]),
-    "lats_mlp_warm_start_frame": (_I, [_VP, _VP, ctypes.c_float, _U32, _U32, _I, _VP]),
-    "lats_mlp_record_guided_frame": (_I, [_VP, _VP, _VP, _U32, _U32, _I, _VP]),
+    "lats_mlp_warm_start_frame": (_I, [_VP, _VP, ctypes.c_float, _U32, _U32, _U32, _I, _VP]),
+    "lats_mlp_record_guided_frame": (_I, [_VP, _VP, _VP, _U32, _U32, _U32, _I, _VP]),
 }
 
 _LIB = None

@@ -436,12 +436,41 @@ __global__ __launch_bounds__(kThreads) void k_mlp_features(FeatArgs f) {
     for (int c = kIn; c < kInP; c++) out[c] = 0;
 }
 
-// x = x_pred + s0 * prediction
-__global__ void k_mlp_apply(float* x, const float* y, uint32_t nv, float s0, const uint32_t* meta) {
+// One smoothing pass of the predicted correction over the spring 1-ring:
+// out[s] = (y[s] + mean over unpinned spring neighbours of y) / 2 (pinned
+// neighbours count as zero, their correction is the inertial one).
+__global__ __launch_bounds__(kThreads) void k_mlp_smooth(const VbdArgs a, const ColourInfo* colours,
+                                                         const uint32_t* colour, const float* y, float* out,
+                                                         uint32_t nv) {
+    const uint32_t s = blockIdx.x * kThreads + threadIdx.x;
+    if (s >= nv) return;
+    const uint32_t edeg = a.meta[s] & 0xffu;
+    const ColourInfo co = colours[colour[s]];
+    const uint32_t i = s - co.start, n = co.count;
+    float m0 = 0.f, m1 = 0.f, m2 = 0.f;
+    for (uint32_t j = 0; j < edeg; j++) {
+        const uint32_t o = a.ell_e[co.eoff + j * n + i].x;
+        if ((a.meta[o] >> 16) & 1u) continue;
+        m0 += y[(size_t)o * 4];
+        m1 += y[(size_t)o * 4 + 1];
+        m2 += y[(size_t)o * 4 + 2];
+    }
+    const float w = edeg ? 0.5f / (float)edeg : 0.f;
+    const float self = edeg ? 0.5f : 1.f;
+    out[(size_t)s * 4] = self * y[(size_t)s * 4] + w * m0;
+    out[(size_t)s * 4 + 1] = self * y[(size_t)s * 4 + 1] + w * m1;
+    out[(size_t)s * 4 + 2] = self * y[(size_t)s * 4 + 2] + w * m2;
+    out[(size_t)s * 4 + 3] = 0.f;
+}
+
+// x = x_pred + alpha s0 y (pinned vertices keep the inertial guess)
+__global__ void k_mlp_apply_scaled(float* x, const float* xpred, const float* y, uint32_t nv, float alpha_s0,
+                                   const uint32_t* meta) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nv) return;
-    if ((meta[s] >> 16) & 1u) return;                             // pinned vertices keep the inertial guess
-    for (int c = 0; c < 3; c++) x[3 * s + c] = x[3 * s + c] + s0 * y[(size_t)s * 4 + c];
+    const bool pinned = (meta[s] >> 16) & 1u;
+    for (int c = 0; c < 3; c++)
+        x[3 * s + c] = pinned ? xpred[3 * s + c] : xpred[3 * s + c] + alpha_s0 * y[(size_t)s * 4 + c];
 }
 
 // targets of a frame: (x_end - x_pred) / s0
@@ -514,10 +543,11 @@ struct lats_mlp {
     // bound solver context and its per-vertex work buffers
     lats_vbd* bound = nullptr;
     uint64_t bound_uid = 0;                    // uid of the bound context (pointers can be reused)
-    uint32_t last_sweeps = 0;                  // sweeps of the bound context's previous frame
     float s0 = 0.01f;
     DBuf<uint16_t> X;
-    DBuf<float> Y, T, xpred, corr1, corr2;
+    DBuf<float> Y, Y2, T, xpred, corr1, corr2;
+    uint32_t smooth_passes = 0;                // 1-ring smoothing passes of the prediction
+    int half_step = 0;                         // retry at half the prediction before falling back
     DBuf<uint32_t> colour;
     // recorded training rows
     DBuf<uint16_t> dsX;
@@ -624,14 +654,14 @@ int copy_frame_state(lats_vbd* dst, lats_vbd* src) {
 }
 
 // One frame of the bound context c: predict; learned guess (use_model) kept
-// only if its energy is <= the inertial guess's; colour sweeps until the
-// energy is <= target (first check after about as many sweeps as the last
-// frame needed, then every check_every); history; update velocity.
+// only if its energy is <= the inertial guess's; at least first_check colour
+// sweeps from a kept guess, then sweeps until the energy is <= target
+// (checked every check_every); history; update velocity.
 // With ref: the target is the energy ref reaches after the full iteration
 // count from the same frame start, and the frame is recorded as training
 // rows (features of c, correction of ref's solve).
-int warm_frame(lats_mlp* m, lats_vbd* c, lats_vbd* ref, float target, uint32_t max_sweeps, uint32_t check_every,
-               int use_model, float* out) {
+int warm_frame(lats_mlp* m, lats_vbd* c, lats_vbd* ref, float target, uint32_t max_sweeps, uint32_t first_check,
+               uint32_t check_every, int use_model, float* out) {
     if (!check_every) check_every = 1;
     const uint32_t V = c->V;
     int rc;
@@ -669,18 +699,40 @@ int warm_frame(lats_mlp* m, lats_vbd* c, lats_vbd* ref, float target, uint32_t m
     e_warm = e_pred;
     if (use_model) {
         enqueue_forward(m, m->X.p, RowMap{}, V, m->Y.p, nullptr, nullptr, c->stream);
-        hipLaunchKernelGGL(k_mlp_apply, dim3(nblocks(V)), dim3(kThreads), 0, c->stream, c->x.p, m->Y.p, V, m->s0,
-                           c->meta.p);
+        float* y = m->Y.p;
+        for (uint32_t p = 0; p < m->smooth_passes; p++) {
+            float* o = (y == m->Y.p) ? m->Y2.p : m->Y.p;
+            hipLaunchKernelGGL(k_mlp_smooth, dim3(nblocks(V)), dim3(kThreads), 0, c->stream, c->args(c->x.p),
+                               c->colours_d.p, m->colour.p, y, o, V);
+            y = o;
+        }
+        hipLaunchKernelGGL(k_mlp_apply_scaled, dim3(nblocks(V)), dim3(kThreads), 0, c->stream, c->x.p, m->xpred.p, y,
+                           V, m->s0, c->meta.p);
         rc = lats_vbd_energy_now(c, &e_warm);
         if (rc) return rc;
         kept = e_warm <= e_pred;
+        if (!kept && m->half_step) {
+            hipLaunchKernelGGL(k_mlp_apply_scaled, dim3(nblocks(V)), dim3(kThreads), 0, c->stream, c->x.p,
+                               m->xpred.p, y, V, 0.5f * m->s0, c->meta.p);
+            float e_half = 0.f;
+            rc = lats_vbd_energy_now(c, &e_half);
+            if (rc) return rc;
+            if (e_half <= e_pred) {
+                kept = true;
+                e_warm = e_half;
+            }
+        }
         if (!kept)
             HIPCHK(hipMemcpyAsync(c->x.p, m->xpred.p, (size_t)3 * V * 4, hipMemcpyDeviceToDevice, c->stream));
     }
     float e = kept ? e_warm : e_pred;
     uint32_t sweeps = 0;
-    const uint32_t first = m->last_sweeps > 2 * check_every ? m->last_sweeps - check_every : check_every;
-    while (sweeps < max_sweeps && !(checks && e <= target)) {
+    // From the learned guess the first check comes after first_check sweeps
+    // (a few sweeps always smooth the guess's high-frequency error), then
+    // every check_every.  From the inertial guess the frame is the plain
+    // solve, so its first check comes after the plain iteration count.
+    const uint32_t first = kept ? std::max(first_check, 1u) : std::max(c->prm.constraint_iter_count, 1u);
+    while (sweeps < max_sweeps && !(checks && e <= target && sweeps >= (kept ? first_check : 0u))) {
         const uint32_t k = std::min(sweeps == 0 ? first : check_every, max_sweeps - sweeps);
         rc = lats_vbd_iterate(c, k);
         if (rc) return rc;
@@ -701,7 +753,6 @@ int warm_frame(lats_mlp* m, lats_vbd* c, lats_vbd* ref, float target, uint32_t m
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->stream));
     if (ref) m->ds_rows += V;
-    m->last_sweeps = sweeps;
     if (out) {
         out[0] = (float)sweeps;
         out[1] = kept ? 1.f : 0.f;
@@ -861,6 +912,7 @@ int lats_mlp_bind(lats_mlp* m, lats_vbd* c) {
     const uint32_t V = c->V;
     HIPCHK(m->X.alloc((size_t)V * kInP));
     HIPCHK(m->Y.alloc((size_t)V * 4));
+    HIPCHK(m->Y2.alloc((size_t)V * 4));
     HIPCHK(m->T.alloc((size_t)V * 4));
     HIPCHK(m->xpred.alloc((size_t)3 * V));
     HIPCHK(m->corr1.upload(std::vector<float>((size_t)3 * V, 0.f)));
@@ -877,7 +929,14 @@ int lats_mlp_bind(lats_mlp* m, lats_vbd* c) {
     m->s0 = (c->E && sum > 0.0) ? (float)(sum / c->E) : 0.01f;
     m->bound = c;
     m->bound_uid = c->uid;
-    m->last_sweeps = 0;
+    return LATS_OK;
+}
+
+int lats_mlp_set_guess_options(lats_mlp* m, uint32_t smooth_passes, int half_step) {
+    if (!m) return fail(LATS_ERR_INVALID_ARGUMENT, "null argument");
+    if (smooth_passes > 8) return fail(LATS_ERR_INVALID_ARGUMENT, "at most 8 smoothing passes");
+    m->smooth_passes = smooth_passes;
+    m->half_step = half_step != 0;
     return LATS_OK;
 }
 
@@ -1047,21 +1106,21 @@ int lats_mlp_train(lats_mlp* m, uint32_t steps, uint32_t batch_rows, float lr, u
     return LATS_OK;
 }
 
-int lats_mlp_warm_start_frame(lats_mlp* m, lats_vbd* c, float target, uint32_t max_sweeps, uint32_t check_every,
-                              int use_model, float* out) {
+int lats_mlp_warm_start_frame(lats_mlp* m, lats_vbd* c, float target, uint32_t max_sweeps, uint32_t first_check,
+                              uint32_t check_every, int use_model, float* out) {
     int rc = check_bound(m, c);
     if (rc) return rc;
-    return warm_frame(m, c, nullptr, target, max_sweeps, check_every, use_model, out);
+    return warm_frame(m, c, nullptr, target, max_sweeps, first_check, check_every, use_model, out);
 }
 
-int lats_mlp_record_guided_frame(lats_mlp* m, lats_vbd* c, lats_vbd* ref, uint32_t max_sweeps, uint32_t check_every,
-                                 int use_model, float* out) {
+int lats_mlp_record_guided_frame(lats_mlp* m, lats_vbd* c, lats_vbd* ref, uint32_t max_sweeps, uint32_t first_check,
+                                 uint32_t check_every, int use_model, float* out) {
     int rc = check_bound(m, c);
     if (rc) return rc;
     if (!ref || ref == c) return fail(LATS_ERR_INVALID_ARGUMENT, "a second context of the same mesh is needed");
     if (ref->V != c->V || ref->C != c->C || ref->perm != c->perm || ref->ccount != c->ccount)
         return fail(LATS_ERR_INVALID_ARGUMENT, "reference context is not built from the same mesh");
-    return warm_frame(m, c, ref, 0.f, max_sweeps, check_every, use_model, out);
+    return warm_frame(m, c, ref, 0.f, max_sweeps, first_check, check_every, use_model, out);
 }
 
 }  // extern "C"

@@ -48,6 +48,7 @@ struct lats_vbd {
     lats_scene_params prm{};
     float h = 0.f, h2inv = 0.f, damp = 0.f, h_energy = 0.f;
     std::vector<uint32_t> cstart, ccount, eoff, boff;
+    std::vector<lats::ColourInfo> colours;   // host copy of colours_d
     std::vector<uint32_t> perm, inv;
     int strategy = LATS_STRATEGY_AUTO;
     int energy_bending = 1;

@@ -46,18 +46,19 @@ constexpr int kBlock = 256;
 
 // One colour pass, G lanes per vertex (G = 1: one thread per vertex).
 template <int G>
-__global__ __launch_bounds__(kBlock) void k_vbd_colour(VbdArgs a, uint32_t cs, uint32_t n, uint32_t eoff,
-                                                       uint32_t boff) {
+__global__ __launch_bounds__(kBlock) void k_vbd_colour(VbdArgs a, ColourInfo ci) {
     const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
     const F3 ext = F3{a.fp->ext_cur[0], a.fp->ext_cur[1], a.fp->ext_cur[2]};
+    const uint32_t n = ci.count;
     if (G == 1) {
         if (t >= n) return;
-        vbd_vertex_ext(a, cs + t, t, n, eoff, boff, ext);
+        vbd_vertex_ext(a, ci.start + t, t, n, ci.eoff, ci.boff, ext);
     } else {
         const uint32_t i = t / G, lane = t % G;
         if (i - (i % (64 / G)) >= n) return;      // the whole wave is past the colour
         const bool valid = i < n;
-        vbd_vertex_group<G>(a, cs + (valid ? i : 0u), valid ? i : 0u, n, eoff, boff, ext, lane, valid);
+        vbd_vertex_group<G>(a, ci.start + (valid ? i : 0u), valid ? i : 0u, n, ci.eoff, ci.boff, ci.ew, ci.bw, ext,
+                            lane, valid);
     }
 }
 
@@ -231,7 +232,7 @@ __global__ __launch_bounds__(kBlock) void k_vbd_persistent(PersistArgs p) {
                     for (uint32_t i = t0 / G; i - (i % (64 / G)) < ci.count; i += stride / G) {
                         const bool valid = i < ci.count;
                         vbd_vertex_group<G>(p.a, ci.start + (valid ? i : 0u), valid ? i : 0u, ci.count, ci.eoff,
-                                            ci.boff, ext, t0 % G, valid);
+                                            ci.boff, ci.ew, ci.bw, ext, t0 % G, valid);
                     }
                 }
                 grid_barrier(p.barrier, target);
@@ -511,10 +512,10 @@ void enqueue_colour(lats_vbd* c, float* xb, uint32_t col) {
     const dim3 grid(nblocks(threads)), block(kBlock);
     const VbdArgs a = c->args(xb);
     switch (c->lanes) {
-        case 4: hipLaunchKernelGGL(k_vbd_colour<4>, grid, block, 0, c->stream, a, c->cstart[col], n, c->eoff[col], c->boff[col]); break;
-        case 8: hipLaunchKernelGGL(k_vbd_colour<8>, grid, block, 0, c->stream, a, c->cstart[col], n, c->eoff[col], c->boff[col]); break;
-        case 16: hipLaunchKernelGGL(k_vbd_colour<16>, grid, block, 0, c->stream, a, c->cstart[col], n, c->eoff[col], c->boff[col]); break;
-        default: hipLaunchKernelGGL(k_vbd_colour<1>, grid, block, 0, c->stream, a, c->cstart[col], n, c->eoff[col], c->boff[col]); break;
+        case 4: hipLaunchKernelGGL(k_vbd_colour<4>, grid, block, 0, c->stream, a, c->colours[col]); break;
+        case 8: hipLaunchKernelGGL(k_vbd_colour<8>, grid, block, 0, c->stream, a, c->colours[col]); break;
+        case 16: hipLaunchKernelGGL(k_vbd_colour<16>, grid, block, 0, c->stream, a, c->colours[col]); break;
+        default: hipLaunchKernelGGL(k_vbd_colour<1>, grid, block, 0, c->stream, a, c->colours[col]); break;
     }
 }
 
@@ -724,6 +725,7 @@ int lats_vbd_create(const lats_mesh_desc* m, const lats_scene_params* prm, lats_
     c->damp = std::exp(-prm->damping_cloth * c->h);
     c->perm = L.perm;
     c->inv = inv;
+    c->colours = L.colours;
     for (const ColourInfo& ci : L.colours) {
         c->cstart.push_back(ci.start);
         c->ccount.push_back(ci.count);

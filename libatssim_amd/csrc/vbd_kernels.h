@@ -11,9 +11,10 @@
 
 namespace lats {
 
-// One colour of one cloth: slots [start, start + count), ELL bases.
+// One colour of one cloth: slots [start, start + count), ELL bases and
+// ELL widths (largest spring / bending-pair degree in the colour).
 struct ColourInfo {
-    uint32_t start, count, eoff, boff;
+    uint32_t start, count, eoff, boff, ew, bw;
 };
 
 struct FrameParams {
@@ -127,6 +128,16 @@ __device__ __forceinline__ void vbd_vertex_ext(const VbdArgs& a, uint32_t s, uin
 }
 
 
+// One incident spring / bending pair term, zero for lanes past the degree.
+__device__ __forceinline__ void spring_term_of(const VbdArgs& a, F3 xk, bool on, uint2 e, F3 xo, Sym3& t, F3& tf) {
+    t = sym_zero();
+    tf = F3{0.f, 0.f, 0.f};
+    if (on) stretch_values(xk, xo, __uint_as_float(e.y), a.ks, t, tf);
+}
+__device__ __forceinline__ F3 pair_point(const VbdArgs& a, F3 xk, uint32_t o, uint32_t k, uint32_t id) {
+    return (o == k) ? xk : ld3(a.x, id);
+}
+
 // The same update with G lanes per vertex (lane = 0..G-1 inside an aligned
 // group of G lanes).  Lane j computes the terms of incident springs / pairs
 // j, j+G, ...; the terms are then added in incidence order through in-group
@@ -134,9 +145,32 @@ __device__ __forceinline__ void vbd_vertex_ext(const VbdArgs& a, uint32_t s, uin
 // is bit-identical to vbd_vertex_ext.  All G lanes of a group must call this
 // together (valid == false: the group has no vertex; it still takes part in
 // the shuffles and stores nothing).
+//
+// ew / bw are the colour's ELL widths (its largest spring / pair degree).
+// The ELL entries of the first PE spring rounds and PB pair rounds are loaded
+// before the vertex's degrees arrive (padding entries are valid slots), so
+// the neighbour gathers follow after one memory round trip instead of two.
 template <int G>
 __device__ __forceinline__ void vbd_vertex_group(const VbdArgs& a, uint32_t s, uint32_t i, uint32_t n, uint32_t eoff,
-                                                 uint32_t boff, F3 ext, uint32_t lane, bool valid) {
+                                                 uint32_t boff, uint32_t ew, uint32_t bw, F3 ext, uint32_t lane,
+                                                 bool valid) {
+    constexpr uint32_t PE = (8 + G - 1) / G;    // spring rounds prefetched (degrees <= 8)
+    constexpr uint32_t PB = (16 + G - 1) / G;   // pair rounds prefetched (degrees <= 16)
+    uint2 pe[PE];
+    float4 pq[PB];
+    uint4 pi[PB];
+#pragma unroll
+    for (uint32_t k = 0; k < PE; k++) {
+        const uint32_t j = k * G + lane;
+        pe[k] = (valid && j < ew) ? a.ell_e[eoff + j * n + i] : uint2{0u, 0u};
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < PB; k++) {
+        const uint32_t j = k * G + lane;
+        const bool on = valid && j < bw;
+        pq[k] = on ? a.ell_bq[boff + j * n + i] : float4{0.f, 0.f, 0.f, 0.f};
+        pi[k] = on ? a.ell_bi[boff + j * n + i] : uint4{0u, 0u, 0u, 0u};
+    }
     const F3 xk = ld3(a.x, s);
     const F3 x0 = ld3(a.x0, s);
     const F3 v0 = ld3(a.v, s);
@@ -144,20 +178,28 @@ __device__ __forceinline__ void vbd_vertex_group(const VbdArgs& a, uint32_t s, u
     const uint32_t meta = valid ? a.meta[s] : 0u;
     const uint32_t edeg = meta & 0xffu, bdeg = (meta >> 8) & 0xffu;
 
+    // neighbour gathers of the prefetched rounds
+    F3 pxo[PE];
+#pragma unroll
+    for (uint32_t k = 0; k < PE; k++) pxo[k] = (k * G + lane < edeg) ? ld3(a.x, pe[k].x) : F3{0.f, 0.f, 0.f};
+    F3 pp[PB][4];
+#pragma unroll
+    for (uint32_t k = 0; k < PB; k++) {
+        const bool on = k * G + lane < bdeg;
+        const uint32_t o = pi[k].x >> 30;
+        pp[k][0] = on ? pair_point(a, xk, o, 0, pi[k].x & 0x3fffffffu) : xk;
+        pp[k][1] = on ? pair_point(a, xk, o, 1, pi[k].y) : xk;
+        pp[k][2] = on ? pair_point(a, xk, o, 2, pi[k].z) : xk;
+        pp[k][3] = on ? pair_point(a, xk, o, 3, pi[k].w) : xk;
+    }
+
     Sym3 H;
     F3 f;
     inertia_term(xk, x0, v0, m, (meta >> 16) & 1u, a.g, ext, a.h, a.h2inv, H, f);
 
     Sym3 S = sym_zero();
     F3 fs = F3{0.f, 0.f, 0.f};
-    for (uint32_t r = 0; r < edeg; r += G) {
-        const uint32_t j = r + lane;
-        Sym3 t = sym_zero();
-        F3 tf = F3{0.f, 0.f, 0.f};
-        if (j < edeg) {
-            const uint2 e = a.ell_e[eoff + j * n + i];
-            stretch_values(xk, ld3(a.x, e.x), __uint_as_float(e.y), a.ks, t, tf);
-        }
+    auto add_spring_round = [&](uint32_t r, const Sym3& t, const F3& tf) {
         const uint32_t cnt = (edeg - r < (uint32_t)G) ? edeg - r : (uint32_t)G;
         for (uint32_t q = 0; q < cnt; q++) {
             S.xx = S.xx + __shfl(t.xx, (int)q, G);
@@ -170,13 +212,49 @@ __device__ __forceinline__ void vbd_vertex_group(const VbdArgs& a, uint32_t s, u
             fs.y = fs.y + __shfl(tf.y, (int)q, G);
             fs.z = fs.z + __shfl(tf.z, (int)q, G);
         }
+    };
+#pragma unroll
+    for (uint32_t k = 0; k < PE; k++) {
+        const uint32_t r = k * G;
+        if (r >= edeg) break;
+        Sym3 t;
+        F3 tf;
+        spring_term_of(a, xk, r + lane < edeg, pe[k], pxo[k], t, tf);
+        add_spring_round(r, t, tf);
+    }
+    for (uint32_t r = PE * G; r < edeg; r += G) {
+        const uint32_t j = r + lane;
+        const bool on = j < edeg;
+        const uint2 e = on ? a.ell_e[eoff + j * n + i] : uint2{0u, 0u};
+        Sym3 t;
+        F3 tf;
+        spring_term_of(a, xk, on, e, on ? ld3(a.x, e.x) : F3{0.f, 0.f, 0.f}, t, tf);
+        add_spring_round(r, t, tf);
     }
     sym_add(H, S);
     f = add3(f, fs);
 
     float hb = 0.f;
     F3 fb = F3{0.f, 0.f, 0.f};
-    for (uint32_t r = 0; r < bdeg; r += G) {
+    auto add_pair_round = [&](uint32_t r, float th, const F3& tf) {
+        const uint32_t cnt = (bdeg - r < (uint32_t)G) ? bdeg - r : (uint32_t)G;
+        for (uint32_t q = 0; q < cnt; q++) {
+            hb = hb + __shfl(th, (int)q, G);
+            fb.x = fb.x + __shfl(tf.x, (int)q, G);
+            fb.y = fb.y + __shfl(tf.y, (int)q, G);
+            fb.z = fb.z + __shfl(tf.z, (int)q, G);
+        }
+    };
+#pragma unroll
+    for (uint32_t k = 0; k < PB; k++) {
+        const uint32_t r = k * G;
+        if (r >= bdeg) break;
+        float th = 0.f;
+        F3 tf = F3{0.f, 0.f, 0.f};
+        if (r + lane < bdeg) bending_term(pq[k], pi[k].x >> 30, pp[k][0], pp[k][1], pp[k][2], pp[k][3], a.kb, th, tf);
+        add_pair_round(r, th, tf);
+    }
+    for (uint32_t r = PB * G; r < bdeg; r += G) {
         const uint32_t j = r + lane;
         float th = 0.f;
         F3 tf = F3{0.f, 0.f, 0.f};
@@ -185,19 +263,10 @@ __device__ __forceinline__ void vbd_vertex_group(const VbdArgs& a, uint32_t s, u
             const float4 q = a.ell_bq[k];
             const uint4 id = a.ell_bi[k];
             const uint32_t o = id.x >> 30;
-            const F3 p0 = (o == 0) ? xk : ld3(a.x, id.x & 0x3fffffffu);
-            const F3 p1 = (o == 1) ? xk : ld3(a.x, id.y);
-            const F3 p2 = (o == 2) ? xk : ld3(a.x, id.z);
-            const F3 p3 = (o == 3) ? xk : ld3(a.x, id.w);
-            bending_term(q, o, p0, p1, p2, p3, a.kb, th, tf);
+            bending_term(q, o, pair_point(a, xk, o, 0, id.x & 0x3fffffffu), pair_point(a, xk, o, 1, id.y),
+                         pair_point(a, xk, o, 2, id.z), pair_point(a, xk, o, 3, id.w), a.kb, th, tf);
         }
-        const uint32_t cnt = (bdeg - r < (uint32_t)G) ? bdeg - r : (uint32_t)G;
-        for (uint32_t q = 0; q < cnt; q++) {
-            hb = hb + __shfl(th, (int)q, G);
-            fb.x = fb.x + __shfl(tf.x, (int)q, G);
-            fb.y = fb.y + __shfl(tf.y, (int)q, G);
-            fb.z = fb.z + __shfl(tf.z, (int)q, G);
-        }
+        add_pair_round(r, th, tf);
     }
     H.xx = H.xx + hb;
     H.yy = H.yy + hb;

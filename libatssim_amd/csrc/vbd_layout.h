@@ -91,7 +91,7 @@ inline bool append_cloth(const lats_mesh_desc& m, ClothLayout& L, std::vector<ui
             err = "adjacency too large";
             return false;
         }
-        L.colours.push_back(ColourInfo{start, n, (uint32_t)eoff, (uint32_t)boff});
+        L.colours.push_back(ColourInfo{start, n, (uint32_t)eoff, (uint32_t)boff, emax, bmax});
         L.ell_e.resize(eoff + (size_t)emax * n, uint2{0, 0});
         L.ell_bq.resize(boff + (size_t)bmax * n, float4{0, 0, 0, 0});
         L.ell_bi.resize(boff + (size_t)bmax * n, uint4{0, 0, 0, 0});

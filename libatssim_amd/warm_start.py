@@ -30,6 +30,8 @@ from ._lib import check, lib
 NUM_FEATURES = 48
 NUM_OUTPUTS = 3
 HIDDEN = 128
+FIRST_CHECK = 4     # sweeps a kept learned guess always gets
+CHECK_EVERY = 2     # sweeps between energy checks after that
 
 # offsets of the parameter blocks in the flat float[43408] vector
 OFF_W1 = 0
@@ -126,6 +128,10 @@ class WarmStartMLP:
         check(lib().lats_mlp_bind(self._h, gpu._h))
         self._bound = gpu
 
+    def set_guess_options(self, smooth_passes=0, half_step=False):
+        """1-ring smoothing passes of the prediction; half-length retry before the fallback."""
+        check(lib().lats_mlp_set_guess_options(self._h, int(smooth_passes), int(bool(half_step))))
+
     def reset_history(self):
         check(lib().lats_mlp_reset_history(self._h))
 
@@ -168,21 +174,24 @@ class WarmStartMLP:
         check(lib().lats_mlp_train(self._h, int(steps), int(batch_rows), float(lr), int(seed), ctypes.byref(loss)))
         return loss.value
 
-    def warm_start_frame(self, gpu, target=0.0, max_sweeps=100, check_every=5, use_model=True):
+    def warm_start_frame(self, gpu, target=0.0, max_sweeps=100, first_check=FIRST_CHECK, check_every=CHECK_EVERY,
+                         use_model=True):
         """One frame of `gpu` from the learned guess (if it lowers the energy)
-        or the inertial guess, sweeping until the energy reaches `target`."""
+        or the inertial guess, sweeping until the energy reaches `target`
+        (a kept guess gets at least first_check sweeps)."""
         out = np.zeros(6, np.float32)
-        check(lib().lats_mlp_warm_start_frame(self._h, gpu._h, float(target), int(max_sweeps), int(check_every),
-                                              int(bool(use_model)), _p(out)))
+        check(lib().lats_mlp_warm_start_frame(self._h, gpu._h, float(target), int(max_sweeps), int(first_check),
+                                              int(check_every), int(bool(use_model)), _p(out)))
         return _frame_result(out)
 
-    def record_guided_frame(self, gpu, ref, max_sweeps=200, check_every=5, use_model=True):
+    def record_guided_frame(self, gpu, ref, max_sweeps=200, first_check=FIRST_CHECK, check_every=CHECK_EVERY,
+                            use_model=True):
         """One warm-started frame of `gpu` whose target comes from the plain
         solve of `ref` (same mesh) from the same frame start; the frame is
         recorded as training rows (features of gpu, correction of ref)."""
         out = np.zeros(6, np.float32)
-        check(lib().lats_mlp_record_guided_frame(self._h, gpu._h, ref._h, int(max_sweeps), int(check_every),
-                                                 int(bool(use_model)), _p(out)))
+        check(lib().lats_mlp_record_guided_frame(self._h, gpu._h, ref._h, int(max_sweeps), int(first_check),
+                                                 int(check_every), int(bool(use_model)), _p(out)))
         return _frame_result(out)
 
 
@@ -191,11 +200,12 @@ def _frame_result(out):
             "e_learned": float(out[3]), "e_final": float(out[4]), "target": float(out[5])}
 
 
-def record_scenarios(mlp, scenarios, params, frames, lanes=8):
-    """Plain-solver trajectories -> training rows.  scenarios: [(cloth, wind)]."""
+def record_scenarios(mlp, scenarios, params, frames, lanes=8, record_every=1):
+    """Plain-solver trajectories -> training rows (every record_every-th
+    frame is recorded, the others are plain frames).  scenarios: [(cloth, wind)]."""
     from .vbd import GpuVBD
     from . import _lib
-    total = sum(c.num_verts for c, _ in scenarios) * frames
+    total = sum(c.num_verts for c, _ in scenarios) * ((frames + record_every - 1) // record_every)
     mlp.reserve_rows(mlp.dataset_rows() + total)
     for cloth, wind in scenarios:
         g = GpuVBD(cloth, params, _lib.LATS_STRATEGY_GRAPH)
@@ -203,19 +213,23 @@ def record_scenarios(mlp, scenarios, params, frames, lanes=8):
         mlp.bind(g)
         for f in range(frames):
             g.set_external_force(wind[f % len(wind)])
-            mlp.record_frame(g)
+            if f % record_every == 0:
+                mlp.record_frame(g)
+            else:
+                mlp.warm_start_frame(g, 0.0, params.constraint_iter_count, use_model=False)
         g.synchronize()
         g.close()
     return mlp.dataset_rows()
 
 
-def record_guided(mlp, scenarios, params, frames, lanes=8, check_every=5, use_model=True):
+def record_guided(mlp, scenarios, params, frames, lanes=8, first_check=FIRST_CHECK, check_every=CHECK_EVERY,
+                  use_model=True, record_every=1):
     """Rows from the states the warm-started solver visits: each scenario is
-    advanced by warm-started frames, each frame's target is the plain solve
-    from the same frame start (a second context)."""
+    advanced by warm-started frames, each recorded frame's target is the
+    plain solve from the same frame start (a second context)."""
     from .vbd import GpuVBD
     from . import _lib
-    total = sum(c.num_verts for c, _ in scenarios) * frames
+    total = sum(c.num_verts for c, _ in scenarios) * ((frames + record_every - 1) // record_every)
     mlp.reserve_rows(mlp.dataset_rows() + total)
     stats = []
     for cloth, wind in scenarios:
@@ -226,26 +240,40 @@ def record_guided(mlp, scenarios, params, frames, lanes=8, check_every=5, use_mo
         mlp.bind(run)
         for f in range(frames):
             run.set_external_force(wind[f % len(wind)])
-            stats.append(mlp.record_guided_frame(run, ref, 2 * params.constraint_iter_count, check_every, use_model))
+            if f % record_every == 0:
+                stats.append(mlp.record_guided_frame(run, ref, 2 * params.constraint_iter_count, first_check,
+                                                     check_every, use_model))
+            else:
+                # target of an unrecorded frame: the plain solve on ref, as when recording
+                x, v = run.state()
+                ref.set_state(x, v)
+                ref.set_external_force(wind[f % len(wind)])
+                ref.frame_begin()
+                ref.iterate(params.constraint_iter_count)
+                stats.append(mlp.warm_start_frame(run, ref.energy_now(), 2 * params.constraint_iter_count,
+                                                  first_check, check_every, use_model))
         run.close()
         ref.close()
     return stats
 
 
-def train_recipe(mlp, scenarios, params, frames, plain_steps=2000, guided_rounds=1, guided_steps=2000,
-                 batch_rows=16384, lr=1e-3, seed=0, log=None):
+def train_recipe(mlp, scenarios, params, frames, plain_steps=2000, guided_rounds=3, guided_steps=2000,
+                 batch_rows=16384, lr=1e-3, seed=0, smooth_passes=1, half_step=True, record_every=1, log=None):
     """Plain trajectories -> feature scales -> Adam; then rounds of rows from
     warm-started trajectories (targets from the plain solve) -> Adam on all
-    rows.  Returns the mean loss of the last chunk of steps."""
+    rows (the states a warm-started solver visits differ from the plain
+    solver's, so a network trained on plain rows alone degrades over a run).
+    Returns the mean loss of the last chunk of steps."""
     import time
     t0 = time.perf_counter()
-    record_scenarios(mlp, scenarios, params, frames)
+    mlp.set_guess_options(smooth_passes, half_step)
+    record_scenarios(mlp, scenarios, params, frames, record_every=record_every)
     mlp.fit_feature_scale()
     loss = _train_chunks(mlp, plain_steps, batch_rows, lr, seed)
     if log:
         log(f"plain rows {mlp.dataset_rows()} loss {loss:.3e} ({time.perf_counter() - t0:.1f} s)")
     for r in range(guided_rounds):
-        stats = record_guided(mlp, scenarios, params, frames)
+        stats = record_guided(mlp, scenarios, params, frames, record_every=record_every)
         loss = _train_chunks(mlp, guided_steps, batch_rows, lr, seed + 1000 * (r + 1))
         if log:
             sw = [s["sweeps"] for s in stats]
@@ -265,7 +293,8 @@ def _train_chunks(mlp, steps, batch_rows, lr, seed, chunk=250):
     return loss
 
 
-def evaluate(mlp, cloth, params, wind, frames, check_every=5, use_model=True, lanes=8, max_sweeps=None):
+def evaluate(mlp, cloth, params, wind, frames, first_check=FIRST_CHECK, check_every=CHECK_EVERY, use_model=True,
+             lanes=8, max_sweeps=None):
     """Frames of `cloth` solved from the learned guess until each frame's
     energy reaches the plain solver's energy after its full iteration count
     from the same frame start (computed on a second context).
@@ -293,7 +322,7 @@ def evaluate(mlp, cloth, params, wind, frames, check_every=5, use_model=True, la
         run.set_external_force(w)
         run.synchronize()
         t0 = time.perf_counter()
-        r = mlp.warm_start_frame(run, target, max_sweeps, check_every, use_model)
+        r = mlp.warm_start_frame(run, target, max_sweeps, first_check, check_every, use_model)
         r["wall_ms"] = (time.perf_counter() - t0) * 1e3
         r["target"] = target
         out.append(r)

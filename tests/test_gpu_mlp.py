@@ -9,7 +9,7 @@ from oracle import mlp_oracle as O
 from libatssim_amd import GpuVBD, Params, configs
 from libatssim_amd import _lib
 from libatssim_amd.warm_start import (NUM_PARAMS, OFF_B1, OFF_B4, OFF_W1, OFF_W4, WarmStartMLP, evaluate,
-                                      record_scenarios, unpack_params)
+                                      train_recipe, unpack_params)
 
 pytestmark = pytest.mark.gpu
 
@@ -154,7 +154,7 @@ def test_fallback_reproduces_plain_solver(gpu_available):
     for f in range(4):
         run.set_external_force(wind[f])
         plain.set_external_force(wind[f])
-        r = m.warm_start_frame(run, 0.0, params.constraint_iter_count, params.constraint_iter_count, True)
+        r = m.warm_start_frame(run, 0.0, params.constraint_iter_count, 4, 2, True)
         plain.physics_step()
         assert not r["kept"] and r["e_learned"] > r["e_inertial"]
         assert r["sweeps"] == params.constraint_iter_count
@@ -274,18 +274,15 @@ def test_features_match_numpy(gpu_available):
 def test_trained_warm_start_needs_fewer_sweeps(gpu_available):
     """What the trained network gains: sweeps to the plain solver's
     100-iteration energy on frames of a scenario it was not trained on."""
-    _case("trained warm start gain (48x48, 4 scenarios x 40 frames)")
+    _case("trained warm start gain (48x48, 4 scenarios x 40 frames, 2 guided rounds)")
     n, iters = 48, 100
     params = Params(constraint_iter_count=iters)
     scen = [configs.warm_c3_scenario(k, n=n, frames=40) for k in range(4)]
     m = WarmStartMLP(2024)
-    record_scenarios(m, scen, params, 40)
-    m.fit_feature_scale()
-    for k in range(6):
-        m.train(250, 4096, 1e-3 if k < 4 else 3e-4, 100 + k)
+    train_recipe(m, scen, params, 40, plain_steps=1000, guided_rounds=2, guided_steps=1000, batch_rows=4096)
     cloth, wind = configs.warm_c3_scenario(9, n=n, frames=30)
-    learned = evaluate(m, cloth, params, wind, 30, check_every=5, use_model=True)
-    plain = evaluate(m, cloth, params, wind, 30, check_every=5, use_model=False)
+    learned = evaluate(m, cloth, params, wind, 30, use_model=True)
+    plain = evaluate(m, cloth, params, wind, 30, use_model=False)
     sl = np.array([r["sweeps"] for r in learned[5:]])
     sp = np.array([r["sweeps"] for r in plain[5:]])
     print("sweeps learned", sl.tolist(), "plain", sp.tolist(), flush=True)
