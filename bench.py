@@ -81,7 +81,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", default="grid_c1", choices=["grid_c1", "batch_c2", "copy_calibration"],
+    ap.add_argument("--workload", default="grid_c1", choices=["grid_c1", "batch_c2", "copy_calibration",
+                                                               "mlp_kernels"],
                     help="grid_c1 (default, the metric's workload), batch_c2 (512 cloths) or "
                          "copy_calibration (a plain 256 MiB device copy, the reference for HBM counters)")
     ap.add_argument("--strategy", default="auto", choices=["auto", "graph", "persistent"])
@@ -102,6 +103,8 @@ def main():
         return bench_batch(args)
     if args.workload == "copy_calibration":
         return copy_calibration(args)
+    if args.workload == "mlp_kernels":
+        return bench_mlp(args)
 
     K, W = args.steps, args.warmup
     cloth, params, _, wind = configs.grid_c1(frames=W + K)
@@ -187,6 +190,34 @@ def main():
     print(json.dumps(out), flush=True)
 
 
+BF16_DENSE_PEAK_TFLOPS = 2500.0
+
+
+def _mlp_roofline(rows, ms):
+    """The MFMA forward kernel (k_mlp_forward): algorithmic FLOPs of `rows`
+    rows (unpadded 48-128-128-128-3) / its mean duration, vs dense BF16 peak."""
+    from libatssim_amd.warm_start import FLOPS_PER_ROW
+    tflops = rows * FLOPS_PER_ROW / (ms * 1e-3) / 1e12
+    return {"kernel": "k_mlp_forward", "bound": "mfma", "rows": rows, "mean_launch_us": round(ms * 1e3, 3),
+            "achieved": round(tflops, 2), "peak": BF16_DENSE_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(tflops / BF16_DENSE_PEAK_TFLOPS, 5)}
+
+
+def bench_mlp(args):
+    """The warm-start network's kernels alone on 65,536 random rows (the
+    256x256 cloth's vertex count): inference forward and a training step."""
+    import numpy as np
+    from libatssim_amd.warm_start import WarmStartMLP
+    rows = 65536
+    m = WarmStartMLP(2024)
+    rng = np.random.default_rng(0)
+    m.append_rows(rng.standard_normal((rows, 48)).astype(np.float32),
+                  rng.standard_normal((rows, 3)).astype(np.float32) * 0.05)
+    fwd_ms, step_ms = m.time_kernels(rows, args.steps)
+    print(json.dumps({"workload": "mlp_kernels", "rows": rows, "forward": _mlp_roofline(rows, fwd_ms),
+                      "train_step_ms": round(step_ms, 4), "iters": args.steps}), flush=True)
+
+
 def bench_warm_start(args, cloth, params, wind, plain_ms):
     """Learned warm start on the same cloth and wind: per frame, wall time of
     the warm-started solve until the energy reaches the plain solver's
@@ -202,8 +233,10 @@ def bench_warm_start(args, cloth, params, wind, plain_ms):
     train_recipe(m, scen, params, args.warm_frames, plain_steps=steps, guided_rounds=3, guided_steps=steps,
                  record_every=args.warm_record_every)
     train_s = time.perf_counter() - t0
-    frames = len(wind)
-    res = evaluate(m, cloth, params, wind, frames)
+    full_wind = configs.grid_c1(frames=60)[3]
+    frames = len(full_wind)
+    res = evaluate(m, cloth, params, full_wind, frames)
+    fwd_ms, step_ms = m.time_kernels(cloth.num_verts, 20)   # after the evaluation: timing steps train
     wall = np.array([r["wall_ms"] for r in res])
     sweeps = np.array([r["sweeps"] for r in res])
     return {
@@ -215,6 +248,8 @@ def bench_warm_start(args, cloth, params, wind, plain_ms):
         "max_sweeps": int(sweeps.max()),
         "kept_fraction": round(float(np.mean([r["kept"] for r in res])), 3),
         "target": "per-frame energy of the plain 100-iteration solve from the same frame start",
+        "mlp_forward": _mlp_roofline(cloth.num_verts, fwd_ms),
+        "mlp_train_step_ms": round(step_ms, 4),
         "training": {"scenarios": args.warm_scenarios, "frames": args.warm_frames,
                      "record_every": args.warm_record_every, "guided_rounds": 3, "adam_steps": 4 * steps,
                      "batch_rows": 16384, "rows": m.dataset_rows(), "seconds": round(train_s, 1)},

@@ -327,6 +327,11 @@ int lats_mlp_fit_feature_scale(lats_mlp* m, float* scale48);
 /* `steps` Adam steps (lr, beta 0.9 / 0.999) on batches of batch_rows rows
  * drawn with replacement from splitmix64(seed + step). */
 int lats_mlp_train(lats_mlp* m, uint32_t steps, uint32_t batch_rows, float lr, uint64_t seed, double* mean_loss);
+/* Measurement: mean duration (HIP events on the network's stream) of one
+ * inference forward pass over the first `rows` recorded rows, and of one
+ * whole training step on them (forward, backward, reduction, Adam at
+ * lr 1e-4, BF16 weight packing: the parameters change). */
+int lats_mlp_time_kernels(lats_mlp* m, uint32_t rows, uint32_t iters, float* forward_ms, float* step_ms);
 /* One frame of the bound context: predict, learned guess (use_model != 0)
  * kept only if its energy is <= the inertial guess's, then colour sweeps
  * until the energy is <= target (target <= 0: run max_sweeps), at most

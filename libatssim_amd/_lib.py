@@ -172,6 +172,7 @@ SIGNATURES = {
     "lats_mlp_dataset_read": (_I, [_VP, ctypes.c_uint64, _U32, _VP, _VP]),
     "lats_mlp_dataset_clear": (_I, [_VP]),
     "lats_mlp_fit_feature_scale": (_I, [_VP, _VP]),
+    "lats_mlp_time_kernels": (_I, [_VP, _U32, _U32, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]),
     "lats_mlp_train": (_I, [_VP, _U32, _U32, ctypes.c_float, ctypes.c_uint64, ctypes.POINTER(ctypes.c_double)]),
     "lats_mlp_warm_start_frame": (_I, [_VP, _VP, ctypes.c_float, _U32, _U32, _U32, _I, _VP]),
     "lats_mlp_record_guided_frame": (_I, [_VP, _VP, _VP, _U32, _U32, _U32, _I, _VP]),

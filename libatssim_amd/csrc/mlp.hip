@@ -1077,6 +1077,47 @@ int lats_mlp_fit_feature_scale(lats_mlp* m, float* scale48) {
     return LATS_OK;
 }
 
+int lats_mlp_time_kernels(lats_mlp* m, uint32_t rows, uint32_t iters, float* forward_ms, float* step_ms) {
+    if (!m || !forward_ms || !step_ms) return fail(LATS_ERR_INVALID_ARGUMENT, "null argument");
+    if (rows == 0 || iters == 0 || rows > m->ds_rows)
+        return fail(LATS_ERR_INVALID_ARGUMENT, "rows must be in [1, recorded rows] and iters positive");
+    int rc = ensure_train_buffers(m, rows);
+    if (rc) return rc;
+    hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventCreate(&e2));
+    // forward passes over the first `rows` recorded rows (inference: no Z / H)
+    enqueue_forward(m, m->dsX.p, RowMap{}, rows, m->tY.p, nullptr, nullptr, m->stream);   // warm-up
+    hipError_t e = hipEventRecord(e0, m->stream);
+    for (uint32_t k = 0; k < iters && e == hipSuccess; k++)
+        enqueue_forward(m, m->dsX.p, RowMap{}, rows, m->tY.p, nullptr, nullptr, m->stream);
+    if (e == hipSuccess) e = hipEventRecord(e1, m->stream);
+    // whole training steps (forward with Z / H, backward, reduction, Adam, weight packing)
+    const float b1 = 0.9f, b2 = 0.999f;
+    for (uint32_t k = 0; k < iters && e == hipSuccess; k++) {
+        enqueue_gradient(m, m->dsX.p, m->dsT.p, RowMap{}, rows, m->tY.p, m->tZ.p, m->tH.p, m->stream);
+        m->step++;
+        const float c1 = 1.f - std::pow(b1, (float)m->step), c2 = 1.f - std::pow(b2, (float)m->step);
+        hipLaunchKernelGGL(k_mlp_adam, dim3((kNParam + 255) / 256), dim3(256), 0, m->stream, m->P.p, m->Mom.p,
+                           m->Vel.p, m->G.p, 1e-4f, b1, b2, 1e-8f, c1, c2);
+        enqueue_pack(m, m->stream);
+    }
+    if (e == hipSuccess) e = hipEventRecord(e2, m->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
+    float a = 0.f, b = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&a, e0, e1);
+    if (e == hipSuccess) e = hipEventElapsedTime(&b, e1, e2);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipEventDestroy(e2);
+    if (e != hipSuccess) return fail(LATS_ERR_HIP, hipGetErrorString(e));
+    HIPCHK(hipGetLastError());
+    *forward_ms = a / (float)iters;
+    *step_ms = b / (float)iters;
+    return LATS_OK;
+}
+
 int lats_mlp_train(lats_mlp* m, uint32_t steps, uint32_t batch_rows, float lr, uint64_t seed, double* mean_loss) {
     if (!m) return fail(LATS_ERR_INVALID_ARGUMENT, "null argument");
     if (m->ds_rows == 0) return fail(LATS_ERR_INVALID_ARGUMENT, "no recorded rows");

@@ -43,6 +43,8 @@ OFF_B3 = OFF_W3 + HIDDEN * HIDDEN
 OFF_W4 = OFF_B3 + HIDDEN
 OFF_B4 = OFF_W4 + HIDDEN * 16
 NUM_PARAMS = OFF_B4 + 16
+# multiply-adds x 2 of one row's forward pass (unpadded layer sizes)
+FLOPS_PER_ROW = 2 * (NUM_FEATURES * HIDDEN + 2 * HIDDEN * HIDDEN + HIDDEN * NUM_OUTPUTS)
 
 
 def _p(a):
@@ -168,6 +170,13 @@ class WarmStartMLP:
         s = np.zeros(NUM_FEATURES, np.float32)
         check(lib().lats_mlp_fit_feature_scale(self._h, _p(s)))
         return s
+
+    def time_kernels(self, rows, iters=20):
+        """(ms of one inference forward over `rows` recorded rows, ms of one training step on them)."""
+        f = ctypes.c_float()
+        t = ctypes.c_float()
+        check(lib().lats_mlp_time_kernels(self._h, int(rows), int(iters), ctypes.byref(f), ctypes.byref(t)))
+        return f.value, t.value
 
     def train(self, steps, batch_rows=8192, lr=1e-3, seed=0):
         loss = ctypes.c_double()
