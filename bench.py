@@ -85,7 +85,8 @@ def main():
                                                                "mlp_kernels"],
                     help="grid_c1 (default, the metric's workload), batch_c2 (512 cloths) or "
                          "copy_calibration (a plain 256 MiB device copy, the reference for HBM counters)")
-    ap.add_argument("--strategy", default="auto", choices=["auto", "graph", "persistent"])
+    ap.add_argument("--strategy", default="auto", choices=["auto", "graph", "persistent", "flow"])
+    ap.add_argument("--tiles", type=int, default=0, help="flow strategy: vertex tiles (0 = default)")
     ap.add_argument("--lanes", type=int, default=0, help="lanes per vertex (1, 4, 8, 16); 0 = autotuned")
     ap.add_argument("--no-warm-start", action="store_true", help="skip the learned warm-start measurement")
     ap.add_argument("--warm-scenarios", type=int, default=8, help="training scenarios of the warm start")
@@ -113,8 +114,9 @@ def main():
     if args.strategy == "auto":
         strategy, blocks = g.autotune(frames=2)
     else:
-        strategy = _lib.LATS_STRATEGY_GRAPH if args.strategy == "graph" else _lib.LATS_STRATEGY_PERSISTENT
-        g.set_strategy(strategy)
+        strategy = {"graph": _lib.LATS_STRATEGY_GRAPH, "persistent": _lib.LATS_STRATEGY_PERSISTENT,
+                    "flow": _lib.LATS_STRATEGY_FLOW}[args.strategy]
+        g.set_strategy(strategy, persistent_blocks=args.tiles if args.strategy == "flow" else None)
         blocks = g.strategy()[1]
     if args.lanes:
         g.set_lanes_per_vertex(args.lanes)
@@ -132,10 +134,10 @@ def main():
     value = vu / wall
 
     b_sweep = algorithmic_bytes_per_sweep(t.num_verts, t.num_edges, t.num_bending)
-    if strategy == _lib.LATS_STRATEGY_PERSISTENT:
+    if strategy in (_lib.LATS_STRATEGY_PERSISTENT, _lib.LATS_STRATEGY_FLOW):
         # one cooperative launch runs the K timed frames: its duration is the
         # HIP-event time of the timed region on the solver's stream
-        kernel = "k_vbd_persistent"
+        kernel = "k_vbd_persistent" if strategy == _lib.LATS_STRATEGY_PERSISTENT else "k_vbd_flow"
         mean_ms = gpu_ms
         bytes_per_launch = b_sweep * iters * K
     else:
@@ -176,8 +178,9 @@ def main():
                         "top row pinned, seeded wind, 100 VBD iterations/frame, one frame per step",
             "colours": t.num_clusters,
             "iterations": iters,
-            "strategy": "persistent" if strategy == _lib.LATS_STRATEGY_PERSISTENT else "graph",
-            "persistent_blocks": blocks,
+            "strategy": {_lib.LATS_STRATEGY_PERSISTENT: "persistent", _lib.LATS_STRATEGY_FLOW: "flow"}.get(
+                strategy, "graph"),
+            "persistent_blocks_or_tiles": blocks,
             "lanes_per_vertex": g.lanes_per_vertex(),
             "gpu_ms_per_step": gpu_ms / K,
         },

@@ -116,6 +116,8 @@ typedef struct lats_vbd lats_vbd;
 #define LATS_STRATEGY_AUTO 0        /* pick the fastest for the mesh                    */
 #define LATS_STRATEGY_GRAPH 1       /* one kernel per colour, frame captured in a hipGraph */
 #define LATS_STRATEGY_PERSISTENT 2  /* one cooperative kernel per frame, grid barrier per colour */
+#define LATS_STRATEGY_FLOW 3        /* one cooperative kernel per run of frames, tiles of vertices hand
+                                       each colour pass on to their neighbouring tiles (no grid barrier) */
 
 /* Creates the context on the current HIP device; uploads everything. */
 int lats_vbd_create(const lats_mesh_desc* mesh, const lats_scene_params* params, lats_vbd** out);
@@ -127,12 +129,13 @@ int lats_vbd_set_strategy(lats_vbd* ctx, int strategy);
 int lats_vbd_set_lanes_per_vertex(lats_vbd* ctx, uint32_t lanes);
 int lats_vbd_get_lanes_per_vertex(lats_vbd* ctx, uint32_t* lanes);
 /* Grid size of the persistent strategy (0: one thread per vertex of the
- * largest colour, capped at the co-resident limit). */
+ * largest colour, capped at the co-resident limit); for the flow strategy
+ * the number of vertex tiles, one block each (0: min(256, V / 128)). */
 int lats_vbd_set_persistent_blocks(lats_vbd* ctx, uint32_t blocks);
 int lats_vbd_get_strategy(lats_vbd* ctx, int* strategy, uint32_t* persistent_blocks);
-/* Autotuner: times `frames` frames with the graph strategy and with the
- * persistent strategy at several grid sizes, for 1/4/8/16 lanes per vertex,
- * keeps the fastest for later
+/* Autotuner: times `frames` frames with the graph strategy, the
+ * persistent strategy at several grid sizes and the flow strategy at
+ * several tile counts, for 1/4/8/16 lanes per vertex, keeps the fastest for later
  * lats_vbd_physics_step / lats_vbd_run_frames calls, and restores the state. */
 int lats_vbd_autotune(lats_vbd* ctx, uint32_t frames, int* chosen_strategy, uint32_t* chosen_blocks);
 /* Include the bending term in the recorded energy (the reference's GPU

@@ -29,6 +29,7 @@ LATS_TOPO_COLOUR = 7
 LATS_STRATEGY_AUTO = 0
 LATS_STRATEGY_GRAPH = 1
 LATS_STRATEGY_PERSISTENT = 2
+LATS_STRATEGY_FLOW = 3
 
 LATS_FN_ADDITIONAL_ROOT = 0
 LATS_FN_ADDITIONAL_TERMINAL = 1

@@ -88,6 +88,12 @@ struct lats_vbd {
     uint32_t persist_max_lanes = 0;
     uint32_t persist_max_blocks = 0;  // co-residency limit (queried on first use)
 
+    // flow strategy: tiles, neighbour lists, per-tile phase counters
+    uint32_t flow_tiles = 0;          // tiles of the built layout (0 = none)
+    uint32_t flow_base = 0;           // phase every tile has completed
+    lats::DBuf<uint2> flow_range;
+    lats::DBuf<uint32_t> flow_nb_off, flow_nb, flow_progress, flow_error;
+
     // graph per (record) flag; graphs of k colour sweeps (lats_vbd_iterate)
     hipGraphExec_t graph[2] = {nullptr, nullptr};
     std::map<uint32_t, hipGraphExec_t> sweep_graphs;

@@ -258,6 +258,149 @@ __global__ __launch_bounds__(kBlock) void k_vbd_persistent(PersistArgs p) {
     }
 }
 
+// --- frame-resident colour passes with neighbour hand-off ("flow") ---------
+// One cooperative launch runs whole frames.  The cloth is cut into tiles of
+// consecutive vertex ids, one block each.  Phase q of a launch (predict,
+// then every (iteration, colour) pass) may start on a tile once every
+// neighbouring tile (a tile holding a vertex that this tile's updates read,
+// or that reads this tile's vertices) has completed phase q - 1: per-tile
+// progress counters replace the grid barrier.  That is enough for the
+// Gauss-Seidel order: a neighbour's vertices of an earlier colour are already
+// updated, those of a later colour are not yet (a neighbour cannot start
+// q + 1 before this tile completed q), so every update reads exactly what the
+// colour-by-colour kernel sequence reads and the result is bit-identical.
+// Positions, x_start and velocity move through device-scope (L2-coherent
+// across XCDs) loads and stores; a tile publishes a phase after its stores
+// have completed.  Waits are bounded: on a time-out the block raises the
+// error word and leaves, and the others leave when they see it.
+struct FlowArgs {
+    VbdArgs a;                 // a.x positions (x0, v read through a as well)
+    float* x0;
+    float* v;
+    const FrameParams* fp;
+    const ColourInfo* colours;
+    const uint2* trange;       // [tiles][colours]: index range inside the colour
+    const uint32_t* nb_off;    // [tiles + 1]
+    const uint32_t* nb;        // neighbouring tiles
+    uint32_t* progress;        // [tiles]: last phase the tile completed
+    uint32_t* error;           // 1 after a wait timed out
+    uint32_t base;             // phase every tile completed before this launch
+    uint32_t ncolours, iters, nframes;
+    float damp;
+};
+
+constexpr uint32_t kFlowSpinLimit = 1u << 22;
+
+__device__ __forceinline__ uint32_t ld_agent(const uint32_t* p) {
+    return __hip_atomic_load(const_cast<uint32_t*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <int G, class Mem>
+__global__ __launch_bounds__(kBlock) void k_vbd_flow(FlowArgs p) {
+    __shared__ uint32_t s_abort;
+    const uint32_t t = blockIdx.x, tid = threadIdx.x, C = p.ncolours;
+    if (tid == 0) s_abort = 0u;
+    __syncthreads();
+    const uint32_t nb0 = p.nb_off[t], nb1 = p.nb_off[t + 1];
+    const uint32_t frame0 = p.fp->frame;   // advanced only after the launch (k_flow_finish)
+    // every neighbouring tile has completed phase `need` (false: give up)
+    auto wait_for = [&](uint32_t need) -> bool {
+        for (uint32_t j = nb0 + tid; j < nb1; j += kBlock) {
+            const uint32_t* flag = p.progress + p.nb[j];
+            uint32_t spins = 0;
+            while (ld_agent(flag) < need) {
+                if (++spins > kFlowSpinLimit || ld_agent(p.error) != 0u) {
+                    __hip_atomic_store(p.error, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    s_abort = 1u;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(2);
+            }
+        }
+        __syncthreads();
+        return s_abort == 0u;
+    };
+    // this tile completed phase `q`: its coherent stores are done, then the flag
+    auto publish = [&](uint32_t q) {
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        __builtin_amdgcn_s_waitcnt(0);
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        __syncthreads();
+        if (tid == 0) __hip_atomic_store(p.progress + t, q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    };
+    uint32_t q = p.base;
+    for (uint32_t f = 0; f < p.nframes; f++) {
+        const float* src = p.fp->forces ? p.fp->forces + 3 * (size_t)(frame0 + f) : p.fp->ext;
+        const F3 ext = F3{src[0], src[1], src[2]};
+        // predict (Core::predict_position): own vertices, after the neighbours
+        // finished reading them in the previous frame's last pass
+        if (!wait_for(q)) return;
+        for (uint32_t c = 0; c < C; c++) {
+            const uint2 r = p.trange[t * C + c];
+            const uint32_t cs = p.colours[c].start;
+            for (uint32_t i = r.x + tid; i < r.y; i += kBlock) {
+                const uint32_t s = cs + i;
+                F3 vpred = add3(Mem::ld(p.v, s), scale3(p.a.h, F3{0.f, -9.8f, 0.f}));
+                if ((p.a.meta[s] >> 16) & 1u) vpred = F3{0.f, 0.f, 0.f};
+                Mem::st(p.a.x, s, add3(Mem::ld(p.x0, s), scale3(p.a.h, vpred)));
+            }
+        }
+        publish(++q);
+        for (uint32_t it = 0; it < p.iters; it++) {
+            for (uint32_t c = 0; c < C; c++) {
+                if (!wait_for(q)) return;
+                const ColourInfo ci = p.colours[c];
+                const uint2 r = p.trange[t * C + c];
+                if (G == 1) {
+                    for (uint32_t i = r.x + tid; i < r.y; i += kBlock)
+                        vbd_vertex_ext<Mem>(p.a, ci.start + i, i, ci.count, ci.eoff, ci.boff, ext);
+                } else {
+                    // groups of G lanes; a wave leaves together
+                    for (uint32_t i0 = r.x; i0 < r.y; i0 += kBlock / G) {
+                        const uint32_t i = i0 + tid / G;
+                        if (i0 + (tid / 64) * (64 / G) >= r.y) break;
+                        const bool valid = i < r.y;
+                        vbd_vertex_group<G, Mem>(p.a, ci.start + (valid ? i : r.x), valid ? i : r.x, ci.count,
+                                                         ci.eoff, ci.boff, ci.ew, ci.bw, ext, tid % G, valid);
+                    }
+                }
+                publish(++q);
+            }
+        }
+        // update velocity (Core::update_velocity): own vertices only; no
+        // other tile reads x_start or v
+        for (uint32_t c = 0; c < C; c++) {
+            const uint2 r = p.trange[t * C + c];
+            const uint32_t cs = p.colours[c].start;
+            for (uint32_t i = r.x + tid; i < r.y; i += kBlock) {
+                const uint32_t s = cs + i;
+                const F3 xk = Mem::ld(p.a.x, s);
+                const F3 dx = sub3(xk, Mem::ld(p.x0, s));
+                F3 vel = F3{dx.x / p.a.h, dx.y / p.a.h, dx.z / p.a.h};
+                vel = F3{vel.x * p.damp, vel.y * p.damp, vel.z * p.damp};
+                Mem::st(p.v, s, vel);
+                Mem::st(p.x0, s, xk);
+            }
+        }
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        __builtin_amdgcn_s_waitcnt(0);
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        __syncthreads();
+    }
+}
+
+// After a flow launch: advance the frame counter and publish the last
+// frame's external force (as the per-frame kernels leave them).
+__global__ void k_flow_finish(FrameParams* fp, uint32_t nframes) {
+    if (threadIdx.x != 0) return;
+    const uint32_t last = fp->frame + nframes - 1;
+    const float* src = fp->forces ? fp->forces + 3 * (size_t)last : fp->ext;
+    fp->ext_cur[0] = src[0];
+    fp->ext_cur[1] = src[1];
+    fp->ext_cur[2] = src[2];
+    fp->frame = fp->frame + nframes;
+}
+
 __global__ void k_zero_u32(uint32_t* p, uint32_t n) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < n) p[t] = 0u;
@@ -641,6 +784,137 @@ int run_persistent(lats_vbd* c, uint32_t nframes) {
     return LATS_OK;
 }
 
+const void* flow_kernel(uint32_t lanes) {
+    switch (lanes) {
+        case 4: return (const void*)k_vbd_flow<4, CoherentMem>;
+        case 8: return (const void*)k_vbd_flow<8, CoherentMem>;
+        case 16: return (const void*)k_vbd_flow<16, CoherentMem>;
+        default: return (const void*)k_vbd_flow<1, CoherentMem>;
+    }
+}
+
+// Most blocks of the flow kernel that can be co-resident (cooperative launch).
+int flow_capacity(lats_vbd* c, uint32_t* cap) {
+    int per_cu = 0, ncu = 0;
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, flow_kernel(c->lanes), kBlock, 0));
+    HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device));
+    *cap = (uint32_t)std::max(0, per_cu * ncu);
+    return LATS_OK;
+}
+
+// Tiles of consecutive vertex ids: per (tile, colour) the index range inside
+// the colour (a colour's slots are in ascending vertex id, so a tile's
+// vertices of one colour are contiguous) and the neighbouring tiles (any
+// tile sharing a spring or a bending pair with it).
+int flow_setup(lats_vbd* c, uint32_t tiles) {
+    const uint32_t V = c->V, C = c->C;
+    tiles = std::max(1u, std::min(tiles, V));
+    const uint32_t tv = (V + tiles - 1) / tiles;
+    tiles = (V + tv - 1) / tv;
+    if (c->flow_tiles == tiles) return LATS_OK;
+    HIPCHK(hipStreamSynchronize(c->stream));   // the old counters may still be in use
+    std::vector<uint2> range((size_t)tiles * C);
+    for (uint32_t k = 0; k < C; k++) {
+        const uint32_t* ids = c->perm.data() + c->cstart[k];
+        const uint32_t n = c->ccount[k];
+        for (uint32_t t = 0; t < tiles; t++) {
+            const uint32_t lo = (uint32_t)(std::lower_bound(ids, ids + n, t * tv) - ids);
+            const uint32_t hi = (uint32_t)(std::lower_bound(ids, ids + n, (t + 1) * tv) - ids);
+            range[(size_t)t * C + k] = make_uint2(lo, hi);
+        }
+    }
+    std::vector<uint2> es(c->E);
+    std::vector<uint4> bs(c->B);
+    if (c->E) HIPCHK(hipMemcpy(es.data(), c->e_slots.p, c->E * sizeof(uint2), hipMemcpyDeviceToHost));
+    if (c->B) HIPCHK(hipMemcpy(bs.data(), c->b_slots.p, c->B * sizeof(uint4), hipMemcpyDeviceToHost));
+    auto tile_of = [&](uint32_t slot) { return c->perm[slot] / tv; };
+    std::vector<std::vector<uint32_t>> nbs(tiles);
+    auto link = [&](uint32_t a, uint32_t b) {
+        if (a != b) {
+            nbs[a].push_back(b);
+            nbs[b].push_back(a);
+        }
+    };
+    for (const uint2& e : es) link(tile_of(e.x), tile_of(e.y));
+    for (const uint4& b : bs) {
+        const uint32_t q[4] = {tile_of(b.x), tile_of(b.y), tile_of(b.z), tile_of(b.w)};
+        for (int i = 0; i < 4; i++)
+            for (int j = i + 1; j < 4; j++) link(q[i], q[j]);
+    }
+    std::vector<uint32_t> off(tiles + 1, 0), nb;
+    for (uint32_t t = 0; t < tiles; t++) {
+        std::sort(nbs[t].begin(), nbs[t].end());
+        nbs[t].erase(std::unique(nbs[t].begin(), nbs[t].end()), nbs[t].end());
+        off[t + 1] = off[t] + (uint32_t)nbs[t].size();
+        nb.insert(nb.end(), nbs[t].begin(), nbs[t].end());
+    }
+    if (nb.empty()) nb.push_back(0);
+    HIPCHK(c->flow_range.upload(range));
+    HIPCHK(c->flow_nb_off.upload(off));
+    HIPCHK(c->flow_nb.upload(nb));
+    HIPCHK(c->flow_progress.upload(std::vector<uint32_t>(tiles, 0u)));
+    HIPCHK(c->flow_error.upload(std::vector<uint32_t>(1, 0u)));
+    c->flow_tiles = tiles;
+    c->flow_base = 0;
+    return LATS_OK;
+}
+
+int run_flow(lats_vbd* c, uint32_t nframes) {
+    if (!nframes) return LATS_OK;
+    if (c->prm.num_substep != 1) return fail(LATS_ERR_UNSUPPORTED, "flow strategy runs one substep per frame");
+    if (c->has_obstacle) return fail(LATS_ERR_UNSUPPORTED, "flow strategy runs cloths without an obstacle");
+    uint32_t cap = 0;
+    int rc = flow_capacity(c, &cap);
+    if (rc) return rc;
+    const uint32_t want = c->persist_blocks ? c->persist_blocks : std::min(256u, std::max(1u, c->V / 128));
+    if (want > cap) return fail(LATS_ERR_UNSUPPORTED, "flow tiles exceed the co-resident blocks of this device");
+    rc = flow_setup(c, want);
+    if (rc) return rc;
+    const uint64_t per_frame = 1 + (uint64_t)c->prm.constraint_iter_count * c->C;
+    const uint32_t chunk = (uint32_t)std::max<uint64_t>(1, (1ull << 30) / per_frame);
+    FlowArgs fa;
+    fa.a = c->args(c->x.p);
+    fa.x0 = c->x0.p;
+    fa.v = c->v.p;
+    fa.fp = c->fp.p;
+    fa.colours = c->colours_d.p;
+    fa.trange = c->flow_range.p;
+    fa.nb_off = c->flow_nb_off.p;
+    fa.nb = c->flow_nb.p;
+    fa.progress = c->flow_progress.p;
+    fa.error = c->flow_error.p;
+    fa.ncolours = c->C;
+    fa.iters = c->prm.constraint_iter_count;
+    fa.damp = c->damp;
+    for (uint32_t done = 0; done < nframes;) {
+        fa.nframes = std::min(chunk, nframes - done);
+        if ((uint64_t)c->flow_base + fa.nframes * per_frame >= (1ull << 31)) {
+            hipLaunchKernelGGL(k_zero_u32, dim3(nblocks(c->flow_tiles)), dim3(kBlock), 0, c->stream,
+                               c->flow_progress.p, c->flow_tiles);
+            c->flow_base = 0;
+        }
+        fa.base = c->flow_base;
+        void* kargs[] = {&fa};
+        HIPCHK(hipLaunchCooperativeKernel(flow_kernel(c->lanes), dim3(c->flow_tiles), dim3(kBlock), kargs, 0,
+                                          c->stream));
+        hipLaunchKernelGGL(k_flow_finish, dim3(1), dim3(64), 0, c->stream, c->fp.p, fa.nframes);
+        c->flow_base += (uint32_t)(fa.nframes * per_frame);
+        done += fa.nframes;
+    }
+    return LATS_OK;
+}
+
+// After a synchronisation: a flow launch whose wait timed out leaves an
+// error word; report it and reset the tile counters.
+int flow_check(lats_vbd* c) {
+    if (c->strategy != LATS_STRATEGY_FLOW || !c->flow_tiles) return LATS_OK;
+    uint32_t err = 0;
+    HIPCHK(hipMemcpy(&err, c->flow_error.p, sizeof(err), hipMemcpyDeviceToHost));
+    if (!err) return LATS_OK;
+    c->flow_tiles = 0;   // next run rebuilds counters from zero
+    return fail(LATS_ERR_HIP, "flow strategy: a tile waited too long for its neighbours (frame results invalid)");
+}
+
 int get_graph(lats_vbd* c, bool record, hipGraphExec_t* out) {
     if (record) {
         int rc = ensure_energy_rows(c, c->prm.constraint_iter_count + 1);
@@ -665,6 +939,7 @@ int get_graph(lats_vbd* c, bool record, hipGraphExec_t* out) {
 
 int run_frame(lats_vbd* c, bool record) {
     if (!record && c->strategy == LATS_STRATEGY_PERSISTENT) return run_persistent(c, 1);
+    if (!record && c->strategy == LATS_STRATEGY_FLOW) return run_flow(c, 1);
     hipGraphExec_t g;
     int rc = get_graph(c, record, &g);
     if (rc) return rc;
@@ -789,10 +1064,10 @@ int lats_vbd_destroy(lats_vbd* c) {
 
 int lats_vbd_set_strategy(lats_vbd* c, int strategy) {
     if (!c) return fail(LATS_ERR_INVALID_ARGUMENT, "null context");
-    if (strategy < LATS_STRATEGY_AUTO || strategy > LATS_STRATEGY_PERSISTENT)
+    if (strategy < LATS_STRATEGY_AUTO || strategy > LATS_STRATEGY_FLOW)
         return fail(LATS_ERR_INVALID_ARGUMENT, "unknown strategy");
-    if (strategy == LATS_STRATEGY_PERSISTENT && c->prm.num_substep != 1)
-        return fail(LATS_ERR_UNSUPPORTED, "persistent strategy runs one substep per frame");
+    if ((strategy == LATS_STRATEGY_PERSISTENT || strategy == LATS_STRATEGY_FLOW) && c->prm.num_substep != 1)
+        return fail(LATS_ERR_UNSUPPORTED, "persistent and flow strategies run one substep per frame");
     c->strategy = strategy;
     return LATS_OK;
 }
@@ -922,6 +1197,13 @@ int lats_vbd_autotune(lats_vbd* c, uint32_t frames, int* chosen, uint32_t* chose
         if (rc) { c->lanes = keep_lanes; return rc; }
         for (uint32_t g : {g0 / 2, g0, 2 * g0, c->persist_max_blocks})
             if (g >= 1 && g <= c->persist_max_blocks) cands.push_back({LATS_STRATEGY_PERSISTENT, g, lanes});
+        if (lanes == 1 || c->has_obstacle) continue;
+        uint32_t fcap = 0;
+        c->lanes = lanes;
+        rc = flow_capacity(c, &fcap);
+        if (rc) { c->lanes = keep_lanes; return rc; }
+        for (uint32_t tiles : {128u, 256u, 512u, 1024u})
+            if (tiles <= fcap && tiles * 32u <= c->V) cands.push_back({LATS_STRATEGY_FLOW, tiles, lanes});
     }
     c->lanes = keep_lanes;
     float best = 3.4e38f;
@@ -1013,7 +1295,7 @@ int lats_vbd_physics_step(lats_vbd* c, int record) {
     HIPCHK(hipEventRecord(c->ev1, c->stream));
     HIPCHK(hipEventSynchronize(c->ev1));
     HIPCHK(hipEventElapsedTime(&c->last_ms, c->ev0, c->ev1));
-    return LATS_OK;
+    return flow_check(c);
 }
 
 int lats_vbd_get_energy(lats_vbd* c, float* out, uint32_t cap, uint32_t* count) {
@@ -1038,6 +1320,10 @@ int lats_vbd_run_frames(lats_vbd* c, uint32_t nframes, const float* forces) {
         HIPCHK(hipEventRecord(c->ev0, c->stream));
         rc = run_persistent(c, nframes);
         if (rc) return rc;
+    } else if (c->strategy == LATS_STRATEGY_FLOW) {
+        HIPCHK(hipEventRecord(c->ev0, c->stream));
+        rc = run_flow(c, nframes);
+        if (rc) return rc;
     } else {
         hipGraphExec_t g;
         rc = get_graph(c, false, &g);
@@ -1048,6 +1334,8 @@ int lats_vbd_run_frames(lats_vbd* c, uint32_t nframes, const float* forces) {
     HIPCHK(hipEventRecord(c->ev1, c->stream));
     HIPCHK(hipEventSynchronize(c->ev1));
     HIPCHK(hipEventElapsedTime(&c->last_ms, c->ev0, c->ev1));
+    rc = flow_check(c);
+    if (rc) return rc;
     if (forces) {
         rc = set_frame_params(c, nullptr, 0, forces + 3 * (size_t)(nframes ? nframes - 1 : 0));
         if (rc) return rc;

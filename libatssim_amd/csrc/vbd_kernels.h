@@ -52,16 +52,43 @@ __device__ __forceinline__ void st3(float* p, uint32_t s, F3 a) {
     p[3 * s + 2] = a.z;
 }
 
+// How a vertex update reads and writes the per-vertex state (x, x_start, v).
+// PlainMem: ordinary cached accesses (kernel boundaries order the colour
+// passes).  CoherentMem: device-scope relaxed atomic accesses, coherent
+// across the L2s of all XCDs without cache maintenance, for kernels in
+// which blocks hand positions to each other inside one launch.
+struct PlainMem {
+    static __device__ __forceinline__ F3 ld(const float* p, uint32_t s) { return ld3(p, s); }
+    static __device__ __forceinline__ void st(float* p, uint32_t s, F3 a) { st3(p, s, a); }
+};
+template <int Scope>
+struct ScopedMem {
+    static __device__ __forceinline__ float ld1(const float* p) {
+        return __hip_atomic_load(const_cast<float*>(p), __ATOMIC_RELAXED, Scope);
+    }
+    static __device__ __forceinline__ void st1(float* p, float v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, Scope); }
+    static __device__ __forceinline__ F3 ld(const float* p, uint32_t s) {
+        return F3{ld1(p + 3 * s), ld1(p + 3 * s + 1), ld1(p + 3 * s + 2)};
+    }
+    static __device__ __forceinline__ void st(float* p, uint32_t s, F3 a) {
+        st1(p + 3 * s, a.x);
+        st1(p + 3 * s + 1, a.y);
+        st1(p + 3 * s + 2, a.z);
+    }
+};
+using CoherentMem = ScopedMem<__HIP_MEMORY_SCOPE_AGENT>;
+
 // One VBD vertex update: VBD::compute_inertia + compute_stretch_mass_spring +
 // compute_bending_quadratic (xpbd_constraints.h:4027-4205) summed as
 // Hf = inertia; Hf += sum(stretch); Hf += sum(bending)
 // (example2_async_iter.cpp:939-992), then the guarded 3x3 Newton step of
 // vbd_step (:993-1014).  i = index inside the colour, n = colour size.
+template <class Mem = PlainMem>
 __device__ __forceinline__ void vbd_vertex_ext(const VbdArgs& a, uint32_t s, uint32_t i, uint32_t n, uint32_t eoff,
                                                uint32_t boff, F3 ext) {
-    const F3 xk = ld3(a.x, s);
-    const F3 x0 = ld3(a.x0, s);
-    const F3 v0 = ld3(a.v, s);
+    const F3 xk = Mem::ld(a.x, s);
+    const F3 x0 = Mem::ld(a.x0, s);
+    const F3 v0 = Mem::ld(a.v, s);
     const float m = a.mass[s];
     const uint32_t meta = a.meta[s];
     const uint32_t edeg = meta & 0xffu, bdeg = (meta >> 8) & 0xffu;
@@ -76,7 +103,7 @@ __device__ __forceinline__ void vbd_vertex_ext(const VbdArgs& a, uint32_t s, uin
     F3 fs = F3{0.f, 0.f, 0.f};
     for (uint32_t j = 0; j < edeg; j++) {
         const uint2 e = a.ell_e[eoff + j * n + i];
-        const F3 xo = ld3(a.x, e.x);
+        const F3 xo = Mem::ld(a.x, e.x);
         stretch_term(xk, xo, __uint_as_float(e.y), a.ks, S, fs);
     }
     sym_add(H, S);
@@ -90,10 +117,10 @@ __device__ __forceinline__ void vbd_vertex_ext(const VbdArgs& a, uint32_t s, uin
         const float4 q = a.ell_bq[k];
         const uint4 id = a.ell_bi[k];
         const uint32_t o = id.x >> 30;
-        const F3 p0 = (o == 0) ? xk : ld3(a.x, id.x & 0x3fffffffu);
-        const F3 p1 = (o == 1) ? xk : ld3(a.x, id.y);
-        const F3 p2 = (o == 2) ? xk : ld3(a.x, id.z);
-        const F3 p3 = (o == 3) ? xk : ld3(a.x, id.w);
+        const F3 p0 = (o == 0) ? xk : Mem::ld(a.x, id.x & 0x3fffffffu);
+        const F3 p1 = (o == 1) ? xk : Mem::ld(a.x, id.y);
+        const F3 p2 = (o == 2) ? xk : Mem::ld(a.x, id.z);
+        const F3 p3 = (o == 3) ? xk : Mem::ld(a.x, id.w);
         bending_term(q, o, p0, p1, p2, p3, a.kb, hb, fb);
     }
     H.xx = H.xx + hb;
@@ -118,13 +145,13 @@ __device__ __forceinline__ void vbd_vertex_ext(const VbdArgs& a, uint32_t s, uin
                                    {H.xz + Hc[2][0], H.yz + Hc[2][1], H.zz + Hc[2][2]}};
             f = add3(f, fc);
             F3 dx;
-            if (newton_step_full(M, f, dx)) st3(a.x, s, add3(xk, dx));
+            if (newton_step_full(M, f, dx)) Mem::st(a.x, s, add3(xk, dx));
             return;
         }
     }
 
     F3 dx;
-    if (newton_step(H, f, dx)) st3(a.x, s, add3(xk, dx));
+    if (newton_step(H, f, dx)) Mem::st(a.x, s, add3(xk, dx));
 }
 
 
@@ -134,8 +161,9 @@ __device__ __forceinline__ void spring_term_of(const VbdArgs& a, F3 xk, bool on,
     tf = F3{0.f, 0.f, 0.f};
     if (on) stretch_values(xk, xo, __uint_as_float(e.y), a.ks, t, tf);
 }
+template <class Mem>
 __device__ __forceinline__ F3 pair_point(const VbdArgs& a, F3 xk, uint32_t o, uint32_t k, uint32_t id) {
-    return (o == k) ? xk : ld3(a.x, id);
+    return (o == k) ? xk : Mem::ld(a.x, id);
 }
 
 // The same update with G lanes per vertex (lane = 0..G-1 inside an aligned
@@ -150,7 +178,7 @@ __device__ __forceinline__ F3 pair_point(const VbdArgs& a, F3 xk, uint32_t o, ui
 // The ELL entries of the first PE spring rounds and PB pair rounds are loaded
 // before the vertex's degrees arrive (padding entries are valid slots), so
 // the neighbour gathers follow after one memory round trip instead of two.
-template <int G>
+template <int G, class Mem = PlainMem>
 __device__ __forceinline__ void vbd_vertex_group(const VbdArgs& a, uint32_t s, uint32_t i, uint32_t n, uint32_t eoff,
                                                  uint32_t boff, uint32_t ew, uint32_t bw, F3 ext, uint32_t lane,
                                                  bool valid) {
@@ -171,9 +199,9 @@ __device__ __forceinline__ void vbd_vertex_group(const VbdArgs& a, uint32_t s, u
         pq[k] = on ? a.ell_bq[boff + j * n + i] : float4{0.f, 0.f, 0.f, 0.f};
         pi[k] = on ? a.ell_bi[boff + j * n + i] : uint4{0u, 0u, 0u, 0u};
     }
-    const F3 xk = ld3(a.x, s);
-    const F3 x0 = ld3(a.x0, s);
-    const F3 v0 = ld3(a.v, s);
+    const F3 xk = Mem::ld(a.x, s);
+    const F3 x0 = Mem::ld(a.x0, s);
+    const F3 v0 = Mem::ld(a.v, s);
     const float m = a.mass[s];
     const uint32_t meta = valid ? a.meta[s] : 0u;
     const uint32_t edeg = meta & 0xffu, bdeg = (meta >> 8) & 0xffu;
@@ -181,16 +209,16 @@ __device__ __forceinline__ void vbd_vertex_group(const VbdArgs& a, uint32_t s, u
     // neighbour gathers of the prefetched rounds
     F3 pxo[PE];
 #pragma unroll
-    for (uint32_t k = 0; k < PE; k++) pxo[k] = (k * G + lane < edeg) ? ld3(a.x, pe[k].x) : F3{0.f, 0.f, 0.f};
+    for (uint32_t k = 0; k < PE; k++) pxo[k] = (k * G + lane < edeg) ? Mem::ld(a.x, pe[k].x) : F3{0.f, 0.f, 0.f};
     F3 pp[PB][4];
 #pragma unroll
     for (uint32_t k = 0; k < PB; k++) {
         const bool on = k * G + lane < bdeg;
         const uint32_t o = pi[k].x >> 30;
-        pp[k][0] = on ? pair_point(a, xk, o, 0, pi[k].x & 0x3fffffffu) : xk;
-        pp[k][1] = on ? pair_point(a, xk, o, 1, pi[k].y) : xk;
-        pp[k][2] = on ? pair_point(a, xk, o, 2, pi[k].z) : xk;
-        pp[k][3] = on ? pair_point(a, xk, o, 3, pi[k].w) : xk;
+        pp[k][0] = on ? pair_point<Mem>(a, xk, o, 0, pi[k].x & 0x3fffffffu) : xk;
+        pp[k][1] = on ? pair_point<Mem>(a, xk, o, 1, pi[k].y) : xk;
+        pp[k][2] = on ? pair_point<Mem>(a, xk, o, 2, pi[k].z) : xk;
+        pp[k][3] = on ? pair_point<Mem>(a, xk, o, 3, pi[k].w) : xk;
     }
 
     Sym3 H;
@@ -228,7 +256,7 @@ __device__ __forceinline__ void vbd_vertex_group(const VbdArgs& a, uint32_t s, u
         const uint2 e = on ? a.ell_e[eoff + j * n + i] : uint2{0u, 0u};
         Sym3 t;
         F3 tf;
-        spring_term_of(a, xk, on, e, on ? ld3(a.x, e.x) : F3{0.f, 0.f, 0.f}, t, tf);
+        spring_term_of(a, xk, on, e, on ? Mem::ld(a.x, e.x) : F3{0.f, 0.f, 0.f}, t, tf);
         add_spring_round(r, t, tf);
     }
     sym_add(H, S);
@@ -263,8 +291,8 @@ __device__ __forceinline__ void vbd_vertex_group(const VbdArgs& a, uint32_t s, u
             const float4 q = a.ell_bq[k];
             const uint4 id = a.ell_bi[k];
             const uint32_t o = id.x >> 30;
-            bending_term(q, o, pair_point(a, xk, o, 0, id.x & 0x3fffffffu), pair_point(a, xk, o, 1, id.y),
-                         pair_point(a, xk, o, 2, id.z), pair_point(a, xk, o, 3, id.w), a.kb, th, tf);
+            bending_term(q, o, pair_point<Mem>(a, xk, o, 0, id.x & 0x3fffffffu), pair_point<Mem>(a, xk, o, 1, id.y),
+                         pair_point<Mem>(a, xk, o, 2, id.z), pair_point<Mem>(a, xk, o, 3, id.w), a.kb, th, tf);
         }
         add_pair_round(r, th, tf);
     }
@@ -289,12 +317,12 @@ __device__ __forceinline__ void vbd_vertex_group(const VbdArgs& a, uint32_t s, u
                                    {H.xz + Hc[2][0], H.yz + Hc[2][1], H.zz + Hc[2][2]}};
             f = add3(f, fc);
             F3 dx;
-            if (newton_step_full(M, f, dx)) st3(a.x, s, add3(xk, dx));
+            if (newton_step_full(M, f, dx)) Mem::st(a.x, s, add3(xk, dx));
             return;
         }
     }
     F3 dx;
-    if (newton_step(H, f, dx)) st3(a.x, s, add3(xk, dx));
+    if (newton_step(H, f, dx)) Mem::st(a.x, s, add3(xk, dx));
 }
 
 }  // namespace lats

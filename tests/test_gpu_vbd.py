@@ -305,9 +305,59 @@ def test_persistent_c1_and_autotune(gpu_available):
     b = GpuVBD(cloth, params)
     x_before = b.state()[0]
     st, nb = b.autotune(frames=1)
-    assert st in (_lib.LATS_STRATEGY_GRAPH, _lib.LATS_STRATEGY_PERSISTENT)
+    assert st in (_lib.LATS_STRATEGY_GRAPH, _lib.LATS_STRATEGY_PERSISTENT, _lib.LATS_STRATEGY_FLOW)
     assert np.array_equal(b.state()[0], x_before)          # autotune restores the state
     b.set_strategy(_lib.LATS_STRATEGY_PERSISTENT)
     a.run_frames(3, wind)
     b.run_frames(3, wind)
     assert np.array_equal(a.state()[0], b.state()[0])
+
+
+@pytest.mark.parametrize("tiles,lanes", [(1, 8), (4, 1), (4, 8), (16, 4), (32, 8), (32, 16)])
+def test_flow_strategy_bit_exact(gpu_available, tiles, lanes):
+    """Tiles handing colour passes to their neighbours == graph path, bit for bit."""
+    _print_case(f"flow strategy, {tiles} tiles, {lanes} lanes, grid_c0 + wind")
+    cloth, params, _, _ = configs.grid_c0()
+    wind = configs.make_wind(6, cloth.num_verts, 2.0, 5) * 30.0
+    a = GpuVBD(cloth, params)
+    a.set_strategy(_lib.LATS_STRATEGY_GRAPH)
+    b = GpuVBD(cloth, params)
+    b.set_strategy(_lib.LATS_STRATEGY_FLOW, persistent_blocks=tiles)
+    b.set_lanes_per_vertex(lanes)
+    a.run_frames(6, wind)
+    b.run_frames(6, wind)
+    assert np.array_equal(a.state()[0], b.state()[0])
+    assert np.array_equal(a.state()[1], b.state()[1])
+    # single frames, a graph frame in between (counters carry over), then flow again
+    for k, strategy in enumerate((_lib.LATS_STRATEGY_FLOW, _lib.LATS_STRATEGY_GRAPH, _lib.LATS_STRATEGY_FLOW)):
+        b.set_strategy(strategy)
+        a.set_external_force(wind[k])
+        b.set_external_force(wind[k])
+        a.physics_step()
+        b.physics_step()
+        assert np.array_equal(a.state()[0], b.state()[0]), k
+        assert np.array_equal(a.state()[1], b.state()[1]), k
+
+
+def test_flow_c1_bit_exact(gpu_available):
+    _print_case("flow strategy on grid_c1 (default tiles), 4 windy frames")
+    cloth, params, _, wind = configs.grid_c1(frames=4)
+    a = GpuVBD(cloth, params)
+    a.set_strategy(_lib.LATS_STRATEGY_GRAPH)
+    a.set_lanes_per_vertex(8)
+    b = GpuVBD(cloth, params)
+    b.set_strategy(_lib.LATS_STRATEGY_FLOW)
+    b.set_lanes_per_vertex(8)
+    a.run_frames(4, wind)
+    b.run_frames(4, wind)
+    assert np.array_equal(a.state()[0], b.state()[0])
+    assert np.array_equal(a.state()[1], b.state()[1])
+
+
+def test_flow_rejects_obstacles_and_oversized_grids(gpu_available):
+    _print_case("flow strategy argument checks")
+    cloth, params, _, _ = configs.grid_c0()
+    b = GpuVBD(cloth, params)
+    b.set_strategy(_lib.LATS_STRATEGY_FLOW, persistent_blocks=100000)
+    with pytest.raises(_lib.LatsError):
+        b.run_frames(1)
