@@ -81,7 +81,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", default="grid_c1", choices=["grid_c1", "batch_c2", "copy_calibration",
+    ap.add_argument("--workload", default="grid_c1", choices=["grid_c1", "batch_c2", "sphere_c4", "copy_calibration",
                                                                "mlp_kernels"],
                     help="grid_c1 (default, the metric's workload), batch_c2 (512 cloths) or "
                          "copy_calibration (a plain 256 MiB device copy, the reference for HBM counters)")
@@ -106,6 +106,8 @@ def main():
         return copy_calibration(args)
     if args.workload == "mlp_kernels":
         return bench_mlp(args)
+    if args.workload == "sphere_c4":
+        return bench_sphere(args)
 
     K, W = args.steps, args.warmup
     cloth, params, _, wind = configs.grid_c1(frames=W + K)
@@ -257,6 +259,53 @@ def bench_warm_start(args, cloth, params, wind, plain_ms):
                      "record_every": args.warm_record_every, "guided_rounds": 3, "adam_steps": 4 * steps,
                      "batch_rows": 16384, "rows": m.dataset_rows(), "seconds": round(train_s, 1)},
     }
+
+
+def bench_sphere(args):
+    """1024x1024 cloth dropped on the sphere (contact detection every frame,
+    50 iterations per frame): plain GPU VBD throughput and the colour kernel's
+    roofline at a million vertices."""
+    from libatssim_amd import GpuVBD, configs, _lib
+    cloth, params, (ox, of), ct = configs.sphere_scene()
+    t = cloth.topology
+    g = GpuVBD(cloth, params)
+    g.set_obstacle(ox, of, ct["thickness"], ct["stiffness"], ct["radius"], ct["cell"])
+    if args.strategy == "auto":
+        strategy, blocks = g.autotune(frames=1)
+    else:
+        g.set_strategy(_lib.LATS_STRATEGY_GRAPH)
+        strategy, blocks = g.strategy()
+    if args.lanes:
+        g.set_lanes_per_vertex(args.lanes)
+    K, W = args.steps, args.warmup
+    if W:
+        g.run_frames(W)
+    g.synchronize()
+    t0 = time.perf_counter()
+    g.run_frames(K)
+    wall = time.perf_counter() - t0
+    gpu_ms = g.last_elapsed_ms()
+    iters = params.constraint_iter_count
+    counts, _ = g.contacts()
+    b_sweep = algorithmic_bytes_per_sweep(t.num_verts, t.num_edges, t.num_bending)
+    mean_ms, launches, _ = g.time_colour_kernels(10)
+    achieved = b_sweep / t.num_clusters / (mean_ms * 1e-3) / 1e9
+    out = {
+        "metric": "VBD vertex-updates/s (1024x1024 cloth on a sphere)",
+        "value": t.num_verts * iters * K / wall, "unit": "vertex-updates/s", "n_gpus": 1, "steps": K, "warmup": W,
+        "ms_per_step": wall * 1e3 / K, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"sphere_c4: {t.num_verts} verts, {t.num_clusters} colours, {iters} iterations/frame, "
+                               f"contact detection every frame", "gpu_ms_per_step": gpu_ms / K,
+                   "strategy": {_lib.LATS_STRATEGY_PERSISTENT: "persistent", _lib.LATS_STRATEGY_FLOW: "flow"}.get(
+                       strategy, "graph"),
+                   "lanes_per_vertex": g.lanes_per_vertex(), "contact_vertices": int((counts > 0).sum())},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": "k_vbd_colour",
+                     "mean_launch_us": round(mean_ms * 1e3, 3),
+                     "algorithmic_bytes_per_launch": round(b_sweep / t.num_clusters, 1)},
+    }
+    print(json.dumps(out), flush=True)
 
 
 def bench_batch(args):

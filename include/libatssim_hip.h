@@ -260,8 +260,12 @@ int lats_batch_get_state(lats_batch* b, uint32_t cloth, float* x, float* v);
  * (0: as many as fit). */
 int lats_batch_set_launch(lats_batch* b, uint32_t block_threads, uint32_t blocks);
 int lats_batch_get_launch(lats_batch* b, uint32_t* block_threads, uint32_t* blocks);
-/* Autotuner of the cloth-to-block mapping: times `frames` frames for each
- * candidate, keeps the fastest, restores the state. */
+/* Lanes per vertex of the batch kernel (1, 4 or 8; bit-identical results). */
+int lats_batch_set_lanes_per_vertex(lats_batch* b, uint32_t lanes);
+int lats_batch_get_lanes_per_vertex(lats_batch* b, uint32_t* lanes);
+/* Autotuner of the cloth-to-block mapping (lanes per vertex x block size x
+ * resident blocks): times `frames` frames for each candidate, keeps the
+ * fastest, restores the state. */
 int lats_batch_autotune(lats_batch* b, uint32_t frames, uint32_t* block_threads, uint32_t* blocks);
 int lats_batch_run_frames(lats_batch* b, uint32_t num_frames);
 int lats_batch_last_elapsed_ms(lats_batch* b, float* ms);

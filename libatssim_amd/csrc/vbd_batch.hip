@@ -54,6 +54,8 @@ struct BatchArgs {
 
 __global__ void k_batch_reset(uint32_t* work) { *work = 0u; }
 
+// G lanes per vertex (1: one thread per vertex; 4 / 8: vbd_vertex_group).
+template <int G>
 __global__ __launch_bounds__(1024) void k_vbd_batch(BatchArgs p) {
     __shared__ uint32_t s_item;
     const F3 ext = F3{p.fp->ext[0], p.fp->ext[1], p.fp->ext[2]};
@@ -98,8 +100,19 @@ __global__ __launch_bounds__(1024) void k_vbd_batch(BatchArgs p) {
         for (uint32_t it = 0; it < p.iters; it++) {
             for (uint32_t c = 0; c < ci.ncol; c++) {
                 const ColourInfo co = p.colours[ci.cbase + c];
-                for (uint32_t i = tid; i < co.count; i += nt)
-                    vbd_vertex_ext(a, co.start + i, i, co.count, co.eoff, co.boff, ext);
+                if (G == 1) {
+                    for (uint32_t i = tid; i < co.count; i += nt)
+                        vbd_vertex_ext(a, co.start + i, i, co.count, co.eoff, co.boff, ext);
+                } else {
+                    // groups of G lanes; a wave leaves the colour together
+                    for (uint32_t i0 = 0; i0 < co.count; i0 += nt / G) {
+                        if (i0 + (tid / 64) * (64 / G) >= co.count) break;
+                        const uint32_t i = i0 + tid / G;
+                        const bool valid = i < co.count;
+                        vbd_vertex_group<G>(a, co.start + (valid ? i : 0u), valid ? i : 0u, co.count, co.eoff,
+                                            co.boff, co.ew, co.bw, ext, tid % G, valid);
+                    }
+                }
                 __syncthreads();
             }
         }
@@ -161,7 +174,8 @@ struct lats_batch {
     DBuf<ClothInfo> cloths_d;
     DBuf<ColourInfo> colours_d;
     DBuf<FrameParams> fp;
-    uint32_t block_threads = 256, blocks = 0, max_blocks_cache = 0, max_blocks_threads = 0;
+    uint32_t block_threads = 256, blocks = 0, max_blocks_cache = 0, max_blocks_threads = 0, max_blocks_lanes = 0;
+    uint32_t lanes = 1;   // lanes per vertex (1, 4, 8)
     hipGraphExec_t graph = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     float last_ms = 0.f;
@@ -175,13 +189,22 @@ struct lats_batch {
 
 namespace {
 
+const void* batch_kernel(uint32_t lanes) {
+    switch (lanes) {
+        case 4: return (const void*)k_vbd_batch<4>;
+        case 8: return (const void*)k_vbd_batch<8>;
+        default: return (const void*)k_vbd_batch<1>;
+    }
+}
+
 int max_resident_blocks(lats_batch* b, uint32_t threads, uint32_t* out) {
-    if (b->max_blocks_threads != threads) {
+    if (b->max_blocks_threads != threads || b->max_blocks_lanes != b->lanes) {
         int per_cu = 0, ncu = 0;
-        HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_vbd_batch, (int)threads, 0));
+        HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel(b->lanes), (int)threads, 0));
         HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, b->device));
         b->max_blocks_cache = (uint32_t)std::max(1, per_cu * ncu);
         b->max_blocks_threads = threads;
+        b->max_blocks_lanes = b->lanes;
     }
     *out = b->max_blocks_cache;
     return LATS_OK;
@@ -217,7 +240,11 @@ int batch_graph(lats_batch* b, hipGraphExec_t* out) {
         HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
         for (uint32_t sub = 0; sub < b->prm.num_substep; sub++) {
             hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(1), 0, b->stream, b->work.p);
-            hipLaunchKernelGGL(k_vbd_batch, dim3(g), dim3(b->block_threads), 0, b->stream, p);
+            switch (b->lanes) {
+                case 4: hipLaunchKernelGGL(k_vbd_batch<4>, dim3(g), dim3(b->block_threads), 0, b->stream, p); break;
+                case 8: hipLaunchKernelGGL(k_vbd_batch<8>, dim3(g), dim3(b->block_threads), 0, b->stream, p); break;
+                default: hipLaunchKernelGGL(k_vbd_batch<1>, dim3(g), dim3(b->block_threads), 0, b->stream, p); break;
+            }
         }
         HIPCHK(hipStreamEndCapture(b->stream, &graph));
         hipError_t e = hipGraphInstantiate(&b->graph, graph, nullptr, nullptr, 0);
@@ -360,6 +387,22 @@ int lats_batch_set_launch(lats_batch* b, uint32_t block_threads, uint32_t blocks
     return LATS_OK;
 }
 
+int lats_batch_set_lanes_per_vertex(lats_batch* b, uint32_t lanes) {
+    if (!b) return fail(LATS_ERR_INVALID_ARGUMENT, "null batch");
+    if (lanes != 1 && lanes != 4 && lanes != 8) return fail(LATS_ERR_INVALID_ARGUMENT, "lanes per vertex must be 1, 4 or 8");
+    if (lanes != b->lanes) {
+        b->lanes = lanes;
+        if (b->graph) { (void)hipGraphExecDestroy(b->graph); b->graph = nullptr; }
+    }
+    return LATS_OK;
+}
+
+int lats_batch_get_lanes_per_vertex(lats_batch* b, uint32_t* lanes) {
+    if (!b || !lanes) return fail(LATS_ERR_INVALID_ARGUMENT, "null argument");
+    *lanes = b->lanes;
+    return LATS_OK;
+}
+
 int lats_batch_get_launch(lats_batch* b, uint32_t* block_threads, uint32_t* blocks) {
     if (!b) return fail(LATS_ERR_INVALID_ARGUMENT, "null batch");
     uint32_t mx = 0;
@@ -389,8 +432,9 @@ int lats_batch_last_elapsed_ms(lats_batch* b, float* ms) {
     return LATS_OK;
 }
 
-// Times `frames` frames for every (block size, resident blocks) pair of a
-// small grid and keeps the fastest; restores the state afterwards.
+// Times `frames` frames for every (lanes per vertex, block size, resident
+// blocks) triple of a small grid and keeps the fastest; restores the state
+// afterwards.
 int lats_batch_autotune(lats_batch* b, uint32_t frames, uint32_t* best_threads, uint32_t* best_blocks) {
     if (!b) return fail(LATS_ERR_INVALID_ARGUMENT, "null batch");
     if (!frames) frames = 1;
@@ -404,22 +448,28 @@ int lats_batch_autotune(lats_batch* b, uint32_t frames, uint32_t* best_threads, 
     HIPCHK(hipMemcpy(sx0, b->x0.p, nf * 4, hipMemcpyDeviceToDevice));
     HIPCHK(hipMemcpy(sv, b->v.p, nf * 4, hipMemcpyDeviceToDevice));
     float best = 3.4e38f;
-    uint32_t bt = b->block_threads, bb = b->blocks;
+    uint32_t bt = b->block_threads, bb = b->blocks, bl = b->lanes;
     int rc = LATS_OK;
-    for (uint32_t threads : {128u, 256u, 512u, 1024u}) {
-        uint32_t mx = 0;
-        rc = max_resident_blocks(b, threads, &mx);
+    for (uint32_t lanes : {1u, 4u, 8u}) {
+        rc = lats_batch_set_lanes_per_vertex(b, lanes);
         if (rc) break;
-        for (uint32_t blocks : {mx / 4, mx / 2, mx}) {
-            if (blocks == 0) continue;
-            rc = lats_batch_set_launch(b, threads, blocks);
-            if (!rc) rc = lats_batch_run_frames(b, 1);
-            if (!rc) rc = lats_batch_run_frames(b, frames);
+        for (uint32_t threads : {128u, 256u, 512u, 1024u}) {
+            uint32_t mx = 0;
+            rc = max_resident_blocks(b, threads, &mx);
             if (rc) break;
-            if (b->last_ms < best) { best = b->last_ms; bt = threads; bb = blocks; }
+            for (uint32_t blocks : {mx / 4, mx / 2, mx}) {
+                if (blocks == 0) continue;
+                rc = lats_batch_set_launch(b, threads, blocks);
+                if (!rc) rc = lats_batch_run_frames(b, 1);
+                if (!rc) rc = lats_batch_run_frames(b, frames);
+                if (rc) break;
+                if (b->last_ms < best) { best = b->last_ms; bt = threads; bb = blocks; bl = lanes; }
+            }
+            if (rc) break;
         }
         if (rc) break;
     }
+    if (!rc) rc = lats_batch_set_lanes_per_vertex(b, bl);
     hipError_t e1 = hipMemcpy(b->x.p, sx, nf * 4, hipMemcpyDeviceToDevice);
     hipError_t e2 = hipMemcpy(b->x0.p, sx0, nf * 4, hipMemcpyDeviceToDevice);
     hipError_t e3 = hipMemcpy(b->v.p, sv, nf * 4, hipMemcpyDeviceToDevice);

@@ -430,6 +430,14 @@ class GpuBatch:
         check(lib().lats_batch_get_launch(self._h, ctypes.byref(t), ctypes.byref(b)))
         return t.value, b.value
 
+    def set_lanes_per_vertex(self, lanes):
+        check(lib().lats_batch_set_lanes_per_vertex(self._h, int(lanes)))
+
+    def lanes_per_vertex(self):
+        n = ctypes.c_uint32()
+        check(lib().lats_batch_get_lanes_per_vertex(self._h, ctypes.byref(n)))
+        return n.value
+
     def autotune(self, frames=1):
         t = ctypes.c_uint32()
         b = ctypes.c_uint32()

@@ -19,12 +19,13 @@ def _small_batch(num, seed, nmax=40, iters=20):
     return cloths, ks, kb, Params(constraint_iter_count=iters)
 
 
-@pytest.mark.parametrize("threads", [64, 256, 1024])
-def test_batch_matches_single_cloth(gpu_available, threads):
-    _print_case(f"batch of 6 cloths, {threads} threads per cloth-block")
+@pytest.mark.parametrize("threads,lanes", [(64, 1), (256, 1), (1024, 1), (256, 4), (1024, 4), (128, 8), (1024, 8)])
+def test_batch_matches_single_cloth(gpu_available, threads, lanes):
+    _print_case(f"batch of 6 cloths, {threads} threads per cloth-block, {lanes} lanes per vertex")
     cloths, ks, kb, params = _small_batch(6, seed=11)
     b = GpuBatch(cloths, params, ks, kb)
     b.set_launch(threads, 0)
+    b.set_lanes_per_vertex(lanes)
     b.run_frames(3)
     for k, cl in enumerate(cloths):
         p = Params(constraint_iter_count=params.constraint_iter_count, stiffness_stretch_spring=float(ks[k]),
