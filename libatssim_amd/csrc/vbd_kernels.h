@@ -45,11 +45,42 @@ struct VbdArgs {
     float thickness;
 };
 
-__device__ __forceinline__ F3 ld3(const float* p, uint32_t s) { return F3{p[3 * s], p[3 * s + 1], p[3 * s + 2]}; }
+// 12-byte position records, read and written with one dwordx3 access.
+struct __attribute__((packed, aligned(4))) P3 {
+    float x, y, z;
+};
+__device__ __forceinline__ F3 ld3(const float* p, uint32_t s) {
+    const P3 v = *reinterpret_cast<const P3*>(p + 3 * (size_t)s);
+    return F3{v.x, v.y, v.z};
+}
 __device__ __forceinline__ void st3(float* p, uint32_t s, F3 a) {
-    p[3 * s] = a.x;
-    p[3 * s + 1] = a.y;
-    p[3 * s + 2] = a.z;
+    *reinterpret_cast<P3*>(p + 3 * (size_t)s) = P3{a.x, a.y, a.z};
+}
+
+// Value of lane (this lane + q) of the same row of 16 lanes (DPP row shift,
+// a VALU move; q is a constant after unrolling).  Lane 0 of an aligned group
+// of G <= 16 lanes reads lane q of its own group.
+__device__ __forceinline__ float lane_plus(float v, int q) {
+#define LATS_ROW_SHL(n) __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x100 + (n), 0xF, 0xF, true))
+    switch (q) {
+        case 1: return LATS_ROW_SHL(1);
+        case 2: return LATS_ROW_SHL(2);
+        case 3: return LATS_ROW_SHL(3);
+        case 4: return LATS_ROW_SHL(4);
+        case 5: return LATS_ROW_SHL(5);
+        case 6: return LATS_ROW_SHL(6);
+        case 7: return LATS_ROW_SHL(7);
+        case 8: return LATS_ROW_SHL(8);
+        case 9: return LATS_ROW_SHL(9);
+        case 10: return LATS_ROW_SHL(10);
+        case 11: return LATS_ROW_SHL(11);
+        case 12: return LATS_ROW_SHL(12);
+        case 13: return LATS_ROW_SHL(13);
+        case 14: return LATS_ROW_SHL(14);
+        case 15: return LATS_ROW_SHL(15);
+        default: return v;
+    }
+#undef LATS_ROW_SHL
 }
 
 // How a vertex update reads and writes the per-vertex state (x, x_start, v).
@@ -168,9 +199,9 @@ __device__ __forceinline__ F3 pair_point(const VbdArgs& a, F3 xk, uint32_t o, ui
 
 // The same update with G lanes per vertex (lane = 0..G-1 inside an aligned
 // group of G lanes).  Lane j computes the terms of incident springs / pairs
-// j, j+G, ...; the terms are then added in incidence order through in-group
-// shuffles, so every sum is formed in the reference's order and the result
-// is bit-identical to vbd_vertex_ext.  All G lanes of a group must call this
+// j, j+G, ...; lane 0 then adds them in incidence order, reading the other
+// lanes' terms with DPP row shifts, so every sum is formed in the reference's
+// order and the result is bit-identical to vbd_vertex_ext.  All G lanes of a group must call this
 // together (valid == false: the group has no vertex; it still takes part in
 // the shuffles and stores nothing).
 //
@@ -227,18 +258,26 @@ __device__ __forceinline__ void vbd_vertex_group(const VbdArgs& a, uint32_t s, u
 
     Sym3 S = sym_zero();
     F3 fs = F3{0.f, 0.f, 0.f};
+    // only lane 0 of the group uses the sums: it adds the terms of lanes
+    // 0, 1, ..., cnt-1 in that order
     auto add_spring_round = [&](uint32_t r, const Sym3& t, const F3& tf) {
         const uint32_t cnt = (edeg - r < (uint32_t)G) ? edeg - r : (uint32_t)G;
-        for (uint32_t q = 0; q < cnt; q++) {
-            S.xx = S.xx + __shfl(t.xx, (int)q, G);
-            S.xy = S.xy + __shfl(t.xy, (int)q, G);
-            S.xz = S.xz + __shfl(t.xz, (int)q, G);
-            S.yy = S.yy + __shfl(t.yy, (int)q, G);
-            S.yz = S.yz + __shfl(t.yz, (int)q, G);
-            S.zz = S.zz + __shfl(t.zz, (int)q, G);
-            fs.x = fs.x + __shfl(tf.x, (int)q, G);
-            fs.y = fs.y + __shfl(tf.y, (int)q, G);
-            fs.z = fs.z + __shfl(tf.z, (int)q, G);
+#pragma unroll
+        for (int q = 0; q < G; q++) {
+            const float vxx = lane_plus(t.xx, q), vxy = lane_plus(t.xy, q), vxz = lane_plus(t.xz, q);
+            const float vyy = lane_plus(t.yy, q), vyz = lane_plus(t.yz, q), vzz = lane_plus(t.zz, q);
+            const float vfx = lane_plus(tf.x, q), vfy = lane_plus(tf.y, q), vfz = lane_plus(tf.z, q);
+            if ((uint32_t)q < cnt) {
+                S.xx = S.xx + vxx;
+                S.xy = S.xy + vxy;
+                S.xz = S.xz + vxz;
+                S.yy = S.yy + vyy;
+                S.yz = S.yz + vyz;
+                S.zz = S.zz + vzz;
+                fs.x = fs.x + vfx;
+                fs.y = fs.y + vfy;
+                fs.z = fs.z + vfz;
+            }
         }
     };
 #pragma unroll
@@ -266,11 +305,15 @@ __device__ __forceinline__ void vbd_vertex_group(const VbdArgs& a, uint32_t s, u
     F3 fb = F3{0.f, 0.f, 0.f};
     auto add_pair_round = [&](uint32_t r, float th, const F3& tf) {
         const uint32_t cnt = (bdeg - r < (uint32_t)G) ? bdeg - r : (uint32_t)G;
-        for (uint32_t q = 0; q < cnt; q++) {
-            hb = hb + __shfl(th, (int)q, G);
-            fb.x = fb.x + __shfl(tf.x, (int)q, G);
-            fb.y = fb.y + __shfl(tf.y, (int)q, G);
-            fb.z = fb.z + __shfl(tf.z, (int)q, G);
+#pragma unroll
+        for (int q = 0; q < G; q++) {
+            const float vh = lane_plus(th, q), vx = lane_plus(tf.x, q), vy = lane_plus(tf.y, q), vz = lane_plus(tf.z, q);
+            if ((uint32_t)q < cnt) {
+                hb = hb + vh;
+                fb.x = fb.x + vx;
+                fb.y = fb.y + vy;
+                fb.z = fb.z + vz;
+            }
         }
     };
 #pragma unroll
