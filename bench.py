@@ -305,7 +305,42 @@ def bench_sphere(args):
                      "mean_launch_us": round(mean_ms * 1e3, 3),
                      "algorithmic_bytes_per_launch": round(b_sweep / t.num_clusters, 1)},
     }
+    if not args.no_warm_start:
+        out["warm_start"] = bench_sphere_warm_start(args, cloth, params, (ox, of), ct, plain_ms=gpu_ms / K)
     print(json.dumps(out), flush=True)
+
+
+def bench_sphere_warm_start(args, cloth, params, obstacle, ct, plain_ms, scenarios=4, frames=120):
+    """Warm start retrained on the sphere regime (plain and guided rollouts of
+    smaller drops), evaluated on the 1024x1024 drop: per frame, wall time to
+    the plain 50-iteration energy from the same frame start."""
+    import numpy as np
+    from libatssim_amd import configs
+    from libatssim_amd.warm_start import WarmStartMLP, evaluate, train_recipe
+    ox, of = obstacle
+
+    def setup(g):
+        g.set_obstacle(ox, of, ct["thickness"], ct["stiffness"], ct["radius"], ct["cell"])
+
+    t0 = time.perf_counter()
+    scen = [configs.sphere_train_scenario(k) for k in range(scenarios)]
+    m = WarmStartMLP(2025)
+    train_recipe(m, scen, params, frames, plain_steps=2000, guided_rounds=3, guided_steps=2000, record_every=2,
+                 setup=setup)
+    train_s = time.perf_counter() - t0
+    n_eval = min(frames, args.warm_frames * 2)
+    res = evaluate(m, cloth, params, None, n_eval, setup=setup)
+    wall = np.array([r["wall_ms"] for r in res])
+    sweeps = np.array([r["sweeps"] for r in res])
+    return {
+        "ms_per_frame_learned": round(float(wall.mean()), 4), "ms_per_frame_plain": round(plain_ms, 4),
+        "speedup": round(plain_ms / float(wall.mean()), 3), "frames": n_eval,
+        "mean_sweeps": round(float(sweeps.mean()), 2), "max_sweeps": int(sweeps.max()),
+        "kept_fraction": round(float(np.mean([r["kept"] for r in res])), 3),
+        "target": "per-frame energy of the plain 50-iteration solve from the same frame start",
+        "training": {"scenarios": scenarios, "cloth": "384x384, 2 mm", "frames": frames, "record_every": 2,
+                     "guided_rounds": 3, "adam_steps": 8000, "rows": m.dataset_rows(), "seconds": round(train_s, 1)},
+    }
 
 
 def bench_batch(args):

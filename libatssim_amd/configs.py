@@ -11,11 +11,13 @@ batch_c2     512 independent grid cloths, side n ~ U{16..128} (~3.2 M vertices),
              k_stretch log-U[1e3, 1e5], k_bend log-U[1e-3, 1e-1] per cloth,
              1-4 random boundary vertices pinned, m = 1e-3 kg, 100 iterations;
              all draws from splitmix64(seed).
-sphere_c4    1024x1024 horizontal grid (2 mm spacing, 2.05 m wide, no pins)
+sphere_c4    (sphere_scene) 1024x1024 horizontal grid (2 mm spacing, 2.05 m wide, no pins)
              10 cm above a fixed sphere of radius 0.3 m (icosphere, 20,480
              faces); contact thickness 2 mm, k_contact 1e5, detection radius
              2.5 cm (one frame of fall at impact speed), hash cell 5 cm;
-             50 iterations/frame, 120 frames.
+             50 iterations/frame, 120 frames.  Warm-start training drops
+             (sphere_train_scenario): 384x384 cloths of the same spacing,
+             dropped from 6-14 cm, centre offset up to +-10 cm.
 warm_c3      learned warm start on the grid_c1 cloth: training scenarios k =
              0..31 of 200 frames each, per-scenario wind seed and pin set
              (k % 4: 0, 2 top row; 1 two top corners; 3 one to four random
@@ -144,6 +146,23 @@ def sphere_scene(n=1024, spacing=0.002, radius=0.3, drop=0.1, level=5, seed=2025
     ox, of = make_icosphere(level, radius)
     contact = dict(thickness=0.002, stiffness=1e5, radius=0.025, cell=0.05)
     return cloth, Params(constraint_iter_count=iters), (ox, of), contact
+
+
+SEED_C4T = 20250107
+
+
+def sphere_train_scenario(k, n=384, spacing=0.002, radius=0.3, seed=SEED_C4T):
+    """Training drop k of the sphere regime: a smaller cloth of the same
+    spacing, material and contact model, dropped from 6-14 cm with its centre
+    offset by up to +-10 cm from the sphere's top.  No wind."""
+    rng = SplitMix64(seed * 1000003 + k)
+    drop = 0.06 + 0.08 * rng.uniform()
+    ox = (rng.uniform() - 0.5) * 0.2
+    oz = (rng.uniform() - 0.5) * 0.2
+    x, f = horizontal_grid(n, spacing, radius + drop, seed + k)
+    x[:, 0] += np.float32(ox)
+    x[:, 2] += np.float32(oz)
+    return Cloth(x, f, np.float32(1e-3), np.zeros(n * n, np.uint8)), None
 
 
 def square8k_pins(x):

@@ -30,8 +30,10 @@ using namespace lats;
 namespace {
 
 constexpr int kIn = 48, kInP = 64, kHid = 128, kOut = 3, kOutP = 16, kOutK = 32;
-constexpr int kTile = 64;   // rows per block (4 waves x 16)
-constexpr int kT = 256;     // threads per block
+constexpr int kTile = 64;   // rows per block of the backward pass (4 waves x 16)
+constexpr int kT = 256;     // threads per block (backward and small kernels)
+constexpr int kTF = 512;    // threads per block of the forward pass (8 waves share the LDS weights)
+constexpr int kTileF = 128; // rows per forward tile (8 waves x 16)
 
 constexpr int OFF_W1 = 0;
 constexpr int OFF_B1 = OFF_W1 + kInP * kHid;
@@ -43,12 +45,17 @@ constexpr int OFF_W4 = OFF_B3 + kHid;
 constexpr int OFF_B4 = OFF_W4 + kHid * kOutP;
 constexpr int kNParam = OFF_B4 + kOutP;
 
-// bf16 transposed weights for the forward B operand ([out][in])
+// bf16 transposed weights for the forward B operand ([out][in]); each row
+// is padded by 8 elements (16 bytes) so that the 16 rows one MFMA B operand
+// reads from LDS start in different banks
+constexpr int kPad = 8;
+constexpr int kS1 = kInP + kPad;   // row stride of WT1
+constexpr int kSH = kHid + kPad;   // row stride of WT2, WT3, WT4
 constexpr int WT1 = 0;
-constexpr int WT2 = WT1 + kHid * kInP;
-constexpr int WT3 = WT2 + kHid * kHid;
-constexpr int WT4 = WT3 + kHid * kHid;
-constexpr int kNWT = WT4 + kOutP * kHid;
+constexpr int WT2 = WT1 + kHid * kS1;
+constexpr int WT3 = WT2 + kHid * kSH;
+constexpr int WT4 = WT3 + kHid * kSH;
+constexpr int kNWT = WT4 + kOutP * kSH;
 // bf16 row-major weights for dH = dZ W^T ([in][out], W4 padded to 32 outputs)
 constexpr int WB2 = 0;
 constexpr int WB3 = WB2 + kHid * kHid;
@@ -95,9 +102,12 @@ struct RowMap {
     __device__ uint64_t operator()(uint32_t r) const { return seed ? mix64(seed + r) % n : (uint64_t)r; }
 };
 
-__device__ inline float silu(float z) { return z / (1.f + expf(-z)); }
+// logistic sigmoid with the hardware exp and reciprocal (a few ulp; the
+// activations are rounded to BF16 right after)
+__device__ inline float sigmoid(float z) { return __builtin_amdgcn_rcpf(1.f + __expf(-z)); }
+__device__ inline float silu(float z) { return z * sigmoid(z); }
 __device__ inline float dsilu(float z) {
-    const float s = 1.f / (1.f + expf(-z));
+    const float s = sigmoid(z);
     return s * (1.f + z * (1.f - s));
 }
 
@@ -115,31 +125,46 @@ struct FwdArgs {
     uint16_t* H;           // [3][M][128] bf16 activations (training) or null
 };
 
-// one dense layer for the wave's 16 rows: acc = bias + A (16 x K) W^T rows
-template <int K>
-__device__ inline void layer_fwd(const bf16x8* a, const uint16_t* wt, const float* bias, int nblocks, uint32_t l,
-                                 f32x4* out) {
+// one dense layer for the wave's 16 rows: out[nb] = bias + A (16 x K) W^T
+// (W^T rows of stride K + kPad in LDS, biases in LDS).  The NB column blocks
+// are independent MFMA chains; each k step loads all NB B operands first.
+template <int K, int NB>
+__device__ __forceinline__ void layer_fwd(const bf16x8* a, const uint16_t* wt, const float* sbias, uint32_t l,
+                                          f32x4* out) {
     const uint32_t kq = 8 * (l / 16), col = l % 16;
-    for (int nb = 0; nb < nblocks; nb++) {
-        const float bv = bias[nb * 16 + col];
-        f32x4 acc = {bv, bv, bv, bv};
-        for (int kb = 0; kb < K / 32; kb++) acc = mfma(a[kb], ld_bf8(wt + (nb * 16 + col) * K + kb * 32 + kq), acc);
-        out[nb] = acc;
+#pragma unroll
+    for (int nb = 0; nb < NB; nb++) {
+        const float bv = sbias[nb * 16 + col];
+        out[nb] = f32x4{bv, bv, bv, bv};
+    }
+#pragma unroll
+    for (int kb = 0; kb < K / 32; kb++) {
+        bf16x8 b[NB];
+#pragma unroll
+        for (int nb = 0; nb < NB; nb++) b[nb] = ld_bf8(wt + (nb * 16 + col) * (K + kPad) + kb * 32 + kq);
+#pragma unroll
+        for (int nb = 0; nb < NB; nb++) out[nb] = mfma(a[kb], b[nb], out[nb]);
     }
 }
 
-__global__ __launch_bounds__(kT) void k_mlp_forward(FwdArgs a) {
+__global__ __launch_bounds__(kTF) void k_mlp_forward(FwdArgs a) {
     extern __shared__ uint4 smem_raw[];
     uint16_t* sWT = reinterpret_cast<uint16_t*>(smem_raw);
-    uint16_t* sAct = sWT + kNWT;                                  // [4][16][kActStride]
+    uint16_t* sAct = sWT + kNWT;                                  // [8][16][kActStride]
+    float* sBias = reinterpret_cast<float*>(sAct + (kTF / 64) * 16 * kActStride);   // b1, b2, b3, b4
     const uint32_t t = threadIdx.x, w = t / 64, l = t % 64;
-    for (uint32_t i = t; i < kNWT / 8; i += kT)
+    for (uint32_t i = t; i < kNWT / 8; i += kTF)
         reinterpret_cast<uint4*>(sWT)[i] = reinterpret_cast<const uint4*>(a.WT)[i];
+    for (uint32_t i = t; i < 3 * kHid + kOutP; i += kTF) {
+        const uint32_t layer = i / kHid, j = i % kHid;
+        const int off = layer == 0 ? OFF_B1 : layer == 1 ? OFF_B2 : layer == 2 ? OFF_B3 : OFF_B4;
+        sBias[i] = a.P[off + j];
+    }
     __syncthreads();
     uint16_t* act = sAct + w * 16 * kActStride;
     const uint32_t kq = 8 * (l / 16), col = l % 16;
-    for (uint32_t tile = blockIdx.x; tile * kTile < a.M; tile += gridDim.x) {
-        const uint32_t r0 = tile * kTile + w * 16;
+    for (uint32_t tile = blockIdx.x; tile * kTileF < a.M; tile += gridDim.x) {
+        const uint32_t r0 = tile * kTileF + w * 16;
         const uint32_t ra = r0 + col;                              // A-operand row of this lane
         bf16x8 ain[4];
         for (int kb = 0; kb < 2; kb++) {
@@ -147,10 +172,9 @@ __global__ __launch_bounds__(kT) void k_mlp_forward(FwdArgs a) {
             else ain[kb] = __builtin_bit_cast(bf16x8, make_uint4(0, 0, 0, 0));
         }
         f32x4 acc[8];
-        const int boff[3] = {OFF_B1, OFF_B2, OFF_B3};
         for (int layer = 0; layer < 3; layer++) {
-            if (layer == 0) layer_fwd<kInP>(ain, sWT + WT1, a.P + OFF_B1, 8, l, acc);
-            else layer_fwd<kHid>(ain, sWT + (layer == 1 ? WT2 : WT3), a.P + boff[layer], 8, l, acc);
+            if (layer == 0) layer_fwd<kInP, 8>(ain, sWT + WT1, sBias, l, acc);
+            else layer_fwd<kHid, 8>(ain, sWT + (layer == 1 ? WT2 : WT3), sBias + layer * kHid, l, acc);
             __syncthreads();   // every wave has read its previous activations
             for (int nb = 0; nb < 8; nb++)
                 for (int i = 0; i < 4; i++) {
@@ -167,7 +191,7 @@ __global__ __launch_bounds__(kT) void k_mlp_forward(FwdArgs a) {
             for (int kb = 0; kb < 4; kb++) ain[kb] = ld_bf8(act + col * kActStride + kb * 32 + kq);
         }
         f32x4 y[1];
-        layer_fwd<kHid>(ain, sWT + WT4, a.P + OFF_B4, 1, l, y);
+        layer_fwd<kHid, 1>(ain, sWT + WT4, sBias + 3 * kHid, l, y);
         for (int i = 0; i < 4; i++) {
             const uint32_t row = r0 + 4 * (l / 16) + i;
             if (row < a.M && col < 4) a.Y[(size_t)row * 4 + col] = y[0][i];
@@ -314,23 +338,27 @@ __global__ void k_mlp_adam(float* p, float* m, float* v, const float* g, float l
     p[i] = p[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
 }
 
-// bf16 weight copies (transposed for the forward, row-major for dH)
+// bf16 weight copies (transposed and row-padded for the forward, row-major
+// for dH); one thread per element of the largest array
+constexpr int kPackThreads = kHid * kSH;
 __global__ void k_mlp_pack(const float* P, uint16_t* WT, uint16_t* WB) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < (uint32_t)(kHid * kInP)) {                       // WT1[n][k] = W1[k][n]
-        const uint32_t n = i / kInP, k = i % kInP;
-        WT[WT1 + i] = f2bf(P[OFF_W1 + k * kHid + n]);
+    if (i < (uint32_t)(kHid * kS1)) {                        // WT1[n][k] = W1[k][n]
+        const uint32_t n = i / kS1, k = i % kS1;
+        WT[WT1 + i] = k < (uint32_t)kInP ? f2bf(P[OFF_W1 + k * kHid + n]) : (uint16_t)0;
+    }
+    if (i < (uint32_t)(kHid * kSH)) {                        // WT2, WT3
+        const uint32_t n = i / kSH, k = i % kSH;
+        WT[WT2 + i] = k < (uint32_t)kHid ? f2bf(P[OFF_W2 + k * kHid + n]) : (uint16_t)0;
+        WT[WT3 + i] = k < (uint32_t)kHid ? f2bf(P[OFF_W3 + k * kHid + n]) : (uint16_t)0;
+    }
+    if (i < (uint32_t)(kOutP * kSH)) {                       // WT4[n][k] = W4[k][n]
+        const uint32_t n = i / kSH, k = i % kSH;
+        WT[WT4 + i] = k < (uint32_t)kHid ? f2bf(P[OFF_W4 + k * kOutP + n]) : (uint16_t)0;
     }
     if (i < (uint32_t)(kHid * kHid)) {
-        const uint32_t n = i / kHid, k = i % kHid;
-        WT[WT2 + i] = f2bf(P[OFF_W2 + k * kHid + n]);
-        WT[WT3 + i] = f2bf(P[OFF_W3 + k * kHid + n]);
         WB[WB2 + i] = f2bf(P[OFF_W2 + i]);
         WB[WB3 + i] = f2bf(P[OFF_W3 + i]);
-    }
-    if (i < (uint32_t)(kOutP * kHid)) {                      // WT4[n][k] = W4[k][n]
-        const uint32_t n = i / kHid, k = i % kHid;
-        WT[WT4 + i] = f2bf(P[OFF_W4 + k * kOutP + n]);
     }
     if (i < (uint32_t)(kHid * kOutK)) {                      // WB4[h][o] = W4[h][o], o < 16
         const uint32_t h = i / kOutK, o = i % kOutK;
@@ -427,9 +455,19 @@ __global__ __launch_bounds__(kThreads) void k_mlp_features(FeatArgs f) {
     fx[k++] = (float)bdeg * 0.0625f;                    // 42    bending degree
     fx[k++] = (S.xx + S.yy + S.zz) * h2m;               // 43    spring stiffness / inertia
     fx[k++] = hb * h2m;                                 // 44    bending stiffness / inertia
-    fx[k++] = sqrtf(dot3(v0, v0)) * h * is0;            // 45    |v| x one step
-    fx[k++] = sqrtf(dot3(c1, c1)) * is0;                // 46    |last correction|
-    fx[k++] = 1.f;                                      // 47    constant
+    // 45-47 obstacle contacts: sum over the vertex's contacts of the depth
+    // of x_pred inside the contact band times the face normal
+    F3 pc = F3{0.f, 0.f, 0.f};
+    if (a.ccount) {
+        const uint32_t nc = a.ccount[s];
+        for (uint32_t q = 0; q < nc; q++) {
+            const float4 xs = a.ct_xs[q * a.nslots + s];
+            const float4 nn = a.ct_n[q * a.nslots + s];
+            const float proj = dot3(sub3(xp, F3{xs.x, xs.y, xs.z}), F3{nn.x, nn.y, nn.z});
+            if (proj < a.thickness) pc = add3(pc, scale3(a.thickness - proj, F3{nn.x, nn.y, nn.z}));
+        }
+    }
+    put3(scale3(is0, pc));                              // 45-47 contact push
     (void)x0;
     uint16_t* out = f.X + (size_t)s * kInP;
     for (int c = 0; c < kIn; c++) out[c] = f2bf(fx[c] / f.scale[c]);
@@ -526,7 +564,8 @@ __global__ void k_mlp_rescale(uint16_t* X, uint64_t rows, const float* mul) {
     X[r * kInP + c] = f2bf(bf2f(X[r * kInP + c]) * mul[c]);
 }
 
-constexpr size_t kFwdLds = (size_t)kNWT * 2 + (size_t)4 * 16 * kActStride * 2;
+constexpr size_t kFwdLds =
+    (size_t)kNWT * 2 + (size_t)(kTF / 64) * 16 * kActStride * 2 + (3 * kHid + kOutP) * sizeof(float);
 constexpr size_t kBwdLds = (size_t)kTile * kHid * 4 * 2 + (size_t)kTile * kActStride * 2 + 4 * sizeof(double);
 constexpr uint32_t kGrid = 256;   // blocks of the training kernels (one partial gradient each)
 
@@ -565,10 +604,11 @@ struct lats_mlp {
 namespace {
 
 void enqueue_pack(lats_mlp* m, hipStream_t st) {
-    hipLaunchKernelGGL(k_mlp_pack, dim3((kHid * kHid + 255) / 256), dim3(256), 0, st, m->P.p, m->WT.p, m->WB.p);
+    hipLaunchKernelGGL(k_mlp_pack, dim3((kPackThreads + 255) / 256), dim3(256), 0, st, m->P.p, m->WT.p, m->WB.p);
 }
 
-uint32_t fwd_grid(uint32_t M) { return std::max(1u, std::min((M + kTile - 1) / kTile, kGrid)); }
+uint32_t fwd_grid(uint32_t M) { return std::max(1u, std::min((M + kTileF - 1) / kTileF, kGrid)); }
+uint32_t bwd_grid(uint32_t M) { return std::max(1u, std::min((M + kTile - 1) / kTile, kGrid)); }
 
 void enqueue_forward(lats_mlp* m, const uint16_t* X, RowMap map, uint32_t M, float* Y, float* Z, uint16_t* H,
                      hipStream_t st) {
@@ -581,7 +621,7 @@ void enqueue_forward(lats_mlp* m, const uint16_t* X, RowMap map, uint32_t M, flo
     a.Y = Y;
     a.Z = Z;
     a.H = H;
-    hipLaunchKernelGGL(k_mlp_forward, dim3(fwd_grid(M)), dim3(kT), kFwdLds, st, a);
+    hipLaunchKernelGGL(k_mlp_forward, dim3(fwd_grid(M)), dim3(kTF), kFwdLds, st, a);
 }
 
 // forward (keeping Z, H) + backward + fixed-order reduction into m->G
@@ -600,7 +640,7 @@ void enqueue_gradient(lats_mlp* m, const uint16_t* X, const float* T, RowMap map
     b.part = m->part.p;
     b.loss_part = m->loss_part.p;
     b.inv_count = 1.f / (3.f * (float)M);
-    const uint32_t g = fwd_grid(M);
+    const uint32_t g = bwd_grid(M);
     hipLaunchKernelGGL(k_mlp_backward, dim3(g), dim3(kT), kBwdLds, st, b);
     hipLaunchKernelGGL(k_mlp_reduce, dim3((kNParam + 255) / 256), dim3(256), 0, st, m->part.p, g, m->G.p);
 }
@@ -893,7 +933,7 @@ int lats_mlp_gradient_host(lats_mlp* m, const float* X, const float* T, uint32_t
     if (rc) return rc;
     enqueue_gradient(m, dX.p, dT.p, RowMap{}, M, m->tY.p, m->tZ.p, m->tH.p, m->stream);
     HIPCHK(hipGetLastError());
-    const uint32_t g = fwd_grid(M);
+    const uint32_t g = bwd_grid(M);
     std::vector<double> lp(g);
     HIPCHK(hipMemcpyAsync(grad, m->G.p, kNParam * sizeof(float), hipMemcpyDeviceToHost, m->stream));
     HIPCHK(hipMemcpyAsync(lp.data(), m->loss_part.p, g * sizeof(double), hipMemcpyDeviceToHost, m->stream));
@@ -1124,7 +1164,7 @@ int lats_mlp_train(lats_mlp* m, uint32_t steps, uint32_t batch_rows, float lr, u
     if (batch_rows == 0) return fail(LATS_ERR_INVALID_ARGUMENT, "batch_rows must be positive");
     int rc = ensure_train_buffers(m, batch_rows);
     if (rc) return rc;
-    const uint32_t g = fwd_grid(batch_rows);
+    const uint32_t g = bwd_grid(batch_rows);
     std::vector<double> lp(g);
     double sum = 0.0;
     const float b1 = 0.9f, b2 = 0.999f;

@@ -18,8 +18,9 @@ as the displacement h^2/m * f they cause over one step):
   24-26 bending force, 27-29 gravity + external force at x_pred; 30 pinned;
   31-38 colour one-hot; 39 mean spring strain; 40 max |strain|;
   41 spring degree / 8; 42 bending degree / 16; 43 spring stiffness trace,
-  44 bending stiffness, both * h^2/m; 45 |v| h; 46 |last correction|;
-  47 constant 1.  Each column is divided by its feature scale.
+  44 bending stiffness, both * h^2/m; 45-47 obstacle contacts: the sum over
+  the vertex's contacts of (thickness - (x_pred - x_s).n) n where positive
+  (zero without an obstacle).  Each column is divided by its feature scale.
 """
 import ctypes
 
@@ -209,9 +210,11 @@ def _frame_result(out):
             "e_learned": float(out[3]), "e_final": float(out[4]), "target": float(out[5])}
 
 
-def record_scenarios(mlp, scenarios, params, frames, lanes=8, record_every=1):
+def record_scenarios(mlp, scenarios, params, frames, lanes=8, record_every=1, setup=None):
     """Plain-solver trajectories -> training rows (every record_every-th
-    frame is recorded, the others are plain frames).  scenarios: [(cloth, wind)]."""
+    frame is recorded, the others are plain frames).  scenarios: [(cloth,
+    wind)], wind None for no external force; setup(gpu) prepares each new
+    context (an obstacle, say)."""
     from .vbd import GpuVBD
     from . import _lib
     total = sum(c.num_verts for c, _ in scenarios) * ((frames + record_every - 1) // record_every)
@@ -219,9 +222,12 @@ def record_scenarios(mlp, scenarios, params, frames, lanes=8, record_every=1):
     for cloth, wind in scenarios:
         g = GpuVBD(cloth, params, _lib.LATS_STRATEGY_GRAPH)
         g.set_lanes_per_vertex(lanes)
+        if setup:
+            setup(g)
         mlp.bind(g)
         for f in range(frames):
-            g.set_external_force(wind[f % len(wind)])
+            if wind is not None:
+                g.set_external_force(wind[f % len(wind)])
             if f % record_every == 0:
                 mlp.record_frame(g)
             else:
@@ -232,7 +238,7 @@ def record_scenarios(mlp, scenarios, params, frames, lanes=8, record_every=1):
 
 
 def record_guided(mlp, scenarios, params, frames, lanes=8, first_check=FIRST_CHECK, check_every=CHECK_EVERY,
-                  use_model=True, record_every=1):
+                  use_model=True, record_every=1, setup=None):
     """Rows from the states the warm-started solver visits: each scenario is
     advanced by warm-started frames, each recorded frame's target is the
     plain solve from the same frame start (a second context)."""
@@ -246,9 +252,13 @@ def record_guided(mlp, scenarios, params, frames, lanes=8, first_check=FIRST_CHE
         ref = GpuVBD(cloth, params, _lib.LATS_STRATEGY_GRAPH)
         run.set_lanes_per_vertex(lanes)
         ref.set_lanes_per_vertex(lanes)
+        if setup:
+            setup(run)
+            setup(ref)
         mlp.bind(run)
         for f in range(frames):
-            run.set_external_force(wind[f % len(wind)])
+            if wind is not None:
+                run.set_external_force(wind[f % len(wind)])
             if f % record_every == 0:
                 stats.append(mlp.record_guided_frame(run, ref, 2 * params.constraint_iter_count, first_check,
                                                      check_every, use_model))
@@ -256,7 +266,8 @@ def record_guided(mlp, scenarios, params, frames, lanes=8, first_check=FIRST_CHE
                 # target of an unrecorded frame: the plain solve on ref, as when recording
                 x, v = run.state()
                 ref.set_state(x, v)
-                ref.set_external_force(wind[f % len(wind)])
+                if wind is not None:
+                    ref.set_external_force(wind[f % len(wind)])
                 ref.frame_begin()
                 ref.iterate(params.constraint_iter_count)
                 stats.append(mlp.warm_start_frame(run, ref.energy_now(), 2 * params.constraint_iter_count,
@@ -267,7 +278,8 @@ def record_guided(mlp, scenarios, params, frames, lanes=8, first_check=FIRST_CHE
 
 
 def train_recipe(mlp, scenarios, params, frames, plain_steps=2000, guided_rounds=3, guided_steps=2000,
-                 batch_rows=16384, lr=1e-3, seed=0, smooth_passes=1, half_step=True, record_every=1, log=None):
+                 batch_rows=16384, lr=1e-3, seed=0, smooth_passes=1, half_step=True, record_every=1, setup=None,
+                 log=None):
     """Plain trajectories -> feature scales -> Adam; then rounds of rows from
     warm-started trajectories (targets from the plain solve) -> Adam on all
     rows (the states a warm-started solver visits differ from the plain
@@ -276,13 +288,13 @@ def train_recipe(mlp, scenarios, params, frames, plain_steps=2000, guided_rounds
     import time
     t0 = time.perf_counter()
     mlp.set_guess_options(smooth_passes, half_step)
-    record_scenarios(mlp, scenarios, params, frames, record_every=record_every)
+    record_scenarios(mlp, scenarios, params, frames, record_every=record_every, setup=setup)
     mlp.fit_feature_scale()
     loss = _train_chunks(mlp, plain_steps, batch_rows, lr, seed)
     if log:
         log(f"plain rows {mlp.dataset_rows()} loss {loss:.3e} ({time.perf_counter() - t0:.1f} s)")
     for r in range(guided_rounds):
-        stats = record_guided(mlp, scenarios, params, frames, record_every=record_every)
+        stats = record_guided(mlp, scenarios, params, frames, record_every=record_every, setup=setup)
         loss = _train_chunks(mlp, guided_steps, batch_rows, lr, seed + 1000 * (r + 1))
         if log:
             sw = [s["sweeps"] for s in stats]
@@ -303,7 +315,7 @@ def _train_chunks(mlp, steps, batch_rows, lr, seed, chunk=250):
 
 
 def evaluate(mlp, cloth, params, wind, frames, first_check=FIRST_CHECK, check_every=CHECK_EVERY, use_model=True,
-             lanes=8, max_sweeps=None):
+             lanes=8, max_sweeps=None, setup=None):
     """Frames of `cloth` solved from the learned guess until each frame's
     energy reaches the plain solver's energy after its full iteration count
     from the same frame start (computed on a second context).
@@ -318,17 +330,20 @@ def evaluate(mlp, cloth, params, wind, frames, first_check=FIRST_CHECK, check_ev
     ref = GpuVBD(cloth, params, _lib.LATS_STRATEGY_GRAPH)
     run.set_lanes_per_vertex(lanes)
     ref.set_lanes_per_vertex(lanes)
+    if setup:
+        setup(run)
+        setup(ref)
     mlp.bind(run)
     out = []
     for f in range(frames):
-        w = wind[f % len(wind)]
         x, v = run.state()
         ref.set_state(x, v)
-        ref.set_external_force(w)
+        if wind is not None:
+            ref.set_external_force(wind[f % len(wind)])
+            run.set_external_force(wind[f % len(wind)])
         ref.frame_begin()
         ref.iterate(iters)
         target = ref.energy_now()
-        run.set_external_force(w)
         run.synchronize()
         t0 = time.perf_counter()
         r = mlp.warm_start_frame(run, target, max_sweeps, first_check, check_every, use_model)
