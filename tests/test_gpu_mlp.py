@@ -238,9 +238,7 @@ def _numpy_features(cloth, params, x0, v, corr1, corr2, wind_force):
     F[:, 42] = bdeg / 16
     F[:, 43] = tr * h2m
     F[:, 44] = hb * h2m
-    F[:, 45] = np.linalg.norm(v, axis=1) * h / s0
-    F[:, 46] = np.linalg.norm(corr1, axis=1) / s0
-    F[:, 47] = 1.0
+    # 45-47: obstacle contacts, zero without an obstacle
     return F, xp
 
 
