@@ -88,6 +88,7 @@ def main():
     ap.add_argument("--strategy", default="auto", choices=["auto", "graph", "persistent", "flow"])
     ap.add_argument("--tiles", type=int, default=0, help="flow strategy: vertex tiles (0 = default)")
     ap.add_argument("--lanes", type=int, default=0, help="lanes per vertex (1, 4, 8, 16); 0 = autotuned")
+    ap.add_argument("--regions", type=int, default=0, help="colour-pass cloth bands (1..8); 0 = autotuned")
     ap.add_argument("--no-warm-start", action="store_true", help="skip the learned warm-start measurement")
     ap.add_argument("--warm-scenarios", type=int, default=8, help="training scenarios of the warm start")
     ap.add_argument("--warm-frames", type=int, default=60, help="frames per training scenario")
@@ -123,6 +124,8 @@ def main():
     if args.lanes:
         g.set_lanes_per_vertex(args.lanes)
         strategy, blocks = g.strategy()
+    if args.regions:
+        g.set_regions(args.regions)
     if W:
         g.run_frames(W, wind[:W])
     g.synchronize()
@@ -184,6 +187,7 @@ def main():
                 strategy, "graph"),
             "persistent_blocks_or_tiles": blocks,
             "lanes_per_vertex": g.lanes_per_vertex(),
+            "regions": g.regions(),
             "gpu_ms_per_step": gpu_ms / K,
         },
         "roofline": roofline,
@@ -277,6 +281,8 @@ def bench_sphere(args):
         strategy, blocks = g.strategy()
     if args.lanes:
         g.set_lanes_per_vertex(args.lanes)
+    if args.regions:
+        g.set_regions(args.regions)
     K, W = args.steps, args.warmup
     if W:
         g.run_frames(W)
@@ -299,7 +305,8 @@ def bench_sphere(args):
                                f"contact detection every frame", "gpu_ms_per_step": gpu_ms / K,
                    "strategy": {_lib.LATS_STRATEGY_PERSISTENT: "persistent", _lib.LATS_STRATEGY_FLOW: "flow"}.get(
                        strategy, "graph"),
-                   "lanes_per_vertex": g.lanes_per_vertex(), "contact_vertices": int((counts > 0).sum())},
+                   "lanes_per_vertex": g.lanes_per_vertex(), "regions": g.regions(),
+                   "contact_vertices": int((counts > 0).sum())},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": "k_vbd_colour",
                      "mean_launch_us": round(mean_ms * 1e3, 3),

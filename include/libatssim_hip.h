@@ -128,6 +128,13 @@ int lats_vbd_set_strategy(lats_vbd* ctx, int strategy);
  * parallel and summed in the reference's order (results are identical). */
 int lats_vbd_set_lanes_per_vertex(lats_vbd* ctx, uint32_t lanes);
 int lats_vbd_get_lanes_per_vertex(lats_vbd* ctx, uint32_t* lanes);
+/* Block mapping of the colour passes (graph strategy): the cloth is cut into
+ * `regions` (1..8) bands of consecutive vertex ids and block b of every
+ * colour pass works on band b % regions.  With 8, each XCD updates the same
+ * band in every pass (workgroups go to the XCDs round-robin).  Results are
+ * identical for every value; default 1. */
+int lats_vbd_set_regions(lats_vbd* ctx, uint32_t regions);
+int lats_vbd_get_regions(lats_vbd* ctx, uint32_t* regions);
 /* Grid size of the persistent strategy (0: one thread per vertex of the
  * largest colour, capped at the co-resident limit); for the flow strategy
  * the number of vertex tiles, one block each (0: min(256, V / 128)). */

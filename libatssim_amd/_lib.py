@@ -113,6 +113,8 @@ SIGNATURES = {
     "lats_vbd_set_energy_bending": (_I, [_VP, _I]),
     "lats_vbd_set_persistent_blocks": (_I, [_VP, _U32]),
     "lats_vbd_set_lanes_per_vertex": (_I, [_VP, _U32]),
+    "lats_vbd_set_regions": (_I, [_VP, _U32]),
+    "lats_vbd_get_regions": (_I, [_VP, ctypes.POINTER(_U32)]),
     "lats_vbd_frame_begin": (_I, [_VP]),
     "lats_vbd_iterate": (_I, [_VP, _U32]),
     "lats_vbd_frame_end": (_I, [_VP]),

@@ -45,19 +45,27 @@ namespace {
 constexpr int kBlock = 256;
 
 // One colour pass, G lanes per vertex (G = 1: one thread per vertex).
+// The colour is cut into `rs.n` regions of consecutive vertex ids (regions
+// of the cloth); block b works on region b % n.  Workgroups are handed to the
+// XCDs round-robin, so with n = 8 each XCD updates the same band of the cloth
+// in every colour pass and the neighbours it gathers are mostly lines its own
+// L2 already holds; n = 1 is the plain linear mapping.
 template <int G>
-__global__ __launch_bounds__(kBlock) void k_vbd_colour(VbdArgs a, ColourInfo ci) {
-    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+__global__ __launch_bounds__(kBlock) void k_vbd_colour(VbdArgs a, ColourInfo ci, RegionSplit rs) {
+    const uint32_t r = blockIdx.x % rs.n, j = blockIdx.x / rs.n;
+    const uint32_t lo = rs.bound[r], hi = rs.bound[r + 1];
     const F3 ext = F3{a.fp->ext_cur[0], a.fp->ext_cur[1], a.fp->ext_cur[2]};
     const uint32_t n = ci.count;
     if (G == 1) {
-        if (t >= n) return;
-        vbd_vertex_ext(a, ci.start + t, t, n, ci.eoff, ci.boff, ext);
+        const uint32_t i = lo + j * kBlock + threadIdx.x;
+        if (i >= hi) return;
+        vbd_vertex_ext(a, ci.start + i, i, n, ci.eoff, ci.boff, ext);
     } else {
-        const uint32_t i = t / G, lane = t % G;
-        if (i - (i % (64 / G)) >= n) return;      // the whole wave is past the colour
-        const bool valid = i < n;
-        vbd_vertex_group<G>(a, ci.start + (valid ? i : 0u), valid ? i : 0u, n, ci.eoff, ci.boff, ci.ew, ci.bw, ext,
+        const uint32_t first = lo + j * (kBlock / G) + (threadIdx.x / 64) * (64 / G);
+        if (first >= hi) return;                  // the whole wave is past the region
+        const uint32_t i = lo + j * (kBlock / G) + threadIdx.x / G, lane = threadIdx.x % G;
+        const bool valid = i < hi;
+        vbd_vertex_group<G>(a, ci.start + (valid ? i : lo), valid ? i : lo, n, ci.eoff, ci.boff, ci.ew, ci.bw, ext,
                             lane, valid);
     }
 }
@@ -649,16 +657,35 @@ void enqueue_energy(lats_vbd* c, const float* xb, uint32_t row) {
                        c->partial.p + (size_t)row * c->energy_nblk);
 }
 
+// Cuts every colour into `n` regions of consecutive vertex ids (a colour's
+// slots are in ascending vertex id, so a region is an index range).
+void split_regions(lats_vbd* c, uint32_t n) {
+    const uint32_t per = (c->V + n - 1) / n;
+    c->region_split.assign(c->C, RegionSplit{});
+    for (uint32_t k = 0; k < c->C; k++) {
+        const uint32_t* ids = c->perm.data() + c->colours[k].start;
+        const uint32_t cnt = c->colours[k].count;
+        RegionSplit& rs = c->region_split[k];
+        rs.n = n;
+        for (uint32_t r = 0; r <= n; r++)
+            rs.bound[r] = r == n ? cnt : (uint32_t)(std::lower_bound(ids, ids + cnt, r * per) - ids);
+    }
+    c->regions = n;
+}
+
 void enqueue_colour(lats_vbd* c, float* xb, uint32_t col) {
-    const uint32_t n = c->ccount[col];
-    const uint64_t threads = (uint64_t)n * c->lanes;
-    const dim3 grid(nblocks(threads)), block(kBlock);
+    const RegionSplit& rs = c->region_split[col];
+    uint32_t most = 0;
+    for (uint32_t r = 0; r < rs.n; r++) most = std::max(most, rs.bound[r + 1] - rs.bound[r]);
+    const uint32_t per_block = kBlock / c->lanes;
+    const dim3 grid(std::max(1u, rs.n * ((most + per_block - 1) / per_block))), block(kBlock);
     const VbdArgs a = c->args(xb);
+    const ColourInfo& ci = c->colours[col];
     switch (c->lanes) {
-        case 4: hipLaunchKernelGGL(k_vbd_colour<4>, grid, block, 0, c->stream, a, c->colours[col]); break;
-        case 8: hipLaunchKernelGGL(k_vbd_colour<8>, grid, block, 0, c->stream, a, c->colours[col]); break;
-        case 16: hipLaunchKernelGGL(k_vbd_colour<16>, grid, block, 0, c->stream, a, c->colours[col]); break;
-        default: hipLaunchKernelGGL(k_vbd_colour<1>, grid, block, 0, c->stream, a, c->colours[col]); break;
+        case 4: hipLaunchKernelGGL(k_vbd_colour<4>, grid, block, 0, c->stream, a, ci, rs); break;
+        case 8: hipLaunchKernelGGL(k_vbd_colour<8>, grid, block, 0, c->stream, a, ci, rs); break;
+        case 16: hipLaunchKernelGGL(k_vbd_colour<16>, grid, block, 0, c->stream, a, ci, rs); break;
+        default: hipLaunchKernelGGL(k_vbd_colour<1>, grid, block, 0, c->stream, a, ci, rs); break;
     }
 }
 
@@ -1001,6 +1028,7 @@ int lats_vbd_create(const lats_mesh_desc* m, const lats_scene_params* prm, lats_
     c->perm = L.perm;
     c->inv = inv;
     c->colours = L.colours;
+    split_regions(c, 1);
     for (const ColourInfo& ci : L.colours) {
         c->cstart.push_back(ci.start);
         c->ccount.push_back(ci.count);
@@ -1135,6 +1163,23 @@ int lats_vbd_set_lanes_per_vertex(lats_vbd* c, uint32_t lanes) {
     return LATS_OK;
 }
 
+int lats_vbd_set_regions(lats_vbd* c, uint32_t regions) {
+    if (!c) return fail(LATS_ERR_INVALID_ARGUMENT, "null context");
+    if (regions < 1 || regions > kMaxRegions)
+        return fail(LATS_ERR_INVALID_ARGUMENT, "regions must be between 1 and 8");
+    if (regions != c->regions) {
+        split_regions(c, regions);
+        c->drop_graphs();
+    }
+    return LATS_OK;
+}
+
+int lats_vbd_get_regions(lats_vbd* c, uint32_t* regions) {
+    if (!c || !regions) return fail(LATS_ERR_INVALID_ARGUMENT, "null argument");
+    *regions = c->regions;
+    return LATS_OK;
+}
+
 int lats_vbd_get_lanes_per_vertex(lats_vbd* c, uint32_t* lanes) {
     if (!c || !lanes) return fail(LATS_ERR_INVALID_ARGUMENT, "null argument");
     *lanes = c->lanes;
@@ -1182,11 +1227,12 @@ int lats_vbd_autotune(lats_vbd* c, uint32_t frames, int* chosen, uint32_t* chose
         HIPCHK(hipMemcpy(c->fp.p, &fps, sizeof(fps), hipMemcpyHostToDevice));
         return LATS_OK;
     };
-    struct Cand { int strategy; uint32_t blocks; uint32_t lanes; };
+    struct Cand { int strategy; uint32_t blocks; uint32_t lanes; uint32_t regions = 1; };
     std::vector<Cand> cands;
-    const uint32_t keep_lanes = c->lanes;
+    const uint32_t keep_lanes = c->lanes, keep_regions = c->regions;
     for (uint32_t lanes : {1u, 4u, 8u, 16u}) {
-        cands.push_back({LATS_STRATEGY_GRAPH, 0, lanes});
+        cands.push_back({LATS_STRATEGY_GRAPH, 0, lanes, 1});
+        cands.push_back({LATS_STRATEGY_GRAPH, 0, lanes, kMaxRegions});
         if (c->prm.num_substep != 1) continue;
         uint32_t g0 = 0;
         const uint32_t keep = c->persist_blocks;
@@ -1210,20 +1256,23 @@ int lats_vbd_autotune(lats_vbd* c, uint32_t frames, int* chosen, uint32_t* chose
     Cand win = cands[0];
     const int keep_strategy = c->strategy;
     const uint32_t keep_blocks = c->persist_blocks;
+    auto use = [&](uint32_t lanes, uint32_t regions) {
+        if (c->lanes != lanes || c->regions != regions) {
+            c->lanes = lanes;
+            if (c->regions != regions) split_regions(c, regions);
+            c->drop_graphs();
+        }
+    };
     for (const Cand& k : cands) {
         c->strategy = k.strategy;
         c->persist_blocks = k.blocks;
-        if (c->lanes != k.lanes) {
-            c->lanes = k.lanes;
-            c->drop_graphs();
-        }
+        use(k.lanes, k.regions);
         int rc = lats_vbd_run_frames(c, 1, nullptr);   // warm-up / capture
         if (!rc) rc = lats_vbd_run_frames(c, frames, nullptr);
         if (rc) {
             c->strategy = keep_strategy;
             c->persist_blocks = keep_blocks;
-            c->lanes = keep_lanes;
-            c->drop_graphs();
+            use(keep_lanes, keep_regions);
             restore();
             return rc;
         }
@@ -1233,10 +1282,7 @@ int lats_vbd_autotune(lats_vbd* c, uint32_t frames, int* chosen, uint32_t* chose
     }
     c->strategy = win.strategy;
     c->persist_blocks = win.blocks;
-    if (c->lanes != win.lanes) {
-        c->lanes = win.lanes;
-        c->drop_graphs();
-    }
+    use(win.lanes, win.regions);
     if (chosen) *chosen = win.strategy;
     if (chosen_blocks) *chosen_blocks = win.blocks;
     return LATS_OK;

@@ -17,6 +17,14 @@ struct ColourInfo {
     uint32_t start, count, eoff, boff, ew, bw;
 };
 
+// A colour cut into n regions of consecutive vertex ids: region r holds the
+// colour's indices [bound[r], bound[r + 1]).
+constexpr uint32_t kMaxRegions = 8;
+struct RegionSplit {
+    uint32_t n;
+    uint32_t bound[kMaxRegions + 1];
+};
+
 struct FrameParams {
     const float* forces;   // per-frame external force (3 floats / frame) or null
     uint32_t frame;        // frame counter, advanced by the last kernel of a frame
@@ -259,25 +267,23 @@ __device__ __forceinline__ void vbd_vertex_group(const VbdArgs& a, uint32_t s, u
     Sym3 S = sym_zero();
     F3 fs = F3{0.f, 0.f, 0.f};
     // only lane 0 of the group uses the sums: it adds the terms of lanes
-    // 0, 1, ..., cnt-1 in that order
-    auto add_spring_round = [&](uint32_t r, const Sym3& t, const F3& tf) {
-        const uint32_t cnt = (edeg - r < (uint32_t)G) ? edeg - r : (uint32_t)G;
+    // 0, 1, ..., G-1 in that order.  Lanes past the degree hold +0 terms, and
+    // adding +0 leaves a sum that started at +0 unchanged (in round-to-nearest
+    // such a sum is never -0), so the sums equal the reference's sums over the
+    // incident springs alone.  Unconditional adds let every DPP move fold into
+    // its add (one v_add_f32_dpp per term).
+    auto add_spring_round = [&](const Sym3& t, const F3& tf) {
 #pragma unroll
         for (int q = 0; q < G; q++) {
-            const float vxx = lane_plus(t.xx, q), vxy = lane_plus(t.xy, q), vxz = lane_plus(t.xz, q);
-            const float vyy = lane_plus(t.yy, q), vyz = lane_plus(t.yz, q), vzz = lane_plus(t.zz, q);
-            const float vfx = lane_plus(tf.x, q), vfy = lane_plus(tf.y, q), vfz = lane_plus(tf.z, q);
-            if ((uint32_t)q < cnt) {
-                S.xx = S.xx + vxx;
-                S.xy = S.xy + vxy;
-                S.xz = S.xz + vxz;
-                S.yy = S.yy + vyy;
-                S.yz = S.yz + vyz;
-                S.zz = S.zz + vzz;
-                fs.x = fs.x + vfx;
-                fs.y = fs.y + vfy;
-                fs.z = fs.z + vfz;
-            }
+            S.xx = S.xx + lane_plus(t.xx, q);
+            S.xy = S.xy + lane_plus(t.xy, q);
+            S.xz = S.xz + lane_plus(t.xz, q);
+            S.yy = S.yy + lane_plus(t.yy, q);
+            S.yz = S.yz + lane_plus(t.yz, q);
+            S.zz = S.zz + lane_plus(t.zz, q);
+            fs.x = fs.x + lane_plus(tf.x, q);
+            fs.y = fs.y + lane_plus(tf.y, q);
+            fs.z = fs.z + lane_plus(tf.z, q);
         }
     };
 #pragma unroll
@@ -287,7 +293,7 @@ __device__ __forceinline__ void vbd_vertex_group(const VbdArgs& a, uint32_t s, u
         Sym3 t;
         F3 tf;
         spring_term_of(a, xk, r + lane < edeg, pe[k], pxo[k], t, tf);
-        add_spring_round(r, t, tf);
+        add_spring_round(t, tf);
     }
     for (uint32_t r = PE * G; r < edeg; r += G) {
         const uint32_t j = r + lane;
@@ -296,24 +302,20 @@ __device__ __forceinline__ void vbd_vertex_group(const VbdArgs& a, uint32_t s, u
         Sym3 t;
         F3 tf;
         spring_term_of(a, xk, on, e, on ? Mem::ld(a.x, e.x) : F3{0.f, 0.f, 0.f}, t, tf);
-        add_spring_round(r, t, tf);
+        add_spring_round(t, tf);
     }
     sym_add(H, S);
     f = add3(f, fs);
 
     float hb = 0.f;
     F3 fb = F3{0.f, 0.f, 0.f};
-    auto add_pair_round = [&](uint32_t r, float th, const F3& tf) {
-        const uint32_t cnt = (bdeg - r < (uint32_t)G) ? bdeg - r : (uint32_t)G;
+    auto add_pair_round = [&](float th, const F3& tf) {   // +0 past the degree, as above
 #pragma unroll
         for (int q = 0; q < G; q++) {
-            const float vh = lane_plus(th, q), vx = lane_plus(tf.x, q), vy = lane_plus(tf.y, q), vz = lane_plus(tf.z, q);
-            if ((uint32_t)q < cnt) {
-                hb = hb + vh;
-                fb.x = fb.x + vx;
-                fb.y = fb.y + vy;
-                fb.z = fb.z + vz;
-            }
+            hb = hb + lane_plus(th, q);
+            fb.x = fb.x + lane_plus(tf.x, q);
+            fb.y = fb.y + lane_plus(tf.y, q);
+            fb.z = fb.z + lane_plus(tf.z, q);
         }
     };
 #pragma unroll
@@ -323,7 +325,7 @@ __device__ __forceinline__ void vbd_vertex_group(const VbdArgs& a, uint32_t s, u
         float th = 0.f;
         F3 tf = F3{0.f, 0.f, 0.f};
         if (r + lane < bdeg) bending_term(pq[k], pi[k].x >> 30, pp[k][0], pp[k][1], pp[k][2], pp[k][3], a.kb, th, tf);
-        add_pair_round(r, th, tf);
+        add_pair_round(th, tf);
     }
     for (uint32_t r = PB * G; r < bdeg; r += G) {
         const uint32_t j = r + lane;
@@ -337,7 +339,7 @@ __device__ __forceinline__ void vbd_vertex_group(const VbdArgs& a, uint32_t s, u
             bending_term(q, o, pair_point<Mem>(a, xk, o, 0, id.x & 0x3fffffffu), pair_point<Mem>(a, xk, o, 1, id.y),
                          pair_point<Mem>(a, xk, o, 2, id.z), pair_point<Mem>(a, xk, o, 3, id.w), a.kb, th, tf);
         }
-        add_pair_round(r, th, tf);
+        add_pair_round(th, tf);
     }
     H.xx = H.xx + hb;
     H.yy = H.yy + hb;

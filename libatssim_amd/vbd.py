@@ -187,6 +187,15 @@ class GpuVBD:
         if persistent_blocks is not None:
             check(lib().lats_vbd_set_persistent_blocks(self._h, int(persistent_blocks)))
 
+    def set_regions(self, regions):
+        """Bands of the colour-pass block mapping (1..8, results identical)."""
+        check(lib().lats_vbd_set_regions(self._h, int(regions)))
+
+    def regions(self):
+        n = ctypes.c_uint32()
+        check(lib().lats_vbd_get_regions(self._h, ctypes.byref(n)))
+        return n.value
+
     def set_lanes_per_vertex(self, lanes):
         check(lib().lats_vbd_set_lanes_per_vertex(self._h, int(lanes)))
 

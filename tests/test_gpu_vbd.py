@@ -313,6 +313,47 @@ def test_persistent_c1_and_autotune(gpu_available):
     assert np.array_equal(a.state()[0], b.state()[0])
 
 
+@pytest.mark.parametrize("lanes", [1, 4, 8, 16])
+def test_region_mapping_bit_exact(gpu_available, lanes):
+    """Colour passes with blocks mapped to 8 cloth bands == the linear mapping."""
+    _print_case(f"region block mapping, {lanes} lanes, grid_c1 + wind, 3 frames")
+    cloth, params, _, wind = configs.grid_c1(frames=3)
+    a = GpuVBD(cloth, params)
+    a.set_strategy(_lib.LATS_STRATEGY_GRAPH)
+    a.set_lanes_per_vertex(lanes)
+    b = GpuVBD(cloth, params)
+    b.set_strategy(_lib.LATS_STRATEGY_GRAPH)
+    b.set_lanes_per_vertex(lanes)
+    b.set_regions(8)
+    assert b.regions() == 8
+    a.run_frames(3, wind)
+    b.run_frames(3, wind)
+    assert np.array_equal(a.state()[0], b.state()[0])
+    assert np.array_equal(a.state()[1], b.state()[1])
+
+
+@pytest.mark.parametrize("n,regions", [(2, 8), (3, 8), (5, 3), (17, 7)])
+def test_region_mapping_small_cloths(gpu_available, n, regions):
+    """Bands with no vertex of a colour (cloths smaller than the band count)."""
+    _print_case(f"region block mapping, {n}x{n} cloth, {regions} regions")
+    cloth = configs.grid_cloth(n, "top", seed=n)
+    params = Params(constraint_iter_count=10)
+    b = GpuVBD(cloth, params)
+    b.set_strategy(_lib.LATS_STRATEGY_GRAPH)
+    b.set_lanes_per_vertex(8)
+    b.set_regions(regions)
+    o = oracle_sim(cloth, params)
+    for _ in range(3):
+        b.physics_step()
+        o.step()
+    assert np.array_equal(b.state()[0], o.state()[0].astype(np.float32))
+    with pytest.raises(_lib.LatsError):
+        b.set_regions(0)
+    with pytest.raises(_lib.LatsError):
+        b.set_regions(9)
+    assert b.regions() == regions
+
+
 @pytest.mark.parametrize("tiles,lanes", [(1, 8), (4, 1), (4, 8), (16, 4), (32, 8), (32, 16)])
 def test_flow_strategy_bit_exact(gpu_available, tiles, lanes):
     """Tiles handing colour passes to their neighbours == graph path, bit for bit."""
