@@ -91,9 +91,18 @@ struct Task {
     bool is_computation_time_constant = false;
     std::vector<Implementation> list_implementation;
 
-    uint buffer_left = ~0u, buffer_right = ~0u, buffer_idx = ~0u;
+    // asynchronous-iteration fields, set by the post-pass of scheduler_dag
+    // (launcher.h:605-615): the position buffer the task iterates, the task
+    // (and its buffer) on the same processor it continues from ("left") or
+    // hands a copy to ("right"), and the tasks of the other processor whose
+    // buffers it merges ("ins")
+    uint buffer_left = ~0u, task_left = ~0u;
+    uint buffer_right = ~0u, task_right = ~0u;
+    uint buffer_idx = ~0u;
     bool is_allocated_to_main_device = true;
-    std::vector<uint> buffer_ins;
+    bool is_first_iterative_task = false;
+    std::vector<uint> buffer_ins, buffer_outs;
+    std::vector<uint> task_ins, task_outs;
 
     std::vector<uint> predecessors;
     std::vector<uint> successors;
@@ -128,8 +137,32 @@ struct ScheduleEvent {
     bool operator<(const ScheduleEvent& r) const { return end != r.end ? end < r.end : start < r.start; }
 };
 
+// A run of consecutive tasks of one processor's schedule [start_idx,
+// end_idx] that first waits for event `wait` of the other processor and, when
+// done, signals (its own index in `signal`) (launcher.h:818-832).
 struct LaunchEvent {
     uint start_idx = 0, end_idx = 0, wait = ~0u, signal = ~0u;
+    LaunchEvent() = default;
+    LaunchEvent(uint s, uint e, uint w, uint g) : start_idx(s), end_idx(e), wait(w), signal(g) {}
+};
+
+// How LaunchModeHetero synchronises the two processors (the reference uses a
+// MTLSharedEvent and command buffers, launcher.cpp:918-1111).  Processor 0 is
+// the calling host thread; processor 1 is an asynchronous queue:
+//   gpu_begin(e, wait):  start queue event e; if wait != -1u the queue first
+//                        waits until the host has signalled a value >= wait + 1
+//   gpu_end(e):          close event e (its completion can be waited for)
+//   cpu_wait_gpu(e):     block the host until queue event e has completed
+//   cpu_signal(v):       host signal value v (monotonic)
+//   gpu_finish():        block until every queue event has completed
+//   gpu_elapsed_ms(e):   queue time of event e (refits the computation matrix)
+struct HeteroHooks {
+    std::function<void(uint, uint)> gpu_begin = [](uint, uint) {};
+    std::function<void(uint)> gpu_end = [](uint) {};
+    std::function<void(uint)> cpu_wait_gpu = [](uint) {};
+    std::function<void(uint)> cpu_signal = [](uint) {};
+    std::function<void()> gpu_finish = []() {};
+    std::function<float(uint)> gpu_elapsed_ms = [](uint) { return 0.f; };
 };
 
 const float inf = 99999.0f;
@@ -168,13 +201,31 @@ public:
     float fn_get_communication_cost(uint proc, uint pred_node, uint node) const;
     float fn_get_inner_communication_cost(uint proc) const;
 
+    // Computation matrix from a (function id, cluster) -> per-processor cost
+    // table; tasks without an implementation on a processor cost inf there
+    // (launcher.cpp:645-704).  cost_total, if empty, receives the per-processor sums.
+    void profile_from(const std::vector<std::pair<FunctionID, uint>>& map_tasks,
+                      const std::vector<std::vector<double>>& map_costs, std::vector<double>& cost_total);
+    // Segments both processors' schedules into LaunchEvents with cross-processor
+    // waits and signals (launcher.cpp:3364-3714; two processors only).
+    bool make_wait_events();
+    const std::vector<std::vector<LaunchEvent>>& get_launch_events() const { return launch_events; }
+
     // ---- execution
-    void launch(LaunchMode mode, const std::function<LaunchParam(const Task&)>& task_to_param);
+    // LaunchModeHetero needs `hooks`; the other modes run on the calling thread.
+    void launch(LaunchMode mode, const std::function<LaunchParam(const Task&)>& task_to_param,
+                const HeteroHooks* hooks = nullptr);
 
     // ---- results / reporting
     float get_scheduled_end_time() const;
     std::vector<float> get_scheduled_speedups() const;   // (single-device cost - makespan) / makespan
     std::string proc_schedules_string() const;
+    // The schedule in the reference's Gantt dump format
+    // (print_schedule_to_graph_xpbd, launcher.cpp:3915-4126): per processor
+    // (colour, task, buffer, start, end), then the cross-processor inputs and
+    // the same-processor "left" links as (proc, task, proc, task, end, start).
+    std::string schedule_graph_string() const;
+    void print_schedule_to_graph_xpbd() const;
     void print_proc_schedules() const;
     void print_speedups_to_each_device() const;
     void update_costs_from_computation_matrix();
@@ -195,6 +246,8 @@ public:
 
 private:
     void compute_ranku(uint num_procs);
+    void async_post_pass();
+    void refit_runtime_cost(const std::vector<float>& cpu_ms, const std::vector<float>& gpu_ms);
     ScheduleEvent compute_eft(uint node, uint proc) const;
     double avg_cost(uint tid, uint num_procs) const;
 
@@ -205,6 +258,8 @@ private:
     std::vector<ScheduleEvent> task_schedules;
     std::vector<double> ranku;
     std::vector<ListSchedule> proc_schedules;
+    std::vector<std::vector<LaunchEvent>> launch_events;
+    std::vector<std::vector<uint>> list_cmd_idx;
     bool print_detail = false;
     bool use_check = true;
 };
