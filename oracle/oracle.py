@@ -60,6 +60,10 @@ def lib():
                                            ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_int]
         L.orc_sim_contacts.restype = ctypes.c_uint32
         L.orc_sim_contacts.argtypes = [ctypes.c_void_p, _u32p, _i32p, ctypes.c_uint32]
+        L.orc_sim_async_begin.argtypes = [ctypes.c_void_p]
+        L.orc_sim_dispatch.argtypes = [ctypes.c_void_p] + [ctypes.c_uint32] * 7 + [_u32p, ctypes.c_int, ctypes.c_int]
+        L.orc_sim_energies.restype = ctypes.c_uint32
+        L.orc_sim_energies.argtypes = [ctypes.c_void_p, _f32p, ctypes.c_uint32]
         L.orc_template_vf.restype = ctypes.c_int
         L.orc_template_vf.argtypes = [_f32p, ctypes.c_float, _f32p, _f32p]
         _LIB = L
@@ -162,6 +166,21 @@ class Sim:
 
     def energy(self):
         return lib().orc_sim_energy(self._h)
+
+    # asynchronous frame (CpuSolver::physics_step_vbd_async + fn_dispatch)
+    def async_begin(self):
+        lib().orc_sim_async_begin(self._h)
+
+    def dispatch(self, fn, iter_idx, cluster_idx, buffer_idx, left, right, main, ins=(), record=False):
+        ins = np.ascontiguousarray(ins, np.uint32)
+        lib().orc_sim_dispatch(self._h, int(fn), int(iter_idx), int(cluster_idx), int(buffer_idx), int(left),
+                               int(right), len(ins), _ptr(ins, _u32p) if len(ins) else None, int(main), int(record))
+
+    def energies(self):
+        n = lib().orc_sim_energies(self._h, None, 0)
+        e = np.zeros(max(n, 1), np.float32)
+        lib().orc_sim_energies(self._h, _ptr(e, _f32p), n)
+        return e[:n]
 
     def set_obstacle(self, x, faces, thickness, stiffness, radius, max_contacts=4):
         self._obs = (np.ascontiguousarray(x, np.float32), np.ascontiguousarray(faces, np.int32))

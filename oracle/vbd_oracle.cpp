@@ -251,6 +251,9 @@ template <typename T> struct Sim {
     int max_contacts = 0;
     struct Contact { int fid; V3<T> xs, n; };
     std::vector<std::vector<Contact>> contacts;  // per vertex, ascending fid
+    // asynchronous iteration: 32 iterate + 32 begin position buffers,
+    // indexed modulo 32 (example2_async_iter.cpp:1147-1157)
+    std::vector<V3<T>> iter_buf[32], begin_buf[32];
 };
 
 template <typename T> static inline T substep_dt(const Params& p) {
@@ -588,6 +591,138 @@ template <typename T> static void physics_step(Sim<T>& s, bool record) {
     }
 }
 
+// One colour pass (a2-a5) over cluster `c` on the positions in s.x
+// (example2_async_iter.cpp:939-1014, as in solve_vbd_iteration).
+template <typename T> static void solve_cluster(Sim<T>& s, u32 c) {
+    const T h = substep_dt<T>(s.p);
+    const auto& cl = s.topo->clusters[c];
+    const long n = (long)cl.size();
+#pragma omp parallel for schedule(static, 256) num_threads(s.nthreads)
+    for (long i = 0; i < n; i++) s.hf[cl[i]] = compute_inertia(s, cl[i], h);
+#pragma omp parallel for schedule(static, 32) num_threads(s.nthreads)
+    for (long i = 0; i < n; i++) { Hf<T> a = compute_stretch(s, cl[i]); hf_add(s.hf[cl[i]], a); }
+#pragma omp parallel for schedule(static, 32) num_threads(s.nthreads)
+    for (long i = 0; i < n; i++) { Hf<T> a = compute_bending(s, cl[i]); hf_add(s.hf[cl[i]], a); }
+    if (!s.contacts.empty()) {
+#pragma omp parallel for schedule(static, 32) num_threads(s.nthreads)
+        for (long i = 0; i < n; i++) { Hf<T> a = compute_contact(s, cl[i]); hf_add(s.hf[cl[i]], a); }
+    }
+#pragma omp parallel for schedule(static, 32) num_threads(s.nthreads)
+    for (long i = 0; i < n; i++) vbd_step(s, cl[i]);
+}
+
+// Core::read_and_solve_conflict (xpbd_constraints.h:815-869) for one vertex:
+// both iterates moved away from the common begin position => begin + w (sum
+// of both moves); one moved => that one; written to self and begin.
+template <typename T>
+static void read_and_solve_conflict(u32 vid, std::vector<V3<T>>& begin, std::vector<V3<T>>& self,
+                                    const std::vector<V3<T>>& other, T weight) {
+    const V3<T> ps = self[vid], po = other[vid];
+    V3<T> w;
+    if (weight > T(1.19209290E-7f)) {
+        const V3<T> init = begin[vid];
+        const V3<T> ds = ps - init, da = po - init;
+        w = init;
+        const bool act_a = dot(ds, ds) != T(0), act_b = dot(da, da) != T(0);
+        if (act_a && act_b) w = init + weight * (ds + da);
+        else if (act_a) w = ps;
+        else if (act_b) w = po;
+    } else {
+        w = T(0.5f) * (ps + po);
+    }
+    self[vid] = w;
+    begin[vid] = w;
+}
+
+// Function ids of the dispatched tasks (include/libatssim_hip.h LATS_FN_*).
+enum : u32 { FN_ROOT = 0, FN_TERMINAL = 1, FN_PREDICT = 2, FN_ALL_IN_ONE = 3, FN_LAST_NODE = 4, FN_UPDATE_VELOCITY = 5,
+             FN_RESET = 6 };
+constexpr u32 kNoBuffer = 0xFFFFFFFFu, kInputBuffer = 0x0FFFFFFFu;
+
+struct DispatchParam {
+    u32 fn, iter, cluster, buffer, left, right, nin;
+    const u32* ins;
+    bool main;
+};
+
+// CpuSolver::fn_dispatch (example2_async_iter.cpp:1144-1304): the task's
+// colour pass runs on its iterate buffer between the prev hook (merge the
+// inputs with the left buffer, or copy the last input / the predicted
+// positions in) and the post hook (copy to the right buffer); energies are
+// recorded where the reference's hooks record them.
+template <typename T> static void dispatch(Sim<T>& s, const DispatchParam& p, bool record) {
+    const size_t nv = s.x.size();
+    auto iter = [&](u32 b) -> std::vector<V3<T>>& {
+        auto& v = s.iter_buf[b % 32];
+        if (v.size() != nv) v.assign(nv, mk(T(0), T(0), T(0)));
+        return v;
+    };
+    auto begin = [&](u32 b) -> std::vector<V3<T>>& {
+        auto& v = s.begin_buf[b % 32];
+        if (v.size() != nv) v.assign(nv, mk(T(0), T(0), T(0)));
+        return v;
+    };
+    auto copy_to_start_and_iter = [&](const std::vector<V3<T>>& src, u32 out) {
+        const std::vector<V3<T>> tmp = src;   // src may be one of the outputs
+        begin(out) = tmp;
+        iter(out) = tmp;
+    };
+    auto energy_of = [&](std::vector<V3<T>>& buf) {
+        std::swap(s.x, buf);
+        s.energy.push_back((float)compute_energy(s));
+        std::swap(s.x, buf);
+    };
+    auto prev = [&]() {
+        if (p.nin && p.left != kNoBuffer) {
+            for (u32 k = 0; k < p.nin; k++) {
+                const u32 in = p.ins[k];
+                std::vector<V3<T>>& bb = p.main ? begin(in) : begin(p.left);
+                std::vector<V3<T>>& self = iter(in);
+                const std::vector<V3<T>>& other = iter(p.left);
+                for (size_t v = 0; v < nv; v++) read_and_solve_conflict((u32)v, bb, self, other, T(0.5f));
+            }
+        } else if (p.nin) {
+            copy_to_start_and_iter(iter(p.ins[p.nin - 1]), p.buffer);
+        } else if (p.left == kInputBuffer) {
+            copy_to_start_and_iter(s.x, p.buffer);
+        }
+        if (record && p.iter == 0 && p.cluster == 0) energy_of(iter(p.buffer));
+    };
+    auto post = [&]() {
+        if (p.right != kNoBuffer) copy_to_start_and_iter(iter(p.buffer), p.right);
+        if (record && ((p.fn == FN_ALL_IN_ONE && p.cluster + 1 == (u32)s.topo->clusters.size()) || p.fn == FN_LAST_NODE))
+            energy_of(iter(p.buffer));
+    };
+    switch (p.fn) {
+        case FN_PREDICT: predict_position(s); break;
+        case FN_UPDATE_VELOCITY: update_velocity(s); break;
+        case FN_ALL_IN_ONE: {
+            prev();
+            std::vector<V3<T>>& buf = iter(p.buffer);
+            std::swap(s.x, buf);
+            solve_cluster(s, p.cluster);
+            std::swap(s.x, buf);
+            post();
+            break;
+        }
+        case FN_LAST_NODE:
+            prev();
+            post();
+            s.x = iter(p.buffer);
+            break;
+        default: break;
+    }
+}
+
+// physics_step_vbd_async frame bracket (example2_async_iter.cpp:1305-1313, 1471-1479)
+template <typename T> static void async_begin(Sim<T>& s) {
+    s.x_frame_start = s.x;
+    s.v_frame_start = s.v;
+    s.x_start = s.x_frame_start;
+    s.v_start = s.v_frame_start;
+    s.energy.clear();
+}
+
 }  // namespace orc
 
 // ---------------------------------------------------------------------------
@@ -783,6 +918,25 @@ int orc_template_vf(const float* p4, float min_distance, float* distance, float*
     *distance = d;
     for (int k = 0; k < 3; k++) bary[k] = b[k];
     return hit ? 1 : 0;
+}
+
+// Asynchronous frame: orc_sim_async_begin, then the scheduled tasks in launch
+// order through orc_sim_dispatch, then the recorded energies.
+void orc_sim_async_begin(orc_sim* s) {
+    if (s->fp64) async_begin(s->s64);
+    else async_begin(s->s32);
+}
+void orc_sim_dispatch(orc_sim* s, u32 fn, u32 iter, u32 cluster, u32 buffer, u32 left, u32 right, u32 nin,
+                      const u32* ins, int main_device, int record) {
+    const DispatchParam p{fn, iter, cluster, buffer, left, right, nin, ins, main_device != 0};
+    if (s->fp64) dispatch(s->s64, p, record != 0);
+    else dispatch(s->s32, p, record != 0);
+}
+u32 orc_sim_energies(const orc_sim* s, float* energy, u32 cap) {
+    const std::vector<float>& e = s->fp64 ? s->s64.energy : s->s32.energy;
+    const u32 n = std::min<u32>(cap, (u32)e.size());
+    if (energy) std::copy(e.begin(), e.begin() + n, energy);
+    return (u32)e.size();
 }
 
 int orc_max_threads(void) {

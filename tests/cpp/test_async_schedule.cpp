@@ -13,6 +13,7 @@
 #include <cstdlib>
 #include <vector>
 
+#include "../../include/libatssim_hip.h"
 #include "host/launcher.h"
 #include "host/vbd_slot.h"
 
@@ -83,6 +84,28 @@ int main(int argc, char** argv) {
         for (size_t t = 0; t < tasks.size(); t++)
             if (when[t] == (int)k) std::printf(" %zu", t);
     std::printf("\n");
+    // the launch sequence for replays: k, scheduled processor, then the
+    // LaunchParam fields (C-ABI function id, iteration, cluster, buffer,
+    // left, right, main device, inputs)
+    auto c_fn = [](Launcher::FunctionID f) -> int {
+        switch (f) {
+            case Launcher::id_xpbd_predict_position: return LATS_FN_PREDICT_POSITION;
+            case Launcher::id_vbd_all_in_one: return LATS_FN_VBD_ALL_IN_ONE;
+            case Launcher::id_xpbd_constraint_last_node: return LATS_FN_CONSTRAINT_LAST_NODE;
+            case Launcher::id_xpbd_update_velocity: return LATS_FN_UPDATE_VELOCITY;
+            default: return -1;
+        }
+    };
+    for (size_t k = 0; k < params.size(); k++)
+        for (size_t t = 0; t < tasks.size(); t++) {
+            if (when[t] != (int)k) continue;
+            const auto& p = params[k];
+            std::printf("PARAM %zu %u %d %u %u %u %u %u %d %zu", k, s.get_task_schedules()[t].proc, c_fn(p.function_id),
+                        p.iter_idx, p.cluster_idx, p.buffer_idx, p.left_buffer_idx, p.right_buffer_idx,
+                        p.is_allocated_to_main_device ? 1 : 0, p.input_buffer_idxs.size());
+            for (unsigned b : p.input_buffer_idxs) std::printf(" %u", b);
+            std::printf("\n");
+        }
     for (size_t t = 0; t < tasks.size(); t++) {
         // the root and terminal the scheduler adds carry empty implementations
         if (tasks[t].func_id == Launcher::id_additional_root || tasks[t].func_id == Launcher::id_additional_terminal)

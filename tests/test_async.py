@@ -13,6 +13,7 @@ import json
 import os
 import subprocess
 
+import numpy as np
 import pytest
 
 from tests.helpers import GOLDEN
@@ -74,3 +75,58 @@ def test_async_schedules_run_in_dependency_order(iters, clusters):
         colour, tid, buf = t[0], t[1], t[2]
         if colour >= 14:               # vbd_all_in_one tasks
             assert buf != 4294967295, t
+
+
+# ---------------------------------------------------------------------------
+# the asynchronous frame on the oracle vs the reference's published history
+# ---------------------------------------------------------------------------
+def launch_sequence(iters, clusters):
+    """The LaunchModeSequeceHetero sequence of the async schedule: a list of
+    (proc, fn, iter, cluster, buffer, left, right, main, ins)."""
+    r = subprocess.run([BIN, str(iters), str(clusters)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and "PASS" in r.stdout, r.stdout + r.stderr
+    seq = []
+    for line in r.stdout.split("\n"):
+        if not line.startswith("PARAM "):
+            continue
+        v = [int(t) for t in line.split()[1:]]
+        k, proc, fn, it, cl, buf, left, right, main, nin = v[:10]
+        ins = v[10:10 + nin]
+        if fn >= 0:
+            seq.append((proc, fn, it, cl, buf, left, right, main, tuple(ins)))
+    return seq
+
+
+def replay_on_oracle(sim, seq, record):
+    sim.async_begin()
+    for proc, fn, it, cl, buf, left, right, main, ins in seq:
+        sim.dispatch(fn, it, cl, buf, left, right, main, ins, record=record)
+    return sim.energies()
+
+
+def test_square8k_async_frame_matches_published():
+    """example2's main (examples/example2_async_iter.cpp:1752-1805): 30 frames,
+    then one asynchronous frame from that state with the energy recorded at
+    the reference's hook points: 102 values, published as list_async
+    (documents/example2_async_convergence.py:82-85).  The oracle replays the
+    async schedule task by task (CpuSolver::fn_dispatch with its 32+32
+    position buffers and read_and_solve_conflict merges)."""
+    from oracle import oracle as O
+    from libatssim_amd import configs
+    from tests.helpers import square8k_mesh
+    x, f = square8k_mesh()
+    cloth = configs.square8k_cloth(x, f)
+    p = configs.square8k_params()
+    m = O.Mesh(x, f)
+    assert m.nclusters == 10
+    seq = launch_sequence(100, 10)
+    s = O.Sim(m, cloth.mass, cloth.fixed, implicit_dt=p.implicit_dt, iter_count=100,
+              k_stretch=p.stiffness_stretch_spring, k_bending=p.stiffness_quadratic_bending,
+              damping=p.damping_cloth, threads=O.max_threads())
+    for _ in range(30):
+        s.step()
+    e = replay_on_oracle(s, seq, record=True)
+    pub = np.array(json.load(open(os.path.join(GOLDEN, "square8k_published.json")))["list_async"])
+    assert len(e) == len(pub) == 102
+    rel = np.abs(e - pub) / np.abs(pub)
+    assert rel.max() < 5e-6, (rel.max(), np.argmax(rel), e[:5], pub[:5])
