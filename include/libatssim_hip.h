@@ -18,6 +18,7 @@
 #ifndef LIBATSSIM_HIP_H
 #define LIBATSSIM_HIP_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -32,6 +33,7 @@ extern "C" {
 #define LATS_ERR_NO_DEVICE 3
 #define LATS_ERR_UNSUPPORTED 4
 #define LATS_ERR_OUT_OF_MEMORY 5
+#define LATS_ERR_TIMEOUT 6          /* a bounded host wait gave up */
 
 /* Message of the last failing call on this thread ("" if none). */
 const char* lats_last_error(void);
@@ -244,8 +246,29 @@ int lats_vbd_host_signal(lats_vbd* ctx, uint32_t value);
  * host until the marker has passed. */
 int lats_vbd_record_marker(lats_vbd* ctx, uint32_t marker);
 int lats_vbd_wait_marker(lats_vbd* ctx, uint32_t marker);
+/* Non-blocking: *done = 1 once the marker has passed (for bounded waits). */
+int lats_vbd_query_marker(lats_vbd* ctx, uint32_t marker, int* done);
 /* Elapsed GPU ms between two markers (≙ GPUEndTime - GPUStartTime). */
 int lats_vbd_marker_elapsed_ms(lats_vbd* ctx, uint32_t m0, uint32_t m1, float* ms);
+
+/* ---- position buffers across processors ------------------------------
+ * The reference shares every buffer between CPU and GPU (unified memory,
+ * SharedArray, example3_hybrid_cpu_gpu.cpp:16-92); here the context owns
+ * them in device memory and a heterogeneous launch moves a buffer across at
+ * a cross-processor edge.  Both calls are enqueued on the context's stream
+ * (order them with markers / host waits), use float[3V] in vertex order on
+ * the host side and need pinned host memory (lats_host_alloc) to run
+ * asynchronously. */
+#define LATS_BUF_ITER 0     /* iterate buffer `idx` (sa_async_iter_positions) */
+#define LATS_BUF_BEGIN 1    /* begin buffer `idx` (sa_async_begin_positions)  */
+#define LATS_BUF_X 2        /* sa_x (idx ignored)                             */
+#define LATS_BUF_X_START 3  /* sa_x_start                                     */
+#define LATS_BUF_V 4        /* sa_v                                           */
+int lats_vbd_buffer_download(lats_vbd* ctx, int which, uint32_t idx, float* host);
+int lats_vbd_buffer_upload(lats_vbd* ctx, int which, uint32_t idx, const float* host);
+/* Pinned (page-locked) host memory for the copies above. */
+int lats_host_alloc(size_t bytes, void** out);
+int lats_host_free(void* p);
 
 /* ======================================================================
  * Batched cloths: many independent cloths of uneven sizes, one persistent

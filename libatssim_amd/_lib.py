@@ -80,6 +80,10 @@ class MeshDesc(ctypes.Structure):
     ]
 
 
+# position buffers across processors (lats_vbd_buffer_download / _upload)
+LATS_BUF_ITER, LATS_BUF_BEGIN, LATS_BUF_X, LATS_BUF_X_START, LATS_BUF_V = range(5)
+
+
 class LaunchParam(ctypes.Structure):
     _fields_ = [
         ("function_id", ctypes.c_uint32),
@@ -141,6 +145,11 @@ SIGNATURES = {
     "lats_vbd_record_marker": (_I, [_VP, _U32]),
     "lats_vbd_wait_marker": (_I, [_VP, _U32]),
     "lats_vbd_marker_elapsed_ms": (_I, [_VP, _U32, _U32, ctypes.POINTER(ctypes.c_float)]),
+    "lats_vbd_query_marker": (_I, [_VP, _U32, ctypes.POINTER(_I)]),
+    "lats_vbd_buffer_download": (_I, [_VP, _I, _U32, _VP]),
+    "lats_vbd_buffer_upload": (_I, [_VP, _I, _U32, _VP]),
+    "lats_host_alloc": (_I, [ctypes.c_size_t, ctypes.POINTER(_VP)]),
+    "lats_host_free": (_I, [_VP]),
     "lats_vbd_device_positions": (_I, [_VP, ctypes.POINTER(_VP)]),
     "lats_vbd_copy_permutation": (_I, [_VP, _VP]),
     "lats_vbd_time_colour_kernels": (_I, [_VP, _U32, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(_U32),

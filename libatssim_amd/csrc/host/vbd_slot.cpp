@@ -1,6 +1,9 @@
 // vbd_slot.cpp -- see vbd_slot.h.
 #include "vbd_slot.h"
 
+#include <chrono>
+#include <thread>
+
 namespace lats {
 
 namespace {
@@ -37,6 +40,40 @@ void VbdGpuSlot::fn_dispatch(const Launcher::LaunchParam& p) {
     c.is_allocated_to_main_device = p.is_allocated_to_main_device ? 1u : 0u;
     const int rc = lats_vbd_dispatch(ctx_, &c);
     if (rc != LATS_OK && status_ == LATS_OK) status_ = rc;
+}
+
+Launcher::HeteroHooks VbdGpuSlot::hetero_hooks(double timeout_s) {
+    Launcher::HeteroHooks h;
+    auto keep = [this](int rc) {
+        if (rc != LATS_OK && status_ == LATS_OK) status_ = rc;
+    };
+    h.gpu_begin = [this, keep](Launcher::uint e, Launcher::uint wait) {
+        if (wait != ~0u) keep(lats_vbd_stream_wait_host(ctx_, wait + 1));
+        keep(lats_vbd_record_marker(ctx_, 2 * e));
+    };
+    h.gpu_end = [this, keep](Launcher::uint e) { keep(lats_vbd_record_marker(ctx_, 2 * e + 1)); };
+    h.cpu_wait_gpu = [this, keep, timeout_s](Launcher::uint e) {
+        const auto t0 = std::chrono::steady_clock::now();
+        int done = 0;
+        while (true) {
+            const int rc = lats_vbd_query_marker(ctx_, 2 * e + 1, &done);
+            if (rc != LATS_OK) { keep(rc); return; }
+            if (done) return;
+            if (std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() > timeout_s) {
+                lats_vbd_host_signal(ctx_, 0xFFFFFFFFu);   // let the queue drain
+                keep(LATS_ERR_TIMEOUT);
+                return;
+            }
+            std::this_thread::sleep_for(std::chrono::microseconds(20));
+        }
+    };
+    h.cpu_signal = [this, keep](Launcher::uint v) { keep(lats_vbd_host_signal(ctx_, v)); };
+    h.gpu_finish = [this, keep]() { keep(lats_vbd_synchronize(ctx_)); };
+    h.gpu_elapsed_ms = [this](Launcher::uint e) {
+        float ms = 0.f;
+        return lats_vbd_marker_elapsed_ms(ctx_, 2 * e, 2 * e + 1, &ms) == LATS_OK ? ms : 0.f;
+    };
+    return h;
 }
 
 Launcher::LaunchParam vbd_task_to_param(const Launcher::Task& t) {

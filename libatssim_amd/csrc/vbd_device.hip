@@ -615,8 +615,12 @@ int ensure_energy_rows(lats_vbd* c, uint32_t rows) {
     HIPCHK(np.alloc((size_t)cap * c->energy_nblk));
     HIPCHK(ne.alloc(cap));
     if (c->energy_rows) {
+        // both the partial sums and the rows already finalised (the dispatch
+        // path finalises one row at a time)
         HIPCHK(hipMemcpyAsync(np.p, c->partial.p, sizeof(double) * (size_t)c->energy_rows * c->energy_nblk,
                               hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(ne.p, c->energy.p, sizeof(float) * (size_t)c->energy_rows, hipMemcpyDeviceToDevice,
+                              c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
     }
     std::swap(c->partial.p, np.p);
@@ -1558,11 +1562,85 @@ int lats_vbd_wait_marker(lats_vbd* c, uint32_t marker) {
     HIPCHK(hipEventSynchronize(it->second));
     return LATS_OK;
 }
+int lats_vbd_query_marker(lats_vbd* c, uint32_t marker, int* done) {
+    if (!c || !done) return fail(LATS_ERR_INVALID_ARGUMENT, "null argument");
+    auto it = c->markers.find(marker);
+    if (it == c->markers.end()) return fail(LATS_ERR_INVALID_ARGUMENT, "marker never recorded");
+    const hipError_t e = hipEventQuery(it->second);
+    if (e == hipErrorNotReady) {
+        *done = 0;
+        return LATS_OK;
+    }
+    HIPCHK(e);
+    *done = 1;
+    return LATS_OK;
+}
 int lats_vbd_marker_elapsed_ms(lats_vbd* c, uint32_t m0, uint32_t m1, float* ms) {
     if (!c || !ms) return fail(LATS_ERR_INVALID_ARGUMENT, "null argument");
     auto a = c->markers.find(m0), b = c->markers.find(m1);
     if (a == c->markers.end() || b == c->markers.end()) return fail(LATS_ERR_INVALID_ARGUMENT, "marker never recorded");
     HIPCHK(hipEventElapsedTime(ms, a->second, b->second));
+    return LATS_OK;
+}
+
+// Position buffers of the asynchronous frame as the host sees them (vertex
+// order); copies run on the context's stream through scratch3.
+static int buffer_of(lats_vbd* c, int which, uint32_t idx, float** out) {
+    switch (which) {
+        case LATS_BUF_ITER:
+        case LATS_BUF_BEGIN: {
+            if (idx == LATS_BUFFER_DEFAULT || idx == LATS_BUFFER_NONE) {
+                if (which == LATS_BUF_BEGIN) return fail(LATS_ERR_INVALID_ARGUMENT, "the default buffer has no begin copy");
+                *out = c->x.p;
+                return LATS_OK;
+            }
+            const int rc = ensure_async(c, idx);
+            if (rc) return rc;
+            *out = which == LATS_BUF_ITER ? iter_buffer(c, idx) : begin_buffer(c, idx);
+            return LATS_OK;
+        }
+        case LATS_BUF_X: *out = c->x.p; return LATS_OK;
+        case LATS_BUF_X_START: *out = c->x0.p; return LATS_OK;
+        case LATS_BUF_V: *out = c->v.p; return LATS_OK;
+        default: return fail(LATS_ERR_INVALID_ARGUMENT, "unknown buffer kind");
+    }
+}
+
+int lats_vbd_buffer_download(lats_vbd* c, int which, uint32_t idx, float* host) {
+    if (!c || !host) return fail(LATS_ERR_INVALID_ARGUMENT, "null argument");
+    float* src = nullptr;
+    int rc = buffer_of(c, which, idx, &src);
+    if (rc) return rc;
+    // vertex v sits in slot inv[v]
+    hipLaunchKernelGGL(k_gather_rows, dim3(nblocks(c->V)), dim3(kBlock), 0, c->stream, src, c->inv_d.p, c->scratch3.p,
+                       c->V);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(host, c->scratch3.p, 3 * (size_t)c->V * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    return LATS_OK;
+}
+
+int lats_vbd_buffer_upload(lats_vbd* c, int which, uint32_t idx, const float* host) {
+    if (!c || !host) return fail(LATS_ERR_INVALID_ARGUMENT, "null argument");
+    float* dst = nullptr;
+    int rc = buffer_of(c, which, idx, &dst);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(c->scratch3.p, host, 3 * (size_t)c->V * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    // slot s holds vertex perm[s]
+    hipLaunchKernelGGL(k_gather_rows, dim3(nblocks(c->V)), dim3(kBlock), 0, c->stream, c->scratch3.p, c->perm_d.p, dst,
+                       c->V);
+    HIPCHK(hipGetLastError());
+    return LATS_OK;
+}
+
+int lats_host_alloc(size_t bytes, void** out) {
+    if (!out) return fail(LATS_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    HIPCHK(hipHostMalloc(out, bytes, hipHostMallocDefault));
+    return LATS_OK;
+}
+
+int lats_host_free(void* p) {
+    if (p) HIPCHK(hipHostFree(p));
     return LATS_OK;
 }
 

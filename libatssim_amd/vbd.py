@@ -311,6 +311,19 @@ class GpuVBD:
     def set_record_energy(self, on):
         check(lib().lats_vbd_set_record_energy(self._h, int(bool(on))))
 
+    def buffer(self, which, idx=0):
+        """Copy of a position buffer (float32 [V, 3], vertex order); synchronises."""
+        out = np.zeros((self.cloth.num_verts, 3), np.float32)
+        check(lib().lats_vbd_buffer_download(self._h, int(which), int(idx), _p(out)))
+        self.synchronize()
+        return out
+
+    def set_buffer(self, which, idx, values):
+        """Overwrites a position buffer from float32 [V, 3] (vertex order); synchronises."""
+        arr = np.ascontiguousarray(values, np.float32).reshape(self.cloth.num_verts, 3)
+        check(lib().lats_vbd_buffer_upload(self._h, int(which), int(idx), _p(arr)))
+        self.synchronize()
+
     def permutation(self):
         p = np.zeros(self.cloth.num_verts, np.uint32)
         check(lib().lats_vbd_copy_permutation(self._h, _p(p)))

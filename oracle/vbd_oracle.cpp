@@ -723,6 +723,27 @@ template <typename T> static void async_begin(Sim<T>& s) {
     s.energy.clear();
 }
 
+// Position buffers (float[3V], vertex order): 0 iterate idx, 1 begin idx,
+// 2 x, 3 x_start, 4 v (the numbering of LATS_BUF_*).
+template <typename T> static std::vector<V3<T>>& buffer_ref(Sim<T>& s, int which, u32 idx) {
+    std::vector<V3<T>>* b = &s.x;
+    if (which == 0) b = &s.iter_buf[idx % 32];
+    else if (which == 1) b = &s.begin_buf[idx % 32];
+    else if (which == 3) b = &s.x_start;
+    else if (which == 4) b = &s.v;
+    if (b->size() != s.x.size()) b->assign(s.x.size(), mk(T(0), T(0), T(0)));
+    return *b;
+}
+template <typename T> static void buffer_get(Sim<T>& s, int which, u32 idx, float* out) {
+    const auto& b = buffer_ref(s, which, idx);
+    for (size_t i = 0; i < b.size(); i++)
+        for (int c = 0; c < 3; c++) out[3 * i + c] = (float)b[i][c];
+}
+template <typename T> static void buffer_set(Sim<T>& s, int which, u32 idx, const float* in) {
+    auto& b = buffer_ref(s, which, idx);
+    for (size_t i = 0; i < b.size(); i++)
+        for (int c = 0; c < 3; c++) b[i][c] = T(in[3 * i + c]);
+}
 }  // namespace orc
 
 // ---------------------------------------------------------------------------
@@ -931,6 +952,14 @@ void orc_sim_dispatch(orc_sim* s, u32 fn, u32 iter, u32 cluster, u32 buffer, u32
     const DispatchParam p{fn, iter, cluster, buffer, left, right, nin, ins, main_device != 0};
     if (s->fp64) dispatch(s->s64, p, record != 0);
     else dispatch(s->s32, p, record != 0);
+}
+void orc_sim_buffer_get(orc_sim* s, int which, u32 idx, float* out) {
+    if (s->fp64) buffer_get(s->s64, which, idx, out);
+    else buffer_get(s->s32, which, idx, out);
+}
+void orc_sim_buffer_set(orc_sim* s, int which, u32 idx, const float* in) {
+    if (s->fp64) buffer_set(s->s64, which, idx, in);
+    else buffer_set(s->s32, which, idx, in);
 }
 u32 orc_sim_energies(const orc_sim* s, float* energy, u32 cap) {
     const std::vector<float>& e = s->fp64 ? s->s64.energy : s->s32.energy;

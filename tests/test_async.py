@@ -130,3 +130,14 @@ def test_square8k_async_frame_matches_published():
     assert len(e) == len(pub) == 102
     rel = np.abs(e - pub) / np.abs(pub)
     assert rel.max() < 5e-6, (rel.max(), np.argmax(rel), e[:5], pub[:5])
+
+
+def test_two_memory_harness_without_device():
+    """The heterogeneous-launch harness with both processors on oracles
+    (tests/cpp/test_hetero_vbd.cpp, "cpu" mode): runs, and the two-memory
+    replay stays close to the shared-buffer one."""
+    path = os.path.join(os.path.dirname(BIN), "test_hetero_vbd")
+    if not os.path.exists(path):
+        pytest.fail(f"{path} not built (make -C libatssim_amd/csrc)")
+    r = subprocess.run([path, "24", "20", "2", "cpu"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and "PASS" in r.stdout, r.stdout + r.stderr

@@ -171,7 +171,14 @@ def test_dispatch_slot_matches_physics_step(gpu_available, buffer_idx):
         assert np.array_equal(a.state()[1], b.state()[1])
     # energy hooks: before iteration 0 of cluster 0, after the last cluster of
     # every iteration, and at the last node
-    assert len(b.energy()) == 3 * (1 + 10 + 1)
+    e = b.energy()
+    assert len(e) == 3 * (1 + 10 + 1)
+    # rows finalised one at a time survive the history's growth (8 -> 24 -> 56 rows)
+    assert np.all(np.isfinite(e)) and np.all(e > 0), e
+    for fr in range(3):
+        ef = e[12 * fr:12 * fr + 12].astype(np.float64)
+        assert np.all(np.diff(ef[:11]) <= 1e-6 * ef.max()), (fr, ef)
+        assert ef[11] == ef[10]            # the last node re-evaluates the final iterate
 
 
 def test_read_and_solve_conflict_merge(gpu_available):
