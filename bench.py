@@ -82,7 +82,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", default="grid_c1", choices=["grid_c1", "batch_c2", "sphere_c4", "copy_calibration",
-                                                               "mlp_kernels"],
+                                                               "mlp_kernels", "comm_matrix"],
                     help="grid_c1 (default, the metric's workload), batch_c2 (512 cloths) or "
                          "copy_calibration (a plain 256 MiB device copy, the reference for HBM counters)")
     ap.add_argument("--strategy", default="auto", choices=["auto", "graph", "persistent", "flow"])
@@ -109,6 +109,8 @@ def main():
         return bench_mlp(args)
     if args.workload == "sphere_c4":
         return bench_sphere(args)
+    if args.workload == "comm_matrix":
+        return bench_comm_matrix(args)
 
     K, W = args.steps, args.warmup
     cloth, params, _, wind = configs.grid_c1(frames=W + K)
@@ -380,6 +382,90 @@ def bench_batch(args):
                    "blocks": blocks, "gpu_ms_per_step": gpu_ms / K},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": "k_vbd_batch"},
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_comm_matrix(args):
+    """The scheduler's cost inputs measured on this GPU for a square8K-sized
+    cloth (91x91 grid, 8,281 vertices), as the reference hard-codes them for
+    its Apple machine (communication_cost_matrix_uma, example3_hybrid_cpu_gpu.cpp:
+    2341-2345; GPU column of the computation matrix, :2276-2311), in ms:
+      cpu_to_gpu  host signal -> the stream passes its wait -> iterate + begin
+                  buffers uploaded (pinned, vertex order) -> marker seen by the host
+      gpu_to_cpu  iterate + begin buffers downloaded -> marker seen by the host
+      gpu_gpu     one dispatch of the smallest cluster (the per-task floor)
+      gpu_cluster per-cluster all-in-one task (mean of repeated dispatches)."""
+    import ctypes
+    import numpy as np
+    from libatssim_amd import GpuVBD, Params, configs, _lib
+    L = _lib.lib()
+    cloth = configs.grid_cloth(91, "top", seed=3)
+    g = GpuVBD(cloth, Params(constraint_iter_count=100))
+    g.physics_step()
+    V = cloth.num_verts
+    nbytes = 3 * V * 4
+    ptrs = []
+    for _ in range(2):
+        p = ctypes.c_void_p()
+        _lib.check(L.lats_host_alloc(nbytes, ctypes.byref(p)))
+        ptrs.append(p)
+    h = g.handle
+    reps = max(args.steps, 20)
+
+    def wait(marker):
+        done = ctypes.c_int(0)
+        while not done.value:
+            _lib.check(L.lats_vbd_query_marker(h, marker, ctypes.byref(done)))
+
+    # predict once so the first cluster task can copy sa_x into buffer 0
+    g.dispatch(_lib.LATS_FN_PREDICT_POSITION)
+    g.dispatch(_lib.LATS_FN_VBD_ALL_IN_ONE, iter_idx=0, cluster_idx=0, buffer_idx=0,
+               left_buffer_idx=_lib.LATS_BUFFER_DEFAULT)
+    # the staging starts with the buffer's own contents (uploads then rewrite the same positions)
+    _lib.check(L.lats_vbd_buffer_download(h, _lib.LATS_BUF_ITER, 0, ptrs[0]))
+    _lib.check(L.lats_vbd_buffer_download(h, _lib.LATS_BUF_BEGIN, 0, ptrs[1]))
+    g.synchronize()
+    c2g, g2c = [], []
+    value = 0
+    for r in range(reps + 3):
+        value += 1
+        _lib.check(L.lats_vbd_stream_wait_host(h, value))
+        _lib.check(L.lats_vbd_buffer_upload(h, _lib.LATS_BUF_ITER, 0, ptrs[0]))
+        _lib.check(L.lats_vbd_buffer_upload(h, _lib.LATS_BUF_BEGIN, 0, ptrs[1]))
+        _lib.check(L.lats_vbd_record_marker(h, 1))
+        time.sleep(0.002)                       # the stream is parked on its wait
+        t0 = time.perf_counter()
+        _lib.check(L.lats_vbd_host_signal(h, value))
+        wait(1)
+        t1 = time.perf_counter()
+        _lib.check(L.lats_vbd_buffer_download(h, _lib.LATS_BUF_ITER, 0, ptrs[0]))
+        _lib.check(L.lats_vbd_buffer_download(h, _lib.LATS_BUF_BEGIN, 0, ptrs[1]))
+        _lib.check(L.lats_vbd_record_marker(h, 2))
+        wait(2)
+        t2 = time.perf_counter()
+        if r >= 3:
+            c2g.append((t1 - t0) * 1e3)
+            g2c.append((t2 - t1) * 1e3)
+    sizes = [len(c) for c in cloth.topology.clusters()]
+    per_cluster = []
+    for c in range(len(sizes)):
+        _lib.check(L.lats_vbd_record_marker(h, 10))
+        for _ in range(reps):
+            g.dispatch(_lib.LATS_FN_VBD_ALL_IN_ONE, iter_idx=1, cluster_idx=c, buffer_idx=0)
+        _lib.check(L.lats_vbd_record_marker(h, 11))
+        g.synchronize()
+        ms = ctypes.c_float()
+        _lib.check(L.lats_vbd_marker_elapsed_ms(h, 10, 11, ctypes.byref(ms)))
+        per_cluster.append(ms.value / reps)
+    for p in ptrs:
+        L.lats_host_free(p)
+    out = {
+        "workload": "comm_matrix", "cloth": "91x91 grid", "verts": V, "clusters": sizes,
+        "cpu_to_gpu_ms": round(float(np.median(c2g)), 4), "gpu_to_cpu_ms": round(float(np.median(g2c)), 4),
+        "gpu_gpu_ms": round(min(per_cluster), 4), "gpu_cluster_ms": [round(v, 4) for v in per_cluster],
+        "reference_apple_uma_ms": {"cpu_to_gpu": 0.220, "gpu_to_cpu": 0.145, "cpu_cpu": 0.002, "gpu_gpu": 0.01},
+        "repeats": reps,
     }
     print(json.dumps(out), flush=True)
 

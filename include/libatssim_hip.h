@@ -59,6 +59,24 @@ typedef struct lats_topology lats_topology;
 int lats_topology_build(const float* x, uint32_t num_verts, const int32_t* faces, uint32_t num_faces,
                         lats_topology** out);
 void lats_topology_destroy(lats_topology* t);
+
+/* ---- Wavefront OBJ (mesh input and frame export) ----------------------
+ * Reading keeps what SimMesh::read_mesh_file keeps of a file
+ * (src/Utils/mesh_reader.cpp:11-200, tinyobjloader defaults): `v` positions
+ * in file order; faces fan-triangulated from their first corner, 1-based or
+ * negative indices, `v/vt/vn` corners; a triangle repeating a vertex is
+ * dropped (mesh_reader.cpp:172-182).  Writing is the format of
+ * SolverInterface::save_mesh_to_obj (examples/example2_async_iter.cpp:1649-1710). */
+typedef struct lats_obj lats_obj;
+int lats_obj_read(const char* path, lats_obj** out);
+int lats_obj_parse(const char* text, lats_obj** out);
+/* counts: vertices, triangles, triangles dropped */
+int lats_obj_counts(const lats_obj* obj, uint32_t counts[3]);
+/* x: float[3 * vertices], faces: int32[3 * triangles] (0-based); either may be null */
+int lats_obj_copy(const lats_obj* obj, float* x, int32_t* faces);
+void lats_obj_destroy(lats_obj* obj);
+int lats_obj_write(const char* path, const float* x, uint32_t num_verts, const int32_t* faces, uint32_t num_faces,
+                   uint32_t mesh_id);
 /* counts[0..3] = num_verts, num_edges, num_bending_edges, num_clusters */
 int lats_topology_counts(const lats_topology* t, uint32_t counts[4]);
 /* number of elements (not bytes) of array `which` */

@@ -105,6 +105,12 @@ _I = ctypes.c_int
 SIGNATURES = {
     "lats_last_error": (ctypes.c_char_p, []),
     "lats_topology_build": (_I, [_VP, _U32, _VP, _U32, ctypes.POINTER(_VP)]),
+    "lats_obj_read": (_I, [ctypes.c_char_p, ctypes.POINTER(_VP)]),
+    "lats_obj_parse": (_I, [ctypes.c_char_p, ctypes.POINTER(_VP)]),
+    "lats_obj_counts": (_I, [_VP, _VP]),
+    "lats_obj_copy": (_I, [_VP, _VP, _VP]),
+    "lats_obj_destroy": (None, [_VP]),
+    "lats_obj_write": (_I, [ctypes.c_char_p, _VP, _U32, _VP, _U32, _U32]),
     "lats_topology_destroy": (None, [_VP]),
     "lats_topology_counts": (_I, [_VP, _VP]),
     "lats_topology_size": (_I, [_VP, _I, ctypes.POINTER(ctypes.c_uint64)]),

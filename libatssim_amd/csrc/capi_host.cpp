@@ -4,12 +4,17 @@
 
 #include "../../include/libatssim_hip.h"
 #include "host/errors.h"
+#include "host/mesh_io.h"
 #include "host/obstacle.h"
 #include "host/scene.h"
 #include "host/topology.h"
 
 struct lats_topology {
     lats::Topology t;
+};
+
+struct lats_obj {
+    lats::ObjMesh m;
 };
 
 namespace {
@@ -20,6 +25,58 @@ int set_err(int code, const char* msg) {
 }  // namespace
 
 extern "C" {
+
+int lats_obj_read(const char* path, lats_obj** out) {
+    if (!path || !out) return set_err(LATS_ERR_INVALID_ARGUMENT, "null argument");
+    auto* o = new lats_obj();
+    std::string err;
+    if (!lats::read_obj(path, o->m, err)) {
+        delete o;
+        *out = nullptr;
+        return set_err(LATS_ERR_INVALID_ARGUMENT, err.c_str());
+    }
+    *out = o;
+    return LATS_OK;
+}
+
+int lats_obj_parse(const char* text, lats_obj** out) {
+    if (!text || !out) return set_err(LATS_ERR_INVALID_ARGUMENT, "null argument");
+    auto* o = new lats_obj();
+    std::string err;
+    if (!lats::parse_obj(text, o->m, err)) {
+        delete o;
+        *out = nullptr;
+        return set_err(LATS_ERR_INVALID_ARGUMENT, err.c_str());
+    }
+    *out = o;
+    return LATS_OK;
+}
+
+int lats_obj_counts(const lats_obj* o, uint32_t counts[3]) {
+    if (!o || !counts) return set_err(LATS_ERR_INVALID_ARGUMENT, "null argument");
+    counts[0] = (uint32_t)(o->m.x.size() / 3);
+    counts[1] = (uint32_t)(o->m.faces.size() / 3);
+    counts[2] = o->m.dropped;
+    return LATS_OK;
+}
+
+int lats_obj_copy(const lats_obj* o, float* x, int32_t* faces) {
+    if (!o) return set_err(LATS_ERR_INVALID_ARGUMENT, "null argument");
+    if (x) std::memcpy(x, o->m.x.data(), o->m.x.size() * sizeof(float));
+    if (faces) std::memcpy(faces, o->m.faces.data(), o->m.faces.size() * sizeof(int32_t));
+    return LATS_OK;
+}
+
+void lats_obj_destroy(lats_obj* o) { delete o; }
+
+int lats_obj_write(const char* path, const float* x, uint32_t nv, const int32_t* faces, uint32_t nf, uint32_t mesh_id) {
+    if (!path || (nv && !x) || (nf && !faces)) return set_err(LATS_ERR_INVALID_ARGUMENT, "null argument");
+    for (uint32_t k = 0; k < 3 * nf; k++)
+        if (faces[k] < 0 || (uint32_t)faces[k] >= nv) return set_err(LATS_ERR_INVALID_ARGUMENT, "face index out of range");
+    std::string err;
+    if (!lats::write_obj(path, x, nv, faces, nf, mesh_id, err)) return set_err(LATS_ERR_INVALID_ARGUMENT, err.c_str());
+    return LATS_OK;
+}
 
 int lats_topology_build(const float* x, uint32_t nv, const int32_t* faces, uint32_t nf, lats_topology** out) {
     if (!out || (nv && !x) || (nf && !faces)) return set_err(LATS_ERR_INVALID_ARGUMENT, "null argument");

@@ -31,6 +31,40 @@ def make_grid(nx, ny, spacing=0.01, jitter=1e-4, seed=0):
     return x, f
 
 
+def _obj_arrays(h):
+    counts = np.zeros(3, np.uint32)
+    try:
+        check(lib().lats_obj_counts(h, _p(counts)))
+        x = np.zeros((int(counts[0]), 3), np.float32)
+        f = np.zeros((int(counts[1]), 3), np.int32)
+        check(lib().lats_obj_copy(h, _p(x), _p(f)))
+    finally:
+        lib().lats_obj_destroy(h)
+    return x, f, int(counts[2])
+
+
+def read_obj(path):
+    """(positions float32 [V,3], triangles int32 [F,3], dropped) of an OBJ file,
+    read the way the reference's mesh reader does (fan triangulation, faces
+    repeating a vertex dropped)."""
+    h = ctypes.c_void_p()
+    check(lib().lats_obj_read(os.fsencode(path), ctypes.byref(h)))
+    return _obj_arrays(h)
+
+
+def parse_obj(text):
+    h = ctypes.c_void_p()
+    check(lib().lats_obj_parse(text.encode(), ctypes.byref(h)))
+    return _obj_arrays(h)
+
+
+def write_obj(path, x, faces, mesh_id=0):
+    """One frame as OBJ (the reference's save_mesh_to_obj format)."""
+    x = np.ascontiguousarray(x, np.float32)
+    f = np.ascontiguousarray(faces, np.int32)
+    check(lib().lats_obj_write(os.fsencode(path), _p(x), len(x), _p(f), len(f), int(mesh_id)))
+
+
 def make_icosphere(level, radius, centre=(0.0, 0.0, 0.0)):
     """Icosahedron subdivided `level` times on a sphere; faces point outwards."""
     c = np.ascontiguousarray(centre, np.float32)
