@@ -556,7 +556,7 @@ void Scheduler::launch(LaunchMode mode, const std::function<LaunchParam(const Ta
     auto ms_since = [](clock::time_point t0) {
         return std::chrono::duration<float, std::milli>(clock::now() - t0).count();
     };
-    const bool need_events = mode == LaunchModeFakeHetero || mode == LaunchModeSequeceHetero || mode == LaunchModeHetero;
+    const bool need_events = mode != LaunchModeCpu && mode != LaunchModeGpu;
     if (need_events && (launch_events.size() != 2 || (launch_events[0].empty() && launch_events[1].empty()))) {
         std::fprintf(stderr, "Launcher: launching order is empty (make_wait_events first)\n");
         return;
@@ -654,6 +654,75 @@ void Scheduler::launch(LaunchMode mode, const std::function<LaunchParam(const Ta
             refit_runtime_cost(cpu_ms, gpu_ms);
             break;
         }
+        case LaunchModeProgressiveHetero: {
+            // launcher.cpp:1112-1268: LaunchModeHetero with the GPU events
+            // queued in windows of `progressive_window`; a window is queued
+            // when the host first needs one of its events.  (The reference
+            // waits for the value `wait` here, without the +1 of its Hetero
+            // branch, so a GPU event waiting for CPU event 0 would not wait;
+            // the Hetero convention is used.)
+            if (!hooks) {
+                std::fprintf(stderr, "Launcher: LaunchModeProgressiveHetero needs synchronisation hooks\n");
+                return;
+            }
+            const auto& cev = launch_events[0];
+            const auto& gev = launch_events[1];
+            const uint window = std::max(1u, progressive_window);
+            uint queued = 0;
+            auto queue_until = [&](uint last) -> bool {   // queue whole windows up to event `last`
+                while (queued < gev.size() && queued <= last) {
+                    const uint end = std::min<uint>((uint)gev.size(), queued + window);
+                    for (uint e = queued; e < end; e++) {
+                        hooks->gpu_begin(e, gev[e].wait);
+                        const bool ok = run_event(1, gev[e], DeviceTypeGpu);
+                        hooks->gpu_end(e);
+                        if (!ok) return false;
+                    }
+                    queued = end;
+                }
+                return true;
+            };
+            hooks->cpu_signal(0);
+            bool ok = gev.empty() || queue_until(0);
+            for (uint e = 0; ok && e < cev.size(); e++) {
+                if (cev[e].wait != ~0u) {
+                    ok = queue_until(cev[e].wait);
+                    if (!ok) break;
+                    hooks->cpu_wait_gpu(cev[e].wait);
+                }
+                ok = run_event(0, cev[e], DeviceTypeCpu);
+                if (ok && cev[e].signal != ~0u) hooks->cpu_signal(cev[e].signal + 1);
+            }
+            if (ok && !gev.empty()) ok = queue_until((uint)gev.size() - 1);
+            hooks->cpu_signal(0xFFFFFFFFu);
+            hooks->gpu_finish();
+            break;
+        }
+        case LaunchModePartialCPU: {
+            // launcher.cpp:1474-1505: the CPU processor's events alone, each
+            // timed against its profiled cost (reported beyond 1.3x)
+            for (uint e = 0; e < launch_events[0].size(); e++) {
+                const LaunchEvent& ev = launch_events[0][e];
+                float profiled = 0.f;
+                for (uint i = ev.start_idx; i <= ev.end_idx; i++) profiled += computation_matrix[proc_schedules[0][i].task_id][0];
+                const auto t0 = clock::now();
+                if (!run_event(0, ev, DeviceTypeCpu)) return;
+                const float ms = ms_since(t0);
+                if (profiled > 0.f && ms / profiled > 1.3f)
+                    std::fprintf(stderr, "Launcher: CPU event %u took %.3f ms, profiled %.3f ms\n", e, ms, profiled);
+            }
+            break;
+        }
+        case LaunchModePartialGPU:
+            // launcher.cpp:1506-1581: the GPU processor's events alone
+            for (uint e = 0; e < launch_events[1].size(); e++) {
+                if (hooks) hooks->gpu_begin(e, ~0u);
+                const bool ok = run_event(1, launch_events[1][e], DeviceTypeGpu);
+                if (hooks) hooks->gpu_end(e);
+                if (!ok) break;
+            }
+            if (hooks) hooks->gpu_finish();
+            break;
         default:
             std::fprintf(stderr, "Launcher: launch mode %d is not available in this build\n", (int)mode);
     }

@@ -177,10 +177,10 @@ public:
     enum LaunchMode {
         LaunchModeCpu,            // every task on its CPU implementation, topological/rank order
         LaunchModeGpu,            // every task on its GPU implementation (CPU fallback if missing)
-        LaunchModePartialCPU,
-        LaunchModePartialGPU,
-        LaunchModeHetero,
-        LaunchModeProgressiveHetero,
+        LaunchModePartialCPU,     // the CPU processor's events only
+        LaunchModePartialGPU,     // the GPU processor's events only
+        LaunchModeHetero,         // both processors at once: queued GPU events behind host-value waits
+        LaunchModeProgressiveHetero,   // Hetero with the GPU events queued window by window
         LaunchModeFakeHetero,     // scheduled allocation, run synchronously in schedule order
         LaunchModeSequeceHetero,  // scheduled order, every task on its CPU implementation
     };
@@ -239,6 +239,7 @@ public:
 
     std::vector<ListCost> computation_matrix;
     std::vector<double> summary_of_costs_each_device;
+    uint progressive_window = 15;   // GPU events queued at a time in LaunchModeProgressiveHetero
     std::vector<std::vector<float>> communication_speed_matrix = {{0, 1, 1}, {1, 0, 1}, {1, 1, 0}};
     std::vector<std::vector<float>> communication_cost_matrix_uma = {{0.002f, 0.220f}, {0.145f, 0.01f}};
     std::vector<float> communication_startup = {0, 0, 0};

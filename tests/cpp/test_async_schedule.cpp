@@ -120,6 +120,23 @@ int main(int argc, char** argv) {
                 return 1;
             }
     }
-    std::printf("PASS %zu tasks\n", params.size());
+    // the partial modes run one processor's scheduled tasks
+    const size_t n_all = params.size();
+    size_t on_proc[2] = {0, 0};
+    for (size_t t = 0; t < tasks.size(); t++)
+        if (tasks[t].func_id != Launcher::id_additional_root && tasks[t].func_id != Launcher::id_additional_terminal)
+            on_proc[s.get_task_schedules()[t].proc]++;
+    params.clear();
+    s.launch(Launcher::Scheduler::LaunchModePartialCPU, lats::vbd_task_to_param);
+    const size_t n_cpu = params.size();
+    params.clear();
+    s.launch(Launcher::Scheduler::LaunchModePartialGPU, lats::vbd_task_to_param);
+    const size_t n_gpu = params.size();
+    std::printf("PARTIAL cpu %zu of %zu, gpu %zu of %zu\n", n_cpu, on_proc[0], n_gpu, on_proc[1]);
+    if (n_cpu != on_proc[0] || n_gpu != on_proc[1] || n_cpu + n_gpu != n_all) {
+        std::printf("FAIL partial launch modes\n");
+        return 1;
+    }
+    std::printf("PASS %zu tasks\n", n_all);
     return 0;
 }

@@ -260,17 +260,20 @@ int main(int argc, char** argv) {
     orc_sim* o_gpu2 = orc_sim_create(om, p9, i2, mass.data(), fixed.data(), 0);
     orc_sim* o_cpu_h = orc_sim_create(om, p9, i2, mass.data(), fixed.data(), 0);
     orc_sim* o_cpu_f = orc_sim_create(om, p9, i2, mass.data(), fixed.data(), 0);
-    lats_vbd *g_h = nullptr, *g_f = nullptr;
+    orc_sim* o_cpu_p = orc_sim_create(om, p9, i2, mass.data(), fixed.data(), 0);
+    lats_vbd *g_h = nullptr, *g_f = nullptr, *g_p = nullptr;
     if (!cpu_only) {
         CHECK(lats_vbd_create(&d, &prm, &g_h));
         CHECK(lats_vbd_create(&d, &prm, &g_f));
+        CHECK(lats_vbd_create(&d, &prm, &g_p));
     }
     // a few synchronous frames first, identical everywhere
     for (int f = 0; f < 3; f++) {
-        for (orc_sim* o : {o_shared, o_cpu2, o_gpu2, o_cpu_h, o_cpu_f}) orc_sim_step(o, 0, nullptr, 0);
+        for (orc_sim* o : {o_shared, o_cpu2, o_gpu2, o_cpu_h, o_cpu_f, o_cpu_p}) orc_sim_step(o, 0, nullptr, 0);
         if (cpu_only) continue;
         CHECK(lats_vbd_physics_step(g_h, 0));
         CHECK(lats_vbd_physics_step(g_f, 0));
+        CHECK(lats_vbd_physics_step(g_p, 0));
     }
     std::vector<float> a(3 * V), b(3 * V), tmp(3 * V);
     auto same_state = [&](orc_sim* o, lats_vbd* g) {
@@ -281,8 +284,8 @@ int main(int argc, char** argv) {
     };
     if (!cpu_only && !same_state(o_cpu_h, g_h)) { std::printf("FAIL synchronous frames differ\n"); return 1; }
 
-    OracleSide s_shared(o_shared), s_cpu2(o_cpu2), s_gpu2(o_gpu2), s_cpu_h(o_cpu_h), s_cpu_f(o_cpu_f);
-    GpuSide s_gh(g_h), s_gf(g_f);
+    OracleSide s_shared(o_shared), s_cpu2(o_cpu2), s_gpu2(o_gpu2), s_cpu_h(o_cpu_h), s_cpu_f(o_cpu_f), s_cpu_p(o_cpu_p);
+    GpuSide s_gh(g_h), s_gf(g_f), s_gp(g_p);
     int bad = 0;
     for (uint32_t f = 0; f < frames; f++) {
         // the reference's semantics: one memory, SequeceHetero order
@@ -332,6 +335,14 @@ int main(int argc, char** argv) {
         rf.launch(Launcher::Scheduler::LaunchModeFakeHetero, &hooks_f);
         CHECK(s_gf.slot.status());
         CHECK(lats_vbd_synchronize(g_f));
+        // CPU oracle + GPU, GPU events queued three at a time
+        FrameRun rp(V, iters, cnt[3]);
+        rp.sides[0] = &s_cpu_p;
+        rp.sides[1] = &s_gp;
+        rp.s.progressive_window = 3;
+        const Launcher::HeteroHooks hooks_p = s_gp.slot.hetero_hooks(60.0);
+        rp.launch(Launcher::Scheduler::LaunchModeProgressiveHetero, &hooks_p);
+        CHECK(s_gp.slot.status());
 
         // update_velocity ran on the CPU side: hand the frame state to the GPU
         // sides for the next frame
@@ -344,8 +355,12 @@ int main(int argc, char** argv) {
             orc_sim_buffer_get(o_cpu_f, which, 0, tmp.data());
             CHECK(lats_vbd_buffer_upload(g_f, which, 0, tmp.data()));
             CHECK(lats_vbd_synchronize(g_f));
+            orc_sim_buffer_get(o_cpu_p, which, 0, tmp.data());
+            CHECK(lats_vbd_buffer_upload(g_p, which, 0, tmp.data()));
+            CHECK(lats_vbd_synchronize(g_p));
         }
-        std::vector<float> x2(3 * V), xh(3 * V), xf(3 * V), xs(3 * V), v2(3 * V), vh(3 * V);
+        std::vector<float> x2(3 * V), xh(3 * V), xf(3 * V), xp(3 * V), xs(3 * V), v2(3 * V), vh(3 * V);
+        orc_sim_buffer_get(o_cpu_p, LATS_BUF_X, 0, xp.data());
         orc_sim_buffer_get(o_cpu2, LATS_BUF_X, 0, x2.data());
         orc_sim_buffer_get(o_cpu_h, LATS_BUF_X, 0, xh.data());
         orc_sim_buffer_get(o_cpu_f, LATS_BUF_X, 0, xf.data());
@@ -357,12 +372,13 @@ int main(int argc, char** argv) {
             dmax = std::fmax(dmax, std::fabs((double)x2[i] - xs[i]));
             xmax = std::fmax(xmax, std::fabs((double)xs[i]));
         }
-        const bool eq_h = xh == x2 && vh == v2, eq_f = xf == x2;
-        std::printf("frame %u: hetero == two-memory replay: %s, fake-hetero: %s; vs shared-buffer replay max |dx| %.3g "
-                    "(rel %.3g), energy %.6f vs %.6f\n",
-                    f, eq_h ? "bit-exact" : "DIFFERENT", eq_f ? "bit-exact" : "DIFFERENT", dmax, dmax / xmax,
-                    orc_sim_energy(o_cpu_h), orc_sim_energy(o_shared));
-        if (!eq_h || !eq_f) bad = 1;
+        const bool eq_h = xh == x2 && vh == v2, eq_f = xf == x2, eq_p = xp == x2;
+        std::printf("frame %u: hetero == two-memory replay: %s, fake-hetero: %s, progressive: %s; vs shared-buffer "
+                    "replay max |dx| %.3g (rel %.3g), energy %.6f vs %.6f\n",
+                    f, eq_h ? "bit-exact" : "DIFFERENT", eq_f ? "bit-exact" : "DIFFERENT",
+                    eq_p ? "bit-exact" : "DIFFERENT", dmax, dmax / xmax, orc_sim_energy(o_cpu_h),
+                    orc_sim_energy(o_shared));
+        if (!eq_h || !eq_f || !eq_p) bad = 1;
         if (dmax / xmax > 5e-2) { std::printf("FAIL far from the shared-buffer replay\n"); bad = 1; }
         if (f == 0) {
             const auto& ev = rh.s.get_launch_events();
@@ -372,7 +388,8 @@ int main(int argc, char** argv) {
     }
     if (g_h) lats_vbd_destroy(g_h);
     if (g_f) lats_vbd_destroy(g_f);
-    for (orc_sim* o : {o_shared, o_cpu2, o_gpu2, o_cpu_h, o_cpu_f}) orc_sim_destroy(o);
+    if (g_p) lats_vbd_destroy(g_p);
+    for (orc_sim* o : {o_shared, o_cpu2, o_gpu2, o_cpu_h, o_cpu_f, o_cpu_p}) orc_sim_destroy(o);
     orc_mesh_destroy(om);
     lats_topology_destroy(topo);
     std::printf(bad ? "FAIL\n" : "PASS\n");

@@ -82,7 +82,8 @@ HETERO_BIN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "bi
 @pytest.mark.parametrize("n,iters,frames", [(24, 20, 2), (48, 100, 2)])
 def test_hetero_launch_cpu_oracle_plus_gpu(gpu_available, n, iters, frames):
     """LaunchModeHetero with the CPU oracle as processor 0 and the HIP context
-    as processor 1 (concurrent, host-value waits and markers), and
+    as processor 1 (concurrent, host-value waits and markers),
+    LaunchModeProgressiveHetero (GPU events queued three at a time) and
     LaunchModeFakeHetero, each == the two-memory oracle replay bit for bit
     (tests/cpp/test_hetero_vbd.cpp)."""
     import subprocess
@@ -92,4 +93,5 @@ def test_hetero_launch_cpu_oracle_plus_gpu(gpu_available, n, iters, frames):
     r = subprocess.run([HETERO_BIN, str(n), str(iters), str(frames)], capture_output=True, text=True, timeout=300)
     print(r.stdout[-3000:])
     assert r.returncode == 0 and "PASS" in r.stdout, r.stdout + r.stderr
-    assert r.stdout.count("hetero == two-memory replay: bit-exact, fake-hetero: bit-exact") == frames
+    assert r.stdout.count("hetero == two-memory replay: bit-exact, fake-hetero: bit-exact, progressive: bit-exact") \
+        == frames
