@@ -143,6 +143,21 @@ typedef struct lats_vbd lats_vbd;
 int lats_vbd_create(const lats_mesh_desc* mesh, const lats_scene_params* params, lats_vbd** out);
 int lats_vbd_destroy(lats_vbd* ctx);
 int lats_vbd_set_strategy(lats_vbd* ctx, int strategy);
+/* Solver of physics_step / run_frames / dispatch: VBD (default) or XPBD
+ * (CpuSolver::physics_step_xpbd, example2_async_iter.cpp:1017-1081: predict,
+ * multipliers reset, per iteration every spring cluster then every bending
+ * cluster, update velocity; constraint clusters from the reference's greedy
+ * constraint colouring, built on the first switch).  XPBD runs the graph
+ * strategy, ignores external forces (the reference's XPBD has none) and
+ * rejects obstacles and the asynchronous buffers. */
+#define LATS_SOLVER_VBD 0
+#define LATS_SOLVER_XPBD 1
+int lats_vbd_set_solver(lats_vbd* ctx, int solver);
+int lats_vbd_get_solver(lats_vbd* ctx, int* solver);
+/* XPBD constraint clusters as a packed CSR (n+1 offsets, then constraint
+ * ids in the reference's edge / bending-pair numbering); which: 0 springs,
+ * 1 bending pairs.  *size receives the element count; copies min(cap, size). */
+int lats_vbd_xpbd_clusters(lats_vbd* ctx, int which, uint32_t* csr, uint64_t cap, uint64_t* size);
 /* Lanes per vertex of the colour passes (1, 4, 8, 16): with G > 1 the
  * incident springs and bending pairs of a vertex are gathered by G lanes in
  * parallel and summed in the reference's order (results are identical). */
@@ -225,7 +240,9 @@ int lats_vbd_energy_now(lats_vbd* ctx, float* energy);
 #define LATS_FN_VBD_ALL_IN_ONE 3        /* id_vbd_all_in_one            */
 #define LATS_FN_CONSTRAINT_LAST_NODE 4  /* id_xpbd_constraint_last_node */
 #define LATS_FN_UPDATE_VELOCITY 5       /* id_xpbd_update_velocity      */
-#define LATS_FN_RESET_CONSTRAINS 6      /* id_xpbd_reset_constrains (no state in VBD) */
+#define LATS_FN_RESET_CONSTRAINS 6      /* id_xpbd_reset_constrains (XPBD multipliers; no state in VBD) */
+#define LATS_FN_XPBD_STRETCH 7          /* id_xpbd_constraint_stretch_mass_spring, cluster_idx */
+#define LATS_FN_XPBD_BENDING 8          /* id_xpbd_constraint_bending, cluster_idx */
 
 #define LATS_BUFFER_DEFAULT 0x0FFFFFFFu /* default_buffer_mask / input_buffer_mask, launcher.h:836-837 */
 #define LATS_BUFFER_NONE 0xFFFFFFFFu    /* -1u                                   */

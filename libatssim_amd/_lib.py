@@ -38,6 +38,9 @@ LATS_FN_VBD_ALL_IN_ONE = 3
 LATS_FN_CONSTRAINT_LAST_NODE = 4
 LATS_FN_UPDATE_VELOCITY = 5
 LATS_FN_RESET_CONSTRAINS = 6
+LATS_FN_XPBD_STRETCH = 7
+LATS_FN_XPBD_BENDING = 8
+LATS_SOLVER_VBD, LATS_SOLVER_XPBD = 0, 1
 
 LATS_BUFFER_DEFAULT = 0x0FFFFFFF
 LATS_BUFFER_NONE = 0xFFFFFFFF
@@ -124,6 +127,9 @@ SIGNATURES = {
     "lats_vbd_set_persistent_blocks": (_I, [_VP, _U32]),
     "lats_vbd_set_lanes_per_vertex": (_I, [_VP, _U32]),
     "lats_vbd_set_regions": (_I, [_VP, _U32]),
+    "lats_vbd_set_solver": (_I, [_VP, _I]),
+    "lats_vbd_get_solver": (_I, [_VP, ctypes.POINTER(_I)]),
+    "lats_vbd_xpbd_clusters": (_I, [_VP, _I, _VP, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)]),
     "lats_vbd_get_regions": (_I, [_VP, ctypes.POINTER(_U32)]),
     "lats_vbd_frame_begin": (_I, [_VP]),
     "lats_vbd_iterate": (_I, [_VP, _U32]),

@@ -22,6 +22,8 @@ const std::map<FunctionID, std::string> taskNames = {
     {id_xpbd_predict_position, "xpbd_predict_position"},
     {id_xpbd_reset_constrains, "xpbd_reset_constrains"},
     {id_xpbd_update_velocity, "xpbd_update_velocity"},
+    {id_xpbd_constraint_stretch_mass_spring, "xpbd_constraint_stretch_mass_spring"},
+    {id_xpbd_constraint_bending, "xpbd_constraint_bending"},
     {id_xpbd_non, "xpbd_non"},
 };
 
@@ -245,7 +247,8 @@ namespace {
 // Task classes of the asynchronous post-pass (launcher.cpp:2957-3030),
 // restricted to the function ids this build registers.
 bool is_constraint_fn(FunctionID f) {
-    return f == id_xpbd_constraint_last_node || f == id_vbd_evaluate_inertia ||
+    return f == id_xpbd_constraint_last_node || f == id_xpbd_constraint_stretch_mass_spring ||
+           f == id_xpbd_constraint_bending || f == id_vbd_evaluate_inertia ||
            f == id_vbd_evaluate_stretch_mass_spring || f == id_vbd_evaluate_bending || f == id_vbd_step ||
            f == id_vbd_all_in_one;
 }

@@ -34,6 +34,8 @@ enum FunctionID : uint {
     id_xpbd_predict_position,
     id_xpbd_reset_constrains,
     id_xpbd_update_velocity,
+    id_xpbd_constraint_stretch_mass_spring,
+    id_xpbd_constraint_bending,
     id_xpbd_non,
 };
 extern const std::map<FunctionID, std::string> taskNames;

@@ -17,6 +17,8 @@ uint32_t to_c_function(Launcher::FunctionID id, bool& ok) {
         case Launcher::id_xpbd_constraint_last_node: return LATS_FN_CONSTRAINT_LAST_NODE;
         case Launcher::id_xpbd_update_velocity: return LATS_FN_UPDATE_VELOCITY;
         case Launcher::id_xpbd_reset_constrains: return LATS_FN_RESET_CONSTRAINS;
+        case Launcher::id_xpbd_constraint_stretch_mass_spring: return LATS_FN_XPBD_STRETCH;
+        case Launcher::id_xpbd_constraint_bending: return LATS_FN_XPBD_BENDING;
         default: ok = false; return 0;
     }
 }
@@ -112,6 +114,33 @@ std::vector<Launcher::uint> register_vbd_dag(Launcher::Scheduler& s, const std::
     const uint upd = s.add_task(Task(id_xpbd_update_velocity, 0, impls));
     for (uint t : prev) s.set_connect(t, last);
     s.set_connect(last, upd);
+    return order;
+}
+
+std::vector<Launcher::uint> register_xpbd_dag(Launcher::Scheduler& s, const std::vector<Launcher::Implementation>& impls,
+                                              Launcher::uint iters, Launcher::uint nstretch, Launcher::uint nbending) {
+    using namespace Launcher;
+    const uint predict = s.add_task(Task(id_xpbd_predict_position, 0, impls));
+    const uint reset = s.add_task(Task(id_xpbd_reset_constrains, 0, impls));
+    s.set_connect(predict, reset);
+    std::vector<uint> order;
+    uint prev = reset;
+    for (uint it = 0; it < iters; it++) {
+        for (uint c = 0; c < nstretch; c++) {
+            const uint t = s.add_task(Task(id_xpbd_constraint_stretch_mass_spring, it, ~0u, c, impls));
+            s.set_connect(prev, t);
+            order.push_back(t);
+            prev = t;
+        }
+        for (uint c = 0; c < nbending; c++) {
+            const uint t = s.add_task(Task(id_xpbd_constraint_bending, it, ~0u, c, impls));
+            s.set_connect(prev, t);
+            order.push_back(t);
+            prev = t;
+        }
+    }
+    const uint upd = s.add_task(Task(id_xpbd_update_velocity, 0, impls));
+    s.set_connect(prev, upd);
     return order;
 }
 

@@ -48,4 +48,14 @@ std::vector<Launcher::uint> register_vbd_dag(Launcher::Scheduler& scheduler,
                                              const std::vector<Launcher::Implementation>& impls,
                                              Launcher::uint constraint_iter_count, Launcher::uint num_clusters);
 
+// Registers one XPBD frame (CpuSolver::physics_step_xpbd,
+// example2_async_iter.cpp:1017-1081, with solve_constraints_XPBD :1510-1534):
+// predict -> reset -> per iteration every spring cluster then every bending
+// cluster (a chain: Gauss-Seidel over constraint colours) -> update velocity.
+// Returns the constraint tasks in order.
+std::vector<Launcher::uint> register_xpbd_dag(Launcher::Scheduler& scheduler,
+                                              const std::vector<Launcher::Implementation>& impls,
+                                              Launcher::uint iters, Launcher::uint num_stretch_clusters,
+                                              Launcher::uint num_bending_clusters);
+
 }  // namespace lats

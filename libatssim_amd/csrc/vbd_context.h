@@ -90,6 +90,17 @@ struct lats_vbd {
     uint32_t persist_max_lanes = 0;
     uint32_t persist_max_blocks = 0;  // co-residency limit (queried on first use)
 
+    // XPBD solver (built on first use): constraints in cluster order, their
+    // multipliers, cluster bounds, 1/m per slot
+    int solver = LATS_SOLVER_VBD;
+    bool xpbd_built = false;
+    std::vector<uint32_t> xs_bound, xb_bound;          // cluster c: [bound[c], bound[c + 1])
+    std::vector<uint32_t> xs_csr, xb_csr;              // packed CSR of constraint ids (reference numbering)
+    lats::DBuf<uint2> xs_edges;
+    lats::DBuf<float> xs_rest, xs_lambda, xb_lambda, x_minv;
+    lats::DBuf<uint4> xb_pairs;
+    lats::DBuf<float4> xb_Q;
+
     // flow strategy: tiles, neighbour lists, per-tile phase counters
     uint32_t flow_tiles = 0;          // tiles of the built layout (0 = none)
     uint32_t flow_base = 0;           // phase every tile has completed

@@ -34,6 +34,7 @@
 #include "vbd_kernels.h"
 #include "vbd_layout.h"
 #include "vbd_context.h"
+#include "xpbd.h"
 
 using namespace lats;
 
@@ -572,6 +573,21 @@ __global__ __launch_bounds__(kBlock) void k_read_and_solve_conflict(float* begin
     st3(begin_self, s, wp);
 }
 
+// --- XPBD: one thread per constraint of one cluster -------------------------
+__global__ __launch_bounds__(kBlock) void k_xpbd_stretch(XpbdArgs a, uint32_t start, uint32_t count) {
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    if (t < count) xpbd_stretch(a, start + t);
+}
+__global__ __launch_bounds__(kBlock) void k_xpbd_bending(XpbdArgs a, uint32_t start, uint32_t count) {
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    if (t < count) xpbd_bending(a, start + t);
+}
+// a fill kernel rather than hipMemsetAsync: frames are captured into graphs
+__global__ __launch_bounds__(kBlock) void k_fill(float* p, float v, uint32_t n) {
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    if (t < n) p[t] = v;
+}
+
 __global__ __launch_bounds__(kBlock) void k_gather_rows(const float* src, const uint32_t* idx, float* dst, uint32_t n) {
     // dst[t] = src[idx[t]] for float3 rows
     const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
@@ -736,8 +752,134 @@ void enqueue_update(lats_vbd* c, int advance) {
                        c->x0.p, c->v.p, c->fp.p, c->h, c->damp, c->V, advance);
 }
 
+XpbdArgs xpbd_args(lats_vbd* c) {
+    XpbdArgs a;
+    a.x = c->x.p;
+    a.minv = c->x_minv.p;
+    a.edges = c->xs_edges.p;
+    a.rest = c->xs_rest.p;
+    a.lambda_s = c->xs_lambda.p;
+    a.pairs = c->xb_pairs.p;
+    a.Q = c->xb_Q.p;
+    a.lambda_b = c->xb_lambda.p;
+    a.ks = c->prm.stiffness_stretch_spring;
+    a.kb = c->prm.stiffness_quadratic_bending;
+    a.h = c->h;
+    return a;
+}
+
+// reset_constrains (example2_async_iter.cpp:698-710)
+void enqueue_xpbd_reset(lats_vbd* c) {
+    if (c->E) hipLaunchKernelGGL(k_fill, dim3(nblocks(c->E)), dim3(kBlock), 0, c->stream, c->xs_lambda.p, 0.0f, c->E);
+    if (c->B) hipLaunchKernelGGL(k_fill, dim3(nblocks(c->B)), dim3(kBlock), 0, c->stream, c->xb_lambda.p, 0.0f, c->B);
+}
+void enqueue_xpbd_stretch(lats_vbd* c, uint32_t k) {
+    const uint32_t lo = c->xs_bound[k], n = c->xs_bound[k + 1] - lo;
+    if (n) hipLaunchKernelGGL(k_xpbd_stretch, dim3(nblocks(n)), dim3(kBlock), 0, c->stream, xpbd_args(c), lo, n);
+}
+void enqueue_xpbd_bending(lats_vbd* c, uint32_t k) {
+    const uint32_t lo = c->xb_bound[k], n = c->xb_bound[k + 1] - lo;
+    if (n) hipLaunchKernelGGL(k_xpbd_bending, dim3(nblocks(n)), dim3(kBlock), 0, c->stream, xpbd_args(c), lo, n);
+}
+
+// CpuSolver::physics_step_xpbd (example2_async_iter.cpp:1017-1081) with the
+// iteration of solve_constraints_XPBD (:1510-1534).
+void enqueue_xpbd_frame(lats_vbd* c, bool record) {
+    const uint32_t nsub = record ? 1u : c->prm.num_substep;
+    for (uint32_t sub = 0; sub < nsub; sub++) {
+        enqueue_predict(c);
+        enqueue_xpbd_reset(c);
+        for (uint32_t it = 0; it < c->prm.constraint_iter_count; it++) {
+            if (record && it == 0) enqueue_energy(c, c->x.p, 0);
+            for (uint32_t k = 0; k + 1 < c->xs_bound.size(); k++) enqueue_xpbd_stretch(c, k);
+            for (uint32_t k = 0; k + 1 < c->xb_bound.size(); k++) enqueue_xpbd_bending(c, k);
+            if (record) enqueue_energy(c, c->x.p, it + 1);
+        }
+        enqueue_update(c, sub + 1 == nsub);
+    }
+    if (record)
+        hipLaunchKernelGGL(k_energy_finalize, dim3(c->prm.constraint_iter_count + 1), dim3(kBlock), 0, c->stream,
+                           c->partial.p, c->energy_nblk, 0u, c->energy.p);
+}
+
+// Constraint clusters and constraints in cluster order, from the element
+// lists already on the device (the reference's edge and pair numbering).
+int build_xpbd(lats_vbd* c) {
+    if (c->xpbd_built) return LATS_OK;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    std::vector<uint2> es(c->E);
+    std::vector<uint4> bs(c->B);
+    std::vector<float> rest(c->E);
+    std::vector<float4> Q(4 * (size_t)c->B);
+    std::vector<float> mass(c->V);
+    std::vector<uint32_t> meta(c->V);
+    if (c->E) HIPCHK(hipMemcpy(es.data(), c->e_slots.p, es.size() * sizeof(uint2), hipMemcpyDeviceToHost));
+    if (c->E) HIPCHK(hipMemcpy(rest.data(), c->rest.p, rest.size() * sizeof(float), hipMemcpyDeviceToHost));
+    if (c->B) HIPCHK(hipMemcpy(bs.data(), c->b_slots.p, bs.size() * sizeof(uint4), hipMemcpyDeviceToHost));
+    if (c->B) HIPCHK(hipMemcpy(Q.data(), c->Qd.p, Q.size() * sizeof(float4), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(mass.data(), c->mass.p, mass.size() * sizeof(float), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(meta.data(), c->meta.p, meta.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    std::vector<uint32_t> ev(2 * (size_t)c->E), bv(4 * (size_t)c->B);
+    for (uint32_t e = 0; e < c->E; e++) {
+        ev[2 * (size_t)e] = es[e].x;
+        ev[2 * (size_t)e + 1] = es[e].y;
+    }
+    for (uint32_t b = 0; b < c->B; b++) {
+        bv[4 * (size_t)b] = bs[b].x;
+        bv[4 * (size_t)b + 1] = bs[b].y;
+        bv[4 * (size_t)b + 2] = bs[b].z;
+        bv[4 * (size_t)b + 3] = bs[b].w;
+    }
+    // slots relabel the vertices; the colouring depends only on which
+    // constraints share a vertex, so it equals the one on vertex ids
+    const auto cs = colour_constraints(ev, 2, c->V);
+    const auto cb = colour_constraints(bv, 4, c->V);
+    auto pack = [](const std::vector<std::vector<uint32_t>>& cl, std::vector<uint32_t>& bound, std::vector<uint32_t>& csr) {
+        const uint32_t n = (uint32_t)cl.size();
+        bound.assign(1, 0u);
+        csr.assign(n + 1, 0u);
+        uint32_t p = n + 1;
+        for (uint32_t k = 0; k < n; k++) {
+            csr[k] = p;
+            p += (uint32_t)cl[k].size();
+            bound.push_back(bound.back() + (uint32_t)cl[k].size());
+        }
+        csr[n] = p;
+        for (const auto& v : cl) csr.insert(csr.end(), v.begin(), v.end());
+    };
+    pack(cs, c->xs_bound, c->xs_csr);
+    pack(cb, c->xb_bound, c->xb_csr);
+    std::vector<uint2> me;
+    std::vector<float> mr;
+    for (const auto& cl : cs)
+        for (uint32_t e : cl) {
+            me.push_back(es[e]);
+            mr.push_back(rest[e]);
+        }
+    std::vector<uint4> mb;
+    std::vector<float4> mq;
+    for (const auto& cl : cb)
+        for (uint32_t b : cl) {
+            mb.push_back(bs[b]);
+            for (int k = 0; k < 4; k++) mq.push_back(Q[4 * (size_t)b + k]);
+        }
+    // example2_async_iter.cpp:223: 0 for pinned vertices, else 1/m
+    std::vector<float> minv(c->V);
+    for (uint32_t s = 0; s < c->V; s++) minv[s] = ((meta[s] >> 16) & 1u) ? 0.0f : 1.0f / mass[s];
+    HIPCHK(c->xs_edges.upload(me));
+    HIPCHK(c->xs_rest.upload(mr));
+    HIPCHK(c->xb_pairs.upload(mb));
+    HIPCHK(c->xb_Q.upload(mq));
+    HIPCHK(c->x_minv.upload(minv));
+    HIPCHK(c->xs_lambda.alloc(std::max(1u, c->E)));
+    HIPCHK(c->xb_lambda.alloc(std::max(1u, c->B)));
+    c->xpbd_built = true;
+    return LATS_OK;
+}
+
 // The whole frame as a kernel sequence (captured into a graph by the caller).
 void enqueue_frame(lats_vbd* c, bool record) {
+    if (c->solver == LATS_SOLVER_XPBD) return enqueue_xpbd_frame(c, record);
     const uint32_t nsub = record ? 1u : c->prm.num_substep;
     for (uint32_t sub = 0; sub < nsub; sub++) {
         enqueue_predict(c);
@@ -969,8 +1111,9 @@ int get_graph(lats_vbd* c, bool record, hipGraphExec_t* out) {
 }
 
 int run_frame(lats_vbd* c, bool record) {
-    if (!record && c->strategy == LATS_STRATEGY_PERSISTENT) return run_persistent(c, 1);
-    if (!record && c->strategy == LATS_STRATEGY_FLOW) return run_flow(c, 1);
+    const bool vbd = c->solver == LATS_SOLVER_VBD;
+    if (vbd && !record && c->strategy == LATS_STRATEGY_PERSISTENT) return run_persistent(c, 1);
+    if (vbd && !record && c->strategy == LATS_STRATEGY_FLOW) return run_flow(c, 1);
     hipGraphExec_t g;
     int rc = get_graph(c, record, &g);
     if (rc) return rc;
@@ -1104,9 +1247,43 @@ int lats_vbd_set_strategy(lats_vbd* c, int strategy) {
     return LATS_OK;
 }
 
+int lats_vbd_set_solver(lats_vbd* c, int solver) {
+    if (!c) return fail(LATS_ERR_INVALID_ARGUMENT, "null context");
+    if (solver != LATS_SOLVER_VBD && solver != LATS_SOLVER_XPBD) return fail(LATS_ERR_INVALID_ARGUMENT, "unknown solver");
+    if (solver == LATS_SOLVER_XPBD) {
+        if (c->has_obstacle) return fail(LATS_ERR_UNSUPPORTED, "the XPBD solver has no obstacle contact");
+        const int rc = build_xpbd(c);
+        if (rc) return rc;
+    }
+    if (solver != c->solver) {
+        HIPCHK(hipStreamSynchronize(c->stream));
+        c->solver = solver;
+        c->drop_graphs();
+    }
+    return LATS_OK;
+}
+
+int lats_vbd_get_solver(lats_vbd* c, int* solver) {
+    if (!c || !solver) return fail(LATS_ERR_INVALID_ARGUMENT, "null argument");
+    *solver = c->solver;
+    return LATS_OK;
+}
+
+int lats_vbd_xpbd_clusters(lats_vbd* c, int which, uint32_t* csr, uint64_t cap, uint64_t* size) {
+    if (!c || !size) return fail(LATS_ERR_INVALID_ARGUMENT, "null argument");
+    if (which != 0 && which != 1) return fail(LATS_ERR_INVALID_ARGUMENT, "which must be 0 (springs) or 1 (bending)");
+    const int rc = build_xpbd(c);
+    if (rc) return rc;
+    const std::vector<uint32_t>& v = which ? c->xb_csr : c->xs_csr;
+    *size = v.size();
+    if (csr) std::memcpy(csr, v.data(), std::min<uint64_t>(cap, v.size()) * sizeof(uint32_t));
+    return LATS_OK;
+}
+
 int lats_vbd_set_obstacle(lats_vbd* c, const float* x, uint32_t nv, const int32_t* faces, uint32_t nf,
                           float thickness, float stiffness, float radius, float cell) {
     if (!c) return fail(LATS_ERR_INVALID_ARGUMENT, "null context");
+    if (nf && c->solver == LATS_SOLVER_XPBD) return fail(LATS_ERR_UNSUPPORTED, "the XPBD solver has no obstacle contact");
     if (nf == 0) {
         c->has_obstacle = false;
     } else {
@@ -1212,6 +1389,7 @@ int lats_vbd_get_strategy(lats_vbd* c, int* strategy, uint32_t* blocks) {
 // at several grid sizes, keeps the fastest, restores the state.
 int lats_vbd_autotune(lats_vbd* c, uint32_t frames, int* chosen, uint32_t* chosen_blocks) {
     if (!c) return fail(LATS_ERR_INVALID_ARGUMENT, "null context");
+    if (c->solver != LATS_SOLVER_VBD) return fail(LATS_ERR_UNSUPPORTED, "the autotuner tunes the VBD colour passes");
     if (!frames) frames = 1;
     const size_t nf = 3 * (size_t)c->V;
     DBuf<float> sx, sx0, sv;
@@ -1366,11 +1544,12 @@ int lats_vbd_run_frames(lats_vbd* c, uint32_t nframes, const float* forces) {
         if (rc) return rc;
     }
     int rc;
-    if (c->strategy == LATS_STRATEGY_PERSISTENT) {
+    const bool vbd = c->solver == LATS_SOLVER_VBD;
+    if (vbd && c->strategy == LATS_STRATEGY_PERSISTENT) {
         HIPCHK(hipEventRecord(c->ev0, c->stream));
         rc = run_persistent(c, nframes);
         if (rc) return rc;
-    } else if (c->strategy == LATS_STRATEGY_FLOW) {
+    } else if (vbd && c->strategy == LATS_STRATEGY_FLOW) {
         HIPCHK(hipEventRecord(c->ev0, c->stream));
         rc = run_flow(c, nframes);
         if (rc) return rc;
@@ -1494,10 +1673,32 @@ int lats_vbd_dispatch(lats_vbd* c, const lats_launch_param* p) {
         return LATS_OK;
     };
     int rc = LATS_OK;
+    auto xpbd_task = [&](bool stretch) -> int {
+        // XPBD tasks iterate sa_x (the reference's XPBD path has no asynchronous buffers)
+        if (p->buffer_idx != LATS_BUFFER_DEFAULT && p->buffer_idx != LATS_BUFFER_NONE)
+            return fail(LATS_ERR_UNSUPPORTED, "XPBD tasks iterate sa_x only");
+        int r = build_xpbd(c);
+        if (r) return r;
+        const size_t nclusters = (stretch ? c->xs_bound.size() : c->xb_bound.size()) - 1;
+        if (p->cluster_idx >= nclusters) return fail(LATS_ERR_INVALID_ARGUMENT, "cluster_idx out of range");
+        if (stretch) enqueue_xpbd_stretch(c, p->cluster_idx);
+        else enqueue_xpbd_bending(c, p->cluster_idx);
+        return LATS_OK;
+    };
     switch (p->function_id) {
         case LATS_FN_ADDITIONAL_ROOT:
         case LATS_FN_ADDITIONAL_TERMINAL:
+            break;
         case LATS_FN_RESET_CONSTRAINS:
+            rc = build_xpbd(c);
+            if (rc) return rc;
+            enqueue_xpbd_reset(c);
+            break;
+        case LATS_FN_XPBD_STRETCH:
+            rc = xpbd_task(true);
+            break;
+        case LATS_FN_XPBD_BENDING:
+            rc = xpbd_task(false);
             break;
         case LATS_FN_PREDICT_POSITION:
             enqueue_predict(c);
@@ -1665,6 +1866,7 @@ int lats_vbd_frame_begin(lats_vbd* c) {
 
 int lats_vbd_iterate(lats_vbd* c, uint32_t sweeps) {
     if (!c) return fail(LATS_ERR_INVALID_ARGUMENT, "null context");
+    if (c->solver != LATS_SOLVER_VBD) return fail(LATS_ERR_UNSUPPORTED, "colour sweeps belong to the VBD solver");
     if (sweeps == 0) return LATS_OK;
     // one graph per distinct sweep count (a handful in practice)
     auto it = c->sweep_graphs.find(sweeps);

@@ -221,6 +221,25 @@ class GpuVBD:
         if persistent_blocks is not None:
             check(lib().lats_vbd_set_persistent_blocks(self._h, int(persistent_blocks)))
 
+    def set_solver(self, solver):
+        """"vbd" (default) or "xpbd" (the reference's XPBD path, same context)."""
+        code = {"vbd": _lib.LATS_SOLVER_VBD, "xpbd": _lib.LATS_SOLVER_XPBD}.get(solver, solver)
+        check(lib().lats_vbd_set_solver(self._h, int(code)))
+
+    def solver(self):
+        v = ctypes.c_int()
+        check(lib().lats_vbd_get_solver(self._h, ctypes.byref(v)))
+        return {_lib.LATS_SOLVER_VBD: "vbd", _lib.LATS_SOLVER_XPBD: "xpbd"}[v.value]
+
+    def xpbd_clusters(self, which):
+        """XPBD constraint clusters (which: 0 springs, 1 bending pairs): list of id arrays."""
+        n = ctypes.c_uint64()
+        check(lib().lats_vbd_xpbd_clusters(self._h, int(which), None, 0, ctypes.byref(n)))
+        csr = np.zeros(n.value, np.uint32)
+        check(lib().lats_vbd_xpbd_clusters(self._h, int(which), _p(csr), n.value, ctypes.byref(n)))
+        k = int(csr[0]) - 1
+        return [csr[csr[i]:csr[i + 1]] for i in range(k)]
+
     def set_regions(self, regions):
         """Bands of the colour-pass block mapping (1..8, results identical)."""
         check(lib().lats_vbd_set_regions(self._h, int(regions)))

@@ -61,6 +61,10 @@ def lib():
         L.orc_sim_contacts.restype = ctypes.c_uint32
         L.orc_sim_contacts.argtypes = [ctypes.c_void_p, _u32p, _i32p, ctypes.c_uint32]
         L.orc_sim_async_begin.argtypes = [ctypes.c_void_p]
+        L.orc_sim_step_xpbd.restype = ctypes.c_uint32
+        L.orc_sim_step_xpbd.argtypes = [ctypes.c_void_p, ctypes.c_int, _f32p, ctypes.c_uint32]
+        L.orc_sim_xpbd_clusters.restype = ctypes.c_uint32
+        L.orc_sim_xpbd_clusters.argtypes = [ctypes.c_void_p, ctypes.c_int, _u32p, ctypes.c_uint32]
         L.orc_sim_dispatch.argtypes = [ctypes.c_void_p] + [ctypes.c_uint32] * 7 + [_u32p, ctypes.c_int, ctypes.c_int]
         L.orc_sim_energies.restype = ctypes.c_uint32
         L.orc_sim_energies.argtypes = [ctypes.c_void_p, _f32p, ctypes.c_uint32]
@@ -166,6 +170,20 @@ class Sim:
 
     def energy(self):
         return lib().orc_sim_energy(self._h)
+
+    # XPBD (CpuSolver::physics_step_xpbd)
+    def step_xpbd(self, record=False):
+        cap = 2 * self.iter_count + 2
+        e = np.zeros(cap, np.float32)
+        n = lib().orc_sim_step_xpbd(self._h, int(record), _ptr(e, _f32p), cap)
+        return e[:n]
+
+    def xpbd_clusters(self, which):
+        n = lib().orc_sim_xpbd_clusters(self._h, int(which), None, 0)
+        p = np.zeros(n, np.uint32)
+        lib().orc_sim_xpbd_clusters(self._h, int(which), _ptr(p, _u32p), n)
+        k = int(p[0]) - 1
+        return [p[p[i]:p[i + 1]] for i in range(k)]
 
     # asynchronous frame (CpuSolver::physics_step_vbd_async + fn_dispatch)
     def async_begin(self):

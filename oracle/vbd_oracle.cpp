@@ -254,6 +254,12 @@ template <typename T> struct Sim {
     // asynchronous iteration: 32 iterate + 32 begin position buffers,
     // indexed modulo 32 (example2_async_iter.cpp:1147-1157)
     std::vector<V3<T>> iter_buf[32], begin_buf[32];
+    // XPBD (example2_async_iter.cpp:453-697): constraint clusters, the
+    // constraints in cluster order ("merged"), their multipliers, 1/m
+    bool xpbd_ready = false;
+    std::vector<std::vector<u32>> cl_stretch, cl_bending;
+    std::vector<int> m_edges, m_bending;
+    std::vector<T> m_rest, m_Q, lambda_s, lambda_b, mass_inv;
 };
 
 template <typename T> static inline T substep_dt(const Params& p) {
@@ -587,6 +593,163 @@ template <typename T> static void physics_step(Sim<T>& s, bool record) {
         predict_position(s);
         if (!s.obs_f.empty()) detect_contacts(s);
         for (u32 it = 0; it < s.p.iter_count; it++) solve_vbd_iteration(s, record, it == 0);
+        update_velocity(s);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// XPBD.  example2_async_iter.cpp:453-697 (constraint
+// colouring, merged arrays), 698-710 (reset), 884-932 (per-cluster solves),
+// 1017-1081 (frame), 1510-1534 (iteration); xpbd_constraints.h:984-1059
+// (solve_stretch_mass_spring_template), 1062-1126 (core_bending_quadratic),
+// 1170-1230 (solve_bending_quadratic_template), 211-236 (get_delta_xpbd_4).
+// ---------------------------------------------------------------------------
+
+// fn_graph_coloring_sequenced_constraint (example2_async_iter.cpp:458-533):
+// repeat { marks = copy(taken); for every untaken constraint in id order
+// whose mark is clear: take it, mark every constraint touching one of its
+// vertices }.
+static std::vector<std::vector<u32>> colour_constraints(u32 n, const std::vector<int>& elems, int per,
+                                                         const std::vector<std::vector<u32>>& vert_adj) {
+    std::vector<std::vector<u32>> out;
+    std::vector<bool> taken(n, false);
+    u32 total = 0;
+    while (total < n) {
+        std::vector<u32> cur;
+        std::vector<bool> marked(taken);
+        for (u32 id = 0; id < n; id++) {
+            if (marked[id]) continue;
+            taken[id] = true;
+            cur.push_back(id);
+            marked[id] = true;
+            for (int j = 0; j < per; j++)
+                for (u32 adj : vert_adj[elems[(size_t)per * id + j]]) marked[adj] = true;
+        }
+        total += (u32)cur.size();
+        out.push_back(cur);
+    }
+    return out;
+}
+
+template <typename T> static void build_xpbd(Sim<T>& s) {
+    if (s.xpbd_ready) return;
+    const Topology& t = *s.topo;
+    const u32 ne = (u32)t.edges.size() / 2, nb = (u32)t.bending.size() / 4;
+    s.cl_stretch = colour_constraints(ne, t.edges, 2, t.adj_edges);
+    s.cl_bending = colour_constraints(nb, t.bending, 4, t.adj_bending);
+    s.m_edges.clear();
+    s.m_rest.clear();
+    for (const auto& c : s.cl_stretch)
+        for (u32 e : c) {
+            s.m_edges.push_back(t.edges[2 * e]);
+            s.m_edges.push_back(t.edges[2 * e + 1]);
+            s.m_rest.push_back(s.rest.rest_len[e]);
+        }
+    s.m_bending.clear();
+    s.m_Q.clear();
+    for (const auto& c : s.cl_bending)
+        for (u32 b : c) {
+            for (int j = 0; j < 4; j++) s.m_bending.push_back(t.bending[4 * b + j]);
+            for (int j = 0; j < 16; j++) s.m_Q.push_back(s.rest.Q[(size_t)b * 16 + j]);
+        }
+    s.lambda_s.assign(ne, T(0));
+    s.lambda_b.assign(nb, T(0));
+    // example2_async_iter.cpp:223: 0 for pinned vertices, else 1/m
+    s.mass_inv.resize(s.mass.size());
+    for (size_t v = 0; v < s.mass.size(); v++) s.mass_inv[v] = s.fixed[v] ? T(0.0f) : T(1.0f) / s.mass[v];
+    s.xpbd_ready = true;
+}
+
+// solve_stretch_mass_spring_template for merged constraint i
+template <typename T> static void xpbd_stretch(Sim<T>& s, u32 i) {
+    const int a = s.m_edges[2 * i], b = s.m_edges[2 * i + 1];
+    const T lambda = s.lambda_s[i];
+    const V3<T> p0 = s.x[a], p1 = s.x[b];
+    const T w0 = s.mass_inv[a], w1 = s.mass_inv[b];
+    const V3<T> diff = p1 - p0;
+    const T lsq = dot(diff, diff);
+    if ((double)lsq < 1e-8) return;
+    const T l = std::sqrt(lsq);
+    const T C = l - s.m_rest[i];
+    const V3<T> dir = diff / l;
+    const V3<T> g1 = -dir;
+    const T k = T(s.p.k_stretch), h = substep_dt<T>(s.p);
+    const T alpha = k < T(1.19209290E-7f) ? T(1e9f) : T(1.f) / (k * h * h);
+    const T denom = alpha + (w0 + w1);
+    const T dl = (-C - alpha * lambda) / denom;
+    V3<T> dx0 = mk(T(0), T(0), T(0)), dx1 = dx0;
+    if (denom != T(0.f)) {
+        dx0 = (dl * w0) * g1;
+        dx1 = ((-dl) * w1) * g1;
+    }
+    s.lambda_s[i] = lambda + dl;
+    s.x[a] = s.x[a] + dx0;
+    s.x[b] = s.x[b] + dx1;
+}
+
+// solve_bending_quadratic_template for merged constraint i (C = the pair's
+// bending energy at unit stiffness, gradient Q x per vertex)
+template <typename T> static void xpbd_bending(Sim<T>& s, u32 i) {
+    const int* e = &s.m_bending[4 * (size_t)i];
+    const T* Q = &s.m_Q[16 * (size_t)i];   // Q[col * 4 + row]
+    T w[4];
+    V3<T> p[4];
+    for (int j = 0; j < 4; j++) {
+        w[j] = s.mass_inv[e[j]];
+        p[j] = s.x[e[j]];
+    }
+    const T lambda = s.lambda_b[i];
+    T sum = T(0.f);
+    for (int ii = 0; ii < 4; ii++)
+        for (int jj = 0; jj < 4; jj++) sum = sum + Q[ii * 4 + jj] * dot(p[ii], p[jj]);
+    const T C = (T(0.5f) * T(1.0f)) * sum;
+    V3<T> g[4];
+    for (int ii = 0; ii < 4; ii++) {
+        V3<T> f = mk(T(0), T(0), T(0));
+        for (int jj = 0; jj < 4; jj++) f = f - Q[ii * 4 + jj] * p[jj];
+        f = T(1.0f) * f;
+        g[ii] = -f;
+    }
+    const T kb = T(s.p.k_bending), h = substep_dt<T>(s.p);
+    const T alpha = kb < T(1.19209290E-7f) ? T(1e12f) : T(1.f) / (kb * h * h);
+    const T denom = w[0] * dot(g[0], g[0]) + w[1] * dot(g[1], g[1]) + w[2] * dot(g[2], g[2]) +
+                    w[3] * dot(g[3], g[3]) + alpha;
+    const T dl = (-C - alpha * lambda) / denom;
+    V3<T> dx[4];
+    for (int j = 0; j < 4; j++) dx[j] = mk(T(0), T(0), T(0));
+    if (denom != T(0.f))
+        for (int j = 0; j < 4; j++) dx[j] = (dl * w[j]) * g[j];
+    s.lambda_b[i] = lambda + dl;
+    for (int j = 0; j < 4; j++) s.x[e[j]] = s.x[e[j]] + dx[j];
+}
+
+// CpuSolver::physics_step_xpbd (example2_async_iter.cpp:1017-1081)
+template <typename T> static void physics_step_xpbd(Sim<T>& s, bool record) {
+    build_xpbd(s);
+    s.x_frame_start = s.x;
+    s.v_frame_start = s.v;
+    s.x_start = s.x_frame_start;
+    s.v_start = s.v_frame_start;
+    s.energy.clear();
+    const u32 nsub = record ? 1u : s.p.num_substep;
+    for (u32 sub = 0; sub < nsub; sub++) {
+        predict_position(s);
+        std::fill(s.lambda_s.begin(), s.lambda_s.end(), T(0.0f));
+        std::fill(s.lambda_b.begin(), s.lambda_b.end(), T(0.0f));
+        for (u32 it = 0; it < s.p.iter_count; it++) {
+            if (record && it == 0) s.energy.push_back((float)compute_energy(s));
+            u32 base = 0;
+            for (const auto& c : s.cl_stretch) {
+                for (u32 k = 0; k < (u32)c.size(); k++) xpbd_stretch(s, base + k);
+                base += (u32)c.size();
+            }
+            base = 0;
+            for (const auto& c : s.cl_bending) {
+                for (u32 k = 0; k < (u32)c.size(); k++) xpbd_bending(s, base + k);
+                base += (u32)c.size();
+            }
+            if (record) s.energy.push_back((float)compute_energy(s));
+        }
         update_velocity(s);
     }
 }
@@ -966,6 +1129,24 @@ u32 orc_sim_energies(const orc_sim* s, float* energy, u32 cap) {
     const u32 n = std::min<u32>(cap, (u32)e.size());
     if (energy) std::copy(e.begin(), e.begin() + n, energy);
     return (u32)e.size();
+}
+
+// XPBD frame (record != 0: one substep with the energy history).
+u32 orc_sim_step_xpbd(orc_sim* s, int record, float* energy, u32 cap) {
+    std::vector<float>* e;
+    if (s->fp64) { physics_step_xpbd(s->s64, record != 0); e = &s->s64.energy; }
+    else { physics_step_xpbd(s->s32, record != 0); e = &s->s32.energy; }
+    const u32 n = std::min<u32>(cap, (u32)e->size());
+    if (energy) std::copy(e->begin(), e->begin() + n, energy);
+    return n;
+}
+// XPBD constraint clusters (which: 0 springs, 1 bending pairs) as a packed CSR.
+u32 orc_sim_xpbd_clusters(orc_sim* s, int which, u32* out, u32 cap) {
+    std::vector<u32> p;
+    if (s->fp64) { build_xpbd(s->s64); p = packed_csr(which ? s->s64.cl_bending : s->s64.cl_stretch); }
+    else { build_xpbd(s->s32); p = packed_csr(which ? s->s32.cl_bending : s->s32.cl_stretch); }
+    if (out) std::copy(p.begin(), p.begin() + std::min<size_t>(cap, p.size()), out);
+    return (u32)p.size();
 }
 
 int orc_max_threads(void) {
