@@ -82,7 +82,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", default="grid_c1", choices=["grid_c1", "batch_c2", "sphere_c4", "copy_calibration",
-                                                               "mlp_kernels", "comm_matrix"],
+                                                               "mlp_kernels", "comm_matrix", "launch_floor",
+                                                               "xpbd_c1"],
                     help="grid_c1 (default, the metric's workload), batch_c2 (512 cloths) or "
                          "copy_calibration (a plain 256 MiB device copy, the reference for HBM counters)")
     ap.add_argument("--strategy", default="auto", choices=["auto", "graph", "persistent", "flow"])
@@ -111,6 +112,10 @@ def main():
         return bench_sphere(args)
     if args.workload == "comm_matrix":
         return bench_comm_matrix(args)
+    if args.workload == "launch_floor":
+        return bench_launch_floor(args)
+    if args.workload == "xpbd_c1":
+        return bench_xpbd(args)
 
     K, W = args.steps, args.warmup
     cloth, params, _, wind = configs.grid_c1(frames=W + K)
@@ -383,6 +388,111 @@ def bench_batch(args):
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": "k_vbd_batch"},
     }
+    print(json.dumps(out), flush=True)
+
+
+# Compulsory bytes of one projection (xpbd.h): a spring reads its slot pair,
+# multiplier, rest length, two inverse masses and two positions, and writes
+# the multiplier and two positions; a bending pair reads its four slots, Q
+# (16 floats), multiplier, four inverse masses and four positions, and
+# writes the multiplier and four positions.
+XPBD_SPRING_BYTES = 8 + 4 + 4 + 8 + 24 + 4 + 24
+XPBD_BENDING_BYTES = 16 + 64 + 4 + 16 + 48 + 4 + 48
+
+
+def bench_xpbd(args):
+    """The XPBD solver on the metric's cloth: 256x256, 100 iterations per
+    frame, every spring cluster then every bending cluster per iteration,
+    one kernel per constraint cluster in a frame graph.  value = constraint
+    projections per second; roofline of the cluster kernel with the larger
+    share of the frame."""
+    from libatssim_amd import GpuVBD, configs
+    K, W = args.steps, args.warmup
+    cloth, params, _, _ = configs.grid_c1(frames=W + K)
+    t = cloth.topology
+    g = GpuVBD(cloth, params)
+    g.set_solver("xpbd")
+    ns, nb = len(g.xpbd_clusters(0)), len(g.xpbd_clusters(1))
+    if W:
+        g.run_frames(W)
+    g.synchronize()
+    t0 = time.perf_counter()
+    g.run_frames(K)
+    g.synchronize()
+    wall = time.perf_counter() - t0
+    gpu_ms = g.last_elapsed_ms()
+    iters = params.constraint_iter_count
+    projections = (t.num_edges + t.num_bending) * iters * K
+    ms_s, ms_b, ls, lb = g.time_xpbd_kernels(10)
+    per_s = t.num_edges * XPBD_SPRING_BYTES / ns
+    per_b = t.num_bending * XPBD_BENDING_BYTES / nb
+    share_s, share_b = ms_s * ns, ms_b * nb
+    kern, per, ms = ("k_xpbd_bending", per_b, ms_b) if share_b >= share_s else ("k_xpbd_stretch", per_s, ms_s)
+    achieved = per / (ms * 1e-3) / 1e9
+    out = {
+        "metric": "XPBD constraint projections/s (256x256 cloth)",
+        "value": projections / wall, "unit": "projections/s", "n_gpus": 1, "steps": K, "warmup": W,
+        "ms_per_step": wall * 1e3 / K, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"xpbd_c1: 256x256 cloth ({t.num_edges} springs, {t.num_bending} bending pairs), "
+                               f"top row pinned, {iters} XPBD iterations/frame, one frame per step",
+                   "spring_clusters": ns, "bending_clusters": nb, "gpu_ms_per_step": gpu_ms / K},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": kern,
+                     "mean_launch_us": round(ms * 1e3, 3), "algorithmic_bytes_per_launch": round(per, 1),
+                     "stretch_launch_us": round(ms_s * 1e3, 3), "bending_launch_us": round(ms_b * 1e3, 3),
+                     "bytes_per_spring": XPBD_SPRING_BYTES, "bytes_per_bending_pair": XPBD_BENDING_BYTES},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline_xpbd(cloth, params)
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline_xpbd(cloth, params, min_seconds=10.0, max_frames=40):
+    from oracle import oracle as O
+    threads = min(O.max_threads(), 16)
+    m = O.Mesh(cloth.x, cloth.faces)
+    s = O.Sim(m, cloth.mass, cloth.fixed, implicit_dt=params.implicit_dt, num_substep=params.num_substep,
+              iter_count=params.constraint_iter_count, k_stretch=params.stiffness_stretch_spring,
+              k_bending=params.stiffness_quadratic_bending, damping=params.damping_cloth, threads=threads)
+    t = cloth.topology
+    frames = 0
+    t0 = time.perf_counter()
+    while True:
+        s.step_xpbd()
+        frames += 1
+        el = time.perf_counter() - t0
+        if el >= min_seconds or frames >= max_frames:
+            break
+    n = (t.num_edges + t.num_bending) * params.constraint_iter_count * frames
+    return {"value": n / el, "unit": "projections/s", "cores": threads, "kind": "port",
+            "sample": f"{frames} XPBD frames of the same 256x256 workload ({n} projections, {el:.1f} s), "
+                      f"oracle/vbd_oracle.cpp FP32, OpenMP per-cluster passes"}
+
+
+def bench_launch_floor(args):
+    """Latency floors at the colour-pass shape, microseconds: an empty kernel
+    and a dependent-gather kernel per launch (graphs of 800 launches, as a
+    frame graph of 7 colours x 100 sweeps replays its passes), and one grid
+    barrier inside a cooperative kernel (800 barriers), at the thread counts
+    of the 256x256 colour passes (8,192-vertex colours x 8 lanes = 65,536
+    threads) and of 256 / 64 blocks for the barrier."""
+    import ctypes
+    from libatssim_amd import _lib
+    L = _lib.lib()
+
+    def floor(kind, threads, launches=800):
+        us = ctypes.c_float()
+        _lib.check(L.lats_time_floor(kind, threads, launches, ctypes.byref(us)))
+        return round(us.value, 3)
+
+    out = {"workload": "launch_floor", "launches_per_graph": 800, "unit": "us",
+           "empty_65536_threads": floor(_lib.LATS_FLOOR_EMPTY, 65536),
+           "empty_64_threads": floor(_lib.LATS_FLOOR_EMPTY, 64),
+           "gather_65536_threads": floor(_lib.LATS_FLOOR_GATHER, 65536),
+           "gather_8192_threads": floor(_lib.LATS_FLOOR_GATHER, 8192),
+           "grid_barrier_256_blocks": floor(_lib.LATS_FLOOR_GRID_BARRIER, 256 * 256),
+           "grid_barrier_64_blocks": floor(_lib.LATS_FLOOR_GRID_BARRIER, 64 * 256)}
     print(json.dumps(out), flush=True)
 
 

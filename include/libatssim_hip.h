@@ -348,6 +348,13 @@ int lats_vbd_copy_permutation(lats_vbd* ctx, uint32_t* perm);
  * updates they performed. */
 int lats_vbd_time_colour_kernels(lats_vbd* ctx, uint32_t iterations, float* mean_ms, uint32_t* launches,
                                  uint64_t* vertex_updates);
+/* Measurement, XPBD solver: `iterations` passes over every spring cluster
+ * (one kernel per cluster) captured in one graph and timed as above, then
+ * the same over every bending cluster; the positions are restored (the
+ * multipliers restart at zero with the next frame).  *_ms = mean
+ * milliseconds per launch, *_launches = launches timed. */
+int lats_vbd_time_xpbd_kernels(lats_vbd* ctx, uint32_t iterations, float* stretch_ms, float* bending_ms,
+                               uint32_t* stretch_launches, uint32_t* bending_launches);
 /* Bytes of device memory held by the context. */
 int lats_vbd_device_bytes(lats_vbd* ctx, uint64_t* bytes);
 
@@ -419,6 +426,19 @@ int lats_mlp_warm_start_frame(lats_mlp* m, lats_vbd* ctx, float target, uint32_t
  * training rows (rows from the states the warm-started solver visits). */
 int lats_mlp_record_guided_frame(lats_mlp* m, lats_vbd* ctx, lats_vbd* ref, uint32_t max_sweeps, uint32_t first_check,
                                  uint32_t check_every, int use_model, float* out);
+
+/* ---- latency floors (measurement; no reference counterpart) --------------
+ * Mean microseconds per launch of one graph of `launches` kernels of
+ * `threads` threads (256 per block) that do nothing (EMPTY) or gather 4
+ * positions each through random indices and store their sum (GATHER: the
+ * dependent ELL -> position chain of a colour pass, without its math); or
+ * mean microseconds per barrier of one cooperative kernel of threads / 256
+ * co-resident blocks crossing `launches` grid-wide barriers (GRID_BARRIER,
+ * bounded waits: LATS_ERR_TIMEOUT if one gives up). */
+#define LATS_FLOOR_EMPTY 0
+#define LATS_FLOOR_GATHER 1
+#define LATS_FLOOR_GRID_BARRIER 2
+int lats_time_floor(int kind, uint32_t threads, uint32_t launches, float* us);
 
 #ifdef __cplusplus
 }

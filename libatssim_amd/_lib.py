@@ -41,6 +41,7 @@ LATS_FN_RESET_CONSTRAINS = 6
 LATS_FN_XPBD_STRETCH = 7
 LATS_FN_XPBD_BENDING = 8
 LATS_SOLVER_VBD, LATS_SOLVER_XPBD = 0, 1
+LATS_FLOOR_EMPTY, LATS_FLOOR_GATHER, LATS_FLOOR_GRID_BARRIER = 0, 1, 2
 
 LATS_BUFFER_DEFAULT = 0x0FFFFFFF
 LATS_BUFFER_NONE = 0xFFFFFFFF
@@ -202,6 +203,9 @@ SIGNATURES = {
     "lats_mlp_train": (_I, [_VP, _U32, _U32, ctypes.c_float, ctypes.c_uint64, ctypes.POINTER(ctypes.c_double)]),
     "lats_mlp_warm_start_frame": (_I, [_VP, _VP, ctypes.c_float, _U32, _U32, _U32, _I, _VP]),
     "lats_mlp_record_guided_frame": (_I, [_VP, _VP, _VP, _U32, _U32, _U32, _I, _VP]),
+    "lats_vbd_time_xpbd_kernels": (_I, [_VP, _U32, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
+                                        ctypes.POINTER(_U32), ctypes.POINTER(_U32)]),
+    "lats_time_floor": (_I, [_I, _U32, _U32, ctypes.POINTER(ctypes.c_float)]),
 }
 
 _LIB = None

@@ -682,14 +682,18 @@ std::vector<uint16_t> to_bf16_rows(const float* X, uint32_t M) {
     return xb;
 }
 
+// The copies are enqueued on dst's stream, ahead of dst's next frame: a
+// synchronous hipMemcpy between device buffers may return before the copy
+// is done, and the null stream it runs on does not order dst's
+// non-blocking stream (its predict could read a half-copied state).
 int copy_frame_state(lats_vbd* dst, lats_vbd* src) {
     const size_t nf = (size_t)3 * src->V * 4;
     HIPCHK(hipStreamSynchronize(src->stream));
+    HIPCHK(hipMemcpyAsync(dst->x.p, src->x.p, nf, hipMemcpyDeviceToDevice, dst->stream));
+    HIPCHK(hipMemcpyAsync(dst->x0.p, src->x0.p, nf, hipMemcpyDeviceToDevice, dst->stream));
+    HIPCHK(hipMemcpyAsync(dst->v.p, src->v.p, nf, hipMemcpyDeviceToDevice, dst->stream));
+    HIPCHK(hipMemcpyAsync(dst->fp.p, src->fp.p, sizeof(FrameParams), hipMemcpyDeviceToDevice, dst->stream));
     HIPCHK(hipStreamSynchronize(dst->stream));
-    HIPCHK(hipMemcpy(dst->x.p, src->x.p, nf, hipMemcpyDeviceToDevice));
-    HIPCHK(hipMemcpy(dst->x0.p, src->x0.p, nf, hipMemcpyDeviceToDevice));
-    HIPCHK(hipMemcpy(dst->v.p, src->v.p, nf, hipMemcpyDeviceToDevice));
-    HIPCHK(hipMemcpy(dst->fp.p, src->fp.p, sizeof(FrameParams), hipMemcpyDeviceToDevice));
     return LATS_OK;
 }
 
@@ -879,8 +883,7 @@ int lats_mlp_get_params(lats_mlp* m, float* p) {
 
 int lats_mlp_set_params(lats_mlp* m, const float* p) {
     if (!m || !p) return fail(LATS_ERR_INVALID_ARGUMENT, "null argument");
-    HIPCHK(hipStreamSynchronize(m->stream));
-    HIPCHK(hipMemcpy(m->P.p, p, kNParam * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpyAsync(m->P.p, p, kNParam * sizeof(float), hipMemcpyHostToDevice, m->stream));
     enqueue_pack(m, m->stream);
     HIPCHK(hipStreamSynchronize(m->stream));
     return LATS_OK;
@@ -897,8 +900,8 @@ int lats_mlp_set_feature_scale(lats_mlp* m, const float* scale48) {
     for (int c = 0; c < kIn; c++)
         if (!(scale48[c] > 0.f) || !std::isfinite(scale48[c]))
             return fail(LATS_ERR_INVALID_ARGUMENT, "feature scales must be finite and positive");
+    HIPCHK(hipMemcpyAsync(m->scale.p, scale48, kIn * sizeof(float), hipMemcpyHostToDevice, m->stream));
     HIPCHK(hipStreamSynchronize(m->stream));
-    HIPCHK(hipMemcpy(m->scale.p, scale48, kIn * sizeof(float), hipMemcpyHostToDevice));
     return LATS_OK;
 }
 
@@ -996,8 +999,10 @@ int lats_mlp_reserve_rows(lats_mlp* m, uint64_t rows) {
     HIPCHK(nt.alloc(rows * 4));
     HIPCHK(hipStreamSynchronize(m->stream));
     if (m->ds_rows) {
-        HIPCHK(hipMemcpy(nx.p, m->dsX.p, m->ds_rows * kInP * 2, hipMemcpyDeviceToDevice));
-        HIPCHK(hipMemcpy(nt.p, m->dsT.p, m->ds_rows * 4 * 4, hipMemcpyDeviceToDevice));
+        // on the network's stream, finished before the old rows are freed
+        HIPCHK(hipMemcpyAsync(nx.p, m->dsX.p, m->ds_rows * kInP * 2, hipMemcpyDeviceToDevice, m->stream));
+        HIPCHK(hipMemcpyAsync(nt.p, m->dsT.p, m->ds_rows * 4 * 4, hipMemcpyDeviceToDevice, m->stream));
+        HIPCHK(hipStreamSynchronize(m->stream));
     }
     std::swap(m->dsX.p, nx.p);
     std::swap(m->dsX.n, nx.n);
@@ -1052,9 +1057,9 @@ int lats_mlp_dataset_append(lats_mlp* m, const float* X, const float* T, uint32_
     for (uint32_t r = 0; r < M; r++)
         for (int c = 0; c < kOut; c++) tb[(size_t)r * 4 + c] = T[(size_t)r * kOut + c];
     const std::vector<uint16_t> xb = to_bf16_rows(X, M);
+    HIPCHK(hipMemcpyAsync(m->dsX.p + m->ds_rows * kInP, xb.data(), xb.size() * 2, hipMemcpyHostToDevice, m->stream));
+    HIPCHK(hipMemcpyAsync(m->dsT.p + m->ds_rows * 4, tb.data(), tb.size() * 4, hipMemcpyHostToDevice, m->stream));
     HIPCHK(hipStreamSynchronize(m->stream));
-    HIPCHK(hipMemcpy(m->dsX.p + m->ds_rows * kInP, xb.data(), xb.size() * 2, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(m->dsT.p + m->ds_rows * 4, tb.data(), tb.size() * 4, hipMemcpyHostToDevice));
     m->ds_rows += M;
     return LATS_OK;
 }
@@ -1111,8 +1116,8 @@ int lats_mlp_fit_feature_scale(lats_mlp* m, float* scale48) {
     hipLaunchKernelGGL(k_mlp_rescale, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, m->stream, m->dsX.p,
                        m->ds_rows, mul.p);
     HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(m->scale.p, sc.data(), kIn * 4, hipMemcpyHostToDevice, m->stream));
     HIPCHK(hipStreamSynchronize(m->stream));
-    HIPCHK(hipMemcpy(m->scale.p, sc.data(), kIn * 4, hipMemcpyHostToDevice));
     if (scale48) std::memcpy(scale48, sc.data(), kIn * 4);
     return LATS_OK;
 }

@@ -151,7 +151,11 @@ template <typename T> struct DBuf {
         n = std::max<size_t>(h.size(), 1);
         hipError_t e = hipMalloc((void**)&p, n * sizeof(T));
         if (e != hipSuccess || h.empty()) return e;
-        return hipMemcpy(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice);
+        e = hipMemcpy(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice);
+        // the copy runs on the null stream, which does not order the
+        // library's non-blocking streams: wait until it has landed
+        if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+        return e;
     }
 };
 
@@ -354,10 +358,10 @@ int lats_batch_set_state(lats_batch* b, uint32_t k, const float* x, const float*
             hv[3 * s + c] = v[3 * (size_t)b->perm[k][s] + c];
         }
     const size_t off = 3 * (size_t)ci.vbase;
+    HIPCHK(hipMemcpyAsync(b->x.p + off, hx.data(), hx.size() * 4, hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipMemcpyAsync(b->x0.p + off, hx.data(), hx.size() * 4, hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipMemcpyAsync(b->v.p + off, hv.data(), hv.size() * 4, hipMemcpyHostToDevice, b->stream));
     HIPCHK(hipStreamSynchronize(b->stream));
-    HIPCHK(hipMemcpy(b->x.p + off, hx.data(), hx.size() * 4, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(b->x0.p + off, hx.data(), hx.size() * 4, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(b->v.p + off, hv.data(), hv.size() * 4, hipMemcpyHostToDevice));
     return LATS_OK;
 }
 
@@ -443,10 +447,12 @@ int lats_batch_autotune(lats_batch* b, uint32_t frames, uint32_t* best_threads, 
     HIPCHK(hipMalloc((void**)&sx, nf * 4));
     HIPCHK(hipMalloc((void**)&sx0, nf * 4));
     HIPCHK(hipMalloc((void**)&sv, nf * 4));
+    // device-to-device copies on the batch's stream (a synchronous hipMemcpy
+    // between device buffers is not ordered with a non-blocking stream)
+    HIPCHK(hipMemcpyAsync(sx, b->x.p, nf * 4, hipMemcpyDeviceToDevice, b->stream));
+    HIPCHK(hipMemcpyAsync(sx0, b->x0.p, nf * 4, hipMemcpyDeviceToDevice, b->stream));
+    HIPCHK(hipMemcpyAsync(sv, b->v.p, nf * 4, hipMemcpyDeviceToDevice, b->stream));
     HIPCHK(hipStreamSynchronize(b->stream));
-    HIPCHK(hipMemcpy(sx, b->x.p, nf * 4, hipMemcpyDeviceToDevice));
-    HIPCHK(hipMemcpy(sx0, b->x0.p, nf * 4, hipMemcpyDeviceToDevice));
-    HIPCHK(hipMemcpy(sv, b->v.p, nf * 4, hipMemcpyDeviceToDevice));
     float best = 3.4e38f;
     uint32_t bt = b->block_threads, bb = b->blocks, bl = b->lanes;
     int rc = LATS_OK;
@@ -470,9 +476,11 @@ int lats_batch_autotune(lats_batch* b, uint32_t frames, uint32_t* best_threads, 
         if (rc) break;
     }
     if (!rc) rc = lats_batch_set_lanes_per_vertex(b, bl);
-    hipError_t e1 = hipMemcpy(b->x.p, sx, nf * 4, hipMemcpyDeviceToDevice);
-    hipError_t e2 = hipMemcpy(b->x0.p, sx0, nf * 4, hipMemcpyDeviceToDevice);
-    hipError_t e3 = hipMemcpy(b->v.p, sv, nf * 4, hipMemcpyDeviceToDevice);
+    hipError_t e1 = hipMemcpyAsync(b->x.p, sx, nf * 4, hipMemcpyDeviceToDevice, b->stream);
+    hipError_t e2 = hipMemcpyAsync(b->x0.p, sx0, nf * 4, hipMemcpyDeviceToDevice, b->stream);
+    hipError_t e3 = hipMemcpyAsync(b->v.p, sv, nf * 4, hipMemcpyDeviceToDevice, b->stream);
+    const hipError_t e4 = hipStreamSynchronize(b->stream);
+    if (e1 == hipSuccess) e1 = e4;
     (void)hipFree(sx);
     (void)hipFree(sx0);
     (void)hipFree(sv);

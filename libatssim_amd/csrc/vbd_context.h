@@ -32,7 +32,11 @@ template <typename T> struct DBuf {
     hipError_t upload(const std::vector<T>& h) {
         hipError_t e = alloc(h.size());
         if (e != hipSuccess || h.empty()) return e;
-        return hipMemcpy(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice);
+        e = hipMemcpy(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice);
+        // the copy runs on the null stream, which does not order the
+        // library's non-blocking streams: wait until it has landed
+        if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+        return e;
     }
 };
 

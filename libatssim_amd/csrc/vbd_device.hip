@@ -1397,16 +1397,20 @@ int lats_vbd_autotune(lats_vbd* c, uint32_t frames, int* chosen, uint32_t* chose
     HIPCHK(sx0.alloc(nf));
     HIPCHK(sv.alloc(nf));
     FrameParams fps;
+    // Device-to-device copies go on the context's stream: a synchronous
+    // hipMemcpy between device buffers may return before the copy is done
+    // and is not ordered with the (non-blocking) stream's later kernels.
+    HIPCHK(hipMemcpyAsync(sx.p, c->x.p, nf * 4, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(sx0.p, c->x0.p, nf * 4, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(sv.p, c->v.p, nf * 4, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(&fps, c->fp.p, sizeof(fps), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    HIPCHK(hipMemcpy(sx.p, c->x.p, nf * 4, hipMemcpyDeviceToDevice));
-    HIPCHK(hipMemcpy(sx0.p, c->x0.p, nf * 4, hipMemcpyDeviceToDevice));
-    HIPCHK(hipMemcpy(sv.p, c->v.p, nf * 4, hipMemcpyDeviceToDevice));
-    HIPCHK(hipMemcpy(&fps, c->fp.p, sizeof(fps), hipMemcpyDeviceToHost));
     auto restore = [&]() -> int {
-        HIPCHK(hipMemcpy(c->x.p, sx.p, nf * 4, hipMemcpyDeviceToDevice));
-        HIPCHK(hipMemcpy(c->x0.p, sx0.p, nf * 4, hipMemcpyDeviceToDevice));
-        HIPCHK(hipMemcpy(c->v.p, sv.p, nf * 4, hipMemcpyDeviceToDevice));
-        HIPCHK(hipMemcpy(c->fp.p, &fps, sizeof(fps), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpyAsync(c->x.p, sx.p, nf * 4, hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(c->x0.p, sx0.p, nf * 4, hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(c->v.p, sv.p, nf * 4, hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(c->fp.p, &fps, sizeof(fps), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
         return LATS_OK;
     };
     struct Cand { int strategy; uint32_t blocks; uint32_t lanes; uint32_t regions = 1; };
@@ -1488,10 +1492,11 @@ int lats_vbd_set_state(lats_vbd* c, const float* x, const float* v) {
             hx[3 * s + k] = x[3 * (size_t)c->perm[s] + k];
             hv[3 * s + k] = v[3 * (size_t)c->perm[s] + k];
         }
+    // stream-ordered copies, complete before hx / hv go out of scope
+    HIPCHK(hipMemcpyAsync(c->x.p, hx.data(), n * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->x0.p, hx.data(), n * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->v.p, hv.data(), n * 4, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    HIPCHK(hipMemcpy(c->x.p, hx.data(), n * 4, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(c->x0.p, hx.data(), n * 4, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(c->v.p, hv.data(), n * 4, hipMemcpyHostToDevice));
     return LATS_OK;
 }
 
@@ -1539,7 +1544,9 @@ int lats_vbd_run_frames(lats_vbd* c, uint32_t nframes, const float* forces) {
     if (!c) return fail(LATS_ERR_INVALID_ARGUMENT, "null context");
     if (forces) {
         if (c->forces.n < 3 * (size_t)nframes) HIPCHK(c->forces.alloc(3 * (size_t)nframes));
-        HIPCHK(hipMemcpy(c->forces.p, forces, 3 * (size_t)nframes * sizeof(float), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpyAsync(c->forces.p, forces, 3 * (size_t)nframes * sizeof(float), hipMemcpyHostToDevice,
+                              c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
         int rc = set_frame_params(c, c->forces.p, 0, nullptr);
         if (rc) return rc;
     }
@@ -1941,6 +1948,56 @@ int lats_vbd_time_colour_kernels(lats_vbd* c, uint32_t iters, float* mean_ms, ui
     *mean_ms = ms / (float)nl;
     if (launches) *launches = nl;
     if (vertex_updates) *vertex_updates = vu;
+    return LATS_OK;
+}
+
+int lats_vbd_time_xpbd_kernels(lats_vbd* c, uint32_t iters, float* stretch_ms, float* bending_ms,
+                               uint32_t* stretch_launches, uint32_t* bending_launches) {
+    if (!c || !stretch_ms || !bending_ms) return fail(LATS_ERR_INVALID_ARGUMENT, "null argument");
+    if (iters == 0) return fail(LATS_ERR_INVALID_ARGUMENT, "iterations must be positive");
+    if (c->solver != LATS_SOLVER_XPBD || !c->xpbd_built)
+        return fail(LATS_ERR_UNSUPPORTED, "select the XPBD solver first (lats_vbd_set_solver)");
+    const size_t nf = 3 * (size_t)c->V;
+    const uint32_t ns = (uint32_t)c->xs_bound.size() - 1, nb = (uint32_t)c->xb_bound.size() - 1;
+    float* out[2] = {stretch_ms, bending_ms};
+    uint32_t* count[2] = {stretch_launches, bending_launches};
+    HIPCHK(hipMemcpyAsync(c->scratch3.p, c->x.p, nf * 4, hipMemcpyDeviceToDevice, c->stream));
+    for (int which = 0; which < 2; which++) {
+        const uint32_t n = which ? nb : ns;
+        uint32_t nl = 0;
+        for (uint32_t k = 0; k < n; k++) {
+            const uint32_t sz = which ? c->xb_bound[k + 1] - c->xb_bound[k] : c->xs_bound[k + 1] - c->xs_bound[k];
+            nl += sz ? iters : 0u;
+        }
+        *out[which] = 0.f;
+        if (count[which]) *count[which] = nl;
+        if (!nl) continue;
+        hipGraph_t graph;
+        HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        for (uint32_t it = 0; it < iters; it++)
+            for (uint32_t k = 0; k < n; k++) {
+                if (which) enqueue_xpbd_bending(c, k);
+                else enqueue_xpbd_stretch(c, k);
+            }
+        HIPCHK(hipStreamEndCapture(c->stream, &graph));
+        hipGraphExec_t exec = nullptr;
+        hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) return fail(LATS_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        float ms = 0.f;
+        e = hipGraphLaunch(exec, c->stream);   // warm-up replay
+        if (e == hipSuccess) e = hipEventRecord(c->ev0, c->stream);
+        if (e == hipSuccess) e = hipGraphLaunch(exec, c->stream);
+        if (e == hipSuccess) e = hipEventRecord(c->ev1, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, c->ev0, c->ev1);
+        (void)hipGraphExecDestroy(exec);
+        if (e != hipSuccess) return fail(LATS_ERR_HIP, hipGetErrorString(e));
+        *out[which] = ms / (float)nl;
+    }
+    // positions back; the multipliers restart at zero with the next frame
+    HIPCHK(hipMemcpyAsync(c->x.p, c->scratch3.p, nf * 4, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
     return LATS_OK;
 }
 

@@ -391,6 +391,15 @@ class GpuVBD:
                                                  ctypes.byref(vu)))
         return ms.value, n.value, vu.value
 
+    def time_xpbd_kernels(self, iterations):
+        """XPBD solver: (mean ms per spring-cluster launch, mean ms per
+        bending-cluster launch, spring launches, bending launches)."""
+        ms_s, ms_b = ctypes.c_float(), ctypes.c_float()
+        n_s, n_b = ctypes.c_uint32(), ctypes.c_uint32()
+        check(lib().lats_vbd_time_xpbd_kernels(self._h, int(iterations), ctypes.byref(ms_s), ctypes.byref(ms_b),
+                                               ctypes.byref(n_s), ctypes.byref(n_b)))
+        return ms_s.value, ms_b.value, n_s.value, n_b.value
+
     def device_bytes(self):
         b = ctypes.c_uint64()
         check(lib().lats_vbd_device_bytes(self._h, ctypes.byref(b)))

@@ -129,3 +129,24 @@ def test_xpbd_dag_through_scheduler_matches_fused(gpu_available):
     r = subprocess.run([path], capture_output=True, text=True, timeout=120)
     print(r.stdout[-2000:])
     assert r.returncode == 0 and "PASS" in r.stdout, r.stdout + r.stderr
+
+
+def test_xpbd_kernel_timing_restores_state(gpu_available):
+    _print_case("XPBD kernel timing, grid_c0")
+    cloth, _, _, _ = configs.grid_c0()
+    params = Params(constraint_iter_count=4)
+    g = GpuVBD(cloth, params)
+    v = GpuVBD(cloth, params)
+    with pytest.raises(_lib.LatsError):
+        g.time_xpbd_kernels(2)          # VBD solver selected
+    g.set_solver("xpbd")
+    v.set_solver("xpbd")
+    g.physics_step()
+    v.physics_step()
+    ms_s, ms_b, ls, lb = g.time_xpbd_kernels(3)
+    assert ms_s > 0 and ms_b > 0
+    assert ls == 3 * len(g.xpbd_clusters(0)) and lb == 3 * len(g.xpbd_clusters(1))
+    assert np.array_equal(g.state()[0], v.state()[0])
+    g.physics_step()
+    v.physics_step()
+    assert np.array_equal(g.state()[0], v.state()[0])
