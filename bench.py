@@ -48,9 +48,22 @@ def algorithmic_bytes_per_sweep(nv, ne, nb):
     return 53 * nv + 20 * ne + 72 * nb + 8
 
 
-def cpu_baseline(cloth, params, wind, min_seconds=10.0, max_frames=40):
+def _host_cpu():
+    """(CPU model, logical CPUs) of this host."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return model, os.cpu_count()
+
+
+def _oracle_vbd_rate(cloth, params, wind, threads, min_seconds, max_frames):
+    """(vertex updates per second, frames, seconds) of the oracle's FP32 VBD frames."""
     from oracle import oracle as O
-    threads = min(O.max_threads(), 16)
     m = O.Mesh(cloth.x, cloth.faces)
     s = O.Sim(m, cloth.mass, cloth.fixed, implicit_dt=params.implicit_dt, num_substep=params.num_substep,
               iter_count=params.constraint_iter_count, k_stretch=params.stiffness_stretch_spring,
@@ -65,13 +78,28 @@ def cpu_baseline(cloth, params, wind, min_seconds=10.0, max_frames=40):
         if el >= min_seconds or frames >= max_frames:
             break
     vu = cloth.num_verts * params.constraint_iter_count * frames
+    return vu / el, frames, el
+
+
+def cpu_baseline(cloth, params, wind, min_seconds=10.0, max_frames=40):
+    """The oracle on this host's cores (at most 16 threads), and on one thread
+    over a shorter sample."""
+    from oracle import oracle as O
+    threads = min(O.max_threads(), 16)
+    rate, frames, el = _oracle_vbd_rate(cloth, params, wind, threads, min_seconds, max_frames)
+    rate1, frames1, el1 = _oracle_vbd_rate(cloth, params, wind, 1, min_seconds / 2, max_frames)
+    model, ncpu = _host_cpu()
+    vu = cloth.num_verts * params.constraint_iter_count * frames
     return {
-        "value": vu / el,
+        "value": rate,
         "unit": "vertex-updates/s",
         "cores": threads,
         "kind": "port",
         "sample": f"{frames} frames of the same 256x256 workload ({vu} vertex updates, {el:.1f} s), "
                   f"oracle/vbd_oracle.cpp FP32, OpenMP per-colour passes",
+        "one_thread": {"value": rate1, "sample": f"{frames1} frames, {el1:.1f} s"},
+        "host_cpu": model,
+        "host_logical_cpus": ncpu,
     }
 
 
@@ -282,7 +310,9 @@ def bench_sphere(args):
     g = GpuVBD(cloth, params)
     g.set_obstacle(ox, of, ct["thickness"], ct["stiffness"], ct["radius"], ct["cell"])
     if args.strategy == "auto":
-        strategy, blocks = g.autotune(frames=1)
+        # three timed frames per candidate: one frame (~10 ms) is too short to
+        # separate candidates a few per cent apart
+        strategy, blocks = g.autotune(frames=3)
     else:
         g.set_strategy(_lib.LATS_STRATEGY_GRAPH)
         strategy, blocks = g.strategy()
@@ -467,7 +497,8 @@ def cpu_baseline_xpbd(cloth, params, min_seconds=10.0, max_frames=40):
     n = (t.num_edges + t.num_bending) * params.constraint_iter_count * frames
     return {"value": n / el, "unit": "projections/s", "cores": threads, "kind": "port",
             "sample": f"{frames} XPBD frames of the same 256x256 workload ({n} projections, {el:.1f} s), "
-                      f"oracle/vbd_oracle.cpp FP32, OpenMP per-cluster passes"}
+                      f"oracle/vbd_oracle.cpp FP32, OpenMP per-cluster passes",
+            "host_cpu": _host_cpu()[0], "host_logical_cpus": _host_cpu()[1]}
 
 
 def bench_launch_floor(args):
