@@ -33,12 +33,12 @@ METRIC = "VBD vertex-updates/s (256² cloth); ms/frame to fixed residual, learne
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_vbd_colour_grid256_traffic.txt")
 
 
-def read_traffic(path):
-    """HBM bytes per launch from the committed PMC summary (`hbm_bytes_per_launch: N` line)."""
+def read_traffic(path, key="hbm_bytes_per_launch"):
+    """HBM bytes per launch from a committed PMC summary (its `<key>: N` line)."""
     if not os.path.exists(path):
         return None
     for line in open(path):
-        if line.startswith("hbm_bytes_per_launch:"):
+        if line.startswith(key + ":"):
             return float(line.split(":", 1)[1])
     return None
 
@@ -230,7 +230,8 @@ def main():
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(cloth, params, wind)
     if not args.no_warm_start:
-        out["warm_start"] = bench_warm_start(args, cloth, params, wind, plain_ms=gpu_ms / K)
+        out["warm_start"] = bench_warm_start(args, cloth, params, wind, plain_ms=gpu_ms / K,
+                                             lanes=g.lanes_per_vertex(), regions=g.regions())
     print(json.dumps(out), flush=True)
 
 
@@ -262,7 +263,7 @@ def bench_mlp(args):
                       "train_step_ms": round(step_ms, 4), "iters": args.steps}), flush=True)
 
 
-def bench_warm_start(args, cloth, params, wind, plain_ms):
+def bench_warm_start(args, cloth, params, wind, plain_ms, lanes=8, regions=1):
     """Learned warm start on the same cloth and wind: per frame, wall time of
     the warm-started solve until the energy reaches the plain solver's
     100-iteration energy from the same frame start (computed on a second
@@ -279,7 +280,8 @@ def bench_warm_start(args, cloth, params, wind, plain_ms):
     train_s = time.perf_counter() - t0
     full_wind = configs.grid_c1(frames=60)[3]
     frames = len(full_wind)
-    res = evaluate(m, cloth, params, full_wind, frames)
+    # the warm-started frames run the plain frame's colour-pass configuration
+    res = evaluate(m, cloth, params, full_wind, frames, lanes=lanes, regions=regions)
     fwd_ms, step_ms = m.time_kernels(cloth.num_verts, 20)   # after the evaluation: timing steps train
     wall = np.array([r["wall_ms"] for r in res])
     sweeps = np.array([r["sweeps"] for r in res])
@@ -350,11 +352,13 @@ def bench_sphere(args):
                      "algorithmic_bytes_per_launch": round(b_sweep / t.num_clusters, 1)},
     }
     if not args.no_warm_start:
-        out["warm_start"] = bench_sphere_warm_start(args, cloth, params, (ox, of), ct, plain_ms=gpu_ms / K)
+        out["warm_start"] = bench_sphere_warm_start(args, cloth, params, (ox, of), ct, plain_ms=gpu_ms / K,
+                                                    lanes=g.lanes_per_vertex(), regions=g.regions())
     print(json.dumps(out), flush=True)
 
 
-def bench_sphere_warm_start(args, cloth, params, obstacle, ct, plain_ms, scenarios=4, frames=120):
+def bench_sphere_warm_start(args, cloth, params, obstacle, ct, plain_ms, scenarios=4, frames=120, lanes=8,
+                            regions=1):
     """Warm start retrained on the sphere regime (plain and guided rollouts of
     smaller drops), evaluated on the 1024x1024 drop: per frame, wall time to
     the plain 50-iteration energy from the same frame start."""
@@ -373,7 +377,7 @@ def bench_sphere_warm_start(args, cloth, params, obstacle, ct, plain_ms, scenari
                  setup=setup)
     train_s = time.perf_counter() - t0
     n_eval = min(frames, args.warm_frames * 2)
-    res = evaluate(m, cloth, params, None, n_eval, setup=setup)
+    res = evaluate(m, cloth, params, None, n_eval, setup=setup, lanes=lanes, regions=regions)
     wall = np.array([r["wall_ms"] for r in res])
     sweeps = np.array([r["sweeps"] for r in res])
     return {
@@ -414,7 +418,7 @@ def bench_batch(args):
         "n_gpus": 1, "steps": K, "warmup": W, "ms_per_step": wall * 1e3 / K,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"batch_c2: {n} cloths, {nv} verts, 100 iterations/frame", "block_threads": threads,
-                   "blocks": blocks, "gpu_ms_per_step": gpu_ms / K},
+                   "blocks": blocks, "lanes_per_vertex": b.lanes_per_vertex(), "gpu_ms_per_step": gpu_ms / K},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": "k_vbd_batch"},
     }
@@ -468,7 +472,9 @@ def bench_xpbd(args):
                                f"top row pinned, {iters} XPBD iterations/frame, one frame per step",
                    "spring_clusters": ns, "bending_clusters": nb, "gpu_ms_per_step": gpu_ms / K},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": kern,
+                     "frac": round(achieved / HBM_PEAK_GBS, 5),
+                     "traffic": read_traffic(os.path.join(ROOT, "profiles", "k_xpbd_grid256_traffic.txt"),
+                                             "hbm_bytes_per_launch_" + kern), "kernel": kern,
                      "mean_launch_us": round(ms * 1e3, 3), "algorithmic_bytes_per_launch": round(per, 1),
                      "stretch_launch_us": round(ms_s * 1e3, 3), "bending_launch_us": round(ms_b * 1e3, 3),
                      "bytes_per_spring": XPBD_SPRING_BYTES, "bytes_per_bending_pair": XPBD_BENDING_BYTES},

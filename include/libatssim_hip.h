@@ -175,9 +175,10 @@ int lats_vbd_get_regions(lats_vbd* ctx, uint32_t* regions);
  * the number of vertex tiles, one block each (0: min(256, V / 128)). */
 int lats_vbd_set_persistent_blocks(lats_vbd* ctx, uint32_t blocks);
 int lats_vbd_get_strategy(lats_vbd* ctx, int* strategy, uint32_t* persistent_blocks);
-/* Autotuner: times `frames` frames with the graph strategy, the
- * persistent strategy at several grid sizes and the flow strategy at
- * several tile counts, for 1/4/8/16 lanes per vertex, keeps the fastest for later
+/* Autotuner: times `frames` frames with the graph strategy (1 or 8 cloth
+ * bands), the persistent strategy at several grid sizes and the flow
+ * strategy at several tile counts, for 1/4/8/16 lanes per vertex, in two
+ * passes (a candidate's best time counts), keeps the fastest for later
  * lats_vbd_physics_step / lats_vbd_run_frames calls, and restores the state. */
 int lats_vbd_autotune(lats_vbd* ctx, uint32_t frames, int* chosen_strategy, uint32_t* chosen_blocks);
 /* Include the bending term in the recorded energy (the reference's GPU

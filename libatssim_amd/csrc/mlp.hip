@@ -584,7 +584,7 @@ struct lats_mlp {
     uint64_t bound_uid = 0;                    // uid of the bound context (pointers can be reused)
     float s0 = 0.01f;
     DBuf<uint16_t> X;
-    DBuf<float> Y, Y2, T, xpred, corr1, corr2;
+    DBuf<float> Y, Y2, T, xpred, xhalf, corr1, corr2;
     uint32_t smooth_passes = 0;                // 1-ring smoothing passes of the prediction
     int half_step = 0;                         // retry at half the prediction before falling back
     DBuf<uint32_t> colour;
@@ -736,12 +736,10 @@ int warm_frame(lats_mlp* m, lats_vbd* c, lats_vbd* ref, float target, uint32_t m
         HIPCHK(hipGetLastError());
     }
     const bool checks = target > 0.f;
-    if (use_model || checks) {
-        rc = lats_vbd_energy_now(c, &e_pred);
-        if (rc) return rc;
-    }
-    e_warm = e_pred;
     if (use_model) {
+        // learned guess in x (and half of it in xhalf), then the energies of
+        // the inertial guess, the learned guess and its half with one host
+        // round trip
         enqueue_forward(m, m->X.p, RowMap{}, V, m->Y.p, nullptr, nullptr, c->stream);
         float* y = m->Y.p;
         for (uint32_t p = 0; p < m->smooth_passes; p++) {
@@ -752,22 +750,28 @@ int warm_frame(lats_mlp* m, lats_vbd* c, lats_vbd* ref, float target, uint32_t m
         }
         hipLaunchKernelGGL(k_mlp_apply_scaled, dim3(nblocks(V)), dim3(kThreads), 0, c->stream, c->x.p, m->xpred.p, y,
                            V, m->s0, c->meta.p);
-        rc = lats_vbd_energy_now(c, &e_warm);
-        if (rc) return rc;
-        kept = e_warm <= e_pred;
-        if (!kept && m->half_step) {
-            hipLaunchKernelGGL(k_mlp_apply_scaled, dim3(nblocks(V)), dim3(kThreads), 0, c->stream, c->x.p,
+        if (m->half_step)
+            hipLaunchKernelGGL(k_mlp_apply_scaled, dim3(nblocks(V)), dim3(kThreads), 0, c->stream, m->xhalf.p,
                                m->xpred.p, y, V, 0.5f * m->s0, c->meta.p);
-            float e_half = 0.f;
-            rc = lats_vbd_energy_now(c, &e_half);
-            if (rc) return rc;
-            if (e_half <= e_pred) {
-                kept = true;
-                e_warm = e_half;
-            }
+        HIPCHK(hipGetLastError());
+        const float* xs[3] = {m->xpred.p, c->x.p, m->xhalf.p};
+        float en[3] = {0.f, 0.f, 0.f};
+        rc = lats::energies_of(c, xs, m->half_step ? 3u : 2u, en);
+        if (rc) return rc;
+        e_pred = en[0];
+        e_warm = en[1];
+        kept = e_warm <= e_pred;
+        if (!kept && m->half_step && en[2] <= e_pred) {
+            kept = true;
+            e_warm = en[2];
+            HIPCHK(hipMemcpyAsync(c->x.p, m->xhalf.p, (size_t)3 * V * 4, hipMemcpyDeviceToDevice, c->stream));
         }
         if (!kept)
             HIPCHK(hipMemcpyAsync(c->x.p, m->xpred.p, (size_t)3 * V * 4, hipMemcpyDeviceToDevice, c->stream));
+    } else if (checks) {
+        rc = lats_vbd_energy_now(c, &e_pred);
+        if (rc) return rc;
+        e_warm = e_pred;
     }
     float e = kept ? e_warm : e_pred;
     uint32_t sweeps = 0;
@@ -958,6 +962,7 @@ int lats_mlp_bind(lats_mlp* m, lats_vbd* c) {
     HIPCHK(m->Y2.alloc((size_t)V * 4));
     HIPCHK(m->T.alloc((size_t)V * 4));
     HIPCHK(m->xpred.alloc((size_t)3 * V));
+    HIPCHK(m->xhalf.alloc((size_t)3 * V));
     HIPCHK(m->corr1.upload(std::vector<float>((size_t)3 * V, 0.f)));
     HIPCHK(m->corr2.upload(std::vector<float>((size_t)3 * V, 0.f)));
     std::vector<uint32_t> col(V, 0);

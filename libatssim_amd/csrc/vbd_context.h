@@ -167,3 +167,10 @@ struct lats_vbd {
     }
 };
 
+namespace lats {
+// Energies of n position buffers (slot order) of context c, with the frame
+// start, velocities and contacts of its running frame: all n evaluated on
+// c's stream after the work already queued there, one host round trip.
+// out[k] = energy of xs[k].  Defined in vbd_device.hip.
+int energies_of(lats_vbd* c, const float* const* xs, uint32_t n, float* out);
+}  // namespace lats

@@ -1141,6 +1141,21 @@ int set_frame_params(lats_vbd* c, const float* forces_dev, uint32_t frame, const
 
 }  // namespace
 
+namespace lats {
+int energies_of(lats_vbd* c, const float* const* xs, uint32_t n, float* out) {
+    if (!c || !xs || !out || n == 0) return fail(LATS_ERR_INVALID_ARGUMENT, "null argument");
+    int rc = ensure_energy_rows(c, n);
+    if (rc) return rc;
+    for (uint32_t k = 0; k < n; k++) enqueue_energy(c, xs[k], k);
+    hipLaunchKernelGGL(k_energy_finalize, dim3(n), dim3(kBlock), 0, c->stream, c->partial.p, c->energy_nblk, 0u,
+                       c->energy.p);
+    HIPCHK(hipMemcpyAsync(out, c->energy.p, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->energy_rows = 0;
+    return LATS_OK;
+}
+}  // namespace lats
+
 // ===========================================================================
 // C ABI
 // ===========================================================================
@@ -1449,23 +1464,35 @@ int lats_vbd_autotune(lats_vbd* c, uint32_t frames, int* chosen, uint32_t* chose
             c->drop_graphs();
         }
     };
-    for (const Cand& k : cands) {
-        c->strategy = k.strategy;
-        c->persist_blocks = k.blocks;
-        use(k.lanes, k.regions);
-        int rc = lats_vbd_run_frames(c, 1, nullptr);   // warm-up / capture
-        if (!rc) rc = lats_vbd_run_frames(c, frames, nullptr);
-        if (rc) {
-            c->strategy = keep_strategy;
-            c->persist_blocks = keep_blocks;
-            use(keep_lanes, keep_regions);
-            restore();
-            return rc;
+    // Two passes over the candidates, each candidate's best time counts:
+    // the first timings after a context is created can be several per cent
+    // off (clocks and caches settling), enough to swap candidates that close.
+    std::vector<float> best_ms(cands.size(), 3.4e38f);
+    for (int pass = 0; pass < 2; pass++) {
+        for (size_t i = 0; i < cands.size(); i++) {
+            const Cand& k = cands[i];
+            c->strategy = k.strategy;
+            c->persist_blocks = k.blocks;
+            use(k.lanes, k.regions);
+            int rc = lats_vbd_run_frames(c, 1, nullptr);   // warm-up / capture
+            if (!rc) rc = lats_vbd_run_frames(c, frames, nullptr);
+            if (rc) {
+                c->strategy = keep_strategy;
+                c->persist_blocks = keep_blocks;
+                use(keep_lanes, keep_regions);
+                restore();
+                return rc;
+            }
+            best_ms[i] = std::min(best_ms[i], c->last_ms);
+            rc = restore();
+            if (rc) return rc;
         }
-        if (c->last_ms < best) { best = c->last_ms; win = k; }
-        rc = restore();
-        if (rc) return rc;
     }
+    for (size_t i = 0; i < cands.size(); i++)
+        if (best_ms[i] < best) {
+            best = best_ms[i];
+            win = cands[i];
+        }
     c->strategy = win.strategy;
     c->persist_blocks = win.blocks;
     use(win.lanes, win.regions);

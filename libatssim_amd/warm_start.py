@@ -315,10 +315,13 @@ def _train_chunks(mlp, steps, batch_rows, lr, seed, chunk=250):
 
 
 def evaluate(mlp, cloth, params, wind, frames, first_check=FIRST_CHECK, check_every=CHECK_EVERY, use_model=True,
-             lanes=8, max_sweeps=None, setup=None):
+             lanes=8, max_sweeps=None, setup=None, regions=None):
     """Frames of `cloth` solved from the learned guess until each frame's
     energy reaches the plain solver's energy after its full iteration count
     from the same frame start (computed on a second context).
+
+    lanes / regions: colour-pass configuration of the warm-started context
+    (results are identical for every choice; regions None keeps the default).
 
     Returns per-frame dicts (sweeps, kept, energies, target, wall_ms)."""
     import time
@@ -330,6 +333,9 @@ def evaluate(mlp, cloth, params, wind, frames, first_check=FIRST_CHECK, check_ev
     ref = GpuVBD(cloth, params, _lib.LATS_STRATEGY_GRAPH)
     run.set_lanes_per_vertex(lanes)
     ref.set_lanes_per_vertex(lanes)
+    if regions:
+        run.set_regions(regions)
+        ref.set_regions(regions)
     if setup:
         setup(run)
         setup(ref)
