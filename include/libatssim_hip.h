@@ -181,6 +181,10 @@ int lats_vbd_get_strategy(lats_vbd* ctx, int* strategy, uint32_t* persistent_blo
  * passes (a candidate's best time counts), keeps the fastest for later
  * lats_vbd_physics_step / lats_vbd_run_frames calls, and restores the state. */
 int lats_vbd_autotune(lats_vbd* ctx, uint32_t frames, int* chosen_strategy, uint32_t* chosen_blocks);
+/* What the last lats_vbd_autotune measured: *count candidates; out (5 floats
+ * per candidate, min(cap, count) copied) = {strategy, persistent blocks or
+ * flow tiles, lanes per vertex, bands, best milliseconds for `frames` frames}. */
+int lats_vbd_autotune_results(lats_vbd* ctx, float* out, uint32_t cap, uint32_t* count);
 /* Include the bending term in the recorded energy (the reference's GPU
  * energy does, example3.metal:369-395; its CPU energy drops it,
  * example2_async_iter.cpp:832-846).  Default 1. */

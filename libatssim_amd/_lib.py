@@ -205,6 +205,7 @@ SIGNATURES = {
     "lats_mlp_record_guided_frame": (_I, [_VP, _VP, _VP, _U32, _U32, _U32, _I, _VP]),
     "lats_vbd_time_xpbd_kernels": (_I, [_VP, _U32, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
                                         ctypes.POINTER(_U32), ctypes.POINTER(_U32)]),
+    "lats_vbd_autotune_results": (_I, [_VP, _VP, _U32, ctypes.POINTER(_U32)]),
     "lats_time_floor": (_I, [_I, _U32, _U32, ctypes.POINTER(ctypes.c_float)]),
 }
 

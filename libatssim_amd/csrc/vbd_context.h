@@ -90,6 +90,8 @@ struct lats_vbd {
     lats::DBuf<lats::ColourInfo> colours_d;
     lats::DBuf<uint32_t> barrier;
     uint32_t persist_blocks = 0;      // 0 = derive from the largest colour
+    // last autotune: per candidate {strategy, blocks, lanes, regions, best ms}
+    std::vector<float> tune_results;
     uint32_t lanes = 1;               // lanes per vertex (1, 4, 8, 16)
     uint32_t persist_max_lanes = 0;
     uint32_t persist_max_blocks = 0;  // co-residency limit (queried on first use)
@@ -120,6 +122,9 @@ struct lats_vbd {
     std::map<uint32_t, hipEvent_t> markers;
     uint32_t* host_flag = nullptr;    // pinned, mapped
     uint32_t* host_flag_d = nullptr;  // device alias
+    static constexpr uint32_t kHostEnergies = 16;
+    float* host_energy = nullptr;     // pinned, mapped, coherent: energies read by the host
+    float* host_energy_d = nullptr;   // device alias
 
     // every captured graph holds kernel choices and buffer pointers of the
     // moment it was captured: drop them whenever one of those changes
@@ -139,6 +144,7 @@ struct lats_vbd {
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         if (host_flag) (void)hipHostFree(host_flag);
+        if (host_energy) (void)hipHostFree(host_energy);
         if (stream) (void)hipStreamDestroy(stream);
     }
 

@@ -1144,13 +1144,16 @@ int set_frame_params(lats_vbd* c, const float* forces_dev, uint32_t frame, const
 namespace lats {
 int energies_of(lats_vbd* c, const float* const* xs, uint32_t n, float* out) {
     if (!c || !xs || !out || n == 0) return fail(LATS_ERR_INVALID_ARGUMENT, "null argument");
+    if (n > lats_vbd::kHostEnergies) return fail(LATS_ERR_INVALID_ARGUMENT, "too many position buffers");
     int rc = ensure_energy_rows(c, n);
     if (rc) return rc;
     for (uint32_t k = 0; k < n; k++) enqueue_energy(c, xs[k], k);
+    // the final sums go straight to pinned host memory: no copy command
     hipLaunchKernelGGL(k_energy_finalize, dim3(n), dim3(kBlock), 0, c->stream, c->partial.p, c->energy_nblk, 0u,
-                       c->energy.p);
-    HIPCHK(hipMemcpyAsync(out, c->energy.p, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+                       c->host_energy_d);
+    HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->stream));
+    for (uint32_t k = 0; k < n; k++) out[k] = c->host_energy[k];
     c->energy_rows = 0;
     return LATS_OK;
 }
@@ -1239,6 +1242,9 @@ int lats_vbd_create(const lats_mesh_desc* m, const lats_scene_params* prm, lats_
     UPCHK(hipHostMalloc((void**)&c->host_flag, 64, hipHostMallocMapped));
     *c->host_flag = 0;
     UPCHK(hipHostGetDevicePointer((void**)&c->host_flag_d, c->host_flag, 0));
+    UPCHK(hipHostMalloc((void**)&c->host_energy, lats_vbd::kHostEnergies * sizeof(float),
+                        hipHostMallocMapped | hipHostMallocCoherent));
+    UPCHK(hipHostGetDevicePointer((void**)&c->host_energy_d, c->host_energy, 0));
 #undef UPCHK
     c->energy_nblk = nblocks((uint64_t)V + E + B);
     *out = c;
@@ -1488,16 +1494,29 @@ int lats_vbd_autotune(lats_vbd* c, uint32_t frames, int* chosen, uint32_t* chose
             if (rc) return rc;
         }
     }
-    for (size_t i = 0; i < cands.size(); i++)
+    c->tune_results.clear();
+    for (size_t i = 0; i < cands.size(); i++) {
         if (best_ms[i] < best) {
             best = best_ms[i];
             win = cands[i];
         }
+        const Cand& k = cands[i];
+        for (float v : {(float)k.strategy, (float)k.blocks, (float)k.lanes, (float)k.regions, best_ms[i]})
+            c->tune_results.push_back(v);
+    }
     c->strategy = win.strategy;
     c->persist_blocks = win.blocks;
     use(win.lanes, win.regions);
     if (chosen) *chosen = win.strategy;
     if (chosen_blocks) *chosen_blocks = win.blocks;
+    return LATS_OK;
+}
+
+int lats_vbd_autotune_results(lats_vbd* c, float* out, uint32_t cap, uint32_t* count) {
+    if (!c || !count) return fail(LATS_ERR_INVALID_ARGUMENT, "null argument");
+    const uint32_t n = (uint32_t)(c->tune_results.size() / 5);
+    *count = n;
+    if (out) std::memcpy(out, c->tune_results.data(), std::min(cap, n) * 5 * sizeof(float));
     return LATS_OK;
 }
 
@@ -1930,15 +1949,8 @@ int lats_vbd_frame_end(lats_vbd* c) {
 
 int lats_vbd_energy_now(lats_vbd* c, float* e) {
     if (!c || !e) return fail(LATS_ERR_INVALID_ARGUMENT, "null argument");
-    int rc = ensure_energy_rows(c, 1);
-    if (rc) return rc;
-    enqueue_energy(c, c->x.p, 0);
-    hipLaunchKernelGGL(k_energy_finalize, dim3(1), dim3(kBlock), 0, c->stream, c->partial.p, c->energy_nblk, 0u,
-                       c->energy.p);
-    HIPCHK(hipMemcpyAsync(e, c->energy.p, sizeof(float), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
-    c->energy_rows = 0;
-    return LATS_OK;
+    const float* xs[1] = {c->x.p};
+    return lats::energies_of(c, xs, 1, e);
 }
 
 int lats_vbd_time_colour_kernels(lats_vbd* c, uint32_t iters, float* mean_ms, uint32_t* launches,

@@ -269,6 +269,15 @@ class GpuVBD:
         check(lib().lats_vbd_autotune(self._h, int(frames), ctypes.byref(st), ctypes.byref(nb)))
         return st.value, nb.value
 
+    def autotune_results(self):
+        """The last autotune's candidates: float array [n, 5] of (strategy,
+        blocks or tiles, lanes, bands, best ms for the timed frames)."""
+        n = ctypes.c_uint32()
+        check(lib().lats_vbd_autotune_results(self._h, None, 0, ctypes.byref(n)))
+        out = np.zeros((n.value, 5), np.float32)
+        check(lib().lats_vbd_autotune_results(self._h, _p(out), n.value, ctypes.byref(n)))
+        return out
+
     def set_obstacle(self, x, faces, thickness, stiffness, radius, cell):
         """Static triangle obstacle with penalty contact (see lats_vbd_set_obstacle)."""
         self._obs = (np.ascontiguousarray(x, np.float32), np.ascontiguousarray(faces, np.int32))
