@@ -43,6 +43,19 @@ def read_traffic(path, key="hbm_bytes_per_launch"):
     return None
 
 
+def read_colour_traffic(path, lanes, regions):
+    """HBM bytes per colour launch from a committed PMC summary, if it was
+    measured at this lanes / bands configuration (its `config_lanes_regions`
+    line), else None."""
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("config_lanes_regions:"):
+            if [int(v) for v in line.split(":", 1)[1].split()] != [lanes, regions]:
+                return None
+    return read_traffic(path)
+
+
 def algorithmic_bytes_per_sweep(nv, ne, nb):
     """Compulsory bytes of one sweep over all colours (compact layout)."""
     return 53 * nv + 20 * ne + 72 * nb + 8
@@ -187,7 +200,8 @@ def main():
         mean_ms, launches, kvu = g.time_colour_kernels(20)
         bytes_per_launch = b_sweep / t.num_clusters
     achieved = bytes_per_launch / (mean_ms * 1e-3) / 1e9
-    traffic = read_traffic(TRAFFIC_FILE) if kernel == "k_vbd_colour" else None
+    traffic = (read_colour_traffic(TRAFFIC_FILE, g.lanes_per_vertex(), g.regions())
+               if kernel == "k_vbd_colour" else None)
     roofline = {
         "bound": "hbm",
         "achieved": round(achieved, 2),
@@ -355,8 +369,10 @@ def bench_sphere(args):
                    "lanes_per_vertex": g.lanes_per_vertex(), "regions": g.regions(),
                    "contact_vertices": int((counts > 0).sum())},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": "k_vbd_colour",
-                     "mean_launch_us": round(mean_ms * 1e3, 3),
+                     "frac": round(achieved / HBM_PEAK_GBS, 5),
+                     "traffic": read_colour_traffic(os.path.join(ROOT, "profiles", "k_vbd_colour_sphere1024_traffic.txt"),
+                                                    g.lanes_per_vertex(), g.regions()),
+                     "kernel": "k_vbd_colour", "mean_launch_us": round(mean_ms * 1e3, 3),
                      "algorithmic_bytes_per_launch": round(b_sweep / t.num_clusters, 1)},
     }
     if not args.no_warm_start:
