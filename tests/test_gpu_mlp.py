@@ -287,3 +287,30 @@ def test_trained_warm_start_needs_fewer_sweeps(gpu_available):
     assert all(r["e_final"] <= r["target"] for r in learned)
     assert all(r["e_final"] <= r["target"] for r in plain)
     assert sl.mean() <= 0.6 * sp.mean(), (sl.mean(), sp.mean())
+
+
+def test_guided_recording_is_deterministic(gpu_available):
+    """Regression: the guided rows' targets come from a second context whose
+    frame-start state is copied in; that copy must be ordered with the second
+    context's stream.  Two recordings with the same network give the same
+    sweeps, energies and rows."""
+    from libatssim_amd.warm_start import record_guided, record_scenarios, _train_chunks
+    _case("guided recording determinism (128x128, 8 frames, twice)")
+    params = Params(constraint_iter_count=100)
+    scen = [configs.warm_c3_scenario(0, n=128, frames=8)]
+    m = WarmStartMLP(2024)
+    record_scenarios(m, scen, params, 8)
+    m.fit_feature_scale()
+    _train_chunks(m, 100, 8192, 1e-3, 0)
+    n0 = m.dataset_rows()
+    runs = []
+    for _ in range(2):
+        start = m.dataset_rows()
+        stats = record_guided(m, scen, params, 8)
+        X, T = m.dataset(start, m.dataset_rows() - start)
+        runs.append((stats, X, T))
+    (s0, X0, T0), (s1, X1, T1) = runs
+    assert m.dataset_rows() == n0 + 2 * 8 * 128 * 128
+    assert [(r["sweeps"], r["kept"], r["e_final"], r["target"]) for r in s0] == \
+           [(r["sweeps"], r["kept"], r["e_final"], r["target"]) for r in s1]
+    assert np.array_equal(X0, X1) and np.array_equal(T0, T1)
