@@ -585,6 +585,7 @@ struct lats_mlp {
     float s0 = 0.01f;
     DBuf<uint16_t> X;
     DBuf<float> Y, Y2, T, xpred, xhalf, corr1, corr2;
+    hipEvent_t ev_guess = nullptr;             // the guesses' energies are in host memory
     uint32_t smooth_passes = 0;                // 1-ring smoothing passes of the prediction
     int half_step = 0;                         // retry at half the prediction before falling back
     DBuf<uint32_t> colour;
@@ -597,6 +598,7 @@ struct lats_mlp {
     DBuf<uint16_t> tH;
     uint32_t t_cap = 0;
     ~lats_mlp() {
+        if (ev_guess) (void)hipEventDestroy(ev_guess);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -727,6 +729,7 @@ int warm_frame(lats_mlp* m, lats_vbd* c, lats_vbd* ref, float target, uint32_t m
     HIPCHK(hipMemcpyAsync(m->xpred.p, c->x.p, (size_t)3 * V * 4, hipMemcpyDeviceToDevice, c->stream));
     float e_pred = 0.f, e_warm = 0.f;
     bool kept = false;
+    uint32_t spec = 0;   // sweeps already run from the kept learned guess
     if (use_model || ref) enqueue_features(m, c);
     if (ref) {
         HIPCHK(hipMemcpyAsync(m->dsX.p + m->ds_rows * kInP, m->X.p, (size_t)V * kInP * 2, hipMemcpyDeviceToDevice,
@@ -755,12 +758,23 @@ int warm_frame(lats_mlp* m, lats_vbd* c, lats_vbd* ref, float target, uint32_t m
                                m->xpred.p, y, V, 0.5f * m->s0, c->meta.p);
         HIPCHK(hipGetLastError());
         const float* xs[3] = {m->xpred.p, c->x.p, m->xhalf.p};
-        float en[3] = {0.f, 0.f, 0.f};
-        rc = lats::energies_of(c, xs, m->half_step ? 3u : 2u, en);
+        const uint32_t ne = m->half_step ? 3u : 2u;
+        rc = lats::energies_enqueue(c, xs, ne);
         if (rc) return rc;
+        HIPCHK(hipEventRecord(m->ev_guess, c->stream));
+        // While the host waits for the energies, the first sweeps from the
+        // full learned guess are already queued behind them (the guess is
+        // kept almost always); a rejected guess discards them below.
+        spec = std::min(std::max(first_check, 1u), max_sweeps);
+        rc = lats_vbd_iterate(c, spec);
+        if (rc) return rc;
+        HIPCHK(hipEventSynchronize(m->ev_guess));
+        float en[3] = {0.f, 0.f, 0.f};
+        for (uint32_t k = 0; k < ne; k++) en[k] = c->host_energy[k];
         e_pred = en[0];
         e_warm = en[1];
         kept = e_warm <= e_pred;
+        if (!kept) spec = 0;   // restart below from the half-length or the inertial guess
         if (!kept && m->half_step && en[2] <= e_pred) {
             kept = true;
             e_warm = en[2];
@@ -774,7 +788,11 @@ int warm_frame(lats_mlp* m, lats_vbd* c, lats_vbd* ref, float target, uint32_t m
         e_warm = e_pred;
     }
     float e = kept ? e_warm : e_pred;
-    uint32_t sweeps = 0;
+    uint32_t sweeps = spec;
+    if (spec && checks) {
+        rc = lats_vbd_energy_now(c, &e);
+        if (rc) return rc;
+    }
     // From the learned guess the first check comes after first_check sweeps
     // (a few sweeps always smooth the guess's high-frequency error), then
     // every check_every.  From the inertial guess the frame is the plain
@@ -844,6 +862,7 @@ int lats_mlp_create(uint64_t seed, lats_mlp** out) {
     };
     hipError_t e;
     if ((e = hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking)) != hipSuccess) return bad(e);
+    if ((e = hipEventCreateWithFlags(&m->ev_guess, hipEventDisableTiming)) != hipSuccess) return bad(e);
     if ((e = m->P.upload(p)) != hipSuccess) return bad(e);
     if ((e = m->Mom.upload(std::vector<float>(kNParam, 0.f))) != hipSuccess) return bad(e);
     if ((e = m->Vel.upload(std::vector<float>(kNParam, 0.f))) != hipSuccess) return bad(e);

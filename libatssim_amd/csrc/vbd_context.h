@@ -179,4 +179,7 @@ namespace lats {
 // c's stream after the work already queued there, one host round trip.
 // out[k] = energy of xs[k].  Defined in vbd_device.hip.
 int energies_of(lats_vbd* c, const float* const* xs, uint32_t n, float* out);
+// The same without waiting: the energies land in c->host_energy[0..n) once
+// the work queued so far on c's stream has completed (n <= kHostEnergies).
+int energies_enqueue(lats_vbd* c, const float* const* xs, uint32_t n);
 }  // namespace lats

@@ -1142,8 +1142,8 @@ int set_frame_params(lats_vbd* c, const float* forces_dev, uint32_t frame, const
 }  // namespace
 
 namespace lats {
-int energies_of(lats_vbd* c, const float* const* xs, uint32_t n, float* out) {
-    if (!c || !xs || !out || n == 0) return fail(LATS_ERR_INVALID_ARGUMENT, "null argument");
+int energies_enqueue(lats_vbd* c, const float* const* xs, uint32_t n) {
+    if (!c || !xs || n == 0) return fail(LATS_ERR_INVALID_ARGUMENT, "null argument");
     if (n > lats_vbd::kHostEnergies) return fail(LATS_ERR_INVALID_ARGUMENT, "too many position buffers");
     int rc = ensure_energy_rows(c, n);
     if (rc) return rc;
@@ -1152,9 +1152,16 @@ int energies_of(lats_vbd* c, const float* const* xs, uint32_t n, float* out) {
     hipLaunchKernelGGL(k_energy_finalize, dim3(n), dim3(kBlock), 0, c->stream, c->partial.p, c->energy_nblk, 0u,
                        c->host_energy_d);
     HIPCHK(hipGetLastError());
+    c->energy_rows = 0;
+    return LATS_OK;
+}
+
+int energies_of(lats_vbd* c, const float* const* xs, uint32_t n, float* out) {
+    if (!out) return fail(LATS_ERR_INVALID_ARGUMENT, "null argument");
+    int rc = energies_enqueue(c, xs, n);
+    if (rc) return rc;
     HIPCHK(hipStreamSynchronize(c->stream));
     for (uint32_t k = 0; k < n; k++) out[k] = c->host_energy[k];
-    c->energy_rows = 0;
     return LATS_OK;
 }
 }  // namespace lats
