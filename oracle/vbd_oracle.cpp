@@ -598,7 +598,7 @@ template <typename T> static void physics_step(Sim<T>& s, bool record) {
 }
 
 // ---------------------------------------------------------------------------
-// XPBD (SURVEY.md §8 f4).  example2_async_iter.cpp:453-697 (constraint
+// XPBD.  example2_async_iter.cpp:453-697 (constraint
 // colouring, merged arrays), 698-710 (reset), 884-932 (per-cluster solves),
 // 1017-1081 (frame), 1510-1534 (iteration); xpbd_constraints.h:984-1059
 // (solve_stretch_mass_spring_template), 1062-1126 (core_bending_quadratic),

@@ -1,4 +1,4 @@
-"""Pins the oracle's XPBD path (oracle/vbd_oracle.cpp, SURVEY.md §8 f4)
+"""Pins the oracle's XPBD path (oracle/vbd_oracle.cpp)
 before it checks the GPU solver.  The reference publishes no XPBD run, so:
 
   * constraint colouring: a set-based numpy restatement of the greedy
