@@ -137,7 +137,7 @@ def main():
     ap.add_argument("--warm-record-every", type=int, default=1, help="record every k-th training frame")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build runs the VBD hot path on a single MI355X "
+        print("bench.py measures one GPU: the VBD hot path runs on a single MI355X "
               "(multi-GPU sharding is a design note only); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 

@@ -727,7 +727,7 @@ void Scheduler::launch(LaunchMode mode, const std::function<LaunchParam(const Ta
             if (hooks) hooks->gpu_finish();
             break;
         default:
-            std::fprintf(stderr, "Launcher: launch mode %d is not available in this build\n", (int)mode);
+            std::fprintf(stderr, "Launcher: launch mode %d is not available in this library\n", (int)mode);
     }
 }
 

@@ -8,7 +8,7 @@ Scheduler::print_schedule_to_graph_xpbd prints:
   ins       (proc, task, proc, task, end, start) per cross-processor input
   left      (proc, task, proc, task, end, start) per same-processor "left" link
 
-Run in the build container (the reference is not on the GPU box):
+Run where the reference checkout is present (tests read only the fixtures):
     python tests/golden/make_example2_schedules.py
 """
 import ast

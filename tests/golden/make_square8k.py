@@ -1,6 +1,6 @@
 """Writes the square8K fixtures from the reference checkout's data files.
 
-Run in the build container (the reference is not on the GPU box):
+Run where the reference checkout is present (tests read only the fixtures):
     python tests/golden/make_square8k.py
 
 Outputs (data only, no reference source):
