@@ -245,7 +245,7 @@ void Scheduler::scheduler_dag() {
 namespace {
 
 // Task classes of the asynchronous post-pass (launcher.cpp:2957-3030),
-// restricted to the function ids this build registers.
+// restricted to the function ids this library registers.
 bool is_constraint_fn(FunctionID f) {
     return f == id_xpbd_constraint_last_node || f == id_xpbd_constraint_stretch_mass_spring ||
            f == id_xpbd_constraint_bending || f == id_vbd_evaluate_inertia ||
