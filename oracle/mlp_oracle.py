@@ -1,7 +1,7 @@
 """FP64 restatement of the warm-start MLP -- TEST INFRASTRUCTURE ONLY.
 
 Only tests/ (and bench.py's CPU leg, if ever) may import this module: it is
-the checker of the library's lats_mlp_* forward and backward kernels, never a
+the test reference for the library's lats_mlp_* forward and backward kernels, never a
 product path.  The network has no counterpart in the reference (whose frames
 start from Core::predict_position, src/Solver/xpbd_constraints.h:321-384), so
 this file restates the library's own definition: 48 features -> 128 -> 128 ->

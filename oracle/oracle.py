@@ -1,6 +1,6 @@
 """ctypes front-end of the CPU oracle -- TEST INFRASTRUCTURE ONLY.
 
-Loaded by tests/, by the smoke() entry point (as the checker) and by
+Loaded by tests/, by the smoke() entry point (to check its result) and by
 bench.py's cpu_baseline leg.  The product path never imports this module.
 
 The oracle is a C++ restatement of the reference's VBD cloth path

@@ -1,7 +1,7 @@
 // vbd_oracle.cpp -- TEST INFRASTRUCTURE ONLY.
 //
 // CPU restatement of the reference's Vertex Block Descent (VBD) cloth path,
-// used as the checker for the HIP product path.  Only tests/, the smoke()
+// used to check the HIP product path.  Only tests/, the smoke()
 // entry and bench.py's cpu_baseline leg may load this library.  Nothing in
 // the product links it or falls back to it.
 //

@@ -1,5 +1,5 @@
 """Learned warm start: the MFMA forward / backward kernels against the FP64
-checker (oracle/mlp_oracle.py), Adam and the batch sampler against numpy,
+restatement (oracle/mlp_oracle.py), Adam and the batch sampler against numpy,
 the feature kernel against a numpy restatement, the residual-gated fallback
 against the plain solver (bit for bit), and what a trained network gains."""
 import numpy as np

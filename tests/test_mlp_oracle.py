@@ -1,4 +1,4 @@
-"""Pins the FP64 MLP checker (oracle/mlp_oracle.py) on the CPU: BF16
+"""Pins the FP64 MLP restatement (oracle/mlp_oracle.py) on the CPU: BF16
 rounding on known values, the layer wiring on a closed form, and the
 analytic gradient on central finite differences."""
 import numpy as np

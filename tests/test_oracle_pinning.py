@@ -1,4 +1,4 @@
-"""Pins the CPU oracle before it is trusted as the checker.
+"""Pins the CPU oracle before the GPU tests compare against it.
 
 Each function of oracle/vbd_oracle.cpp is checked against something other
 than its own formula:
