@@ -94,7 +94,7 @@ def _oracle_vbd_rate(cloth, params, wind, threads, min_seconds, max_frames):
     return vu / el, frames, el
 
 
-def cpu_baseline(cloth, params, wind, min_seconds=10.0, max_frames=40):
+def cpu_baseline(cloth, params, wind, min_seconds=10.0, max_frames=120):
     """The oracle on this host's cores (at most 16 threads), and on one thread
     over a shorter sample."""
     from oracle import oracle as O
