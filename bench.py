@@ -295,7 +295,14 @@ def bench_warm_start(args, cloth, params, wind, plain_ms, lanes=8, regions=1):
     scen = [configs.warm_c3_scenario(k, n=256, frames=args.warm_frames) for k in range(args.warm_scenarios)]
     m = WarmStartMLP(2024)
     steps = 2000
-    train_recipe(m, scen, params, args.warm_frames, plain_steps=steps, guided_rounds=3, guided_steps=steps,
+    # rounds of rows from warm-started trajectories: 3 -> 13.7 sweeps per
+    # frame, 5 -> 11.6, 8 -> 11.1 (more rounds, states closer to the ones the
+    # warm-started solver visits)
+    # (the dataset holds every pass; at most ~400 M rows = 58 GB, so larger
+    # training sets get fewer rounds, never fewer than 3)
+    rows_per_pass = args.warm_scenarios * (-(-args.warm_frames // args.warm_record_every)) * cloth.num_verts
+    rounds = max(3, min(8, 400_000_000 // rows_per_pass - 1))
+    train_recipe(m, scen, params, args.warm_frames, plain_steps=steps, guided_rounds=rounds, guided_steps=steps,
                  record_every=args.warm_record_every)
     train_s = time.perf_counter() - t0
     full_wind = configs.grid_c1(frames=60)[3]
@@ -317,7 +324,8 @@ def bench_warm_start(args, cloth, params, wind, plain_ms, lanes=8, regions=1):
         "mlp_forward": _mlp_roofline(cloth.num_verts, fwd_ms),
         "mlp_train_step_ms": round(step_ms, 4),
         "training": {"scenarios": args.warm_scenarios, "frames": args.warm_frames,
-                     "record_every": args.warm_record_every, "guided_rounds": 3, "adam_steps": 4 * steps,
+                     "record_every": args.warm_record_every, "guided_rounds": rounds,
+                     "adam_steps": (1 + rounds) * steps,
                      "batch_rows": 16384, "rows": m.dataset_rows(), "seconds": round(train_s, 1)},
     }
 
