@@ -584,7 +584,9 @@ struct lats_mlp {
     uint64_t bound_uid = 0;                    // uid of the bound context (pointers can be reused)
     float s0 = 0.01f;
     DBuf<uint16_t> X;
-    DBuf<float> Y, Y2, T, xpred, xhalf, corr1, corr2;
+    DBuf<float> Y, Y2, T, xpred, xfull, xhalf, corr1, corr2;
+    hipStream_t side = nullptr;                // guess energies, beside the first sweeps
+    hipEvent_t ev_applied = nullptr;           // the guesses are in place
     hipEvent_t ev_guess = nullptr;             // the guesses' energies are in host memory
     uint32_t smooth_passes = 0;                // 1-ring smoothing passes of the prediction
     int half_step = 0;                         // retry at half the prediction before falling back
@@ -598,7 +600,10 @@ struct lats_mlp {
     DBuf<uint16_t> tH;
     uint32_t t_cap = 0;
     ~lats_mlp() {
+        if (side) (void)hipStreamSynchronize(side);
         if (ev_guess) (void)hipEventDestroy(ev_guess);
+        if (ev_applied) (void)hipEventDestroy(ev_applied);
+        if (side) (void)hipStreamDestroy(side);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -751,20 +756,28 @@ int warm_frame(lats_mlp* m, lats_vbd* c, lats_vbd* ref, float target, uint32_t m
                                c->colours_d.p, m->colour.p, y, o, V);
             y = o;
         }
+        // the learned guess goes to x (the sweeps start from it) and to
+        // xfull (its energy is evaluated beside the sweeps)
         hipLaunchKernelGGL(k_mlp_apply_scaled, dim3(nblocks(V)), dim3(kThreads), 0, c->stream, c->x.p, m->xpred.p, y,
                            V, m->s0, c->meta.p);
+        hipLaunchKernelGGL(k_mlp_apply_scaled, dim3(nblocks(V)), dim3(kThreads), 0, c->stream, m->xfull.p,
+                           m->xpred.p, y, V, m->s0, c->meta.p);
         if (m->half_step)
             hipLaunchKernelGGL(k_mlp_apply_scaled, dim3(nblocks(V)), dim3(kThreads), 0, c->stream, m->xhalf.p,
                                m->xpred.p, y, V, 0.5f * m->s0, c->meta.p);
         HIPCHK(hipGetLastError());
-        const float* xs[3] = {m->xpred.p, c->x.p, m->xhalf.p};
+        // The three energies run on the side stream while the first sweeps
+        // from the learned guess run on the context's stream (the guess is
+        // kept almost always; a rejected guess discards them below).  The
+        // energies read xpred, xfull and xhalf and the frame start, none of
+        // which the sweeps write.
+        HIPCHK(hipEventRecord(m->ev_applied, c->stream));
+        HIPCHK(hipStreamWaitEvent(m->side, m->ev_applied, 0));
+        const float* xs[3] = {m->xpred.p, m->xfull.p, m->xhalf.p};
         const uint32_t ne = m->half_step ? 3u : 2u;
-        rc = lats::energies_enqueue(c, xs, ne);
+        rc = lats::energies_enqueue(c, xs, ne, m->side);
         if (rc) return rc;
-        HIPCHK(hipEventRecord(m->ev_guess, c->stream));
-        // While the host waits for the energies, the first sweeps from the
-        // full learned guess are already queued behind them (the guess is
-        // kept almost always); a rejected guess discards them below.
+        HIPCHK(hipEventRecord(m->ev_guess, m->side));
         spec = std::min(std::max(first_check, 1u), max_sweeps);
         rc = lats_vbd_iterate(c, spec);
         if (rc) return rc;
@@ -863,6 +876,8 @@ int lats_mlp_create(uint64_t seed, lats_mlp** out) {
     hipError_t e;
     if ((e = hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking)) != hipSuccess) return bad(e);
     if ((e = hipEventCreateWithFlags(&m->ev_guess, hipEventDisableTiming)) != hipSuccess) return bad(e);
+    if ((e = hipEventCreateWithFlags(&m->ev_applied, hipEventDisableTiming)) != hipSuccess) return bad(e);
+    if ((e = hipStreamCreateWithFlags(&m->side, hipStreamNonBlocking)) != hipSuccess) return bad(e);
     if ((e = m->P.upload(p)) != hipSuccess) return bad(e);
     if ((e = m->Mom.upload(std::vector<float>(kNParam, 0.f))) != hipSuccess) return bad(e);
     if ((e = m->Vel.upload(std::vector<float>(kNParam, 0.f))) != hipSuccess) return bad(e);
@@ -982,6 +997,7 @@ int lats_mlp_bind(lats_mlp* m, lats_vbd* c) {
     HIPCHK(m->T.alloc((size_t)V * 4));
     HIPCHK(m->xpred.alloc((size_t)3 * V));
     HIPCHK(m->xhalf.alloc((size_t)3 * V));
+    HIPCHK(m->xfull.alloc((size_t)3 * V));
     HIPCHK(m->corr1.upload(std::vector<float>((size_t)3 * V, 0.f)));
     HIPCHK(m->corr2.upload(std::vector<float>((size_t)3 * V, 0.f)));
     std::vector<uint32_t> col(V, 0);

@@ -180,6 +180,8 @@ namespace lats {
 // out[k] = energy of xs[k].  Defined in vbd_device.hip.
 int energies_of(lats_vbd* c, const float* const* xs, uint32_t n, float* out);
 // The same without waiting: the energies land in c->host_energy[0..n) once
-// the work queued so far on c's stream has completed (n <= kHostEnergies).
-int energies_enqueue(lats_vbd* c, const float* const* xs, uint32_t n);
+// the work queued on stream st (c's stream if null) has completed
+// (n <= kHostEnergies).  The energy scratch rows are c's: no other energy
+// evaluation of c may run until they have landed.
+int energies_enqueue(lats_vbd* c, const float* const* xs, uint32_t n, hipStream_t st = nullptr);
 }  // namespace lats

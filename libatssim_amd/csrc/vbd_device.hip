@@ -650,7 +650,7 @@ int ensure_energy_rows(lats_vbd* c, uint32_t rows) {
 }
 
 // Enqueue one energy evaluation of buffer xb into history row `row`.
-void enqueue_energy(lats_vbd* c, const float* xb, uint32_t row) {
+void enqueue_energy(lats_vbd* c, const float* xb, uint32_t row, hipStream_t st = nullptr) {
     EnergyArgs a;
     a.x = xb;
     a.x0 = c->x0.p;
@@ -673,7 +673,7 @@ void enqueue_energy(lats_vbd* c, const float* xb, uint32_t row) {
     a.ct_xs = c->ct_xs.p;
     a.ct_n = c->ct_n.p;
     a.thickness = c->thickness;
-    hipLaunchKernelGGL(k_energy_partial, dim3(c->energy_nblk), dim3(kBlock), 0, c->stream, a,
+    hipLaunchKernelGGL(k_energy_partial, dim3(c->energy_nblk), dim3(kBlock), 0, st ? st : c->stream, a,
                        c->partial.p + (size_t)row * c->energy_nblk);
 }
 
@@ -1142,14 +1142,15 @@ int set_frame_params(lats_vbd* c, const float* forces_dev, uint32_t frame, const
 }  // namespace
 
 namespace lats {
-int energies_enqueue(lats_vbd* c, const float* const* xs, uint32_t n) {
+int energies_enqueue(lats_vbd* c, const float* const* xs, uint32_t n, hipStream_t st) {
     if (!c || !xs || n == 0) return fail(LATS_ERR_INVALID_ARGUMENT, "null argument");
     if (n > lats_vbd::kHostEnergies) return fail(LATS_ERR_INVALID_ARGUMENT, "too many position buffers");
     int rc = ensure_energy_rows(c, n);
     if (rc) return rc;
-    for (uint32_t k = 0; k < n; k++) enqueue_energy(c, xs[k], k);
+    if (!st) st = c->stream;
+    for (uint32_t k = 0; k < n; k++) enqueue_energy(c, xs[k], k, st);
     // the final sums go straight to pinned host memory: no copy command
-    hipLaunchKernelGGL(k_energy_finalize, dim3(n), dim3(kBlock), 0, c->stream, c->partial.p, c->energy_nblk, 0u,
+    hipLaunchKernelGGL(k_energy_finalize, dim3(n), dim3(kBlock), 0, st, c->partial.p, c->energy_nblk, 0u,
                        c->host_energy_d);
     HIPCHK(hipGetLastError());
     c->energy_rows = 0;
