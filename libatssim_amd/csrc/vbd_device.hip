@@ -101,81 +101,167 @@ __device__ __forceinline__ uint32_t hash_coord_d(int x, int y, int z, uint32_t T
     return (uint32_t)(v % (long long)T);
 }
 
+// The hash cell of position p: on success [*f0, *f1) is its face range.
+__device__ __forceinline__ bool detect_cell(const DetectArgs& d, F3 p, uint32_t* f0, uint32_t* f1) {
+    const int ix = (int)floorf((p.x - d.ox) * d.inv_cell);
+    const int iy = (int)floorf((p.y - d.oy) * d.inv_cell);
+    const int iz = (int)floorf((p.z - d.oz) * d.inv_cell);
+    if (!(ix >= 0 && iy >= 0 && iz >= 0 && ix < d.dimx && iy < d.dimy && iz < d.dimz)) return false;
+    const uint32_t key = ((uint32_t)ix << 20) | ((uint32_t)iy << 10) | (uint32_t)iz;
+    uint32_t h = hash_coord_d(ix, iy, iz, d.table_size);
+    for (int j = 0; j < 32; j++) {
+        const uint32_t b = d.belongs[h];
+        if (b == key) {
+            *f0 = d.start[h];
+            *f1 = *f0 + d.count[h];
+            return true;
+        }
+        if (b == 0xffffffffu) return false;
+        h = (h + 1) % d.table_size;
+    }
+    return false;
+}
+
+// The kMaxContacts nearest candidates by (distance, face id): kd / kf / kb
+// hold n of them in that order.
+struct NearList {
+    uint32_t n;
+    float kd[kMaxContacts];
+    uint32_t kf[kMaxContacts];
+    float kb[kMaxContacts][3];
+};
+
+__device__ __forceinline__ bool near_before(float da, uint32_t fa, float db, uint32_t fb) {
+    return da < db || (da == db && fa < fb);
+}
+
+// Inserts (ed, ef, eb) into L if it is among the nearest; returns false if
+// it is not.  Static indices only, so the list stays in registers: slot q
+// takes the new entry, the entry of slot q - 1, or keeps its own, decided
+// from the old contents.
+__device__ __forceinline__ bool insert_near(NearList& L, float ed, uint32_t ef, float eb0, float eb1, float eb2) {
+    bool goes_before[kMaxContacts];
+#pragma unroll
+    for (int q = 0; q < kMaxContacts; q++)
+        goes_before[q] = (uint32_t)q >= L.n || near_before(ed, ef, L.kd[q], L.kf[q]);
+    if (!goes_before[kMaxContacts - 1]) return false;
+#pragma unroll
+    for (int q = kMaxContacts - 1; q >= 0; q--) {
+        if (!goes_before[q]) continue;
+        const bool shift = q > 0 && goes_before[q - 1];
+        L.kd[q] = shift ? L.kd[q - 1] : ed;
+        L.kf[q] = shift ? L.kf[q - 1] : ef;
+        L.kb[q][0] = shift ? L.kb[q - 1][0] : eb0;
+        L.kb[q][1] = shift ? L.kb[q - 1][1] : eb1;
+        L.kb[q][2] = shift ? L.kb[q - 1][2] : eb2;
+    }
+    if (L.n < (uint32_t)kMaxContacts) L.n++;
+    return true;
+}
+
+// Face fid against p: inserted if it passes the narrow phase within the
+// radius and is among the nearest.
+__device__ __forceinline__ void consider_face(const DetectArgs& d, F3 p, uint32_t fid, NearList& L) {
+    const F3 A = ld3(d.ob_x, (uint32_t)d.ob_f[3 * fid]);
+    const F3 B = ld3(d.ob_x, (uint32_t)d.ob_f[3 * fid + 1]);
+    const F3 C = ld3(d.ob_x, (uint32_t)d.ob_f[3 * fid + 2]);
+    float dist, bary[3];
+    if (!template_vf(p, A, B, C, d.radius, dist, bary)) return;
+    insert_near(L, dist, fid, bary[0], bary[1], bary[2]);
+}
+
+// Writes slot s's contacts in ascending face id.
+__device__ __forceinline__ void order_pair(NearList& L, int a, int b) {
+    if (L.kf[a] <= L.kf[b]) return;
+    const float td = L.kd[a]; L.kd[a] = L.kd[b]; L.kd[b] = td;
+    const uint32_t tf = L.kf[a]; L.kf[a] = L.kf[b]; L.kf[b] = tf;
+#pragma unroll
+    for (int c = 0; c < 3; c++) { const float tb = L.kb[a][c]; L.kb[a][c] = L.kb[b][c]; L.kb[b][c] = tb; }
+}
+
+__device__ __forceinline__ void write_contacts(const DetectArgs& d, uint32_t s, NearList& L) {
+    static_assert(kMaxContacts == 4, "the sorting network below orders four entries");
+    // unused entries sort last; a 4-input sorting network on the face id
+#pragma unroll
+    for (int q = 0; q < kMaxContacts; q++)
+        if ((uint32_t)q >= L.n) L.kf[q] = 0xffffffffu;
+    order_pair(L, 0, 1);
+    order_pair(L, 2, 3);
+    order_pair(L, 0, 2);
+    order_pair(L, 1, 3);
+    order_pair(L, 1, 2);
+#pragma unroll
+    for (int k = 0; k < kMaxContacts; k++) {
+        if ((uint32_t)k >= L.n) break;
+        const uint32_t fid = L.kf[k];
+        const F3 A = ld3(d.ob_x, (uint32_t)d.ob_f[3 * fid]);
+        const F3 B = ld3(d.ob_x, (uint32_t)d.ob_f[3 * fid + 1]);
+        const F3 C = ld3(d.ob_x, (uint32_t)d.ob_f[3 * fid + 2]);
+        const F3 xs = add3(add3(scale3(L.kb[k][0], A), scale3(L.kb[k][1], B)), scale3(L.kb[k][2], C));
+        d.ct_xs[k * d.nv + s] = float4{xs.x, xs.y, xs.z, d.stiffness * 1.0f};
+        d.ct_n[k * d.nv + s] = float4{d.ob_n[3 * fid], d.ob_n[3 * fid + 1], d.ob_n[3 * fid + 2], 0.f};
+        d.ct_fid[k * d.nv + s] = fid;
+    }
+    d.ccount[s] = L.n;
+}
+
 // Contacts of slot s at its current (predicted) position: faces of the hash
 // cell containing it that pass the narrow phase within `radius`; the
 // kMaxContacts nearest by (distance, face id), stored in ascending face id.
 __device__ __forceinline__ void detect_vertex(const DetectArgs& d, uint32_t s) {
     const F3 p = ld3(d.x, s);
-    const int ix = (int)floorf((p.x - d.ox) * d.inv_cell);
-    const int iy = (int)floorf((p.y - d.oy) * d.inv_cell);
-    const int iz = (int)floorf((p.z - d.oz) * d.inv_cell);
-    uint32_t n = 0;
-    float kd[kMaxContacts];
-    uint32_t kf[kMaxContacts];
-    float kb[kMaxContacts][3];
-    if (ix >= 0 && iy >= 0 && iz >= 0 && ix < d.dimx && iy < d.dimy && iz < d.dimz) {
-        const uint32_t key = ((uint32_t)ix << 20) | ((uint32_t)iy << 10) | (uint32_t)iz;
-        uint32_t h = hash_coord_d(ix, iy, iz, d.table_size);
-        bool found = false;
-        for (int j = 0; j < 32; j++) {
-            const uint32_t b = d.belongs[h];
-            if (b == key) { found = true; break; }
-            if (b == 0xffffffffu) break;
-            h = (h + 1) % d.table_size;
-        }
-        if (found) {
-            const uint32_t f0 = d.start[h], f1 = f0 + d.count[h];
-            for (uint32_t j = f0; j < f1; j++) {
-                const uint32_t fid = d.faces[j];
-                const F3 A = ld3(d.ob_x, (uint32_t)d.ob_f[3 * fid]);
-                const F3 B = ld3(d.ob_x, (uint32_t)d.ob_f[3 * fid + 1]);
-                const F3 C = ld3(d.ob_x, (uint32_t)d.ob_f[3 * fid + 2]);
-                float dist, bary[3];
-                if (!template_vf(p, A, B, C, d.radius, dist, bary)) continue;
-                // insert into the nearest-k list ordered by (distance, face id)
-                uint32_t pos = n;
-                while (pos > 0 && (kd[pos - 1] > dist || (kd[pos - 1] == dist && kf[pos - 1] > fid))) pos--;
-                if (pos >= (uint32_t)kMaxContacts) continue;
-                const uint32_t last = (n < (uint32_t)kMaxContacts) ? n : (uint32_t)kMaxContacts - 1;
-                for (uint32_t q = last; q > pos; q--) {
-                    kd[q] = kd[q - 1];
-                    kf[q] = kf[q - 1];
-                    kb[q][0] = kb[q - 1][0];
-                    kb[q][1] = kb[q - 1][1];
-                    kb[q][2] = kb[q - 1][2];
-                }
-                kd[pos] = dist;
-                kf[pos] = fid;
-                kb[pos][0] = bary[0];
-                kb[pos][1] = bary[1];
-                kb[pos][2] = bary[2];
-                if (n < (uint32_t)kMaxContacts) n++;
-            }
-        }
-    }
-    // ascending face id
-    for (uint32_t i = 1; i < n; i++)
-        for (uint32_t q = i; q > 0 && kf[q - 1] > kf[q]; q--) {
-            const float td = kd[q]; kd[q] = kd[q - 1]; kd[q - 1] = td;
-            const uint32_t tf = kf[q]; kf[q] = kf[q - 1]; kf[q - 1] = tf;
-            for (int c = 0; c < 3; c++) { const float tb = kb[q][c]; kb[q][c] = kb[q - 1][c]; kb[q - 1][c] = tb; }
-        }
-    for (uint32_t k = 0; k < n; k++) {
-        const uint32_t fid = kf[k];
-        const F3 A = ld3(d.ob_x, (uint32_t)d.ob_f[3 * fid]);
-        const F3 B = ld3(d.ob_x, (uint32_t)d.ob_f[3 * fid + 1]);
-        const F3 C = ld3(d.ob_x, (uint32_t)d.ob_f[3 * fid + 2]);
-        const F3 xs = add3(add3(scale3(kb[k][0], A), scale3(kb[k][1], B)), scale3(kb[k][2], C));
-        d.ct_xs[k * d.nv + s] = float4{xs.x, xs.y, xs.z, d.stiffness * 1.0f};
-        d.ct_n[k * d.nv + s] = float4{d.ob_n[3 * fid], d.ob_n[3 * fid + 1], d.ob_n[3 * fid + 2], 0.f};
-        d.ct_fid[k * d.nv + s] = fid;
-    }
-    d.ccount[s] = n;
+    NearList L;
+    L.n = 0;
+    uint32_t f0 = 0, f1 = 0;
+    if (detect_cell(d, p, &f0, &f1))
+        for (uint32_t j = f0; j < f1; j++) consider_face(d, p, d.faces[j], L);
+    write_contacts(d, s, L);
 }
 
-__global__ __launch_bounds__(kBlock) void k_detect(DetectArgs d) {
-    const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
-    if (s < d.nv) detect_vertex(d, s);
+// The same with kDetectLanes lanes per vertex: lane j tests faces j, j+G, ...
+// of the cell and keeps its own nearest list; the G lists are merged by
+// pairwise exchanges (the nearest-k set under the total order (distance,
+// face id) does not depend on how the faces were split), then lane 0
+// writes.  Identical contacts to detect_vertex (used by the persistent kernel).
+constexpr uint32_t kDetectLanes = 8;
+
+// Merges O's entries into L (both ordered by (distance, face id)), keeping
+// the kMaxContacts nearest; O's entries come in order, so the first one
+// that does not make the list ends the merge.
+__device__ __forceinline__ void merge_near(NearList& L, const NearList& O) {
+#pragma unroll
+    for (int e = 0; e < kMaxContacts; e++) {
+        if ((uint32_t)e >= O.n) break;
+        if (!insert_near(L, O.kd[e], O.kf[e], O.kb[e][0], O.kb[e][1], O.kb[e][2])) break;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_detect_group(DetectArgs d) {
+    const uint64_t t = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t lane = threadIdx.x % kDetectLanes;
+    const uint32_t s = (uint32_t)(t / kDetectLanes);
+    const bool valid = s < d.nv;     // groups past the end still take part in the exchanges
+    const F3 p = ld3(d.x, valid ? s : 0u);
+    NearList L;
+    L.n = 0;
+    uint32_t f0 = 0, f1 = 0;
+    if (valid && detect_cell(d, p, &f0, &f1))
+        for (uint32_t j = f0 + lane; j < f1; j += kDetectLanes) consider_face(d, p, d.faces[j], L);
+#pragma unroll
+    for (uint32_t off = 1; off < kDetectLanes; off <<= 1) {
+        NearList O;
+        O.n = __shfl_xor(L.n, off);
+#pragma unroll
+        for (int k = 0; k < kMaxContacts; k++) {
+            O.kd[k] = __shfl_xor(L.kd[k], off);
+            O.kf[k] = __shfl_xor(L.kf[k], off);
+            O.kb[k][0] = __shfl_xor(L.kb[k][0], off);
+            O.kb[k][1] = __shfl_xor(L.kb[k][1], off);
+            O.kb[k][2] = __shfl_xor(L.kb[k][2], off);
+        }
+        merge_near(L, O);
+    }
+    if (valid && lane == 0) write_contacts(d, s, L);
 }
 
 // --- persistent frame kernel ---------------------------------------------
@@ -744,7 +830,8 @@ DetectArgs detect_args(lats_vbd* c, const float* xb) {
 
 void enqueue_detect(lats_vbd* c) {
     if (!c->has_obstacle) return;
-    hipLaunchKernelGGL(k_detect, dim3(nblocks(c->V)), dim3(kBlock), 0, c->stream, detect_args(c, c->x.p));
+    hipLaunchKernelGGL(k_detect_group, dim3(nblocks((uint64_t)c->V * kDetectLanes)), dim3(kBlock), 0, c->stream,
+                       detect_args(c, c->x.p));
 }
 
 void enqueue_update(lats_vbd* c, int advance) {
