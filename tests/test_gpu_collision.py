@@ -102,3 +102,38 @@ def test_sphere_c4_full_size(gpu_available):
     r = np.linalg.norm(x[counts > 0].astype(np.float64), axis=1)
     print("contact vertices", int((counts > 0).sum()), "min radius", r.min())
     assert r.min() > 0.3 - 0.01
+
+
+def test_contacts_match_oracle_dense_cells(gpu_available):
+    """The sphere_c4 sphere and contact parameters (level-5 icosphere, 5 cm
+    cells, 2.5 cm query radius: ~100 faces per cell, so all 8 lanes of the
+    detection kernel find candidates and their nearest lists are merged) on a
+    96x96 piece of the cloth dropped 1.5 cm: contact sets and positions equal
+    the oracle's brute-force query every frame."""
+    _print_case("96x96 cloth on the sphere_c4 sphere vs oracle, 6 frames")
+    n, spacing, radius = 96, 0.002, 0.3
+    x, f = configs.horizontal_grid(n, spacing, radius + 0.015, seed=91)
+    cloth = Cloth(x, f, np.float32(1e-3), np.zeros(n * n, np.uint8))
+    ox, of = make_icosphere(5, radius)
+    ct = dict(thickness=0.002, stiffness=1e5, radius=0.025, cell=0.05)
+    params = Params(constraint_iter_count=20)
+    g = GpuVBD(cloth, params)
+    g.set_lanes_per_vertex(4)
+    g.set_strategy(_lib.LATS_STRATEGY_GRAPH)
+    g.set_obstacle(ox, of, ct["thickness"], ct["stiffness"], ct["radius"], ct["cell"])
+    o = oracle_sim(cloth, params, threads=16)
+    o.set_obstacle(ox, of, ct["thickness"], ct["stiffness"], ct["radius"])
+    touched = 0
+    full = 0
+    for fr in range(6):
+        g.physics_step()
+        o.step()
+        cg, fg = g.contacts()
+        co, fo = o.contacts()
+        assert np.array_equal(cg, co), fr
+        assert np.array_equal(fg, fo), fr
+        touched += int((cg > 0).sum())
+        full += int((cg == 4).sum())
+        assert np.array_equal(g.state()[0], o.state()[0].astype(np.float32)), fr
+    print("contact vertex-frames", touched, "with 4 contacts", full, flush=True)
+    assert touched > 0 and full > 0
