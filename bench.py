@@ -686,4 +686,10 @@ def copy_calibration(args):
 
 
 if __name__ == "__main__":
-    main()
+    try:
+        main()
+    finally:
+        # every context, batch and network is destroyed before the interpreter
+        # (and the HIP runtime) shuts down
+        from libatssim_amd import _lib as _l
+        _l.close_all()

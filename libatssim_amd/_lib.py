@@ -4,8 +4,10 @@ The library is built in-tree (`make -C libatssim_amd/csrc`, or
 `__graft_entry__.build()`).  Loading fails loudly when it is missing: there
 is no CPU fallback for the product path.
 """
+import atexit
 import ctypes
 import os
+import weakref
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libatssim_hip.so")
@@ -227,6 +229,28 @@ def lib():
             fn.argtypes = args
         _LIB = L
     return _LIB
+
+
+# Every live context, batch and network, closed by an atexit hook: Python runs
+# it before the C runtime's exit handlers (and so before the HIP runtime tears
+# itself down), whereas objects left to __del__ at interpreter teardown may be
+# destroyed after that.  Networks close first (they hold bound contexts).
+_LIVE = weakref.WeakSet()
+
+
+def track(obj):
+    """Registers an object with a close() method for closing at exit."""
+    _LIVE.add(obj)
+
+
+def close_all():
+    """Closes every tracked object that is still open."""
+    objs = sorted(list(_LIVE), key=lambda o: getattr(o, "_close_rank", 1))
+    for o in objs:
+        o.close()
+
+
+atexit.register(close_all)
 
 
 def check(rc):

@@ -181,6 +181,7 @@ extern "C" int lats_time_floor(int kind, uint32_t threads, uint32_t launches, fl
         if ((e = hipEventElapsedTime(&t, e0.e, e1.e)) != hipSuccess) break;
         if (rep == 0 || t < ms) ms = t;
     }
+    (void)hipStreamSynchronize(st.s);
     (void)hipGraphExecDestroy(exec);
     if (e != hipSuccess) return fail(LATS_ERR_HIP, std::string("floor graph: ") + hipGetErrorString(e));
     *us = ms * 1e3f / (float)launches;

@@ -183,8 +183,15 @@ struct lats_batch {
     hipGraphExec_t graph = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     float last_ms = 0.f;
+    // a launched frame graph may still be running: drain the stream first
+    void drop_graph() {
+        if (!graph) return;
+        if (stream) (void)hipStreamSynchronize(stream);
+        (void)hipGraphExecDestroy(graph);
+        graph = nullptr;
+    }
     ~lats_batch() {
-        if (graph) (void)hipGraphExecDestroy(graph);
+        drop_graph();
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         if (stream) (void)hipStreamDestroy(stream);
@@ -387,7 +394,7 @@ int lats_batch_set_launch(lats_batch* b, uint32_t block_threads, uint32_t blocks
         return fail(LATS_ERR_INVALID_ARGUMENT, "block_threads must be a multiple of 64 in [64, 1024]");
     b->block_threads = block_threads;
     b->blocks = blocks;
-    if (b->graph) { (void)hipGraphExecDestroy(b->graph); b->graph = nullptr; }
+    b->drop_graph();
     return LATS_OK;
 }
 
@@ -396,7 +403,7 @@ int lats_batch_set_lanes_per_vertex(lats_batch* b, uint32_t lanes) {
     if (lanes != 1 && lanes != 4 && lanes != 8) return fail(LATS_ERR_INVALID_ARGUMENT, "lanes per vertex must be 1, 4 or 8");
     if (lanes != b->lanes) {
         b->lanes = lanes;
-        if (b->graph) { (void)hipGraphExecDestroy(b->graph); b->graph = nullptr; }
+        b->drop_graph();
     }
     return LATS_OK;
 }

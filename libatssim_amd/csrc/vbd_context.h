@@ -127,8 +127,13 @@ struct lats_vbd {
     float* host_energy_d = nullptr;   // device alias
 
     // every captured graph holds kernel choices and buffer pointers of the
-    // moment it was captured: drop them whenever one of those changes
+    // moment it was captured: drop them whenever one of those changes.  A
+    // graph may still be running on the stream (launches return at once), so
+    // the stream drains before any executable is destroyed.
     void drop_graphs() {
+        bool any = !sweep_graphs.empty();
+        for (auto& g : graph) any = any || g;
+        if (any && stream) (void)hipStreamSynchronize(stream);
         for (auto& g : graph)
             if (g) {
                 (void)hipGraphExecDestroy(g);

@@ -267,32 +267,47 @@ __global__ __launch_bounds__(kBlock) void k_detect_group(DetectArgs d) {
 // --- persistent frame kernel ---------------------------------------------
 // One cooperative launch runs whole frames: predict, every (iteration, colour)
 // phase, update velocity; a grid barrier separates dependent phases.  All
-// blocks are co-resident (hipLaunchCooperativeKernel), so the barrier's wait
-// always ends.
+// blocks are co-resident (hipLaunchCooperativeKernel), so every block reaches
+// every barrier; the wait is still bounded: after kBarrierSpinLimit polls a
+// block raises the error word (barrier[1]) and leaves, and every block that
+// sees the word leaves at its next barrier, so the grid always drains.
 struct PersistArgs {
     VbdArgs a;                 // a.x = positions, a.fp = frame parameters
     float* x0;                 // x_start (written by update velocity)
     float* v;                  // velocity (written by update velocity)
     FrameParams* fp;
     const ColourInfo* colours;
-    uint32_t* barrier;         // monotonically increasing arrival counter
+    uint32_t* barrier;         // [0] monotonically increasing arrival counter, [1] error word
     uint32_t nv, ncolours, iters, nframes;
     float damp;
     int detect;                // run contact detection after predict
     DetectArgs det;
 };
 
-__device__ __forceinline__ void grid_barrier(uint32_t* bar, uint32_t& target) {
+constexpr uint32_t kBarrierSpinLimit = 1u << 24;
+
+// false: a block gave up waiting (error word set); the caller leaves the kernel
+__device__ __forceinline__ bool grid_barrier(uint32_t* bar, uint32_t& target) {
+    __shared__ uint32_t s_fail;
     target += gridDim.x;
     __threadfence();           // every wave: its own stores reach device scope
     __syncthreads();
     if (threadIdx.x == 0) {
+        s_fail = 0u;
         atomicAdd(bar, 1u);
-        while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target)
+        uint32_t spins = 0;
+        while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            if (__hip_atomic_load(bar + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u || ++spins > kBarrierSpinLimit) {
+                __hip_atomic_store(bar + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                s_fail = 1u;
+                break;
+            }
             __builtin_amdgcn_s_sleep(1);
+        }
         __threadfence();
     }
     __syncthreads();
+    return s_fail == 0u;
 }
 
 template <int G>
@@ -311,10 +326,10 @@ __global__ __launch_bounds__(kBlock) void k_vbd_persistent(PersistArgs p) {
             if ((p.a.meta[s] >> 16) & 1u) vpred = F3{0.f, 0.f, 0.f};
             st3(p.a.x, s, add3(ld3(p.x0, s), scale3(p.a.h, vpred)));
         }
-        grid_barrier(p.barrier, target);
+        if (!grid_barrier(p.barrier, target)) return;
         if (p.detect) {
             for (uint32_t s = t0; s < p.nv; s += stride) detect_vertex(p.det, s);
-            grid_barrier(p.barrier, target);
+            if (!grid_barrier(p.barrier, target)) return;
         }
         for (uint32_t it = 0; it < p.iters; it++) {
             for (uint32_t c = 0; c < p.ncolours; c++) {
@@ -330,7 +345,7 @@ __global__ __launch_bounds__(kBlock) void k_vbd_persistent(PersistArgs p) {
                                             ci.boff, ci.ew, ci.bw, ext, t0 % G, valid);
                     }
                 }
-                grid_barrier(p.barrier, target);
+                if (!grid_barrier(p.barrier, target)) return;
             }
         }
         // update velocity (Core::update_velocity)
@@ -342,7 +357,7 @@ __global__ __launch_bounds__(kBlock) void k_vbd_persistent(PersistArgs p) {
             st3(p.v, s, vel);
             st3(p.x0, s, xk);
         }
-        grid_barrier(p.barrier, target);
+        if (!grid_barrier(p.barrier, target)) return;
     }
     if (t0 == 0) {
         p.fp->frame = frame0 + p.nframes;
@@ -412,6 +427,9 @@ __global__ __launch_bounds__(kBlock) void k_vbd_flow(FlowArgs p) {
                 __builtin_amdgcn_s_sleep(2);
             }
         }
+        // pairs with the neighbours' release in publish(): their position
+        // stores happen before this block's reads of them
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         __syncthreads();
         return s_abort == 0u;
     };
@@ -420,8 +438,9 @@ __global__ __launch_bounds__(kBlock) void k_vbd_flow(FlowArgs p) {
         __atomic_signal_fence(__ATOMIC_SEQ_CST);
         __builtin_amdgcn_s_waitcnt(0);
         __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");   // every wave's stores before the flag
         __syncthreads();
-        if (tid == 0) __hip_atomic_store(p.progress + t, q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (tid == 0) __hip_atomic_store(p.progress + t, q, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     };
     uint32_t q = p.base;
     for (uint32_t f = 0; f < p.nframes; f++) {
@@ -1036,7 +1055,7 @@ int run_persistent(lats_vbd* c, uint32_t nframes) {
     const uint32_t chunk = (uint32_t)std::max<uint64_t>(1, (1ull << 31) / per_frame);
     for (uint32_t done = 0; done < nframes;) {
         pa.nframes = std::min(chunk, nframes - done);
-        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c->stream, c->barrier.p, 1u);
+        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c->stream, c->barrier.p, 2u);
         void* kargs[] = {&pa};
         HIPCHK(hipLaunchCooperativeKernel(persistent_kernel(c->lanes), dim3(g), dim3(kBlock), kargs, 0, c->stream));
         done += pa.nframes;
@@ -1164,9 +1183,15 @@ int run_flow(lats_vbd* c, uint32_t nframes) {
     return LATS_OK;
 }
 
-// After a synchronisation: a flow launch whose wait timed out leaves an
-// error word; report it and reset the tile counters.
+// After a synchronisation: a persistent or flow launch whose wait timed out
+// leaves an error word; report it (and reset the flow tile counters).
 int flow_check(lats_vbd* c) {
+    if (c->strategy == LATS_STRATEGY_PERSISTENT && c->solver == LATS_SOLVER_VBD) {
+        uint32_t err = 0;
+        HIPCHK(hipMemcpy(&err, c->barrier.p + 1, sizeof(err), hipMemcpyDeviceToHost));
+        if (!err) return LATS_OK;
+        return fail(LATS_ERR_TIMEOUT, "persistent strategy: a grid barrier wait gave up (frame results invalid)");
+    }
     if (c->strategy != LATS_STRATEGY_FLOW || !c->flow_tiles) return LATS_OK;
     uint32_t err = 0;
     HIPCHK(hipMemcpy(&err, c->flow_error.p, sizeof(err), hipMemcpyDeviceToHost));
@@ -1260,18 +1285,17 @@ int energies_of(lats_vbd* c, const float* const* xs, uint32_t n, float* out) {
 extern "C" {
 
 int lats_vbd_create(const lats_mesh_desc* m, const lats_scene_params* prm, lats_vbd** out) {
-    if (!out || !prm) return fail(LATS_ERR_INVALID_ARGUMENT, "null argument");
+    if (!out || !prm || !m) return fail(LATS_ERR_INVALID_ARGUMENT, "null argument");
     *out = nullptr;
     if (prm->num_substep == 0 || !(prm->implicit_dt > 0.f))
         return fail(LATS_ERR_INVALID_ARGUMENT, "implicit_dt must be > 0 and num_substep >= 1");
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(LATS_ERR_NO_DEVICE, "no HIP device");
-
     const uint32_t V = m->num_verts, E = m->num_edges, B = m->num_bending_edges, C = m->num_clusters;
     ClothLayout L;
     std::vector<uint32_t> inv;
     std::string err;
     if (!append_cloth(*m, L, inv, err)) return fail(LATS_ERR_INVALID_ARGUMENT, err);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(LATS_ERR_NO_DEVICE, "no HIP device");
 
     auto c = new lats_vbd();
     c->V = V;
@@ -1327,7 +1351,7 @@ int lats_vbd_create(const lats_mesh_desc* m, const lats_scene_params* prm, lats_
     UPCHK(c->scratch3.alloc(3 * (size_t)V));
     {
         UPCHK(c->colours_d.upload(L.colours));
-        UPCHK(c->barrier.alloc(1));
+        UPCHK(c->barrier.alloc(2));
     }
     FrameParams fp0;
     std::memset(&fp0, 0, sizeof(fp0));
@@ -2075,6 +2099,7 @@ int lats_vbd_time_colour_kernels(lats_vbd* c, uint32_t iters, float* mean_ms, ui
     if (e == hipSuccess) e = hipEventRecord(c->ev1, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e == hipSuccess) e = hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    (void)hipStreamSynchronize(c->stream);   // also after a failed step: nothing may still run the graph
     (void)hipGraphExecDestroy(exec);
     if (e != hipSuccess) return fail(LATS_ERR_HIP, hipGetErrorString(e));
     HIPCHK(hipMemcpyAsync(c->x.p, c->scratch3.p, nf * 4, hipMemcpyDeviceToDevice, c->stream));
@@ -2125,6 +2150,7 @@ int lats_vbd_time_xpbd_kernels(lats_vbd* c, uint32_t iters, float* stretch_ms, f
         if (e == hipSuccess) e = hipEventRecord(c->ev1, c->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
         if (e == hipSuccess) e = hipEventElapsedTime(&ms, c->ev0, c->ev1);
+        (void)hipStreamSynchronize(c->stream);
         (void)hipGraphExecDestroy(exec);
         if (e != hipSuccess) return fail(LATS_ERR_HIP, hipGetErrorString(e));
         *out[which] = ms / (float)nl;

@@ -66,6 +66,37 @@ inline bool append_cloth(const lats_mesh_desc& m, ClothLayout& L, std::vector<ui
         if (vid >= V || inv[vid] != 0xffffffffu) { err = "clusters CSR is not a permutation of the vertices"; return false; }
         inv[vid] = (uint32_t)(base + s);
     }
+    // a colour pass updates its vertices concurrently, so a cluster must be an
+    // independent set of the graph the reference colours
+    // (vert_adj_verts_with_bending, example2_async_iter.cpp:287-303): no
+    // spring and no bending pair may hold two vertices of one cluster
+    {
+        std::vector<uint32_t> colour(V);
+        for (uint32_t k = 0; k < C; k++)
+            for (uint32_t r = cl[k]; r < cl[k + 1]; r++) colour[cl[r]] = k;
+        for (uint32_t e = 0; e < E; e++) {
+            const uint32_t a0 = (uint32_t)m.edges[2 * (size_t)e], a1 = (uint32_t)m.edges[2 * (size_t)e + 1];
+            if (a0 >= V || a1 >= V) { err = "edge vertex out of range"; return false; }
+            if (colour[a0] == colour[a1]) {
+                err = "clusters are not independent: spring " + std::to_string(e) + " joins vertices " +
+                      std::to_string(a0) + " and " + std::to_string(a1) + " of cluster " + std::to_string(colour[a0]);
+                return false;
+            }
+        }
+        for (uint32_t b = 0; b < B; b++) {
+            const int32_t* q4 = &m.bending_edges[4 * (size_t)b];
+            for (int t = 0; t < 4; t++)
+                if ((uint32_t)q4[t] >= V) { err = "bending vertex out of range"; return false; }
+            for (int t = 0; t < 4; t++)
+                for (int u = t + 1; u < 4; u++)
+                    if (colour[q4[t]] == colour[q4[u]]) {
+                        err = "clusters are not independent: bending pair " + std::to_string(b) + " holds vertices " +
+                              std::to_string(q4[t]) + " and " + std::to_string(q4[u]) + " of cluster " +
+                              std::to_string(colour[q4[t]]);
+                        return false;
+                    }
+        }
+    }
     // per-slot arrays
     for (uint32_t s = 0; s < V; s++) {
         const uint32_t vid = cl[C + 1 + s];

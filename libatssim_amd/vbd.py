@@ -201,6 +201,7 @@ class GpuVBD:
         h = ctypes.c_void_p()
         check(lib().lats_vbd_create(ctypes.byref(d), ctypes.byref(self._params), ctypes.byref(h)))
         self._h = h
+        _lib.track(self)
         if strategy != _lib.LATS_STRATEGY_AUTO:
             check(lib().lats_vbd_set_strategy(self._h, strategy))
 
@@ -486,6 +487,7 @@ class GpuBatch:
         check(lib().lats_batch_create(n, descs, None if ks is None else _p(ks), None if kb is None else _p(kb),
                                       ctypes.byref(self._params), ctypes.byref(h)))
         self._h = h
+        _lib.track(self)
         del descs
 
     def close(self):

@@ -26,6 +26,7 @@ import ctypes
 
 import numpy as np
 
+from . import _lib
 from ._lib import check, lib
 
 NUM_FEATURES = 48
@@ -70,6 +71,8 @@ def unpack_params(p):
 class WarmStartMLP:
     """The per-vertex warm-start network and its training rows (on the GPU)."""
 
+    _close_rank = 0   # closed before the contexts it may be bound to
+
     def __init__(self, seed=0):
         n = ctypes.c_uint32()
         check(lib().lats_mlp_num_params(ctypes.byref(n)))
@@ -78,6 +81,7 @@ class WarmStartMLP:
         h = ctypes.c_void_p()
         check(lib().lats_mlp_create(int(seed), ctypes.byref(h)))
         self._h = h
+        _lib.track(self)
         self._bound = None
 
     def close(self):

@@ -37,3 +37,12 @@ def gpu_available():
     if not ok:
         pytest.skip("no HIP device")
     return True
+
+
+def pytest_sessionfinish(session, exitstatus):
+    """Destroy every GPU context the tests left open while the HIP runtime is
+    still up (objects left to garbage collection at interpreter exit may be
+    destroyed after it has shut down)."""
+    mod = sys.modules.get("libatssim_amd._lib")
+    if mod is not None:
+        mod.close_all()
