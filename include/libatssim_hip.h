@@ -235,6 +235,19 @@ int lats_vbd_iterate(lats_vbd* ctx, uint32_t sweeps); /* colour sweeps on the po
 int lats_vbd_frame_end(lats_vbd* ctx);                /* update velocity, advance the frame */
 /* Energy (incremental potential) of the current positions; synchronises. */
 int lats_vbd_energy_now(lats_vbd* ctx, float* energy);
+/* No reference counterpart: for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops).  The energy of
+ * lats_vbd_energy_now, by the same kernels, of the positions x_dev (device
+ * float[3V] in slot order, see lats_vbd_device_positions; NULL: the
+ * context's current positions), with the context's frame start, velocities
+ * and contacts; one float is written to energy_dev (device memory).  Enqueued
+ * on the context's stream; returns without waiting.  Allocates nothing: the
+ * FP64 block partials use one fixed array per GPU of 32768 doubles (256 KiB),
+ * so vertices + springs + bending pairs must be <= 8,388,608
+ * (LATS_ERR_INVALID_ARGUMENT otherwise).  Every call on one GPU uses that
+ * array: calls on different contexts of one GPU whose executions overlap
+ * (different streams) race and must be ordered by the caller. */
+int lats_vbd_energy_noalloc(lats_vbd* ctx, const float* x_dev, float* energy_dev);
 
 /* ---- scheduler slot: GpuSolver::fn_dispatch ------------------------- */
 /* Function ids of the tasks the VBD DAG registers

@@ -138,6 +138,7 @@ SIGNATURES = {
     "lats_vbd_iterate": (_I, [_VP, _U32]),
     "lats_vbd_frame_end": (_I, [_VP]),
     "lats_vbd_energy_now": (_I, [_VP, ctypes.POINTER(ctypes.c_float)]),
+    "lats_vbd_energy_noalloc": (_I, [_VP, _VP, _VP]),
     "lats_vbd_set_obstacle": (_I, [_VP, _VP, _U32, _VP, _U32, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                                    ctypes.c_float]),
     "lats_vbd_get_contacts": (_I, [_VP, _VP, _VP]),

@@ -23,6 +23,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -646,6 +647,13 @@ __global__ __launch_bounds__(kBlock) void k_energy_finalize(const double* partia
     if (threadIdx.x == 0) out[row] = (float)r;
 }
 
+// FP64 block partials of lats_vbd_energy_noalloc: one copy of this array per
+// GPU, held until the process ends.  Host code never passes its name to a
+// kernel (on the host the name is a stand-in, not a device address): it uses
+// the address hipGetSymbolAddress returned for the device (scratch_address).
+constexpr uint32_t kScratchPartials = 32768;   // 256 KiB: energy_nblk <= 32768
+static __device__ __align__(256) double scratch[kScratchPartials];
+
 // --- multi-buffer helpers of the heterogeneous schedule ------------------
 __global__ __launch_bounds__(kBlock) void k_copy_to_2(const float* a, float* b, float* c, uint32_t n) {
     const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
@@ -728,6 +736,30 @@ int fail(int code, const std::string& msg) {
 
 namespace {
 
+// Device address of `scratch` per device, looked up once per device (with
+// that device current) when the first context is created on it.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_addr[kMaxDevices] = {};
+
+int scratch_setup(int device) {
+    if (device < 0 || device >= kMaxDevices) return fail(LATS_ERR_UNSUPPORTED, "device ordinal above 63");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_addr[device]) return LATS_OK;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return fail(LATS_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_addr[device] = static_cast<double*>(p);
+    return LATS_OK;
+}
+
+double* scratch_address(int device) {
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return g_scratch_addr[device];
+}
+
 int ensure_energy_rows(lats_vbd* c, uint32_t rows) {
     if (rows <= c->energy_cap) return LATS_OK;
     const uint32_t cap = std::max(rows, c->energy_cap * 2 + 8);
@@ -754,8 +786,9 @@ int ensure_energy_rows(lats_vbd* c, uint32_t rows) {
     return LATS_OK;
 }
 
-// Enqueue one energy evaluation of buffer xb into history row `row`.
-void enqueue_energy(lats_vbd* c, const float* xb, uint32_t row, hipStream_t st = nullptr) {
+// Enqueue the FP64 block partials (energy_nblk of them) of one energy
+// evaluation of buffer xb into partial[0, energy_nblk).
+void enqueue_energy_partials(lats_vbd* c, const float* xb, double* partial, hipStream_t st) {
     EnergyArgs a;
     a.x = xb;
     a.x0 = c->x0.p;
@@ -778,8 +811,12 @@ void enqueue_energy(lats_vbd* c, const float* xb, uint32_t row, hipStream_t st =
     a.ct_xs = c->ct_xs.p;
     a.ct_n = c->ct_n.p;
     a.thickness = c->thickness;
-    hipLaunchKernelGGL(k_energy_partial, dim3(c->energy_nblk), dim3(kBlock), 0, st ? st : c->stream, a,
-                       c->partial.p + (size_t)row * c->energy_nblk);
+    hipLaunchKernelGGL(k_energy_partial, dim3(c->energy_nblk), dim3(kBlock), 0, st ? st : c->stream, a, partial);
+}
+
+// Enqueue one energy evaluation of buffer xb into history row `row`.
+void enqueue_energy(lats_vbd* c, const float* xb, uint32_t row, hipStream_t st = nullptr) {
+    enqueue_energy_partials(c, xb, c->partial.p + (size_t)row * c->energy_nblk, st);
 }
 
 // Cuts every colour into `n` regions of consecutive vertex ids (a colour's
@@ -1366,6 +1403,10 @@ int lats_vbd_create(const lats_mesh_desc* m, const lats_scene_params* prm, lats_
     UPCHK(hipHostGetDevicePointer((void**)&c->host_energy_d, c->host_energy, 0));
 #undef UPCHK
     c->energy_nblk = nblocks((uint64_t)V + E + B);
+    {
+        const int rc = scratch_setup(c->device);
+        if (rc) return cleanup_fail(rc);
+    }
     *out = c;
     return LATS_OK;
 }
@@ -2070,6 +2111,23 @@ int lats_vbd_energy_now(lats_vbd* c, float* e) {
     if (!c || !e) return fail(LATS_ERR_INVALID_ARGUMENT, "null argument");
     const float* xs[1] = {c->x.p};
     return lats::energies_of(c, xs, 1, e);
+}
+
+// lats_vbd_energy_now's kernels with their block partials in `scratch` and the
+// result left in device memory: no allocation, no host wait.
+int lats_vbd_energy_noalloc(lats_vbd* c, const float* x_dev, float* energy_dev) {
+    if (!c || !energy_dev) return fail(LATS_ERR_INVALID_ARGUMENT, "null argument");
+    if (c->energy_nblk > kScratchPartials)
+        return fail(LATS_ERR_INVALID_ARGUMENT,
+                    "cloth too large for the no-allocation energy: vertices + springs + bending pairs = " +
+                        std::to_string((uint64_t)c->V + c->E + c->B) + " > " +
+                        std::to_string((uint64_t)kScratchPartials * kBlock));
+    double* part = scratch_address(c->device);
+    if (!part) return fail(LATS_ERR_HIP, "scratch address of the context's device was not looked up");
+    enqueue_energy_partials(c, x_dev ? x_dev : c->x.p, part, c->stream);
+    hipLaunchKernelGGL(k_energy_finalize, dim3(1), dim3(kBlock), 0, c->stream, part, c->energy_nblk, 0u, energy_dev);
+    HIPCHK(hipGetLastError());
+    return LATS_OK;
 }
 
 int lats_vbd_time_colour_kernels(lats_vbd* c, uint32_t iters, float* mean_ms, uint32_t* launches,
