@@ -574,7 +574,7 @@ def bench_launch_floor(args):
 
     def floor(kind, threads, launches=800):
         us = ctypes.c_float()
-        _lib.check(L.lats_time_floor(kind, threads, launches, ctypes.byref(us)))
+        _lib.check(_lib.measure_lib().lats_time_floor(kind, threads, launches, ctypes.byref(us)))
         return round(us.value, 3)
 
     out = {"workload": "launch_floor", "launches_per_graph": 800, "unit": "us",

@@ -358,6 +358,11 @@ int lats_batch_last_elapsed_ms(lats_batch* b, float* ms);
  * holds vertex perm[s] (lats_vbd_copy_permutation). */
 int lats_vbd_device_positions(lats_vbd* ctx, void** dptr);
 int lats_vbd_copy_permutation(lats_vbd* ctx, uint32_t* perm);
+/* The context's HIP stream (a hipStream_t): a caller's own kernels and copies
+ * enqueued on it run in order with the context's work, host-value waits and
+ * markers (≙ adding tasks to the reference's command list,
+ * command_list.h:474-480).  Owned by the context. */
+int lats_vbd_get_stream(lats_vbd* ctx, void** stream);
 /* Measurement: captures `iterations` colour sweeps (one VBD kernel launch
  * per colour) in a graph, replays it once to warm up and once between two
  * HIP events on the context's stream, then restores the positions.
@@ -444,19 +449,6 @@ int lats_mlp_warm_start_frame(lats_mlp* m, lats_vbd* ctx, float target, uint32_t
  * training rows (rows from the states the warm-started solver visits). */
 int lats_mlp_record_guided_frame(lats_mlp* m, lats_vbd* ctx, lats_vbd* ref, uint32_t max_sweeps, uint32_t first_check,
                                  uint32_t check_every, int use_model, float* out);
-
-/* ---- latency floors (measurement; no reference counterpart) --------------
- * Mean microseconds per launch of one graph of `launches` kernels of
- * `threads` threads (256 per block) that do nothing (EMPTY) or gather 4
- * positions each through random indices and store their sum (GATHER: the
- * dependent ELL -> position chain of a colour pass, without its math); or
- * mean microseconds per barrier of one cooperative kernel of threads / 256
- * co-resident blocks crossing `launches` grid-wide barriers (GRID_BARRIER,
- * bounded waits: LATS_ERR_TIMEOUT if one gives up). */
-#define LATS_FLOOR_EMPTY 0
-#define LATS_FLOOR_GATHER 1
-#define LATS_FLOOR_GRID_BARRIER 2
-int lats_time_floor(int kind, uint32_t threads, uint32_t launches, float* us);
 
 #ifdef __cplusplus
 }

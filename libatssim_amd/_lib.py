@@ -43,7 +43,6 @@ LATS_FN_RESET_CONSTRAINS = 6
 LATS_FN_XPBD_STRETCH = 7
 LATS_FN_XPBD_BENDING = 8
 LATS_SOLVER_VBD, LATS_SOLVER_XPBD = 0, 1
-LATS_FLOOR_EMPTY, LATS_FLOOR_GATHER, LATS_FLOOR_GRID_BARRIER = 0, 1, 2
 
 LATS_BUFFER_DEFAULT = 0x0FFFFFFF
 LATS_BUFFER_NONE = 0xFFFFFFFF
@@ -168,6 +167,7 @@ SIGNATURES = {
     "lats_host_free": (_I, [_VP]),
     "lats_vbd_device_positions": (_I, [_VP, ctypes.POINTER(_VP)]),
     "lats_vbd_copy_permutation": (_I, [_VP, _VP]),
+    "lats_vbd_get_stream": (_I, [_VP, ctypes.POINTER(_VP)]),
     "lats_vbd_time_colour_kernels": (_I, [_VP, _U32, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(_U32),
                                           ctypes.POINTER(ctypes.c_uint64)]),
     "lats_vbd_device_bytes": (_I, [_VP, ctypes.POINTER(ctypes.c_uint64)]),
@@ -209,7 +209,6 @@ SIGNATURES = {
     "lats_vbd_time_xpbd_kernels": (_I, [_VP, _U32, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
                                         ctypes.POINTER(_U32), ctypes.POINTER(_U32)]),
     "lats_vbd_autotune_results": (_I, [_VP, _VP, _U32, ctypes.POINTER(_U32)]),
-    "lats_time_floor": (_I, [_I, _U32, _U32, ctypes.POINTER(ctypes.c_float)]),
 }
 
 _LIB = None
@@ -252,6 +251,26 @@ def close_all():
 
 
 atexit.register(close_all)
+
+
+MEASURE_PATH = os.path.join(_HERE, "lib", "liblats_measure.so")
+LATS_FLOOR_EMPTY, LATS_FLOOR_GATHER, LATS_FLOOR_GRID_BARRIER = 0, 1, 2
+_MEASURE = None
+
+
+def measure_lib():
+    """The benchmark's latency-floor library (liblats_measure.so, built beside
+    the solver library); lats_time_floor reports errors via lats_last_error."""
+    global _MEASURE
+    if _MEASURE is None:
+        lib()
+        if not os.path.exists(MEASURE_PATH):
+            raise RuntimeError(f"{MEASURE_PATH} is missing: build it with `make -C libatssim_amd/csrc`")
+        M = ctypes.CDLL(MEASURE_PATH)
+        M.lats_time_floor.restype = _I
+        M.lats_time_floor.argtypes = [_I, _U32, _U32, ctypes.POINTER(ctypes.c_float)]
+        _MEASURE = M
+    return _MEASURE
 
 
 def check(rc):

@@ -596,7 +596,7 @@ void Scheduler::launch(LaunchMode mode, const std::function<LaunchParam(const Ta
                     if (!run_event(1, gev[e], gdev)) return false;
                     if (fake && hooks) {
                         hooks->gpu_end(e);
-                        hooks->cpu_wait_gpu(e);
+                        if (!hooks->cpu_wait_gpu(e)) return false;
                         gpu_ms[e] = hooks->gpu_elapsed_ms(e);
                     }
                 }
@@ -640,7 +640,13 @@ void Scheduler::launch(LaunchMode mode, const std::function<LaunchParam(const Ta
             }
             std::vector<float> cpu_ms(cev.size(), 0.f), gpu_ms(gev.size(), 0.f);
             for (uint e = 0; e < cev.size(); e++) {
-                if (cev[e].wait != ~0u) hooks->cpu_wait_gpu(cev[e].wait);
+                if (cev[e].wait != ~0u && !hooks->cpu_wait_gpu(cev[e].wait)) {
+                    // the GPU result this event needs never arrived: run no
+                    // further host event, release every queued wait, drain
+                    hooks->cpu_signal(0xFFFFFFFFu);
+                    hooks->gpu_finish();
+                    return;
+                }
                 const auto t0 = clock::now();
                 const bool ok = run_event(0, cev[e], DeviceTypeCpu);
                 cpu_ms[e] = ms_since(t0);
@@ -691,7 +697,8 @@ void Scheduler::launch(LaunchMode mode, const std::function<LaunchParam(const Ta
                 if (cev[e].wait != ~0u) {
                     ok = queue_until(cev[e].wait);
                     if (!ok) break;
-                    hooks->cpu_wait_gpu(cev[e].wait);
+                    ok = hooks->cpu_wait_gpu(cev[e].wait);
+                    if (!ok) break;
                 }
                 ok = run_event(0, cev[e], DeviceTypeCpu);
                 if (ok && cev[e].signal != ~0u) hooks->cpu_signal(cev[e].signal + 1);

@@ -154,14 +154,16 @@ struct LaunchEvent {
 //   gpu_begin(e, wait):  start queue event e; if wait != -1u the queue first
 //                        waits until the host has signalled a value >= wait + 1
 //   gpu_end(e):          close event e (its completion can be waited for)
-//   cpu_wait_gpu(e):     block the host until queue event e has completed
+//   cpu_wait_gpu(e):     block the host until queue event e has completed;
+//                        false if the wait failed or gave up (the launch then
+//                        runs no further host event and drains the queue)
 //   cpu_signal(v):       host signal value v (monotonic)
 //   gpu_finish():        block until every queue event has completed
 //   gpu_elapsed_ms(e):   queue time of event e (refits the computation matrix)
 struct HeteroHooks {
     std::function<void(uint, uint)> gpu_begin = [](uint, uint) {};
     std::function<void(uint)> gpu_end = [](uint) {};
-    std::function<void(uint)> cpu_wait_gpu = [](uint) {};
+    std::function<bool(uint)> cpu_wait_gpu = [](uint) { return true; };
     std::function<void(uint)> cpu_signal = [](uint) {};
     std::function<void()> gpu_finish = []() {};
     std::function<float(uint)> gpu_elapsed_ms = [](uint) { return 0.f; };

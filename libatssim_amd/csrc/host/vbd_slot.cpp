@@ -54,17 +54,17 @@ Launcher::HeteroHooks VbdGpuSlot::hetero_hooks(double timeout_s) {
         keep(lats_vbd_record_marker(ctx_, 2 * e));
     };
     h.gpu_end = [this, keep](Launcher::uint e) { keep(lats_vbd_record_marker(ctx_, 2 * e + 1)); };
-    h.cpu_wait_gpu = [this, keep, timeout_s](Launcher::uint e) {
+    h.cpu_wait_gpu = [this, keep, timeout_s](Launcher::uint e) -> bool {
         const auto t0 = std::chrono::steady_clock::now();
         int done = 0;
         while (true) {
             const int rc = lats_vbd_query_marker(ctx_, 2 * e + 1, &done);
-            if (rc != LATS_OK) { keep(rc); return; }
-            if (done) return;
+            if (rc != LATS_OK) { keep(rc); return false; }
+            if (done) return true;
             if (std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() > timeout_s) {
-                lats_vbd_host_signal(ctx_, 0xFFFFFFFFu);   // let the queue drain
+                // the launch stops issuing host events and releases the queue
                 keep(LATS_ERR_TIMEOUT);
-                return;
+                return false;
             }
             std::this_thread::sleep_for(std::chrono::microseconds(20));
         }

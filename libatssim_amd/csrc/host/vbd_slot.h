@@ -30,8 +30,9 @@ public:
     // MTLSharedEvent + command buffers, launcher.cpp:918-1111): queue event e
     // is bracketed by markers 2e and 2e + 1, queue waits are
     // lats_vbd_stream_wait_host, host signals lats_vbd_host_signal.  A host
-    // wait gives up after `timeout_s` seconds: it then releases every queued
-    // wait (so the stream drains) and status() reports LATS_ERR_TIMEOUT.
+    // wait gives up after `timeout_s` seconds: the launch then runs no further
+    // host event and releases every queued wait (so the stream drains), and
+    // status() reports LATS_ERR_TIMEOUT.
     Launcher::HeteroHooks hetero_hooks(double timeout_s = 60.0);
 
 private:

@@ -10,8 +10,9 @@
 #include <string>
 #include <vector>
 
-#include "../../include/libatssim_hip.h"
-#include "host/errors.h"
+#include "../../../include/libatssim_hip.h"
+#include "../host/errors.h"
+#include "floors.h"
 
 namespace {
 

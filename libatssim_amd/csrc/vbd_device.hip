@@ -2063,6 +2063,11 @@ int lats_vbd_device_positions(lats_vbd* c, void** d) {
     *d = c->x.p;
     return LATS_OK;
 }
+int lats_vbd_get_stream(lats_vbd* c, void** stream) {
+    if (!c || !stream) return fail(LATS_ERR_INVALID_ARGUMENT, "null argument");
+    *stream = static_cast<void*>(c->stream);
+    return LATS_OK;
+}
 int lats_vbd_copy_permutation(lats_vbd* c, uint32_t* perm) {
     if (!c || !perm) return fail(LATS_ERR_INVALID_ARGUMENT, "null argument");
     std::copy(c->perm.begin(), c->perm.end(), perm);

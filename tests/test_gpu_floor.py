@@ -12,7 +12,7 @@ pytestmark = pytest.mark.gpu
 
 def _floor(kind, threads, launches=50):
     us = ctypes.c_float()
-    _lib.check(_lib.lib().lats_time_floor(kind, threads, launches, ctypes.byref(us)))
+    _lib.check(_lib.measure_lib().lats_time_floor(kind, threads, launches, ctypes.byref(us)))
     return us.value
 
 
@@ -27,8 +27,7 @@ def test_floor_kinds(gpu_available):
 
 
 def test_floor_rejects(gpu_available):
-    L = _lib.lib()
     us = ctypes.c_float()
-    assert L.lats_time_floor(7, 64, 10, ctypes.byref(us)) == 1
-    assert L.lats_time_floor(_lib.LATS_FLOOR_EMPTY, 0, 10, ctypes.byref(us)) == 1
-    assert L.lats_time_floor(_lib.LATS_FLOOR_GRID_BARRIER, 1 << 24, 10, ctypes.byref(us)) == 1
+    assert _lib.measure_lib().lats_time_floor(7, 64, 10, ctypes.byref(us)) == 1
+    assert _lib.measure_lib().lats_time_floor(_lib.LATS_FLOOR_EMPTY, 0, 10, ctypes.byref(us)) == 1
+    assert _lib.measure_lib().lats_time_floor(_lib.LATS_FLOOR_GRID_BARRIER, 1 << 24, 10, ctypes.byref(us)) == 1

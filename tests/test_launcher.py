@@ -29,3 +29,30 @@ def test_vbd_dag_through_scheduler_matches_fused(gpu_available):
     print("[case] VBD DAG via Launcher LaunchModeGpu", flush=True)
     r = _run("test_vbd_dag")
     assert r.returncode == 0 and "PASS" in r.stdout, r.stdout + r.stderr
+
+
+@pytest.mark.gpu
+def test_integration_snippets_match_fused_frames(gpu_available):
+    """The adapter, the hand-written LaunchParam slot and the XPBD DAG of the
+    integration guide, run for two frames each, equal the fused frames."""
+    print("[case] integration snippets via Launcher LaunchModeGpu", flush=True)
+    r = _run("test_integration_snippets")
+    assert r.returncode == 0 and "PASS" in r.stdout, r.stdout + r.stderr
+
+
+def test_hetero_launch_stops_after_failed_host_wait():
+    """A host wait for the queue that fails (a time-out) ends the host side of
+    LaunchModeHetero / ProgressiveHetero: no further host event runs and the
+    queue is released and drained (host-only hooks)."""
+    r = _run("test_hetero_wait_failure")
+    assert r.returncode == 0 and "PASS" in r.stdout, r.stdout + r.stderr
+
+
+@pytest.mark.gpu
+def test_waiting_events_at_depth(gpu_available):
+    """60 alternating host / GPU events x 100 loops on one shared counter
+    (the reference's test_1_waiting_events): every loop ends at 120 and no GPU
+    event completes before the host signal it waits for."""
+    print("[case] host/GPU waiting events, depth 60 x 100 loops", flush=True)
+    r = _run("test_waiting_events", timeout=300)
+    assert r.returncode == 0 and "PASS" in r.stdout, r.stdout + r.stderr
