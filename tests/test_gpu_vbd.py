@@ -86,22 +86,6 @@ def test_square8k_matches_oracle_and_published(gpu_available):
     assert (np.abs(eg - pub) / pub).max() < 5e-6
 
 
-def test_c1_first_frames_bit_exact(gpu_available):
-    _print_case("grid_c1 256x256, 2 windy frames vs oracle")
-    cloth, params, _, wind = configs.grid_c1(frames=2)
-    g = GpuVBD(cloth, params)
-    o = oracle_sim(cloth, params, threads=O.max_threads())
-    for fr in range(2):
-        g.set_external_force(wind[fr])
-        o.set_wind(wind[fr])
-        g.physics_step()
-        o.step()
-        xg = g.state()[0]
-        xo = o.state()[0]
-        assert rel_maxnorm(xg, xo) <= POS_TOL
-        assert np.array_equal(xg, xo.astype(np.float32)), f"frame {fr}"
-
-
 def test_run_frames_matches_stepping(gpu_available):
     _print_case("run_frames with per-frame forces == physics_step loop")
     cloth, params, _, _ = configs.grid_c0()
