@@ -55,6 +55,8 @@ def lib():
         L.orc_sim_set_state.argtypes = [ctypes.c_void_p, _f64p, _f64p]
         L.orc_sim_energy.restype = ctypes.c_double
         L.orc_sim_energy.argtypes = [ctypes.c_void_p]
+        L.orc_sim_set_positions.argtypes = [ctypes.c_void_p, _f64p]
+        L.orc_sim_forces.argtypes = [ctypes.c_void_p, _f64p]
         L.orc_max_threads.restype = ctypes.c_int
         L.orc_sim_set_obstacle.argtypes = [ctypes.c_void_p, ctypes.c_uint32, _f32p, ctypes.c_uint32, _i32p,
                                            ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_int]
@@ -167,6 +169,17 @@ class Sim:
         x = np.ascontiguousarray(x, np.float64)
         v = np.ascontiguousarray(v, np.float64)
         lib().orc_sim_set_state(self._h, _ptr(x, _f64p), _ptr(v, _f64p))
+
+    def set_positions(self, x):
+        """Positions alone; the frame start and velocities stay."""
+        x = np.ascontiguousarray(x, np.float64)
+        lib().orc_sim_set_positions(self._h, _ptr(x, _f64p))
+
+    def forces(self):
+        """Per-vertex force vector of the vertex solve at the current positions."""
+        f = np.zeros(self.mesh.nv * 3, np.float64)
+        lib().orc_sim_forces(self._h, _ptr(f, _f64p))
+        return f.reshape(-1, 3)
 
     def energy(self):
         return lib().orc_sim_energy(self._h)

@@ -986,6 +986,27 @@ template <typename T> static u32 contact_list(const Sim<T>& s, u32* counts, int*
     return tot;
 }
 
+// Checker support: the positions alone (frame start and velocities kept), and
+// per vertex the force vector f of the vertex solve at the current positions
+// (inertia + stretch + bending + contact terms, the f vbd_step solves with).
+template <typename T> static void set_positions(Sim<T>& s, const double* x) {
+    for (size_t i = 0; i < s.x.size(); i++)
+        for (int c = 0; c < 3; c++) s.x[i][c] = (T)x[3 * i + c];
+}
+template <typename T> static void solve_forces(const Sim<T>& s, double* f) {
+    const T h = substep_dt<T>(s.p);
+    for (u32 vid = 0; vid < (u32)s.x.size(); vid++) {
+        Hf<T> a = compute_inertia(s, vid, h);
+        Hf<T> b = compute_stretch(s, vid);
+        hf_add(a, b);
+        b = compute_bending(s, vid);
+        hf_add(a, b);
+        b = compute_contact(s, vid);
+        hf_add(a, b);
+        for (int c = 0; c < 3; c++) f[3 * (size_t)vid + c] = (double)a.f[c];
+    }
+}
+
 extern "C" {
 
 orc_mesh* orc_mesh_create(u32 nv, const float* x, u32 nf, const int* faces) {
@@ -1082,6 +1103,15 @@ void orc_sim_set_state(orc_sim* s, const double* x, const double* v) {
     else set_state(s->s32, x, v);
 }
 double orc_sim_energy(orc_sim* s) { return s->fp64 ? compute_energy(s->s64) : compute_energy(s->s32); }
+
+void orc_sim_set_positions(orc_sim* s, const double* x) {
+    if (s->fp64) set_positions(s->s64, x);
+    else set_positions(s->s32, x);
+}
+void orc_sim_forces(const orc_sim* s, double* f) {
+    if (s->fp64) solve_forces(s->s64, f);
+    else solve_forces(s->s32, f);
+}
 
 // Static triangle obstacle (face normals = normalize(cross(b - a, c - a))).
 void orc_sim_set_obstacle(orc_sim* s, u32 nv, const float* x, u32 nf, const int* f, float thickness, float stiffness,
