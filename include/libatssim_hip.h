@@ -55,10 +55,24 @@ typedef struct lats_topology lats_topology;
 #define LATS_TOPO_ADJ_BENDING_CSR 5  /* uint32 packed CSR                            */
 #define LATS_TOPO_CLUSTERS_CSR 6     /* uint32 packed CSR of colour clusters         */
 #define LATS_TOPO_COLOUR 7           /* uint8[V] colour id per vertex                */
+#define LATS_TOPO_FACES 8            /* int32[3F] triangles                          */
 
 int lats_topology_build(const float* x, uint32_t num_verts, const int32_t* faces, uint32_t num_faces,
                         lats_topology** out);
 void lats_topology_destroy(lats_topology* t);
+/* The topology on disk (replaces save_to_binary / read_from_binary,
+ * src/Utils/mesh_reader.h:321-343, without their hard-coded path): one file
+ * per array in directory `dir` (which must exist), "<name>.dat" holding the
+ * array's raw little-endian elements, the element count given by the file
+ * size.  Files: faces (int32 3F), edges (int32 2E), bending_edges (int32 4B),
+ * edges_rest_state_length (float E), bending_edges_Q (float 16B),
+ * vert_adj_edges, vert_adj_bending_edges, clusterd_per_vertex_bending (uint32
+ * packed CSR as upload_2d_csr_from, examples/example2_async_iter.cpp:99-127),
+ * per_vertex_bending_cluster_id (uint8 V).  Loading checks the arrays against
+ * each other (CSR headers and offsets, id ranges, every vertex in exactly one
+ * cluster, cluster ids) and fails with LATS_ERR_INVALID_ARGUMENT otherwise. */
+int lats_topology_save(const lats_topology* t, const char* dir);
+int lats_topology_load(const char* dir, lats_topology** out);
 
 /* ---- Wavefront OBJ (mesh input and frame export) ----------------------
  * Reading keeps what SimMesh::read_mesh_file keeps of a file

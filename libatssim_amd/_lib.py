@@ -27,6 +27,7 @@ LATS_TOPO_ADJ_EDGES_CSR = 4
 LATS_TOPO_ADJ_BENDING_CSR = 5
 LATS_TOPO_CLUSTERS_CSR = 6
 LATS_TOPO_COLOUR = 7
+LATS_TOPO_FACES = 8
 
 LATS_STRATEGY_AUTO = 0
 LATS_STRATEGY_GRAPH = 1
@@ -117,6 +118,8 @@ SIGNATURES = {
     "lats_obj_destroy": (None, [_VP]),
     "lats_obj_write": (_I, [ctypes.c_char_p, _VP, _U32, _VP, _U32, _U32]),
     "lats_topology_destroy": (None, [_VP]),
+    "lats_topology_save": (_I, [_VP, ctypes.c_char_p]),
+    "lats_topology_load": (_I, [ctypes.c_char_p, ctypes.POINTER(_VP)]),
     "lats_topology_counts": (_I, [_VP, _VP]),
     "lats_topology_size": (_I, [_VP, _I, ctypes.POINTER(ctypes.c_uint64)]),
     "lats_topology_copy": (_I, [_VP, _I, _VP]),

@@ -93,6 +93,26 @@ int lats_topology_build(const float* x, uint32_t nv, const int32_t* faces, uint3
 
 void lats_topology_destroy(lats_topology* t) { delete t; }
 
+int lats_topology_save(const lats_topology* t, const char* dir) {
+    if (!t || !dir) return set_err(LATS_ERR_INVALID_ARGUMENT, "null argument");
+    std::string err;
+    if (!lats::save_topology(t->t, dir, err)) return set_err(LATS_ERR_INVALID_ARGUMENT, err.c_str());
+    return LATS_OK;
+}
+
+int lats_topology_load(const char* dir, lats_topology** out) {
+    if (!dir || !out) return set_err(LATS_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    auto* t = new lats_topology();
+    std::string err;
+    if (!lats::load_topology(dir, t->t, err)) {
+        delete t;
+        return set_err(LATS_ERR_INVALID_ARGUMENT, err.c_str());
+    }
+    *out = t;
+    return LATS_OK;
+}
+
 int lats_topology_counts(const lats_topology* t, uint32_t counts[4]) {
     if (!t || !counts) return set_err(LATS_ERR_INVALID_ARGUMENT, "null argument");
     counts[0] = t->t.num_verts;
@@ -114,6 +134,7 @@ int lats_topology_size(const lats_topology* t, int which, uint64_t* n) {
         case LATS_TOPO_ADJ_BENDING_CSR: *n = T.adj_bending_csr.size(); break;
         case LATS_TOPO_CLUSTERS_CSR: *n = T.clusters_csr.size(); break;
         case LATS_TOPO_COLOUR: *n = T.colour.size(); break;
+        case LATS_TOPO_FACES: *n = T.faces.size(); break;
         default: return set_err(LATS_ERR_INVALID_ARGUMENT, "unknown topology array");
     }
     return LATS_OK;
@@ -132,6 +153,7 @@ int lats_topology_copy(const lats_topology* t, int which, void* dst) {
         case LATS_TOPO_ADJ_BENDING_CSR: cp(T.adj_bending_csr); break;
         case LATS_TOPO_CLUSTERS_CSR: cp(T.clusters_csr); break;
         case LATS_TOPO_COLOUR: cp(T.colour); break;
+        case LATS_TOPO_FACES: cp(T.faces); break;
         default: return set_err(LATS_ERR_INVALID_ARGUMENT, "unknown topology array");
     }
     return LATS_OK;

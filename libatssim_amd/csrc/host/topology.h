@@ -42,6 +42,12 @@ struct Topology {
 bool build_topology(const float* x, uint32_t nv, const int32_t* faces, uint32_t nf, Topology& out,
                     std::string& err);
 
+// The topology on disk: one raw array per "<dir>/<name>.dat" file (the
+// reference's save_to_binary / read_from_binary format, mesh_reader.h:321-343).
+// load_topology checks every array against the others.
+bool save_topology(const Topology& t, const std::string& dir, std::string& err);
+bool load_topology(const std::string& dir, Topology& out, std::string& err);
+
 // Packed-CSR helpers: row i of a packed CSR is data[p[i]] .. data[p[i+1]-1].
 inline uint32_t csr_rows(const std::vector<uint32_t>& p) { return p.empty() ? 0 : (uint32_t)(p[0] - 1); }
 

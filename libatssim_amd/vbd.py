@@ -98,13 +98,37 @@ class Topology:
         "adj_bending_csr": (_lib.LATS_TOPO_ADJ_BENDING_CSR, np.uint32, 1),
         "clusters_csr": (_lib.LATS_TOPO_CLUSTERS_CSR, np.uint32, 1),
         "colour": (_lib.LATS_TOPO_COLOUR, np.uint8, 1),
+        "faces": (_lib.LATS_TOPO_FACES, np.int32, 3),
     }
+    # file name of each array on disk (lats_topology_save / _load)
+    _FILES = {"faces": "faces", "edges": "edges", "bending": "bending_edges", "rest_length": "edges_rest_state_length",
+              "bending_Q": "bending_edges_Q", "adj_edges_csr": "vert_adj_edges",
+              "adj_bending_csr": "vert_adj_bending_edges", "clusters_csr": "clusterd_per_vertex_bending",
+              "colour": "per_vertex_bending_cluster_id"}
 
     def __init__(self, x, faces):
         x = np.ascontiguousarray(x, np.float32).reshape(-1, 3)
         faces = np.ascontiguousarray(faces, np.int32).reshape(-1, 3)
         h = ctypes.c_void_p()
         check(lib().lats_topology_build(_p(x), len(x), _p(faces), len(faces), ctypes.byref(h)))
+        self._take(h)
+
+    @classmethod
+    def load(cls, directory):
+        """The topology saved in `directory` (lats_topology_load, checked)."""
+        h = ctypes.c_void_p()
+        check(lib().lats_topology_load(os.fsencode(directory), ctypes.byref(h)))
+        t = cls.__new__(cls)
+        t._take(h)
+        return t
+
+    def save(self, directory):
+        """One raw array per file, the format lats_topology_save writes."""
+        for name, fname in self._FILES.items():
+            np.ascontiguousarray(getattr(self, name)).tofile(os.path.join(directory, fname + ".dat"))
+
+    def _take(self, h):
+        """Copies every array out of the library handle h and frees it."""
         try:
             c = np.zeros(4, np.uint32)
             check(lib().lats_topology_counts(h, _p(c)))

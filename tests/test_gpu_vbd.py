@@ -400,3 +400,27 @@ def test_flow_rejects_obstacles_and_oversized_grids(gpu_available):
     b.set_strategy(_lib.LATS_STRATEGY_FLOW, persistent_blocks=100000)
     with pytest.raises(_lib.LatsError):
         b.run_frames(1)
+
+
+def test_context_from_saved_topology_bit_exact(gpu_available, tmp_path):
+    """square8K: topology saved to disk and loaded back (lats_topology_save /
+    _load) drives a context whose frames equal the built topology's bit for
+    bit, and the oracle's."""
+    _print_case("square8k from a saved topology, 5 frames")
+    from libatssim_amd import Topology
+    x, f = square8k_mesh()
+    cloth = configs.square8k_cloth(x, f)
+    params = configs.square8k_params()
+    cloth.topology.save(str(tmp_path))
+    loaded = Cloth(cloth.x, cloth.faces, cloth.mass, cloth.fixed, topology=Topology.load(str(tmp_path)))
+    a = GpuVBD(cloth, params)
+    b = GpuVBD(loaded, params)
+    o = oracle_sim(cloth, params, threads=O.max_threads())
+    for _ in range(5):
+        a.physics_step()
+        b.physics_step()
+        o.step()
+    xa, va = a.state()
+    xb, vb = b.state()
+    assert np.array_equal(xa, xb) and np.array_equal(va, vb)
+    assert np.array_equal(xb, o.state()[0].astype(np.float32))
