@@ -284,13 +284,20 @@ def bench_mlp(args):
 
 
 def bench_warm_start(args, cloth, params, wind, plain_ms, lanes=8, regions=1):
-    """Learned warm start on the same cloth and wind: per frame, wall time of
-    the warm-started solve until the energy reaches the plain solver's
-    100-iteration energy from the same frame start (computed on a second
-    context, outside the timed call), against the plain frame graph."""
+    """Learned warm start on the same cloth and wind, against the plain frame
+    graph, per frame, measured two ways:
+
+    oracle_stopped: sweeps until the energy reaches the plain solver's
+      100-iteration energy from the same frame start, which a second context
+      computes beforehand (outside the timed call).  It shows how soon the
+      warm-started solve gets there; a deployed solver does not know that
+      energy.
+    deployable: the target-free stopping rule (energy decrease per sweep <=
+      tol * |E|) with tol calibrated on two training scenarios; the plain
+      energy is computed only to report how many frames reach it."""
     import numpy as np
     from libatssim_amd import configs
-    from libatssim_amd.warm_start import WarmStartMLP, evaluate, train_recipe
+    from libatssim_amd.warm_start import WarmStartMLP, calibrate_stop_rule, evaluate, evaluate_rule, train_recipe
     t0 = time.perf_counter()
     scen = [configs.warm_c3_scenario(k, n=256, frames=args.warm_frames) for k in range(args.warm_scenarios)]
     m = WarmStartMLP(2024)
@@ -304,23 +311,37 @@ def bench_warm_start(args, cloth, params, wind, plain_ms, lanes=8, regions=1):
     rounds = max(3, min(8, 400_000_000 // rows_per_pass - 1))
     train_recipe(m, scen, params, args.warm_frames, plain_steps=steps, guided_rounds=rounds, guided_steps=steps,
                  record_every=args.warm_record_every)
+    cal_frames = min(30, args.warm_frames)
+    tol, tried = calibrate_stop_rule(m, [configs.warm_c3_scenario(k, n=256, frames=cal_frames) for k in range(2)],
+                                     params, cal_frames, lanes=lanes, regions=regions)
     train_s = time.perf_counter() - t0
     full_wind = configs.grid_c1(frames=60)[3]
     frames = len(full_wind)
+
+    def summary(res):
+        wall = np.array([r["wall_ms"] for r in res])
+        sweeps = np.array([r["sweeps"] for r in res])
+        excess = np.array([(r["e_final"] - r["target"]) / abs(r["target"]) for r in res])
+        return {"ms_per_frame": round(float(wall.mean()), 4), "speedup": round(plain_ms / float(wall.mean()), 3),
+                "mean_sweeps": round(float(sweeps.mean()), 2), "max_sweeps": int(sweeps.max()),
+                "kept_fraction": round(float(np.mean([r["kept"] for r in res])), 3),
+                "frames_reaching_plain_energy": round(float(np.mean(excess <= 0)), 3),
+                "max_energy_excess": float(f"{excess.max():.3g}")}
+
     # the warm-started frames run the plain frame's colour-pass configuration
-    res = evaluate(m, cloth, params, full_wind, frames, lanes=lanes, regions=regions)
+    oracle = summary(evaluate(m, cloth, params, full_wind, frames, lanes=lanes, regions=regions))
+    oracle["stop"] = ("first check at or below the plain 100-iteration energy of the same frame start, computed "
+                      "beforehand on a second context (not known to a deployed solver)")
+    deploy = summary(evaluate_rule(m, cloth, params, full_wind, frames, tol, lanes=lanes, regions=regions))
+    deploy["stop"] = "energy decrease per sweep <= tol * |E| since the previous check (no reference solve)"
+    deploy["tol"] = tol
+    deploy["calibration"] = {"scenarios": 2, "frames": cal_frames, "candidates": tried}
     fwd_ms, step_ms = m.time_kernels(cloth.num_verts, 20)   # after the evaluation: timing steps train
-    wall = np.array([r["wall_ms"] for r in res])
-    sweeps = np.array([r["sweeps"] for r in res])
     return {
-        "ms_per_frame_learned": round(float(wall.mean()), 4),
         "ms_per_frame_plain": round(plain_ms, 4),
-        "speedup": round(plain_ms / float(wall.mean()), 3),
         "frames": frames,
-        "mean_sweeps": round(float(sweeps.mean()), 2),
-        "max_sweeps": int(sweeps.max()),
-        "kept_fraction": round(float(np.mean([r["kept"] for r in res])), 3),
-        "target": "per-frame energy of the plain 100-iteration solve from the same frame start",
+        "deployable": deploy,
+        "oracle_stopped": oracle,
         "mlp_forward": _mlp_roofline(cloth.num_verts, fwd_ms),
         "mlp_train_step_ms": round(step_ms, 4),
         "training": {"scenarios": args.warm_scenarios, "frames": args.warm_frames,
@@ -439,7 +460,8 @@ def bench_sphere_warm_start(args, cloth, params, obstacle, ct, plain_ms, scenari
         "ms_per_frame_plain_timed_window": round(plain_ms, 4),
         "mean_sweeps": round(float(sweeps.mean()), 2), "max_sweeps": int(sweeps.max()),
         "kept_fraction": round(float(np.mean([r["kept"] for r in res])), 3),
-        "target": "per-frame energy of the plain 50-iteration solve from the same frame start",
+        "stop": ("oracle-stopped: first check at or below the plain 50-iteration energy of the same frame start, "
+                 "computed beforehand on a second context (not known to a deployed solver)"),
         "training": {"scenarios": scenarios, "cloth": "384x384, 2 mm", "frames": frames, "record_every": 2,
                      "guided_rounds": 3, "adam_steps": 8000, "rows": m.dataset_rows(), "seconds": round(train_s, 1)},
     }

@@ -429,6 +429,15 @@ int lats_mlp_reset_history(lats_mlp* m);
  * spring 1-ring ((own + ring mean) / 2, pinned neighbours as zero), and, with
  * half_step != 0, retried at half length before the inertial fallback. */
 int lats_mlp_set_guess_options(lats_mlp* m, uint32_t smooth_passes, int half_step);
+/* Stopping rule of a warm-started frame that is given no target (target <= 0
+ * in lats_mlp_warm_start_frame, use_model != 0): from a kept guess, sweep
+ * until the energy decrease per sweep since the previous check (the first
+ * check compares with the guess's energy; a rise counts as no decrease) is
+ * <= rel_decrease * |energy|, at most max_sweeps; from the inertial guess,
+ * the plain iteration count (the plain frame).  No reference solve is
+ * involved, so a frame may stop above the energy the plain solve would
+ * reach.  0 turns it off (then such a frame runs max_sweeps); default 0. */
+int lats_mlp_set_stop_rule(lats_mlp* m, float rel_decrease);
 /* Training rows: one plain frame (predict, all iterations, update) of the
  * bound context, recording its features and its correction as rows. */
 int lats_mlp_reserve_rows(lats_mlp* m, uint64_t rows);
@@ -450,7 +459,8 @@ int lats_mlp_train(lats_mlp* m, uint32_t steps, uint32_t batch_rows, float lr, u
 int lats_mlp_time_kernels(lats_mlp* m, uint32_t rows, uint32_t iters, float* forward_ms, float* step_ms);
 /* One frame of the bound context: predict, learned guess (use_model != 0)
  * kept only if its energy is <= the inertial guess's, then colour sweeps
- * until the energy is <= target (target <= 0: run max_sweeps), at most
+ * until the energy is <= target (target <= 0: the stopping rule of
+ * lats_mlp_set_stop_rule, or max_sweeps when it is off), at most
  * max_sweeps; a kept guess gets at least first_check sweeps, then a check
  * every check_every; from the inertial guess the first check comes after
  * the plain iteration count; update.  out[6] = {sweeps, kept,

@@ -197,6 +197,7 @@ SIGNATURES = {
     "lats_mlp_gradient_host": (_I, [_VP, _VP, _VP, _U32, _VP, ctypes.POINTER(ctypes.c_double)]),
     "lats_mlp_bind": (_I, [_VP, _VP]),
     "lats_mlp_reset_history": (_I, [_VP]),
+    "lats_mlp_set_stop_rule": (_I, [_VP, ctypes.c_float]),
     "lats_mlp_set_guess_options": (_I, [_VP, _U32, _I]),
     "lats_mlp_reserve_rows": (_I, [_VP, ctypes.c_uint64]),
     "lats_mlp_record_frame": (_I, [_VP, _VP]),

@@ -590,6 +590,7 @@ struct lats_mlp {
     hipEvent_t ev_guess = nullptr;             // the guesses' energies are in host memory
     uint32_t smooth_passes = 0;                // 1-ring smoothing passes of the prediction
     int half_step = 0;                         // retry at half the prediction before falling back
+    float stop_rel = 0.f;                      // stopping rule without a target (0: off)
     DBuf<uint32_t> colour;
     // recorded training rows
     DBuf<uint16_t> dsX;
@@ -708,6 +709,12 @@ int copy_frame_state(lats_vbd* dst, lats_vbd* src) {
 // only if its energy is <= the inertial guess's; at least first_check colour
 // sweeps from a kept guess, then sweeps until the energy is <= target
 // (checked every check_every); history; update velocity.
+// Without a target (target <= 0) and with the stopping rule on
+// (m->stop_rel > 0): from a kept guess, sweeps until the energy decrease per
+// sweep since the previous check is <= stop_rel * |energy| (the first check,
+// after first_check sweeps, compares with the guess's energy; a rise counts
+// as no decrease); from the inertial guess, the plain iteration count (the
+// plain frame).  No reference solve is involved.
 // With ref: the target is the energy ref reaches after the full iteration
 // count from the same frame start, and the frame is recorded as training
 // rows (features of c, correction of ref's solve).
@@ -743,7 +750,8 @@ int warm_frame(lats_mlp* m, lats_vbd* c, lats_vbd* ref, float target, uint32_t m
                            m->dsT.p + m->ds_rows * 4, V, 1.f / m->s0);
         HIPCHK(hipGetLastError());
     }
-    const bool checks = target > 0.f;
+    const bool rule = !(target > 0.f) && !ref && m->stop_rel > 0.f && use_model;
+    const bool checks = target > 0.f || rule;
     if (use_model) {
         // learned guess in x (and half of it in xhalf), then the energies of
         // the inertial guess, the learned guess and its half with one host
@@ -801,6 +809,8 @@ int warm_frame(lats_mlp* m, lats_vbd* c, lats_vbd* ref, float target, uint32_t m
         e_warm = e_pred;
     }
     float e = kept ? e_warm : e_pred;
+    float e_prev = e;             // energy at the previous check (the rule)
+    uint32_t s_prev = 0;          // sweeps at the previous check
     uint32_t sweeps = spec;
     if (spec && checks) {
         rc = lats_vbd_energy_now(c, &e);
@@ -811,10 +821,20 @@ int warm_frame(lats_mlp* m, lats_vbd* c, lats_vbd* ref, float target, uint32_t m
     // every check_every.  From the inertial guess the frame is the plain
     // solve, so its first check comes after the plain iteration count.
     const uint32_t first = kept ? std::max(first_check, 1u) : std::max(c->prm.constraint_iter_count, 1u);
-    while (sweeps < max_sweeps && !(checks && e <= target && sweeps >= (kept ? first_check : 0u))) {
+    auto done = [&]() {
+        if (!rule) return checks && e <= target && sweeps >= (kept ? first_check : 0u);
+        if (!kept) return sweeps >= c->prm.constraint_iter_count;   // the plain frame
+        if (sweeps < std::max(first_check, 1u) || sweeps == s_prev) return false;
+        return (e_prev - e) / (float)(sweeps - s_prev) <= m->stop_rel * std::fabs(e);
+    };
+    while (sweeps < max_sweeps && !done()) {
         const uint32_t k = std::min(sweeps == 0 ? first : check_every, max_sweeps - sweeps);
         rc = lats_vbd_iterate(c, k);
         if (rc) return rc;
+        if (checks) {
+            e_prev = e;
+            s_prev = sweeps;
+        }
         sweeps += k;
         if (checks) {
             rc = lats_vbd_energy_now(c, &e);
@@ -1020,6 +1040,14 @@ int lats_mlp_set_guess_options(lats_mlp* m, uint32_t smooth_passes, int half_ste
     if (smooth_passes > 8) return fail(LATS_ERR_INVALID_ARGUMENT, "at most 8 smoothing passes");
     m->smooth_passes = smooth_passes;
     m->half_step = half_step != 0;
+    return LATS_OK;
+}
+
+int lats_mlp_set_stop_rule(lats_mlp* m, float rel_decrease) {
+    if (!m) return fail(LATS_ERR_INVALID_ARGUMENT, "null argument");
+    if (!(rel_decrease >= 0.f) || !std::isfinite(rel_decrease))
+        return fail(LATS_ERR_INVALID_ARGUMENT, "the stopping tolerance must be finite and >= 0");
+    m->stop_rel = rel_decrease;
     return LATS_OK;
 }
 

@@ -143,6 +143,11 @@ class WarmStartMLP:
         """1-ring smoothing passes of the prediction; half-length retry before the fallback."""
         check(lib().lats_mlp_set_guess_options(self._h, int(smooth_passes), int(bool(half_step))))
 
+    def set_stop_rule(self, rel_decrease):
+        """Target-free stopping rule (0 turns it off): a kept guess sweeps
+        until the energy decrease per sweep falls to rel_decrease * |energy|."""
+        check(lib().lats_mlp_set_stop_rule(self._h, float(rel_decrease)))
+
     def reset_history(self):
         check(lib().lats_mlp_reset_history(self._h))
 
@@ -323,10 +328,15 @@ def _train_chunks(mlp, steps, batch_rows, lr, seed, chunk=250):
 
 
 def evaluate(mlp, cloth, params, wind, frames, first_check=FIRST_CHECK, check_every=CHECK_EVERY, use_model=True,
-             lanes=8, max_sweeps=None, setup=None, regions=None):
+             lanes=8, max_sweeps=None, setup=None, regions=None, stop_rule=None):
     """Frames of `cloth` solved from the learned guess until each frame's
     energy reaches the plain solver's energy after its full iteration count
     from the same frame start (computed on a second context).
+
+    stop_rule: tolerance of the target-free stopping rule
+    (lats_mlp_set_stop_rule).  The frames then never see the plain target:
+    they stop by the rule, and the target is only reported beside the result
+    (computed before the frame, outside its timing) to check it.
 
     lanes / regions: colour-pass configuration of the warm-started context
     (results are identical for every choice; regions None keeps the default).
@@ -360,10 +370,42 @@ def evaluate(mlp, cloth, params, wind, frames, first_check=FIRST_CHECK, check_ev
         target = ref.energy_now()
         run.synchronize()
         t0 = time.perf_counter()
-        r = mlp.warm_start_frame(run, target, max_sweeps, first_check, check_every, use_model)
+        r = mlp.warm_start_frame(run, 0.0 if stop_rule else target, max_sweeps, first_check, check_every,
+                                 use_model)
         r["wall_ms"] = (time.perf_counter() - t0) * 1e3
+        if stop_rule:
+            r["rule_target"] = r["target"]     # the frame's own estimate
         r["target"] = target
         out.append(r)
     run.close()
     ref.close()
     return out
+
+
+def evaluate_rule(mlp, cloth, params, wind, frames, rel_tol, **kw):
+    """evaluate() with the target-free stopping rule at rel_tol."""
+    mlp.set_stop_rule(rel_tol)
+    try:
+        return evaluate(mlp, cloth, params, wind, frames, stop_rule=rel_tol, **kw)
+    finally:
+        mlp.set_stop_rule(0.0)
+
+
+STOP_RULE_CANDIDATES = (1e-4, 3e-5, 1e-5, 3e-6, 1e-6)
+
+
+def calibrate_stop_rule(mlp, scenarios, params, frames, candidates=STOP_RULE_CANDIDATES, **kw):
+    """Tolerance of the target-free stopping rule, chosen on training
+    `scenarios` ((cloth, wind) pairs) only: the loosest candidate whose share
+    of frames reaching their plain target is within one percentage point of
+    the best candidate's.  Returns (tolerance, per-candidate summaries)."""
+    import numpy as np
+    tried = []
+    for tol in sorted(candidates, reverse=True):
+        res = []
+        for cloth, wind in scenarios:
+            res += evaluate_rule(mlp, cloth, params, wind, frames, tol, **kw)
+        tried.append({"tol": tol, "reached": float(np.mean([r["e_final"] <= r["target"] for r in res])),
+                      "mean_sweeps": float(np.mean([r["sweeps"] for r in res]))})
+    best = max(t["reached"] for t in tried)
+    return next(t["tol"] for t in tried if t["reached"] >= best - 0.01), tried

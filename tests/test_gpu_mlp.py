@@ -314,3 +314,52 @@ def test_guided_recording_is_deterministic(gpu_available):
     assert [(r["sweeps"], r["kept"], r["e_final"], r["target"]) for r in s0] == \
            [(r["sweeps"], r["kept"], r["e_final"], r["target"]) for r in s1]
     assert np.array_equal(X0, X1) and np.array_equal(T0, T1)
+
+
+def test_stop_rule_fallback_is_the_plain_frame(gpu_available):
+    """With the target-free rule on, a rejected guess runs the plain frame."""
+    _case("stop rule: rejected guesses run the plain frame")
+    cloth, params, wind = _small_scene()
+    m = WarmStartMLP(1)
+    p = m.params()
+    W4 = p[OFF_W4:OFF_B4].reshape(128, 16)
+    W4[:, :3] = 1e3 * np.sign(W4[:, :3])
+    m.set_params(p)
+    m.set_stop_rule(1e-5)
+    run = GpuVBD(cloth, params, _lib.LATS_STRATEGY_GRAPH)
+    plain = GpuVBD(cloth, params, _lib.LATS_STRATEGY_GRAPH)
+    m.bind(run)
+    for f in range(3):
+        run.set_external_force(wind[f])
+        plain.set_external_force(wind[f])
+        r = m.warm_start_frame(run, 0.0, 3 * params.constraint_iter_count, 4, 2, True)
+        plain.physics_step()
+        assert not r["kept"] and r["sweeps"] == params.constraint_iter_count
+        assert np.array_equal(run.state()[0], plain.state()[0]), f
+
+
+def test_stop_rule_needs_no_target(gpu_available):
+    """A trained network with the rule on: frames given no target stop
+    between the first check and the cap (rejected guesses: the plain count),
+    and from the same frame start a looser tolerance stops no later (its
+    condition holds wherever the tighter one does)."""
+    _case("stop rule without a target (24x24, 3 scenarios x 30 frames)")
+    from libatssim_amd.warm_start import evaluate_rule
+    n, iters = 24, 60
+    params = Params(constraint_iter_count=iters)
+    scen = [configs.warm_c3_scenario(k, n=n, frames=30) for k in range(3)]
+    m = WarmStartMLP(2024)
+    train_recipe(m, scen, params, 30, plain_steps=800, guided_rounds=1, guided_steps=800, batch_rows=4096)
+    cloth, wind = configs.warm_c3_scenario(7, n=n, frames=20)
+    loose = evaluate_rule(m, cloth, params, wind, 20, 1e-3, max_sweeps=150)
+    tight = evaluate_rule(m, cloth, params, wind, 20, 1e-7, max_sweeps=150)
+    for res in (loose, tight):
+        for r in res:
+            assert r["sweeps"] <= 150
+            if r["kept"]:
+                assert r["sweeps"] >= 10
+            else:
+                assert r["sweeps"] == iters
+    assert sum(r["kept"] for r in loose) > 0
+    assert np.mean([r["sweeps"] for r in loose]) < 150
+    assert loose[0]["sweeps"] <= tight[0]["sweeps"]
