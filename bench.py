@@ -131,6 +131,8 @@ def main():
     ap.add_argument("--tiles", type=int, default=0, help="flow strategy: vertex tiles (0 = default)")
     ap.add_argument("--lanes", type=int, default=0, help="lanes per vertex (1, 4, 8, 16); 0 = autotuned")
     ap.add_argument("--regions", type=int, default=0, help="colour-pass cloth bands (1..8); 0 = autotuned")
+    ap.add_argument("--colour-block", type=int, default=0, help="colour-pass block threads (64, 128, 256); "
+                                                                 "0 = autotuned")
     ap.add_argument("--no-warm-start", action="store_true", help="skip the learned warm-start measurement")
     ap.add_argument("--warm-scenarios", type=int, default=8, help="training scenarios of the warm start")
     ap.add_argument("--warm-frames", type=int, default=60, help="frames per training scenario")
@@ -174,6 +176,8 @@ def main():
         strategy, blocks = g.strategy()
     if args.regions:
         g.set_regions(args.regions)
+    if args.colour_block:
+        g.set_colour_block(args.colour_block)
     if W:
         g.run_frames(W, wind[:W])
     g.synchronize()
@@ -243,6 +247,7 @@ def main():
             "persistent_blocks_or_tiles": blocks,
             "lanes_per_vertex": g.lanes_per_vertex(),
             "regions": g.regions(),
+            "colour_block_threads": g.colour_block(),
             "gpu_ms_per_step": gpu_ms / K,
         },
         "roofline": roofline,
@@ -251,7 +256,8 @@ def main():
         out["cpu_baseline"] = cpu_baseline(cloth, params, wind)
     if not args.no_warm_start:
         out["warm_start"] = bench_warm_start(args, cloth, params, wind, plain_ms=gpu_ms / K,
-                                             lanes=g.lanes_per_vertex(), regions=g.regions())
+                                             lanes=g.lanes_per_vertex(), regions=g.regions(),
+                                             colour_block=g.colour_block())
     print(json.dumps(out), flush=True)
 
 
@@ -283,7 +289,7 @@ def bench_mlp(args):
                       "train_step_ms": round(step_ms, 4), "iters": args.steps}), flush=True)
 
 
-def bench_warm_start(args, cloth, params, wind, plain_ms, lanes=8, regions=1):
+def bench_warm_start(args, cloth, params, wind, plain_ms, lanes=8, regions=1, colour_block=256):
     """Learned warm start on the same cloth and wind, against the plain frame
     graph, per frame, measured two ways:
 
@@ -313,7 +319,7 @@ def bench_warm_start(args, cloth, params, wind, plain_ms, lanes=8, regions=1):
                  record_every=args.warm_record_every)
     cal_frames = min(30, args.warm_frames)
     tol, tried = calibrate_stop_rule(m, [configs.warm_c3_scenario(k, n=256, frames=cal_frames) for k in range(2)],
-                                     params, cal_frames, lanes=lanes, regions=regions)
+                                     params, cal_frames, lanes=lanes, regions=regions, colour_block=colour_block)
     train_s = time.perf_counter() - t0
     full_wind = configs.grid_c1(frames=60)[3]
     frames = len(full_wind)
@@ -329,10 +335,12 @@ def bench_warm_start(args, cloth, params, wind, plain_ms, lanes=8, regions=1):
                 "max_energy_excess": float(f"{excess.max():.3g}")}
 
     # the warm-started frames run the plain frame's colour-pass configuration
-    oracle = summary(evaluate(m, cloth, params, full_wind, frames, lanes=lanes, regions=regions))
+    oracle = summary(evaluate(m, cloth, params, full_wind, frames, lanes=lanes, regions=regions,
+                              colour_block=colour_block))
     oracle["stop"] = ("first check at or below the plain 100-iteration energy of the same frame start, computed "
                       "beforehand on a second context (not known to a deployed solver)")
-    deploy = summary(evaluate_rule(m, cloth, params, full_wind, frames, tol, lanes=lanes, regions=regions))
+    deploy = summary(evaluate_rule(m, cloth, params, full_wind, frames, tol, lanes=lanes, regions=regions,
+                                   colour_block=colour_block))
     deploy["stop"] = "energy decrease per sweep <= tol * |E| since the previous check (no reference solve)"
     deploy["tol"] = tol
     deploy["calibration"] = {"scenarios": 2, "frames": cal_frames, "candidates": tried}

@@ -184,20 +184,26 @@ int lats_vbd_get_lanes_per_vertex(lats_vbd* ctx, uint32_t* lanes);
  * identical for every value; default 1. */
 int lats_vbd_set_regions(lats_vbd* ctx, uint32_t regions);
 int lats_vbd_get_regions(lats_vbd* ctx, uint32_t* regions);
+/* Threads per block of the colour passes (graph strategy): 64, 128 or 256
+ * (default).  A colour pass of a small cloth has few waves; smaller blocks
+ * spread them more evenly over the CUs.  Results are identical. */
+int lats_vbd_set_colour_block(lats_vbd* ctx, uint32_t threads);
+int lats_vbd_get_colour_block(lats_vbd* ctx, uint32_t* threads);
 /* Grid size of the persistent strategy (0: one thread per vertex of the
  * largest colour, capped at the co-resident limit); for the flow strategy
  * the number of vertex tiles, one block each (0: min(256, V / 128)). */
 int lats_vbd_set_persistent_blocks(lats_vbd* ctx, uint32_t blocks);
 int lats_vbd_get_strategy(lats_vbd* ctx, int* strategy, uint32_t* persistent_blocks);
 /* Autotuner: times `frames` frames with the graph strategy (1 or 8 cloth
- * bands), the persistent strategy at several grid sizes and the flow
+ * bands, 64 / 128 / 256-thread blocks), the persistent strategy at several grid sizes and the flow
  * strategy at several tile counts, for 1/4/8/16 lanes per vertex, in two
  * passes (a candidate's best time counts), keeps the fastest for later
  * lats_vbd_physics_step / lats_vbd_run_frames calls, and restores the state. */
 int lats_vbd_autotune(lats_vbd* ctx, uint32_t frames, int* chosen_strategy, uint32_t* chosen_blocks);
-/* What the last lats_vbd_autotune measured: *count candidates; out (5 floats
+/* What the last lats_vbd_autotune measured: *count candidates; out (6 floats
  * per candidate, min(cap, count) copied) = {strategy, persistent blocks or
- * flow tiles, lanes per vertex, bands, best milliseconds for `frames` frames}. */
+ * flow tiles, lanes per vertex, bands, best milliseconds for `frames` frames,
+ * colour-pass block threads}. */
 int lats_vbd_autotune_results(lats_vbd* ctx, float* out, uint32_t cap, uint32_t* count);
 /* Include the bending term in the recorded energy (the reference's GPU
  * energy does, example3.metal:369-395; its CPU energy drops it,

@@ -136,6 +136,8 @@ SIGNATURES = {
     "lats_vbd_get_solver": (_I, [_VP, ctypes.POINTER(_I)]),
     "lats_vbd_xpbd_clusters": (_I, [_VP, _I, _VP, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)]),
     "lats_vbd_get_regions": (_I, [_VP, ctypes.POINTER(_U32)]),
+    "lats_vbd_set_colour_block": (_I, [_VP, _U32]),
+    "lats_vbd_get_colour_block": (_I, [_VP, ctypes.POINTER(_U32)]),
     "lats_vbd_frame_begin": (_I, [_VP]),
     "lats_vbd_iterate": (_I, [_VP, _U32]),
     "lats_vbd_frame_end": (_I, [_VP]),

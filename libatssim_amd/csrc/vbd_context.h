@@ -55,6 +55,7 @@ struct lats_vbd {
     std::vector<lats::ColourInfo> colours;   // host copy of colours_d
     std::vector<lats::RegionSplit> region_split;   // per colour (colour-pass block mapping)
     uint32_t regions = 1;
+    uint32_t colour_block = 256;   // threads per block of the colour passes (64, 128, 256)
     std::vector<uint32_t> perm, inv;
     int strategy = LATS_STRATEGY_AUTO;
     int energy_bending = 1;

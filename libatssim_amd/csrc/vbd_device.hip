@@ -52,20 +52,36 @@ constexpr int kBlock = 256;
 // XCDs round-robin, so with n = 8 each XCD updates the same band of the cloth
 // in every colour pass and the neighbours it gathers are mostly lines its own
 // L2 already holds; n = 1 is the plain linear mapping.
-template <int G>
-__global__ __launch_bounds__(kBlock) void k_vbd_colour(VbdArgs a, ColourInfo ci, RegionSplit rs) {
+// B threads per block (64, 128 or 256): smaller blocks spread a small colour
+// pass's waves more evenly over the CUs (results are identical).
+template <int G, int B>
+__global__ __launch_bounds__(B) void k_vbd_colour(VbdArgs a, ColourInfo ci, RegionSplit rs) {
+    // Every kernel argument is named here first (no instruction), so all of
+    // them arrive in one scalar round trip: the argument block is otherwise
+    // loaded only past the early exit below, after the region bounds.
+    asm volatile("" ::"s"(a.x), "s"(a.x0), "s"(a.v), "s"(a.mass), "s"(a.meta), "s"(a.ell_e), "s"(a.ell_bq),
+                 "s"(a.ell_bi), "s"(a.fp), "s"(ci.start), "s"(ci.count), "s"(ci.eoff), "s"(ci.boff), "s"(ci.ew),
+                 "s"(ci.bw), "s"(rs.n), "s"(rs.bound[kMaxRegions]));
     const uint32_t r = blockIdx.x % rs.n, j = blockIdx.x / rs.n;
-    const uint32_t lo = rs.bound[r], hi = rs.bound[r + 1];
+    // the region's bounds selected from the whole table (an indexed read of
+    // rs.bound[r] is a second round trip, after rs.n)
+    uint32_t lo = rs.bound[0], hi = rs.bound[1];
+#pragma unroll
+    for (uint32_t q = 1; q < kMaxRegions; q++)
+        if (q == r) {
+            lo = rs.bound[q];
+            hi = rs.bound[q + 1];
+        }
     const F3 ext = F3{a.fp->ext_cur[0], a.fp->ext_cur[1], a.fp->ext_cur[2]};
     const uint32_t n = ci.count;
     if (G == 1) {
-        const uint32_t i = lo + j * kBlock + threadIdx.x;
+        const uint32_t i = lo + j * B + threadIdx.x;
         if (i >= hi) return;
         vbd_vertex_ext(a, ci.start + i, i, n, ci.eoff, ci.boff, ext);
     } else {
-        const uint32_t first = lo + j * (kBlock / G) + (threadIdx.x / 64) * (64 / G);
+        const uint32_t first = lo + j * (B / G) + (threadIdx.x / 64) * (64 / G);
         if (first >= hi) return;                  // the whole wave is past the region
-        const uint32_t i = lo + j * (kBlock / G) + threadIdx.x / G, lane = threadIdx.x % G;
+        const uint32_t i = lo + j * (B / G) + threadIdx.x / G, lane = threadIdx.x % G;
         const bool valid = i < hi;
         vbd_vertex_group<G>(a, ci.start + (valid ? i : lo), valid ? i : lo, n, ci.eoff, ci.boff, ci.ew, ci.bw, ext,
                             lane, valid);
@@ -835,19 +851,28 @@ void split_regions(lats_vbd* c, uint32_t n) {
     c->regions = n;
 }
 
+template <int B>
+void launch_colour(lats_vbd* c, const VbdArgs& a, const ColourInfo& ci, const RegionSplit& rs, uint32_t most) {
+    const uint32_t per_block = B / c->lanes;
+    const dim3 grid(std::max(1u, rs.n * ((most + per_block - 1) / per_block))), block(B);
+    switch (c->lanes) {
+        case 4: hipLaunchKernelGGL((k_vbd_colour<4, B>), grid, block, 0, c->stream, a, ci, rs); break;
+        case 8: hipLaunchKernelGGL((k_vbd_colour<8, B>), grid, block, 0, c->stream, a, ci, rs); break;
+        case 16: hipLaunchKernelGGL((k_vbd_colour<16, B>), grid, block, 0, c->stream, a, ci, rs); break;
+        default: hipLaunchKernelGGL((k_vbd_colour<1, B>), grid, block, 0, c->stream, a, ci, rs); break;
+    }
+}
+
 void enqueue_colour(lats_vbd* c, float* xb, uint32_t col) {
     const RegionSplit& rs = c->region_split[col];
     uint32_t most = 0;
     for (uint32_t r = 0; r < rs.n; r++) most = std::max(most, rs.bound[r + 1] - rs.bound[r]);
-    const uint32_t per_block = kBlock / c->lanes;
-    const dim3 grid(std::max(1u, rs.n * ((most + per_block - 1) / per_block))), block(kBlock);
     const VbdArgs a = c->args(xb);
     const ColourInfo& ci = c->colours[col];
-    switch (c->lanes) {
-        case 4: hipLaunchKernelGGL(k_vbd_colour<4>, grid, block, 0, c->stream, a, ci, rs); break;
-        case 8: hipLaunchKernelGGL(k_vbd_colour<8>, grid, block, 0, c->stream, a, ci, rs); break;
-        case 16: hipLaunchKernelGGL(k_vbd_colour<16>, grid, block, 0, c->stream, a, ci, rs); break;
-        default: hipLaunchKernelGGL(k_vbd_colour<1>, grid, block, 0, c->stream, a, ci, rs); break;
+    switch (c->colour_block) {
+        case 64: launch_colour<64>(c, a, ci, rs, most); break;
+        case 128: launch_colour<128>(c, a, ci, rs, most); break;
+        default: launch_colour<256>(c, a, ci, rs, most); break;
     }
 }
 
@@ -1536,6 +1561,23 @@ int lats_vbd_set_regions(lats_vbd* c, uint32_t regions) {
     return LATS_OK;
 }
 
+int lats_vbd_set_colour_block(lats_vbd* c, uint32_t threads) {
+    if (!c) return fail(LATS_ERR_INVALID_ARGUMENT, "null context");
+    if (threads != 64 && threads != 128 && threads != 256)
+        return fail(LATS_ERR_INVALID_ARGUMENT, "colour-pass blocks must have 64, 128 or 256 threads");
+    if (threads != c->colour_block) {
+        c->colour_block = threads;
+        c->drop_graphs();
+    }
+    return LATS_OK;
+}
+
+int lats_vbd_get_colour_block(lats_vbd* c, uint32_t* threads) {
+    if (!c || !threads) return fail(LATS_ERR_INVALID_ARGUMENT, "null argument");
+    *threads = c->colour_block;
+    return LATS_OK;
+}
+
 int lats_vbd_get_regions(lats_vbd* c, uint32_t* regions) {
     if (!c || !regions) return fail(LATS_ERR_INVALID_ARGUMENT, "null argument");
     *regions = c->regions;
@@ -1594,12 +1636,14 @@ int lats_vbd_autotune(lats_vbd* c, uint32_t frames, int* chosen, uint32_t* chose
         HIPCHK(hipStreamSynchronize(c->stream));
         return LATS_OK;
     };
-    struct Cand { int strategy; uint32_t blocks; uint32_t lanes; uint32_t regions = 1; };
+    struct Cand { int strategy; uint32_t blocks; uint32_t lanes; uint32_t regions = 1; uint32_t threads = 256; };
     std::vector<Cand> cands;
-    const uint32_t keep_lanes = c->lanes, keep_regions = c->regions;
+    const uint32_t keep_lanes = c->lanes, keep_regions = c->regions, keep_threads = c->colour_block;
     for (uint32_t lanes : {1u, 4u, 8u, 16u}) {
-        cands.push_back({LATS_STRATEGY_GRAPH, 0, lanes, 1});
-        cands.push_back({LATS_STRATEGY_GRAPH, 0, lanes, kMaxRegions});
+        for (uint32_t threads : {64u, 128u, 256u}) {
+            cands.push_back({LATS_STRATEGY_GRAPH, 0, lanes, 1, threads});
+            cands.push_back({LATS_STRATEGY_GRAPH, 0, lanes, kMaxRegions, threads});
+        }
         if (c->prm.num_substep != 1) continue;
         uint32_t g0 = 0;
         const uint32_t keep = c->persist_blocks;
@@ -1623,9 +1667,10 @@ int lats_vbd_autotune(lats_vbd* c, uint32_t frames, int* chosen, uint32_t* chose
     Cand win = cands[0];
     const int keep_strategy = c->strategy;
     const uint32_t keep_blocks = c->persist_blocks;
-    auto use = [&](uint32_t lanes, uint32_t regions) {
-        if (c->lanes != lanes || c->regions != regions) {
+    auto use = [&](uint32_t lanes, uint32_t regions, uint32_t threads) {
+        if (c->lanes != lanes || c->regions != regions || c->colour_block != threads) {
             c->lanes = lanes;
+            c->colour_block = threads;
             if (c->regions != regions) split_regions(c, regions);
             c->drop_graphs();
         }
@@ -1639,13 +1684,13 @@ int lats_vbd_autotune(lats_vbd* c, uint32_t frames, int* chosen, uint32_t* chose
             const Cand& k = cands[i];
             c->strategy = k.strategy;
             c->persist_blocks = k.blocks;
-            use(k.lanes, k.regions);
+            use(k.lanes, k.regions, k.threads);
             int rc = lats_vbd_run_frames(c, 1, nullptr);   // warm-up / capture
             if (!rc) rc = lats_vbd_run_frames(c, frames, nullptr);
             if (rc) {
                 c->strategy = keep_strategy;
                 c->persist_blocks = keep_blocks;
-                use(keep_lanes, keep_regions);
+                use(keep_lanes, keep_regions, keep_threads);
                 restore();
                 return rc;
             }
@@ -1661,12 +1706,13 @@ int lats_vbd_autotune(lats_vbd* c, uint32_t frames, int* chosen, uint32_t* chose
             win = cands[i];
         }
         const Cand& k = cands[i];
-        for (float v : {(float)k.strategy, (float)k.blocks, (float)k.lanes, (float)k.regions, best_ms[i]})
+        for (float v : {(float)k.strategy, (float)k.blocks, (float)k.lanes, (float)k.regions, best_ms[i],
+                        (float)k.threads})
             c->tune_results.push_back(v);
     }
     c->strategy = win.strategy;
     c->persist_blocks = win.blocks;
-    use(win.lanes, win.regions);
+    use(win.lanes, win.regions, win.threads);
     if (chosen) *chosen = win.strategy;
     if (chosen_blocks) *chosen_blocks = win.blocks;
     return LATS_OK;
@@ -1674,9 +1720,9 @@ int lats_vbd_autotune(lats_vbd* c, uint32_t frames, int* chosen, uint32_t* chose
 
 int lats_vbd_autotune_results(lats_vbd* c, float* out, uint32_t cap, uint32_t* count) {
     if (!c || !count) return fail(LATS_ERR_INVALID_ARGUMENT, "null argument");
-    const uint32_t n = (uint32_t)(c->tune_results.size() / 5);
+    const uint32_t n = (uint32_t)(c->tune_results.size() / 6);
     *count = n;
-    if (out) std::memcpy(out, c->tune_results.data(), std::min(cap, n) * 5 * sizeof(float));
+    if (out) std::memcpy(out, c->tune_results.data(), std::min(cap, n) * 6 * sizeof(float));
     return LATS_OK;
 }
 

@@ -223,41 +223,52 @@ __device__ __forceinline__ void vbd_vertex_group(const VbdArgs& a, uint32_t s, u
                                                  bool valid) {
     constexpr uint32_t PE = (8 + G - 1) / G;    // spring rounds prefetched (degrees <= 8)
     constexpr uint32_t PB = (16 + G - 1) / G;   // pair rounds prefetched (degrees <= 16)
+    // Every load below is unconditional, from an address that is always
+    // valid: ELL rows past the colour's width are clamped to its last row
+    // (a colour without springs / pairs reads entry 0), and padding entries
+    // hold slot 0.  Values a lane does not own are masked where they are
+    // used.  A conditional load (cond ? load : 0) makes the compiler wait for
+    // it before the next one, which serialises the prefetch rounds.
     uint2 pe[PE];
     float4 pq[PB];
     uint4 pi[PB];
 #pragma unroll
     for (uint32_t k = 0; k < PE; k++) {
         const uint32_t j = k * G + lane;
-        pe[k] = (valid && j < ew) ? a.ell_e[eoff + j * n + i] : uint2{0u, 0u};
+        const uint32_t row = j < ew ? j : ew - 1;
+        pe[k] = a.ell_e[ew ? eoff + row * n + i : 0u];
     }
 #pragma unroll
     for (uint32_t k = 0; k < PB; k++) {
         const uint32_t j = k * G + lane;
-        const bool on = valid && j < bw;
-        pq[k] = on ? a.ell_bq[boff + j * n + i] : float4{0.f, 0.f, 0.f, 0.f};
-        pi[k] = on ? a.ell_bi[boff + j * n + i] : uint4{0u, 0u, 0u, 0u};
+        const uint32_t row = j < bw ? j : bw - 1;
+        const uint32_t at = bw ? boff + row * n + i : 0u;
+        pq[k] = a.ell_bq[at];
+        pi[k] = a.ell_bi[at];
     }
     const F3 xk = Mem::ld(a.x, s);
     const F3 x0 = Mem::ld(a.x0, s);
     const F3 v0 = Mem::ld(a.v, s);
     const float m = a.mass[s];
-    const uint32_t meta = valid ? a.meta[s] : 0u;
+    const uint32_t meta_raw = a.meta[s];
+    const uint32_t meta = valid ? meta_raw : 0u;
     const uint32_t edeg = meta & 0xffu, bdeg = (meta >> 8) & 0xffu;
 
-    // neighbour gathers of the prefetched rounds
+    // neighbour gathers of the prefetched rounds (lanes past the degree load
+    // a valid slot and mask the term)
     F3 pxo[PE];
 #pragma unroll
-    for (uint32_t k = 0; k < PE; k++) pxo[k] = (k * G + lane < edeg) ? Mem::ld(a.x, pe[k].x) : F3{0.f, 0.f, 0.f};
+    for (uint32_t k = 0; k < PE; k++) pxo[k] = Mem::ld(a.x, pe[k].x);
     F3 pp[PB][4];
 #pragma unroll
     for (uint32_t k = 0; k < PB; k++) {
-        const bool on = k * G + lane < bdeg;
         const uint32_t o = pi[k].x >> 30;
-        pp[k][0] = on ? pair_point<Mem>(a, xk, o, 0, pi[k].x & 0x3fffffffu) : xk;
-        pp[k][1] = on ? pair_point<Mem>(a, xk, o, 1, pi[k].y) : xk;
-        pp[k][2] = on ? pair_point<Mem>(a, xk, o, 2, pi[k].z) : xk;
-        pp[k][3] = on ? pair_point<Mem>(a, xk, o, 3, pi[k].w) : xk;
+        const F3 q0 = Mem::ld(a.x, pi[k].x & 0x3fffffffu), q1 = Mem::ld(a.x, pi[k].y);
+        const F3 q2 = Mem::ld(a.x, pi[k].z), q3 = Mem::ld(a.x, pi[k].w);
+        pp[k][0] = (o == 0) ? xk : q0;
+        pp[k][1] = (o == 1) ? xk : q1;
+        pp[k][2] = (o == 2) ? xk : q2;
+        pp[k][3] = (o == 3) ? xk : q3;
     }
 
     Sym3 H;

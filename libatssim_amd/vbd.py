@@ -269,6 +269,14 @@ class GpuVBD:
         """Bands of the colour-pass block mapping (1..8, results identical)."""
         check(lib().lats_vbd_set_regions(self._h, int(regions)))
 
+    def set_colour_block(self, threads):
+        check(lib().lats_vbd_set_colour_block(self._h, int(threads)))
+
+    def colour_block(self):
+        t = ctypes.c_uint32()
+        check(lib().lats_vbd_get_colour_block(self._h, ctypes.byref(t)))
+        return t.value
+
     def regions(self):
         n = ctypes.c_uint32()
         check(lib().lats_vbd_get_regions(self._h, ctypes.byref(n)))
@@ -295,11 +303,12 @@ class GpuVBD:
         return st.value, nb.value
 
     def autotune_results(self):
-        """The last autotune's candidates: float array [n, 5] of (strategy,
-        blocks or tiles, lanes, bands, best ms for the timed frames)."""
+        """The last autotune's candidates: float array [n, 6] of (strategy,
+        blocks or tiles, lanes, bands, best ms for the timed frames, colour
+        block threads)."""
         n = ctypes.c_uint32()
         check(lib().lats_vbd_autotune_results(self._h, None, 0, ctypes.byref(n)))
-        out = np.zeros((n.value, 5), np.float32)
+        out = np.zeros((n.value, 6), np.float32)
         check(lib().lats_vbd_autotune_results(self._h, _p(out), n.value, ctypes.byref(n)))
         return out
 

@@ -328,7 +328,7 @@ def _train_chunks(mlp, steps, batch_rows, lr, seed, chunk=250):
 
 
 def evaluate(mlp, cloth, params, wind, frames, first_check=FIRST_CHECK, check_every=CHECK_EVERY, use_model=True,
-             lanes=8, max_sweeps=None, setup=None, regions=None, stop_rule=None):
+             lanes=8, max_sweeps=None, setup=None, regions=None, stop_rule=None, colour_block=None):
     """Frames of `cloth` solved from the learned guess until each frame's
     energy reaches the plain solver's energy after its full iteration count
     from the same frame start (computed on a second context).
@@ -338,8 +338,9 @@ def evaluate(mlp, cloth, params, wind, frames, first_check=FIRST_CHECK, check_ev
     they stop by the rule, and the target is only reported beside the result
     (computed before the frame, outside its timing) to check it.
 
-    lanes / regions: colour-pass configuration of the warm-started context
-    (results are identical for every choice; regions None keeps the default).
+    lanes / regions / colour_block: colour-pass configuration of the
+    warm-started context (results are identical for every choice; None keeps
+    the default).
 
     Returns per-frame dicts (sweeps, kept, energies, target, wall_ms)."""
     import time
@@ -354,6 +355,9 @@ def evaluate(mlp, cloth, params, wind, frames, first_check=FIRST_CHECK, check_ev
     if regions:
         run.set_regions(regions)
         ref.set_regions(regions)
+    if colour_block:
+        run.set_colour_block(colour_block)
+        ref.set_colour_block(colour_block)
     if setup:
         setup(run)
         setup(ref)

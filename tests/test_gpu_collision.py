@@ -66,14 +66,16 @@ def test_contact_energy_matches_oracle(gpu_available):
     assert np.abs(eg.astype(np.float64) - eo).max() / np.abs(eo).max() <= 1e-4
 
 
-@pytest.mark.parametrize("lanes", [4, 8, 16])
-def test_lanes_per_vertex_bit_exact(gpu_available, lanes):
-    """G lanes per vertex (terms summed in incidence order) == one thread per vertex."""
-    _print_case(f"lanes per vertex {lanes}: grid_c0 and grid_c1")
+@pytest.mark.parametrize("lanes,threads", [(4, 256), (8, 256), (16, 256), (1, 64), (8, 64), (4, 128), (16, 128)])
+def test_lanes_per_vertex_bit_exact(gpu_available, lanes, threads):
+    """G lanes per vertex (terms summed in incidence order) in blocks of any
+    size == one thread per vertex in 256-thread blocks."""
+    _print_case(f"lanes per vertex {lanes}, {threads}-thread blocks: grid_c0 and grid_c1")
     cloth, params, _, _ = configs.grid_c0()
     a = GpuVBD(cloth, params)
     b = GpuVBD(cloth, params)
     b.set_lanes_per_vertex(lanes)
+    b.set_colour_block(threads)
     for _ in range(3):
         a.physics_step()
         b.physics_step()
@@ -82,6 +84,8 @@ def test_lanes_per_vertex_bit_exact(gpu_available, lanes):
     a = GpuVBD(cloth, params)
     b = GpuVBD(cloth, params)
     b.set_lanes_per_vertex(lanes)
+    b.set_colour_block(threads)
+    b.set_regions(8)
     a.run_frames(2, wind)
     b.run_frames(2, wind)
     assert np.array_equal(a.state()[0], b.state()[0])

@@ -300,11 +300,11 @@ def test_persistent_c1_and_autotune(gpu_available):
     assert np.array_equal(b.state()[0], x_before)          # autotune restores the state
     # the reported candidates: the kept configuration is the fastest one
     res = b.autotune_results()
-    assert res.shape[1] == 5 and len(res) >= 8 and (res[:, 4] > 0).all()
+    assert res.shape[1] == 6 and len(res) >= 8 and (res[:, 4] > 0).all()
     win = res[np.argmin(res[:, 4])]
     assert int(win[0]) == st and int(win[2]) == b.lanes_per_vertex()
     if st == _lib.LATS_STRATEGY_GRAPH:
-        assert int(win[3]) == b.regions()
+        assert int(win[3]) == b.regions() and int(win[5]) == b.colour_block()
     b.set_strategy(_lib.LATS_STRATEGY_PERSISTENT)
     a.run_frames(3, wind)
     b.run_frames(3, wind)
