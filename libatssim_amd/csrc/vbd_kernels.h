@@ -200,6 +200,18 @@ __device__ __forceinline__ void spring_term_of(const VbdArgs& a, F3 xk, bool on,
     tf = F3{0.f, 0.f, 0.f};
     if (on) stretch_values(xk, xo, __uint_as_float(e.y), a.ks, t, tf);
 }
+// The same, computed on every lane and masked afterwards: a lane past the
+// degree works on a valid placeholder entry (finite: l >= epsilon).  Used
+// where the neighbour was gathered ahead: computing the term unconditionally
+// keeps that gather out of a branch, so it is issued with the others.
+__device__ __forceinline__ void spring_term_masked(const VbdArgs& a, F3 xk, bool on, uint2 e, F3 xo, Sym3& t,
+                                                   F3& tf) {
+    Sym3 u;
+    F3 uf;
+    stretch_values(xk, xo, __uint_as_float(e.y), a.ks, u, uf);
+    t = Sym3{on ? u.xx : 0.f, on ? u.xy : 0.f, on ? u.xz : 0.f, on ? u.yy : 0.f, on ? u.yz : 0.f, on ? u.zz : 0.f};
+    tf = F3{on ? uf.x : 0.f, on ? uf.y : 0.f, on ? uf.z : 0.f};
+}
 template <class Mem>
 __device__ __forceinline__ F3 pair_point(const VbdArgs& a, F3 xk, uint32_t o, uint32_t k, uint32_t id) {
     return (o == k) ? xk : Mem::ld(a.x, id);
@@ -232,17 +244,20 @@ __device__ __forceinline__ void vbd_vertex_group(const VbdArgs& a, uint32_t s, u
     uint2 pe[PE];
     float4 pq[PB];
     uint4 pi[PB];
+    // (index arithmetic without branches: a zero width multiplies the index
+    // by zero, so no load waits behind a branch on the width)
+    const uint32_t e_on = ew != 0u, b_on = bw != 0u;
 #pragma unroll
     for (uint32_t k = 0; k < PE; k++) {
         const uint32_t j = k * G + lane;
         const uint32_t row = j < ew ? j : ew - 1;
-        pe[k] = a.ell_e[ew ? eoff + row * n + i : 0u];
+        pe[k] = a.ell_e[(eoff + row * n + i) * e_on];
     }
 #pragma unroll
     for (uint32_t k = 0; k < PB; k++) {
         const uint32_t j = k * G + lane;
         const uint32_t row = j < bw ? j : bw - 1;
-        const uint32_t at = bw ? boff + row * n + i : 0u;
+        const uint32_t at = (boff + row * n + i) * b_on;
         pq[k] = a.ell_bq[at];
         pi[k] = a.ell_bi[at];
     }
@@ -251,6 +266,9 @@ __device__ __forceinline__ void vbd_vertex_group(const VbdArgs& a, uint32_t s, u
     const F3 v0 = Mem::ld(a.v, s);
     const float m = a.mass[s];
     const uint32_t meta_raw = a.meta[s];
+    // every load above is issued before any gather below (the scheduler
+    // would otherwise issue later rounds' ELL loads after the first wait)
+    __builtin_amdgcn_sched_barrier(0);
     const uint32_t meta = valid ? meta_raw : 0u;
     const uint32_t edeg = meta & 0xffu, bdeg = (meta >> 8) & 0xffu;
 
@@ -270,6 +288,7 @@ __device__ __forceinline__ void vbd_vertex_group(const VbdArgs& a, uint32_t s, u
         pp[k][2] = (o == 2) ? xk : q2;
         pp[k][3] = (o == 3) ? xk : q3;
     }
+    __builtin_amdgcn_sched_barrier(0);   // every gather issued before the arithmetic
 
     Sym3 H;
     F3 f;
@@ -297,13 +316,14 @@ __device__ __forceinline__ void vbd_vertex_group(const VbdArgs& a, uint32_t s, u
             fs.z = fs.z + lane_plus(tf.z, q);
         }
     };
+    // every prefetched round runs (no early exit that the gathers could be
+    // moved behind); a round with no lane on adds +0 terms, a no-op
 #pragma unroll
     for (uint32_t k = 0; k < PE; k++) {
         const uint32_t r = k * G;
-        if (r >= edeg) break;
         Sym3 t;
         F3 tf;
-        spring_term_of(a, xk, r + lane < edeg, pe[k], pxo[k], t, tf);
+        spring_term_masked(a, xk, r + lane < edeg, pe[k], pxo[k], t, tf);
         add_spring_round(t, tf);
     }
     for (uint32_t r = PE * G; r < edeg; r += G) {
