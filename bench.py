@@ -43,17 +43,17 @@ def read_traffic(path, key="hbm_bytes_per_launch"):
     return None
 
 
-def read_colour_traffic(path, lanes, regions):
+def read_colour_traffic(path, lanes, regions, threads):
     """HBM bytes per colour launch from a committed PMC summary, if it was
-    measured at this lanes / bands configuration (its `config_lanes_regions`
-    line), else None."""
+    measured at this lanes / bands / block-threads configuration (its
+    `hbm_bytes_per_launch L R T: N` lines), else None."""
     if not os.path.exists(path):
         return None
+    key = f"hbm_bytes_per_launch {lanes} {regions} {threads}:"
     for line in open(path):
-        if line.startswith("config_lanes_regions:"):
-            if [int(v) for v in line.split(":", 1)[1].split()] != [lanes, regions]:
-                return None
-    return read_traffic(path)
+        if line.startswith(key):
+            return float(line.split(":", 1)[1])
+    return None
 
 
 def algorithmic_bytes_per_sweep(nv, ne, nb):
@@ -208,7 +208,7 @@ def main():
         graph_ms, launches, kvu = g.time_colour_kernels(20)
         bytes_per_launch = b_sweep / t.num_clusters
     achieved = bytes_per_launch / (mean_ms * 1e-3) / 1e9
-    traffic = (read_colour_traffic(TRAFFIC_FILE, g.lanes_per_vertex(), g.regions())
+    traffic = (read_colour_traffic(TRAFFIC_FILE, g.lanes_per_vertex(), g.regions(), g.colour_block())
                if kernel == "k_vbd_colour" else None)
     roofline = {
         "bound": "hbm",
@@ -410,11 +410,11 @@ def bench_sphere(args):
                    "strategy": {_lib.LATS_STRATEGY_PERSISTENT: "persistent", _lib.LATS_STRATEGY_FLOW: "flow"}.get(
                        strategy, "graph"),
                    "lanes_per_vertex": g.lanes_per_vertex(), "regions": g.regions(),
-                   "contact_vertices": int((counts > 0).sum())},
+                   "colour_block_threads": g.colour_block(), "contact_vertices": int((counts > 0).sum())},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5),
                      "traffic": read_colour_traffic(os.path.join(ROOT, "profiles", "k_vbd_colour_sphere1024_traffic.txt"),
-                                                    g.lanes_per_vertex(), g.regions()),
+                                                    g.lanes_per_vertex(), g.regions(), g.colour_block()),
                      "kernel": "k_vbd_colour", "mean_launch_us": round(mean_ms * 1e3, 3),
                      "algorithmic_bytes_per_launch": round(b_sweep / t.num_clusters, 1)},
     }
