@@ -228,21 +228,115 @@ __device__ inline void layer_bwd(const bf16x8* a, const uint16_t* wb, uint32_t l
     }
 }
 
-__global__ __launch_bounds__(kT) void k_mlp_backward(BwdArgs a) {
-    // LDS: dZ of the tile (fp32 [64][128]), its bf16 copy, the layer input (fp32 [64][128])
+// Weight gradients dW = A^T dZ on the matrix cores: the tile's rows are the
+// reduction dimension.  For a 16 x 16 block (input block ib, output block ob)
+// of dW, lane l feeds A[m = l%16][k = 8(l/16)..+7] = input[row k][ib*16 + m]
+// and B[k][n = l%16] = dZ[row k][ob*16 + n], both read as 8 consecutive rows
+// of one column from the transposed BF16 tiles in LDS, and accumulates
+// C[4(l/16) + j][l%16] = dW[ib*16 + 4(l/16) + j][ob*16 + l%16].  Wave w owns
+// blocks w, w + 4, ... of every layer; its accumulators stay in registers
+// over all of the block's tiles.
+constexpr int kTS = kTile + 8;   // row stride of the transposed BF16 tiles (padded)
+
+template <int NI, int NO>
+__device__ __forceinline__ void dw_accumulate(const uint16_t* sinT, const uint16_t* sdzT, uint32_t w, uint32_t l,
+                                              f32x4* acc) {
+    constexpr int NBW = NI * NO / 4;   // blocks per wave
+    const uint32_t kq = 8 * (l / 16), col = l % 16;
+#pragma unroll
+    for (int q = 0; q < NBW; q++) {
+        const uint32_t b = w + 4 * q, ib = b / NO, ob = b % NO;
+#pragma unroll
+        for (int ks = 0; ks < kTile / 32; ks++) {
+            const bf16x8 av = ld_bf8(sinT + (ib * 16 + col) * kTS + ks * 32 + kq);
+            const bf16x8 bv = ld_bf8(sdzT + (ob * 16 + col) * kTS + ks * 32 + kq);
+            acc[q] = mfma(av, bv, acc[q]);
+        }
+    }
+}
+
+template <int NI, int NO>
+__device__ __forceinline__ void dw_store(float* gp, int offW, uint32_t w, uint32_t l, const f32x4* acc) {
+    constexpr int NBW = NI * NO / 4;
+    const uint32_t col = l % 16;
+#pragma unroll
+    for (int q = 0; q < NBW; q++) {
+        const uint32_t b = w + 4 * q, ib = b / NO, ob = b % NO;
+#pragma unroll
+        for (int jj = 0; jj < 4; jj++) gp[offW + (ib * 16 + 4 * (l / 16) + jj) * (NO * 16) + ob * 16 + col] = acc[q][jj];
+    }
+}
+
+__global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_mlp_backward(BwdArgs a) {
+    // LDS: dZ of the tile (fp32 [64][128], bias sums), its bf16 copy row-major
+    // (A operand of dH = dZ W^T) and transposed (B operand of dW), and the
+    // layer input transposed in bf16 (A operand of dW)
     extern __shared__ uint4 smem_raw[];
-    float* sdz = reinterpret_cast<float*>(smem_raw);                 // [64][128]
-    float* sin = sdz + kTile * kHid;                                 // [64][128] layer input (h or x)
-    uint16_t* sdzb = reinterpret_cast<uint16_t*>(sin + kTile * kHid);// [64][kActStride]
-    double* sloss = reinterpret_cast<double*>(sdzb + kTile * kActStride);
+    float* sdz = reinterpret_cast<float*>(smem_raw);                          // [64][128]
+    uint16_t* sdzb = reinterpret_cast<uint16_t*>(sdz + kTile * kHid);        // [64][kActStride]
+    uint16_t* sdzT = sdzb + kTile * kActStride;                               // [128][kTS]
+    uint16_t* sinT = sdzT + kHid * kTS;                                       // [128][kTS]
+    double* sloss = reinterpret_cast<double*>(sinT + kHid * kTS);
     const uint32_t t = threadIdx.x, w = t / 64, l = t % 64;
     const uint32_t kq = 8 * (l / 16), col = l % 16;
-    float* gp = a.part + (size_t)blockIdx.x * kNParam;
-    for (uint32_t i = t; i < (uint32_t)kNParam; i += kT) gp[i] = 0.f;
+    f32x4 acc1[4 * 8 / 4], acc2[8 * 8 / 4], acc3[8 * 8 / 4], acc4[8 * 1 / 4];
+    for (auto& v : acc1) v = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (auto& v : acc2) v = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (auto& v : acc3) v = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (auto& v : acc4) v = f32x4{0.f, 0.f, 0.f, 0.f};
+    float bsum[4] = {0.f, 0.f, 0.f, 0.f};   // thread t < 128: bias gradient of output t, layers 1..4
     double loss = 0.0;
+    // dZ of the previous layer from this wave's dH rows: fp32, bf16 row-major, bf16 transposed
+    auto store_dz = [&](const f32x4* dh, int prev, uint32_t base) {
+        for (int nb = 0; nb < 8; nb++)
+            for (int i = 0; i < 4; i++) {
+                const uint32_t rl = w * 16 + 4 * (l / 16) + i, c = nb * 16 + col, row = base + rl;
+                float dz = 0.f;
+                if (row < a.M) dz = dh[nb][i] * dsilu(a.Z[((size_t)prev * a.M + row) * kHid + c]);
+                const uint16_t b = f2bf(dz);
+                sdz[rl * kHid + c] = dz;
+                sdzb[rl * kActStride + c] = b;
+                sdzT[c * kTS + rl] = b;
+            }
+    };
+    // the input of layer `prev + 1` (H of layer prev, or X), transposed; a
+    // thread reads 8 consecutive columns of one row with one 16-byte load,
+    // consecutive threads take consecutive rows (so their transposed 2-byte
+    // stores land in consecutive LDS words)
+    auto load_input = [&](int prev, uint32_t base) {
+        const uint32_t nc8 = (prev < 0 ? (uint32_t)kInP : (uint32_t)kHid) / 8;
+        for (uint32_t e = t; e < kTile * nc8; e += kT) {
+            const uint32_t r = e % kTile, c0 = 8 * (e / kTile), row = base + r;
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (row < a.M)
+                v = *reinterpret_cast<const uint4*>(prev < 0 ? a.X + a.map(row) * kInP + c0
+                                                             : a.H + ((size_t)prev * a.M + row) * kHid + c0);
+            const uint32_t w4[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                sinT[(c0 + 2 * q) * kTS + r] = (uint16_t)(w4[q] & 0xffffu);
+                sinT[(c0 + 2 * q + 1) * kTS + r] = (uint16_t)(w4[q] >> 16);
+            }
+        }
+    };
+    // column sums of the fp32 dZ tile in a fixed order: 8 interleaved partial
+    // sums (independent chains), combined pairwise
+    auto bias_sum = [&](int layer, uint32_t K) {
+        if (t < K) {
+            float p[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++) p[q] = sdz[q * kHid + t];
+#pragma unroll
+            for (uint32_t r = 8; r < (uint32_t)kTile; r += 8)
+#pragma unroll
+                for (int q = 0; q < 8; q++) p[q] = p[q] + sdz[(r + q) * kHid + t];
+            const float s = ((p[0] + p[1]) + (p[2] + p[3])) + ((p[4] + p[5]) + (p[6] + p[7]));
+            bsum[layer] = bsum[layer] + s;
+        }
+    };
     for (uint32_t tile = blockIdx.x; tile * kTile < a.M; tile += gridDim.x) {
         const uint32_t base = tile * kTile;
-        // dY (fp32, 64 x 16) -> sdz[:, 0:16]; bf16 copy padded to 32 columns
+        // dY (fp32, 64 x 16) -> sdz[:, 0:16]; bf16 copies (row-major padded to 32 columns, transposed)
         for (uint32_t e = t; e < kTile * kOutK; e += kT) {
             const uint32_t r = e / kOutK, c = e % kOutK, row = base + r;
             float d = 0.f;
@@ -251,64 +345,71 @@ __global__ __launch_bounds__(kT) void k_mlp_backward(BwdArgs a) {
                 loss += (double)(diff * diff);
                 d = 2.f * a.inv_count * diff;
             }
-            if (c < (uint32_t)kOutP) sdz[r * kHid + c] = d;
-            sdzb[r * kActStride + c] = f2bf(d);
+            const uint16_t b = f2bf(d);
+            if (c < (uint32_t)kOutP) {
+                sdz[r * kHid + c] = d;
+                sdzT[c * kTS + r] = b;
+            }
+            sdzb[r * kActStride + c] = b;
         }
-        // layer input of W4: H3
-        for (uint32_t e = t; e < kTile * kHid; e += kT) {
-            const uint32_t r = e / kHid, c = e % kHid, row = base + r;
-            sin[e] = row < a.M ? bf2f(a.H[((size_t)2 * a.M + row) * kHid + c]) : 0.f;
-        }
+        load_input(2, base);   // H3, the input of W4
         __syncthreads();
-        for (int layer = 3; layer >= 0; layer--) {
-            const int K = layer == 3 ? kOutP : kHid;            // outputs of this layer
-            const int Kin = layer == 0 ? kInP : kHid;           // inputs of this layer
-            const int offW = layer == 3 ? OFF_W4 : layer == 2 ? OFF_W3 : layer == 1 ? OFF_W2 : OFF_W1;
-            const int offB = layer == 3 ? OFF_B4 : layer == 2 ? OFF_B3 : layer == 1 ? OFF_B2 : OFF_B1;
-            // weight and bias gradients (FP32, fixed row order per entry)
-            for (uint32_t e = t; e < (uint32_t)(Kin * K); e += kT) {
-                const uint32_t i = e / K, o = e % K;
-                float s = 0.f;
-                for (uint32_t r = 0; r < (uint32_t)kTile; r++) s = s + sin[r * kHid + i] * sdz[r * kHid + o];
-                gp[offW + e] += s;
-            }
-            for (uint32_t o = t; o < (uint32_t)K; o += kT) {
-                float s = 0.f;
-                for (uint32_t r = 0; r < (uint32_t)kTile; r++) s = s + sdz[r * kHid + o];
-                gp[offB + o] += s;
-            }
-            if (layer == 0) break;
-            // dH of the layer input: MFMA with the bf16 dZ rows of this wave
-            bf16x8 adz[4];
-            const int kblocks = layer == 3 ? 1 : 4;
-            for (int kb = 0; kb < kblocks; kb++) adz[kb] = ld_bf8(sdzb + (w * 16 + col) * kActStride + kb * 32 + kq);
+        // layer 4 (W4: 128 x 16)
+        dw_accumulate<8, 1>(sinT, sdzT, w, l, acc4);
+        bias_sum(3, kOutP);
+        {
+            bf16x8 adz[1];
+            adz[0] = ld_bf8(sdzb + (w * 16 + col) * kActStride + kq);
             f32x4 dh[8];
-            if (layer == 3) layer_bwd<kOutK>(adz, a.WB + WB4, l, dh);
-            else layer_bwd<kHid>(adz, a.WB + (layer == 2 ? WB3 : WB2), l, dh);
-            __syncthreads();   // all reads of sdz/sdzb/sin for this layer are done
-            // dZ of the previous layer = dH * silu'(Z); its input becomes H (or X)
-            const int prev = layer - 1;                         // activation index 0..2
-            for (int nb = 0; nb < 8; nb++)
-                for (int i = 0; i < 4; i++) {
-                    const uint32_t rl = w * 16 + 4 * (l / 16) + i, c = nb * 16 + col, row = base + rl;
-                    float dz = 0.f;
-                    if (row < a.M) dz = dh[nb][i] * dsilu(a.Z[((size_t)prev * a.M + row) * kHid + c]);
-                    sdz[rl * kHid + c] = dz;
-                    sdzb[rl * kActStride + c] = f2bf(dz);
-                }
-            for (uint32_t e = t; e < kTile * kHid; e += kT) {
-                const uint32_t r = e / kHid, c = e % kHid, row = base + r;
-                float v = 0.f;
-                if (row < a.M) {
-                    if (prev == 0) v = c < (uint32_t)kInP ? bf2f(a.X[a.map(row) * kInP + c]) : 0.f;
-                    else v = bf2f(a.H[((size_t)(prev - 1) * a.M + row) * kHid + c]);
-                }
-                sin[e] = v;
-            }
+            layer_bwd<kOutK>(adz, a.WB + WB4, l, dh);
+            __syncthreads();   // all reads of this layer's LDS tiles are done
+            store_dz(dh, 2, base);
+            load_input(1, base);
             __syncthreads();
         }
-        __syncthreads();
+        // layer 3 (W3)
+        dw_accumulate<8, 8>(sinT, sdzT, w, l, acc3);
+        bias_sum(2, kHid);
+        {
+            bf16x8 adz[4];
+            for (int kb = 0; kb < 4; kb++) adz[kb] = ld_bf8(sdzb + (w * 16 + col) * kActStride + kb * 32 + kq);
+            f32x4 dh[8];
+            layer_bwd<kHid>(adz, a.WB + WB3, l, dh);
+            __syncthreads();
+            store_dz(dh, 1, base);
+            load_input(0, base);
+            __syncthreads();
+        }
+        // layer 2 (W2)
+        dw_accumulate<8, 8>(sinT, sdzT, w, l, acc2);
+        bias_sum(1, kHid);
+        {
+            bf16x8 adz[4];
+            for (int kb = 0; kb < 4; kb++) adz[kb] = ld_bf8(sdzb + (w * 16 + col) * kActStride + kb * 32 + kq);
+            f32x4 dh[8];
+            layer_bwd<kHid>(adz, a.WB + WB2, l, dh);
+            __syncthreads();
+            store_dz(dh, 0, base);
+            load_input(-1, base);   // X
+            __syncthreads();
+        }
+        // layer 1 (W1: 64 x 128)
+        dw_accumulate<4, 8>(sinT, sdzT, w, l, acc1);
+        bias_sum(0, kHid);
+        __syncthreads();   // before the next tile overwrites the LDS tiles
     }
+    // every entry of this block's partial is written exactly once
+    float* gp = a.part + (size_t)blockIdx.x * kNParam;
+    dw_store<4, 8>(gp, OFF_W1, w, l, acc1);
+    dw_store<8, 8>(gp, OFF_W2, w, l, acc2);
+    dw_store<8, 8>(gp, OFF_W3, w, l, acc3);
+    dw_store<8, 1>(gp, OFF_W4, w, l, acc4);
+    if (t < (uint32_t)kHid) {
+        gp[OFF_B1 + t] = bsum[0];
+        gp[OFF_B2 + t] = bsum[1];
+        gp[OFF_B3 + t] = bsum[2];
+    }
+    if (t < (uint32_t)kOutP) gp[OFF_B4 + t] = bsum[3];
     // block loss
     for (int off = 32; off > 0; off >>= 1) loss += __shfl_xor(loss, off, 64);
     if (l == 0) sloss[w] = loss;
@@ -316,13 +417,18 @@ __global__ __launch_bounds__(kT) void k_mlp_backward(BwdArgs a) {
     if (t == 0) a.loss_part[blockIdx.x] = (sloss[0] + sloss[1]) + (sloss[2] + sloss[3]);
 }
 
-// grad[i] = sum over blocks (fixed order) of part[b][i]
+// grad[i] = sum over blocks of part[b][i] in a fixed order: 8 interleaved
+// partial sums (b = q, q + 8, ...), combined pairwise
 __global__ void k_mlp_reduce(const float* part, uint32_t nparts, float* grad) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (uint32_t)kNParam) return;
-    float s = 0.f;
-    for (uint32_t b = 0; b < nparts; b++) s += part[(size_t)b * kNParam + i];
-    grad[i] = s;
+    float p[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    uint32_t b = 0;
+    for (; b + 8 <= nparts; b += 8)
+#pragma unroll
+        for (int q = 0; q < 8; q++) p[q] = p[q] + part[(size_t)(b + q) * kNParam + i];
+    for (int q = 0; b + q < nparts; q++) p[q] = p[q] + part[(size_t)(b + q) * kNParam + i];
+    grad[i] = ((p[0] + p[1]) + (p[2] + p[3])) + ((p[4] + p[5]) + (p[6] + p[7]));
 }
 
 // Adam (Kingma & Ba 2015) on the FP32 master parameters
@@ -566,7 +672,8 @@ __global__ void k_mlp_rescale(uint16_t* X, uint64_t rows, const float* mul) {
 
 constexpr size_t kFwdLds =
     (size_t)kNWT * 2 + (size_t)(kTF / 64) * 16 * kActStride * 2 + (3 * kHid + kOutP) * sizeof(float);
-constexpr size_t kBwdLds = (size_t)kTile * kHid * 4 * 2 + (size_t)kTile * kActStride * 2 + 4 * sizeof(double);
+constexpr size_t kBwdLds = (size_t)kTile * kHid * 4 + (size_t)kTile * kActStride * 2 + (size_t)2 * kHid * (kTile + 8) * 2 +
+                           4 * sizeof(double);
 constexpr uint32_t kGrid = 256;   // blocks of the training kernels (one partial gradient each)
 
 }  // namespace

@@ -11,7 +11,8 @@ laid out as libatssim_amd.warm_start.unpack_params describes.
 With bf16=True the values the kernels round to BF16 are rounded the same way
 (round to nearest even): the inputs, every weight, each hidden activation fed
 to the next layer, and the output / pre-activation gradients fed to the
-dH = dZ W^T products; everything else is FP64.  With bf16=False it is plain
+matrix-core products (dH = dZ W^T and the weight gradients dW = A^T dZ; the
+bias gradients sum the unrounded dZ); everything else is FP64.  With bf16=False it is plain
 FP64 calculus, which the finite-difference test pins.
 """
 import numpy as np
@@ -85,7 +86,7 @@ def gradient(p, X, T, bf16_round=True):
     dz = dY
     for li in range(3, -1, -1):
         W, _ = layers[li]
-        grads[li] = (acts[li].T @ dz, dz.sum(axis=0))
+        grads[li] = (acts[li].T @ rnd(dz), dz.sum(axis=0))
         if li == 0:
             break
         Wq = bf16(W) if bf16_round else W
