@@ -137,6 +137,7 @@ def main():
     ap.add_argument("--warm-scenarios", type=int, default=8, help="training scenarios of the warm start")
     ap.add_argument("--warm-frames", type=int, default=60, help="frames per training scenario")
     ap.add_argument("--warm-record-every", type=int, default=1, help="record every k-th training frame")
+    ap.add_argument("--warm-steps", type=int, default=2000, help="Adam steps per training phase of the warm start")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the VBD hot path runs on a single MI355X "
@@ -307,7 +308,7 @@ def bench_warm_start(args, cloth, params, wind, plain_ms, lanes=8, regions=1, co
     t0 = time.perf_counter()
     scen = [configs.warm_c3_scenario(k, n=256, frames=args.warm_frames) for k in range(args.warm_scenarios)]
     m = WarmStartMLP(2024)
-    steps = 2000
+    steps = args.warm_steps
     # rounds of rows from warm-started trajectories: 3 -> 13.7 sweeps per
     # frame, 5 -> 11.6, 8 -> 11.1 (more rounds, states closer to the ones the
     # warm-started solver visits)
