@@ -503,7 +503,10 @@ def bench_batch(args):
         "n_gpus": 1, "steps": K, "warmup": W, "ms_per_step": wall * 1e3 / K,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"batch_c2: {n} cloths, {nv} verts, 100 iterations/frame", "block_threads": threads,
-                   "blocks": blocks, "lanes_per_vertex": b.lanes_per_vertex(), "gpu_ms_per_step": gpu_ms / K},
+                   "blocks": blocks, "lanes_per_vertex": b.lanes_per_vertex(),
+                   "lds_staged_max_verts": b.lds_slots()[0],
+                   "lds_staged_cloths": sum(c.topology.num_verts <= b.lds_slots()[0] for c in cloths),
+                   "gpu_ms_per_step": gpu_ms / K},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": "k_vbd_batch"},
     }

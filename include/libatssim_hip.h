@@ -366,9 +366,17 @@ int lats_batch_get_launch(lats_batch* b, uint32_t* block_threads, uint32_t* bloc
 /* Lanes per vertex of the batch kernel (1, 4 or 8; bit-identical results). */
 int lats_batch_set_lanes_per_vertex(lats_batch* b, uint32_t lanes);
 int lats_batch_get_lanes_per_vertex(lats_batch* b, uint32_t* lanes);
-/* Autotuner of the cloth-to-block mapping (lanes per vertex x block size x
- * resident blocks): times `frames` frames for each candidate, keeps the
- * fastest, restores the state. */
+/* LDS staging: a cloth of at most `slots` vertices keeps its positions in
+ * its block's LDS (12 bytes per vertex) for the whole frame and writes them
+ * back at the frame's end; larger cloths gather from HBM.  0 turns staging
+ * off; more than max_slots (what one block's LDS holds on this device) is
+ * rejected.  The default stages every cloth that fits.  Results are
+ * bit-identical either way. */
+int lats_batch_set_lds_slots(lats_batch* b, uint32_t slots);
+int lats_batch_get_lds_slots(lats_batch* b, uint32_t* slots, uint32_t* max_slots);
+/* Autotuner of the cloth-to-block mapping (LDS staging off / on x lanes per
+ * vertex x block size x resident blocks): times `frames` frames for each
+ * candidate, keeps the fastest, restores the state. */
 int lats_batch_autotune(lats_batch* b, uint32_t frames, uint32_t* block_threads, uint32_t* blocks);
 int lats_batch_run_frames(lats_batch* b, uint32_t num_frames);
 int lats_batch_last_elapsed_ms(lats_batch* b, float* ms);

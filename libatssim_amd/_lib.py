@@ -188,6 +188,8 @@ SIGNATURES = {
     "lats_batch_last_elapsed_ms": (_I, [_VP, ctypes.POINTER(ctypes.c_float)]),
     "lats_batch_set_lanes_per_vertex": (_I, [_VP, _U32]),
     "lats_batch_get_lanes_per_vertex": (_I, [_VP, ctypes.POINTER(_U32)]),
+    "lats_batch_set_lds_slots": (_I, [_VP, _U32]),
+    "lats_batch_get_lds_slots": (_I, [_VP, ctypes.POINTER(_U32), ctypes.POINTER(_U32)]),
     "lats_mlp_create": (_I, [ctypes.c_uint64, ctypes.POINTER(_VP)]),
     "lats_mlp_destroy": (_I, [_VP]),
     "lats_mlp_num_params": (_I, [ctypes.POINTER(_U32)]),

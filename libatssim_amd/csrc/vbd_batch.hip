@@ -6,8 +6,11 @@
 // block barrier between colours, update velocity; it then pulls the next
 // cloth from a work counter.  Cloths are queued largest first (longest
 // processing time first), so the tail of the frame is made of small cloths.
-// Block size and resident blocks are tunable (lats_batch_autotune).  The
-// frame (counter reset + batch kernel) is captured once in a hipGraph.
+// A cloth small enough keeps its positions in the block's LDS for the whole
+// frame, so its passes stream only the static ELL and the own state from
+// HBM.  Block size, resident blocks, lanes per vertex and LDS staging are
+// tunable (lats_batch_autotune).  The frame (counter reset + batch kernel)
+// is captured once in a hipGraph.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -48,20 +51,72 @@ struct BatchArgs {
     uint32_t* work;            // next queue position
     const FrameParams* fp;
     uint32_t ncloths, iters;
+    uint32_t lds_slots;        // largest cloth whose positions are held in LDS (0: none)
     float h, h2inv, damp;
     float g[3];
 };
 
 __global__ void k_batch_reset(uint32_t* work) { *work = 0u; }
 
+// One frame of one cloth by the whole block: predict, every (iteration,
+// colour) pass with a block barrier between colours, update velocity.
+// Mem decides where the cloth's positions live for the frame (PlainMem: in
+// HBM; LdsMem: in LDS, written back to HBM by the last step).
+template <int G, class Mem>
+__device__ __forceinline__ void batch_cloth(const BatchArgs& p, const ClothInfo& ci, const VbdArgs& a, F3 ext,
+                                            Mem mem) {
+    const uint32_t tid = threadIdx.x, nt = blockDim.x;
+    const uint32_t s0 = ci.vbase, s1 = ci.vbase + ci.nv;
+    // Core::predict_position
+    for (uint32_t s = s0 + tid; s < s1; s += nt) {
+        F3 vpred = add3(ld3(p.v, s), scale3(p.h, F3{0.f, -9.8f, 0.f}));
+        if ((p.meta[s] >> 16) & 1u) vpred = F3{0.f, 0.f, 0.f};
+        mem.stx(p.x, s, add3(ld3(p.x0, s), scale3(p.h, vpred)));
+    }
+    __syncthreads();
+    for (uint32_t it = 0; it < p.iters; it++) {
+        for (uint32_t c = 0; c < ci.ncol; c++) {
+            const ColourInfo co = p.colours[ci.cbase + c];
+            if (G == 1) {
+                for (uint32_t i = tid; i < co.count; i += nt)
+                    vbd_vertex_ext(a, co.start + i, i, co.count, co.eoff, co.boff, ext, mem);
+            } else {
+                // groups of G lanes; a wave leaves the colour together
+                for (uint32_t i0 = 0; i0 < co.count; i0 += nt / G) {
+                    if (i0 + (tid / 64) * (64 / G) >= co.count) break;
+                    const uint32_t i = i0 + tid / G;
+                    const bool valid = i < co.count;
+                    vbd_vertex_group<G>(a, co.start + (valid ? i : 0u), valid ? i : 0u, co.count, co.eoff, co.boff,
+                                        co.ew, co.bw, ext, tid % G, valid, mem);
+                }
+            }
+            __syncthreads();
+        }
+    }
+    // Core::update_velocity
+    for (uint32_t s = s0 + tid; s < s1; s += nt) {
+        const F3 xk = mem.ldx(p.x, s);
+        const F3 dx = sub3(xk, ld3(p.x0, s));
+        F3 vel = F3{dx.x / p.h, dx.y / p.h, dx.z / p.h};
+        vel = F3{vel.x * p.damp, vel.y * p.damp, vel.z * p.damp};
+        st3(p.v, s, vel);
+        st3(p.x0, s, xk);
+        if (Mem::kStaged) st3(p.x, s, xk);
+    }
+    __syncthreads();
+}
+
 // G lanes per vertex (1: one thread per vertex; 4 / 8: vbd_vertex_group).
+// A cloth of at most p.lds_slots vertices keeps its positions in the
+// launch's dynamic LDS (12 bytes per slot) for the whole frame; a larger one
+// gathers them from HBM.
 template <int G>
 __global__ __launch_bounds__(1024) void k_vbd_batch(BatchArgs p) {
+    extern __shared__ float s_x[];
     __shared__ uint32_t s_item;
     const F3 ext = F3{p.fp->ext[0], p.fp->ext[1], p.fp->ext[2]};
-    const uint32_t tid = threadIdx.x, nt = blockDim.x;
     for (;;) {
-        if (tid == 0) s_item = atomicAdd(p.work, 1u);
+        if (threadIdx.x == 0) s_item = atomicAdd(p.work, 1u);
         __syncthreads();
         const uint32_t k = s_item;
         __syncthreads();
@@ -89,43 +144,10 @@ __global__ __launch_bounds__(1024) void k_vbd_batch(BatchArgs p) {
         a.ct_n = nullptr;
         a.nslots = 0;
         a.thickness = 0.f;
-        const uint32_t s0 = ci.vbase, s1 = ci.vbase + ci.nv;
-        // Core::predict_position
-        for (uint32_t s = s0 + tid; s < s1; s += nt) {
-            F3 vpred = add3(ld3(p.v, s), scale3(p.h, F3{0.f, -9.8f, 0.f}));
-            if ((p.meta[s] >> 16) & 1u) vpred = F3{0.f, 0.f, 0.f};
-            st3(p.x, s, add3(ld3(p.x0, s), scale3(p.h, vpred)));
-        }
-        __syncthreads();
-        for (uint32_t it = 0; it < p.iters; it++) {
-            for (uint32_t c = 0; c < ci.ncol; c++) {
-                const ColourInfo co = p.colours[ci.cbase + c];
-                if (G == 1) {
-                    for (uint32_t i = tid; i < co.count; i += nt)
-                        vbd_vertex_ext(a, co.start + i, i, co.count, co.eoff, co.boff, ext);
-                } else {
-                    // groups of G lanes; a wave leaves the colour together
-                    for (uint32_t i0 = 0; i0 < co.count; i0 += nt / G) {
-                        if (i0 + (tid / 64) * (64 / G) >= co.count) break;
-                        const uint32_t i = i0 + tid / G;
-                        const bool valid = i < co.count;
-                        vbd_vertex_group<G>(a, co.start + (valid ? i : 0u), valid ? i : 0u, co.count, co.eoff,
-                                            co.boff, co.ew, co.bw, ext, tid % G, valid);
-                    }
-                }
-                __syncthreads();
-            }
-        }
-        // Core::update_velocity
-        for (uint32_t s = s0 + tid; s < s1; s += nt) {
-            const F3 xk = ld3(p.x, s);
-            const F3 dx = sub3(xk, ld3(p.x0, s));
-            F3 vel = F3{dx.x / p.h, dx.y / p.h, dx.z / p.h};
-            vel = F3{vel.x * p.damp, vel.y * p.damp, vel.z * p.damp};
-            st3(p.v, s, vel);
-            st3(p.x0, s, xk);
-        }
-        __syncthreads();
+        if (ci.nv != 0u && ci.nv <= p.lds_slots)
+            batch_cloth<G>(p, ci, a, ext, LdsMem{s_x, ci.vbase, ci.nv - 1u});
+        else
+            batch_cloth<G>(p, ci, a, ext, PlainMem{});
     }
 }
 
@@ -180,6 +202,9 @@ struct lats_batch {
     DBuf<FrameParams> fp;
     uint32_t block_threads = 256, blocks = 0, max_blocks_cache = 0, max_blocks_threads = 0, max_blocks_lanes = 0;
     uint32_t lanes = 1;   // lanes per vertex (1, 4, 8)
+    // positions of cloths with at most lds_slots vertices stay in LDS for the
+    // frame; lds_max = the most slots one block's LDS holds on this device
+    uint32_t lds_slots = 0, lds_max = 0, max_nv = 0, max_blocks_lds = 0;
     hipGraphExec_t graph = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     float last_ms = 0.f;
@@ -208,14 +233,18 @@ const void* batch_kernel(uint32_t lanes) {
     }
 }
 
+size_t lds_bytes(uint32_t slots) { return 12 * (size_t)slots; }
+
 int max_resident_blocks(lats_batch* b, uint32_t threads, uint32_t* out) {
-    if (b->max_blocks_threads != threads || b->max_blocks_lanes != b->lanes) {
+    if (b->max_blocks_threads != threads || b->max_blocks_lanes != b->lanes || b->max_blocks_lds != b->lds_slots) {
         int per_cu = 0, ncu = 0;
-        HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel(b->lanes), (int)threads, 0));
+        HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel(b->lanes), (int)threads,
+                                                            lds_bytes(b->lds_slots)));
         HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, b->device));
         b->max_blocks_cache = (uint32_t)std::max(1, per_cu * ncu);
         b->max_blocks_threads = threads;
         b->max_blocks_lanes = b->lanes;
+        b->max_blocks_lds = b->lds_slots;
     }
     *out = b->max_blocks_cache;
     return LATS_OK;
@@ -243,18 +272,20 @@ int batch_graph(lats_batch* b, hipGraphExec_t* out) {
         p.fp = b->fp.p;
         p.ncloths = b->ncloths;
         p.iters = b->prm.constraint_iter_count;
+        p.lds_slots = b->lds_slots;
         p.h = b->h;
         p.h2inv = b->h2inv;
         p.damp = b->damp;
         for (int k = 0; k < 3; k++) p.g[k] = b->prm.gravity[k];
         hipGraph_t graph;
         HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        const size_t shm = lds_bytes(b->lds_slots);
         for (uint32_t sub = 0; sub < b->prm.num_substep; sub++) {
             hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(1), 0, b->stream, b->work.p);
             switch (b->lanes) {
-                case 4: hipLaunchKernelGGL(k_vbd_batch<4>, dim3(g), dim3(b->block_threads), 0, b->stream, p); break;
-                case 8: hipLaunchKernelGGL(k_vbd_batch<8>, dim3(g), dim3(b->block_threads), 0, b->stream, p); break;
-                default: hipLaunchKernelGGL(k_vbd_batch<1>, dim3(g), dim3(b->block_threads), 0, b->stream, p); break;
+                case 4: hipLaunchKernelGGL(k_vbd_batch<4>, dim3(g), dim3(b->block_threads), shm, b->stream, p); break;
+                case 8: hipLaunchKernelGGL(k_vbd_batch<8>, dim3(g), dim3(b->block_threads), shm, b->stream, p); break;
+                default: hipLaunchKernelGGL(k_vbd_batch<1>, dim3(g), dim3(b->block_threads), shm, b->stream, p); break;
             }
         }
         HIPCHK(hipStreamEndCapture(b->stream, &graph));
@@ -307,6 +338,7 @@ int lats_batch_create(uint32_t num_cloths, const lats_mesh_desc* meshes, const f
         ci.pad0 = ci.pad1 = 0;
         b->cloths.push_back(ci);
         b->perm.emplace_back(L.perm.begin() + vbase, L.perm.end());
+        b->max_nv = std::max(b->max_nv, ci.nv);
     }
     b->total_verts = L.num_slots();
     std::vector<uint32_t> order(num_cloths);
@@ -337,6 +369,23 @@ int lats_batch_create(uint32_t num_cloths, const lats_mesh_desc* meshes, const f
     if ((e = b->fp.upload(std::vector<FrameParams>{fp0})) != hipSuccess) return bad(e, "frame params");
     if ((e = hipEventCreate(&b->ev0)) != hipSuccess) return bad(e, "event");
     if ((e = hipEventCreate(&b->ev1)) != hipSuccess) return bad(e, "event");
+    // LDS left for positions: the device's per-block limit less the largest
+    // static LDS of the kernel's variants; every variant may then use it all
+    int lds_limit = 0;
+    if ((e = hipDeviceGetAttribute(&lds_limit, hipDeviceAttributeMaxSharedMemoryPerBlock, b->device)) != hipSuccess)
+        return bad(e, "LDS limit");
+    size_t lds_static = 0;
+    for (uint32_t lanes : {1u, 4u, 8u}) {
+        hipFuncAttributes fa;
+        if ((e = hipFuncGetAttributes(&fa, batch_kernel(lanes))) != hipSuccess) return bad(e, "kernel attributes");
+        lds_static = std::max(lds_static, fa.sharedSizeBytes);
+    }
+    b->lds_max = lds_limit > (int)lds_static ? (uint32_t)((lds_limit - lds_static) / 12) : 0u;
+    for (uint32_t lanes : {1u, 4u, 8u})
+        if ((e = hipFuncSetAttribute(batch_kernel(lanes), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)lds_bytes(b->lds_max))) != hipSuccess)
+            return bad(e, "LDS size");
+    b->lds_slots = std::min(b->max_nv, b->lds_max);
     *out = b;
     return LATS_OK;
 }
@@ -408,6 +457,25 @@ int lats_batch_set_lanes_per_vertex(lats_batch* b, uint32_t lanes) {
     return LATS_OK;
 }
 
+int lats_batch_set_lds_slots(lats_batch* b, uint32_t slots) {
+    if (!b) return fail(LATS_ERR_INVALID_ARGUMENT, "null batch");
+    if (slots > b->lds_max)
+        return fail(LATS_ERR_INVALID_ARGUMENT, "at most " + std::to_string(b->lds_max) +
+                                                   " cloth vertices fit in one block's LDS on this device");
+    if (slots != b->lds_slots) {
+        b->lds_slots = slots;
+        b->drop_graph();
+    }
+    return LATS_OK;
+}
+
+int lats_batch_get_lds_slots(lats_batch* b, uint32_t* slots, uint32_t* max_slots) {
+    if (!b) return fail(LATS_ERR_INVALID_ARGUMENT, "null batch");
+    if (slots) *slots = b->lds_slots;
+    if (max_slots) *max_slots = b->lds_max;
+    return LATS_OK;
+}
+
 int lats_batch_get_lanes_per_vertex(lats_batch* b, uint32_t* lanes) {
     if (!b || !lanes) return fail(LATS_ERR_INVALID_ARGUMENT, "null argument");
     *lanes = b->lanes;
@@ -443,9 +511,9 @@ int lats_batch_last_elapsed_ms(lats_batch* b, float* ms) {
     return LATS_OK;
 }
 
-// Times `frames` frames for every (lanes per vertex, block size, resident
-// blocks) triple of a small grid and keeps the fastest; restores the state
-// afterwards.
+// Times `frames` frames for every (LDS staging off / on, lanes per vertex,
+// block size, resident blocks) choice of a small grid and keeps the fastest;
+// restores the state afterwards.
 int lats_batch_autotune(lats_batch* b, uint32_t frames, uint32_t* best_threads, uint32_t* best_blocks) {
     if (!b) return fail(LATS_ERR_INVALID_ARGUMENT, "null batch");
     if (!frames) frames = 1;
@@ -461,28 +529,44 @@ int lats_batch_autotune(lats_batch* b, uint32_t frames, uint32_t* best_threads, 
     HIPCHK(hipMemcpyAsync(sv, b->v.p, nf * 4, hipMemcpyDeviceToDevice, b->stream));
     HIPCHK(hipStreamSynchronize(b->stream));
     float best = 3.4e38f;
-    uint32_t bt = b->block_threads, bb = b->blocks, bl = b->lanes;
+    uint32_t bt = b->block_threads, bb = b->blocks, bl = b->lanes, bs = b->lds_slots;
     int rc = LATS_OK;
-    for (uint32_t lanes : {1u, 4u, 8u}) {
-        rc = lats_batch_set_lanes_per_vertex(b, lanes);
+    // LDS staging off, and on for every cloth that fits
+    const uint32_t staged = std::min(b->max_nv, b->lds_max);
+    std::vector<uint32_t> lds_options{0u};
+    if (staged) lds_options.push_back(staged);
+    for (uint32_t slots : lds_options) {
+        rc = lats_batch_set_lds_slots(b, slots);
         if (rc) break;
-        for (uint32_t threads : {128u, 256u, 512u, 1024u}) {
-            uint32_t mx = 0;
-            rc = max_resident_blocks(b, threads, &mx);
+        for (uint32_t lanes : {1u, 4u, 8u}) {
+            rc = lats_batch_set_lanes_per_vertex(b, lanes);
             if (rc) break;
-            for (uint32_t blocks : {mx / 4, mx / 2, mx}) {
-                if (blocks == 0) continue;
-                rc = lats_batch_set_launch(b, threads, blocks);
-                if (!rc) rc = lats_batch_run_frames(b, 1);
-                if (!rc) rc = lats_batch_run_frames(b, frames);
+            for (uint32_t threads : {128u, 256u, 512u, 1024u}) {
+                uint32_t mx = 0;
+                rc = max_resident_blocks(b, threads, &mx);
                 if (rc) break;
-                if (b->last_ms < best) { best = b->last_ms; bt = threads; bb = blocks; bl = lanes; }
+                for (uint32_t blocks : {mx / 4, mx / 2, mx}) {
+                    if (blocks == 0) continue;
+                    rc = lats_batch_set_launch(b, threads, blocks);
+                    if (!rc) rc = lats_batch_run_frames(b, 1);
+                    if (!rc) rc = lats_batch_run_frames(b, frames);
+                    if (rc) break;
+                    if (b->last_ms < best) {
+                        best = b->last_ms;
+                        bt = threads;
+                        bb = blocks;
+                        bl = lanes;
+                        bs = slots;
+                    }
+                }
+                if (rc) break;
             }
             if (rc) break;
         }
         if (rc) break;
     }
     if (!rc) rc = lats_batch_set_lanes_per_vertex(b, bl);
+    if (!rc) rc = lats_batch_set_lds_slots(b, bs);
     hipError_t e1 = hipMemcpyAsync(b->x.p, sx, nf * 4, hipMemcpyDeviceToDevice, b->stream);
     hipError_t e2 = hipMemcpyAsync(b->x0.p, sx0, nf * 4, hipMemcpyDeviceToDevice, b->stream);
     hipError_t e3 = hipMemcpyAsync(b->v.p, sv, nf * 4, hipMemcpyDeviceToDevice, b->stream);

@@ -96,9 +96,18 @@ __device__ __forceinline__ float lane_plus(float v, int q) {
 // passes).  CoherentMem: device-scope relaxed atomic accesses, coherent
 // across the L2s of all XCDs without cache maintenance, for kernels in
 // which blocks hand positions to each other inside one launch.
+//
+// ld / st serve the vertex's own state (x_start, v); ldx / stx serve the
+// positions, own and gathered, so a policy can hold positions elsewhere.
+// Policies are passed by value (PlainMem and ScopedMem carry no state).
+// kStaged: the positions live outside a.x while a frame runs and the frame's
+// last step writes them back.
 struct PlainMem {
+    static constexpr bool kStaged = false;
     static __device__ __forceinline__ F3 ld(const float* p, uint32_t s) { return ld3(p, s); }
     static __device__ __forceinline__ void st(float* p, uint32_t s, F3 a) { st3(p, s, a); }
+    static __device__ __forceinline__ F3 ldx(const float* p, uint32_t s) { return ld3(p, s); }
+    static __device__ __forceinline__ void stx(float* p, uint32_t s, F3 a) { st3(p, s, a); }
 };
 template <int Scope>
 struct ScopedMem {
@@ -114,8 +123,37 @@ struct ScopedMem {
         st1(p + 3 * s + 1, a.y);
         st1(p + 3 * s + 2, a.z);
     }
+    static constexpr bool kStaged = false;
+    static __device__ __forceinline__ F3 ldx(const float* p, uint32_t s) { return ld(p, s); }
+    static __device__ __forceinline__ void stx(float* p, uint32_t s, F3 a) { st(p, s, a); }
 };
 using CoherentMem = ScopedMem<__HIP_MEMORY_SCOPE_AGENT>;
+// LdsMem: the positions of one cloth, slots [vbase, vbase + last], held in
+// LDS (slot s at lds[3 * (s - vbase)]) by the block that owns the cloth for
+// a whole frame; the own state stays in HBM.  A gathered slot outside the
+// cloth (an ELL padding entry, whose term is masked) is clamped into it, so
+// every LDS access stays inside the cloth's 12 * (last + 1) bytes.
+struct LdsMem {
+    static constexpr bool kStaged = true;
+    float* lds;
+    uint32_t vbase, last;
+    __device__ __forceinline__ uint32_t at(uint32_t s) const {
+        const uint32_t i = s - vbase;
+        return 3u * (i < last ? i : last);
+    }
+    static __device__ __forceinline__ F3 ld(const float* p, uint32_t s) { return ld3(p, s); }
+    static __device__ __forceinline__ void st(float* p, uint32_t s, F3 a) { st3(p, s, a); }
+    __device__ __forceinline__ F3 ldx(const float*, uint32_t s) const {
+        const uint32_t k = at(s);
+        return F3{lds[k], lds[k + 1], lds[k + 2]};
+    }
+    __device__ __forceinline__ void stx(float*, uint32_t s, F3 a) const {
+        const uint32_t k = at(s);
+        lds[k] = a.x;
+        lds[k + 1] = a.y;
+        lds[k + 2] = a.z;
+    }
+};
 
 // One VBD vertex update: VBD::compute_inertia + compute_stretch_mass_spring +
 // compute_bending_quadratic (xpbd_constraints.h:4027-4205) summed as
@@ -124,10 +162,10 @@ using CoherentMem = ScopedMem<__HIP_MEMORY_SCOPE_AGENT>;
 // vbd_step (:993-1014).  i = index inside the colour, n = colour size.
 template <class Mem = PlainMem>
 __device__ __forceinline__ void vbd_vertex_ext(const VbdArgs& a, uint32_t s, uint32_t i, uint32_t n, uint32_t eoff,
-                                               uint32_t boff, F3 ext) {
-    const F3 xk = Mem::ld(a.x, s);
-    const F3 x0 = Mem::ld(a.x0, s);
-    const F3 v0 = Mem::ld(a.v, s);
+                                               uint32_t boff, F3 ext, Mem mem = Mem{}) {
+    const F3 xk = mem.ldx(a.x, s);
+    const F3 x0 = mem.ld(a.x0, s);
+    const F3 v0 = mem.ld(a.v, s);
     const float m = a.mass[s];
     const uint32_t meta = a.meta[s];
     const uint32_t edeg = meta & 0xffu, bdeg = (meta >> 8) & 0xffu;
@@ -142,7 +180,7 @@ __device__ __forceinline__ void vbd_vertex_ext(const VbdArgs& a, uint32_t s, uin
     F3 fs = F3{0.f, 0.f, 0.f};
     for (uint32_t j = 0; j < edeg; j++) {
         const uint2 e = a.ell_e[eoff + j * n + i];
-        const F3 xo = Mem::ld(a.x, e.x);
+        const F3 xo = mem.ldx(a.x, e.x);
         stretch_term(xk, xo, __uint_as_float(e.y), a.ks, S, fs);
     }
     sym_add(H, S);
@@ -156,10 +194,10 @@ __device__ __forceinline__ void vbd_vertex_ext(const VbdArgs& a, uint32_t s, uin
         const float4 q = a.ell_bq[k];
         const uint4 id = a.ell_bi[k];
         const uint32_t o = id.x >> 30;
-        const F3 p0 = (o == 0) ? xk : Mem::ld(a.x, id.x & 0x3fffffffu);
-        const F3 p1 = (o == 1) ? xk : Mem::ld(a.x, id.y);
-        const F3 p2 = (o == 2) ? xk : Mem::ld(a.x, id.z);
-        const F3 p3 = (o == 3) ? xk : Mem::ld(a.x, id.w);
+        const F3 p0 = (o == 0) ? xk : mem.ldx(a.x, id.x & 0x3fffffffu);
+        const F3 p1 = (o == 1) ? xk : mem.ldx(a.x, id.y);
+        const F3 p2 = (o == 2) ? xk : mem.ldx(a.x, id.z);
+        const F3 p3 = (o == 3) ? xk : mem.ldx(a.x, id.w);
         bending_term(q, o, p0, p1, p2, p3, a.kb, hb, fb);
     }
     H.xx = H.xx + hb;
@@ -184,13 +222,13 @@ __device__ __forceinline__ void vbd_vertex_ext(const VbdArgs& a, uint32_t s, uin
                                    {H.xz + Hc[2][0], H.yz + Hc[2][1], H.zz + Hc[2][2]}};
             f = add3(f, fc);
             F3 dx;
-            if (newton_step_full(M, f, dx)) Mem::st(a.x, s, add3(xk, dx));
+            if (newton_step_full(M, f, dx)) mem.stx(a.x, s, add3(xk, dx));
             return;
         }
     }
 
     F3 dx;
-    if (newton_step(H, f, dx)) Mem::st(a.x, s, add3(xk, dx));
+    if (newton_step(H, f, dx)) mem.stx(a.x, s, add3(xk, dx));
 }
 
 
@@ -213,8 +251,8 @@ __device__ __forceinline__ void spring_term_masked(const VbdArgs& a, F3 xk, bool
     tf = F3{on ? uf.x : 0.f, on ? uf.y : 0.f, on ? uf.z : 0.f};
 }
 template <class Mem>
-__device__ __forceinline__ F3 pair_point(const VbdArgs& a, F3 xk, uint32_t o, uint32_t k, uint32_t id) {
-    return (o == k) ? xk : Mem::ld(a.x, id);
+__device__ __forceinline__ F3 pair_point(const Mem& mem, const VbdArgs& a, F3 xk, uint32_t o, uint32_t k, uint32_t id) {
+    return (o == k) ? xk : mem.ldx(a.x, id);
 }
 
 // The same update with G lanes per vertex (lane = 0..G-1 inside an aligned
@@ -232,7 +270,7 @@ __device__ __forceinline__ F3 pair_point(const VbdArgs& a, F3 xk, uint32_t o, ui
 template <int G, class Mem = PlainMem>
 __device__ __forceinline__ void vbd_vertex_group(const VbdArgs& a, uint32_t s, uint32_t i, uint32_t n, uint32_t eoff,
                                                  uint32_t boff, uint32_t ew, uint32_t bw, F3 ext, uint32_t lane,
-                                                 bool valid) {
+                                                 bool valid, Mem mem = Mem{}) {
     constexpr uint32_t PE = (8 + G - 1) / G;    // spring rounds prefetched (degrees <= 8)
     constexpr uint32_t PB = (16 + G - 1) / G;   // pair rounds prefetched (degrees <= 16)
     // Every load below is unconditional, from an address that is always
@@ -261,9 +299,9 @@ __device__ __forceinline__ void vbd_vertex_group(const VbdArgs& a, uint32_t s, u
         pq[k] = a.ell_bq[at];
         pi[k] = a.ell_bi[at];
     }
-    const F3 xk = Mem::ld(a.x, s);
-    const F3 x0 = Mem::ld(a.x0, s);
-    const F3 v0 = Mem::ld(a.v, s);
+    const F3 xk = mem.ldx(a.x, s);
+    const F3 x0 = mem.ld(a.x0, s);
+    const F3 v0 = mem.ld(a.v, s);
     const float m = a.mass[s];
     const uint32_t meta_raw = a.meta[s];
     // every load above is issued before any gather below (the scheduler
@@ -276,13 +314,13 @@ __device__ __forceinline__ void vbd_vertex_group(const VbdArgs& a, uint32_t s, u
     // a valid slot and mask the term)
     F3 pxo[PE];
 #pragma unroll
-    for (uint32_t k = 0; k < PE; k++) pxo[k] = Mem::ld(a.x, pe[k].x);
+    for (uint32_t k = 0; k < PE; k++) pxo[k] = mem.ldx(a.x, pe[k].x);
     F3 pp[PB][4];
 #pragma unroll
     for (uint32_t k = 0; k < PB; k++) {
         const uint32_t o = pi[k].x >> 30;
-        const F3 q0 = Mem::ld(a.x, pi[k].x & 0x3fffffffu), q1 = Mem::ld(a.x, pi[k].y);
-        const F3 q2 = Mem::ld(a.x, pi[k].z), q3 = Mem::ld(a.x, pi[k].w);
+        const F3 q0 = mem.ldx(a.x, pi[k].x & 0x3fffffffu), q1 = mem.ldx(a.x, pi[k].y);
+        const F3 q2 = mem.ldx(a.x, pi[k].z), q3 = mem.ldx(a.x, pi[k].w);
         pp[k][0] = (o == 0) ? xk : q0;
         pp[k][1] = (o == 1) ? xk : q1;
         pp[k][2] = (o == 2) ? xk : q2;
@@ -332,7 +370,7 @@ __device__ __forceinline__ void vbd_vertex_group(const VbdArgs& a, uint32_t s, u
         const uint2 e = on ? a.ell_e[eoff + j * n + i] : uint2{0u, 0u};
         Sym3 t;
         F3 tf;
-        spring_term_of(a, xk, on, e, on ? Mem::ld(a.x, e.x) : F3{0.f, 0.f, 0.f}, t, tf);
+        spring_term_of(a, xk, on, e, on ? mem.ldx(a.x, e.x) : F3{0.f, 0.f, 0.f}, t, tf);
         add_spring_round(t, tf);
     }
     sym_add(H, S);
@@ -367,8 +405,8 @@ __device__ __forceinline__ void vbd_vertex_group(const VbdArgs& a, uint32_t s, u
             const float4 q = a.ell_bq[k];
             const uint4 id = a.ell_bi[k];
             const uint32_t o = id.x >> 30;
-            bending_term(q, o, pair_point<Mem>(a, xk, o, 0, id.x & 0x3fffffffu), pair_point<Mem>(a, xk, o, 1, id.y),
-                         pair_point<Mem>(a, xk, o, 2, id.z), pair_point<Mem>(a, xk, o, 3, id.w), a.kb, th, tf);
+            bending_term(q, o, pair_point(mem, a, xk, o, 0, id.x & 0x3fffffffu), pair_point(mem, a, xk, o, 1, id.y),
+                         pair_point(mem, a, xk, o, 2, id.z), pair_point(mem, a, xk, o, 3, id.w), a.kb, th, tf);
         }
         add_pair_round(th, tf);
     }
@@ -393,12 +431,12 @@ __device__ __forceinline__ void vbd_vertex_group(const VbdArgs& a, uint32_t s, u
                                    {H.xz + Hc[2][0], H.yz + Hc[2][1], H.zz + Hc[2][2]}};
             f = add3(f, fc);
             F3 dx;
-            if (newton_step_full(M, f, dx)) Mem::st(a.x, s, add3(xk, dx));
+            if (newton_step_full(M, f, dx)) mem.stx(a.x, s, add3(xk, dx));
             return;
         }
     }
     F3 dx;
-    if (newton_step(H, f, dx)) Mem::st(a.x, s, add3(xk, dx));
+    if (newton_step(H, f, dx)) mem.stx(a.x, s, add3(xk, dx));
 }
 
 }  // namespace lats

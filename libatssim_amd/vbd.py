@@ -566,6 +566,18 @@ class GpuBatch:
         check(lib().lats_batch_get_lanes_per_vertex(self._h, ctypes.byref(n)))
         return n.value
 
+    def set_lds_slots(self, slots):
+        """Cloths of at most `slots` vertices keep their positions in LDS for
+        the frame (0: none; at most lds_slots()[1])."""
+        check(lib().lats_batch_set_lds_slots(self._h, int(slots)))
+
+    def lds_slots(self):
+        """(current LDS staging limit in vertices, the most this device allows)."""
+        n = ctypes.c_uint32()
+        mx = ctypes.c_uint32()
+        check(lib().lats_batch_get_lds_slots(self._h, ctypes.byref(n), ctypes.byref(mx)))
+        return n.value, mx.value
+
     def autotune(self, frames=1):
         t = ctypes.c_uint32()
         b = ctypes.c_uint32()
