@@ -89,19 +89,19 @@ LATS_HD float contact_energy(F3 x, F3 xs, F3 n, float stiff, float thickness) {
 }
 
 // vbd_step on a general 3x3 (contacts make H non-symmetric by rounding).
-LATS_HD bool newton_step_full(const float m[3][3], F3 f, F3& dx) {
+LATS_D bool newton_step_full(const float m[3][3], F3 f, F3& dx) {
     const float det = m[0][0] * (m[1][1] * m[2][2] - m[2][1] * m[1][2]) - m[1][0] * (m[0][1] * m[2][2] - m[2][1] * m[0][2]) +
                       m[2][0] * (m[0][1] * m[1][2] - m[1][1] * m[0][2]);
     if (!(fabsf(det) > 1.19209290E-7f)) return false;
-    const float i00 = (m[1][1] * m[2][2] - m[1][2] * m[2][1]) / det;
-    const float i01 = -(m[1][0] * m[2][2] - m[1][2] * m[2][0]) / det;
-    const float i02 = (m[1][0] * m[2][1] - m[1][1] * m[2][0]) / det;
-    const float i11 = (m[0][0] * m[2][2] - m[0][2] * m[2][0]) / det;
-    const float i12 = -(m[0][0] * m[2][1] - m[0][1] * m[2][0]) / det;
-    const float i22 = (m[0][0] * m[1][1] - m[0][1] * m[1][0]) / det;
-    const float i10 = -(m[0][1] * m[2][2] - m[0][2] * m[2][1]) / det;
-    const float i20 = (m[0][1] * m[1][2] - m[0][2] * m[1][1]) / det;
-    const float i21 = -(m[0][0] * m[1][2] - m[0][2] * m[1][0]) / det;
+    const float num[9] = {(m[1][1] * m[2][2] - m[1][2] * m[2][1]), -(m[1][0] * m[2][2] - m[1][2] * m[2][0]),
+                          (m[1][0] * m[2][1] - m[1][1] * m[2][0]), (m[0][0] * m[2][2] - m[0][2] * m[2][0]),
+                          -(m[0][0] * m[2][1] - m[0][1] * m[2][0]), (m[0][0] * m[1][1] - m[0][1] * m[1][0]),
+                          -(m[0][1] * m[2][2] - m[0][2] * m[2][1]), (m[0][1] * m[1][2] - m[0][2] * m[1][1]),
+                          -(m[0][0] * m[1][2] - m[0][2] * m[1][0])};
+    float q[9];
+    divide_all(num, det, q);
+    const float i00 = q[0], i01 = q[1], i02 = q[2], i11 = q[3], i12 = q[4], i22 = q[5];
+    const float i10 = q[6], i20 = q[7], i21 = q[8];
     dx.x = (i00 * f.x + i10 * f.y) + i20 * f.z;
     dx.y = (i01 * f.x + i11 * f.y) + i21 * f.z;
     dx.z = (i02 * f.x + i12 * f.y) + i22 * f.z;

@@ -45,6 +45,57 @@ LATS_HD void sym_add(Sym3& a, const Sym3& b) {
     a.zz = a.zz + b.zz;
 }
 
+// --- quotients by one divisor --------------------------------------------
+// The IEEE FP32 division the compiler emits for n / d is v_div_scale of d
+// and of n, v_rcp, one reciprocal refinement, a product and two fma
+// corrections, v_div_fmas, v_div_fixup.  When v_div_scale leaves both
+// operands unscaled (it then clears VCC) -- d normal with 1/d normal, n of
+// biased exponent at least 24, n / d normal and their exponents less than
+// 96 apart -- v_div_fmas is a plain fma and v_div_fixup returns its input,
+// so the division is exactly the steps below.  Written out, quotients by
+// one divisor share its refined reciprocal.  The operands are checked
+// together, by magnitude, against a box inside those conditions:
+// 2^-120 <= |d| <= 2^120 and 2^-100, |d| 2^-120 <= |n| <= |d| 2^90; any
+// operand outside it (zeros included) sends every quotient of the call
+// through the full division.  Either way each result is the correctly
+// rounded n / d.
+#define LATS_D __device__ __forceinline__
+struct Divisor {
+    float d, y;   // the divisor and its refined reciprocal
+};
+LATS_D Divisor divisor(float d) {
+    const float r = __builtin_amdgcn_rcpf(d);
+    return Divisor{d, __builtin_fmaf(__builtin_fmaf(-d, r, 1.0f), r, r)};
+}
+LATS_D float quotient(float n, Divisor v) {
+    const float q0 = n * v.y;
+    const float q1 = __builtin_fmaf(__builtin_fmaf(-v.d, q0, n), v.y, q0);
+    return __builtin_fmaf(__builtin_fmaf(-v.d, q1, n), v.y, q1);
+}
+// q[k] = num[k] / den for k < N.
+template <int N>
+LATS_D void divide_all(const float (&num)[N], float den, float (&q)[N]) {
+    const Divisor v = divisor(den);
+    float lo = fabsf(num[0]), hi = lo;
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+        q[k] = quotient(num[k], v);
+        lo = fminf(lo, fabsf(num[k]));
+        hi = fmaxf(hi, fabsf(num[k]));
+    }
+    const float ad = fabsf(den);
+    // (NaN operands fail every comparison)
+    const bool plain = ad >= 0x1p-120f && ad <= 0x1p120f && lo >= 0x1p-100f && lo >= ad * 0x1p-120f &&
+                       hi <= ad * 0x1p90f;
+    if (!plain) {
+        // (an empty volatile asm: the block then has a side effect, so it
+        // is branched over instead of being computed always and selected)
+        asm volatile("");
+#pragma unroll
+        for (int k = 0; k < N; k++) q[k] = num[k] / den;
+    }
+}
+
 // H = (m/h^2) I (+1e9 I if pinned);
 // f = ((-m) h^-2) ((xk - x0) - v0 h) + ((0 + m g) + ext)
 LATS_HD void inertia_term(F3 xk, F3 x0, F3 v0, float m, uint32_t pinned, const float* g, F3 ext, float h,
@@ -62,15 +113,19 @@ LATS_HD void inertia_term(F3 xk, F3 x0, F3 v0, float m, uint32_t pinned, const f
 // One incident spring.  diff = x_other - x_self equals +/- the reference's
 // x[e1] - x[e0] exactly, and the force on the vertex is (k * diff/l) * C in
 // both orientations (IEEE negation commutes with rounding).
-LATS_HD void stretch_term(F3 xk, F3 xo, float L, float k, Sym3& S, F3& fs) {
+LATS_D void stretch_term(F3 xk, F3 xo, float L, float k, Sym3& S, F3& fs) {
     const F3 diff = sub3(xo, xk);
     const float len = sqrtf(dot3(diff, diff));
     const float eps = 1.19209290E-7f;
     const float l = (len < eps) ? eps : len;
     const float C = l - L;
-    const F3 dir = F3{diff.x / l, diff.y / l, diff.z / l};
+    // diff.x / l, diff.y / l, diff.z / l, 1 / l
+    const float num[4] = {diff.x, diff.y, diff.z, 1.f};
+    float q[4];
+    divide_all(num, l, q);
+    const F3 dir = F3{q[0], q[1], q[2]};
     const F3 f = F3{(k * dir.x) * C, (k * dir.y) * C, (k * dir.z) * C};
-    const float xinv = 1.f / l;
+    const float xinv = q[3];
     const float xsq = xinv * xinv;
     const float s1 = k * xsq;
     const float t = 1.f - L * xinv;
@@ -87,7 +142,7 @@ LATS_HD void stretch_term(F3 xk, F3 xo, float L, float k, Sym3& S, F3& fs) {
 }
 
 // The same spring term on its own (equal to stretch_term from a zero sum).
-LATS_HD void stretch_values(F3 xk, F3 xo, float L, float k, Sym3& he, F3& f) {
+LATS_D void stretch_values(F3 xk, F3 xo, float L, float k, Sym3& he, F3& f) {
     he = sym_zero();
     f = F3{0.f, 0.f, 0.f};
     stretch_term(xk, xo, L, k, he, f);
@@ -108,22 +163,21 @@ LATS_HD void bending_term(float4 q, uint32_t o, F3 p0, F3 p1, F3 p2, F3 p3, floa
 }
 
 // vbd_step: det(H) (glm), |det| > Epsilon ? dx = adj(H)/det * f : no move.
-LATS_HD bool newton_step(const Sym3& H, F3 f, F3& dx) {
+LATS_D bool newton_step(const Sym3& H, F3 f, F3& dx) {
     // m[col][row]
     const float m00 = H.xx, m01 = H.xy, m02 = H.xz;
     const float m10 = H.xy, m11 = H.yy, m12 = H.yz;
     const float m20 = H.xz, m21 = H.yz, m22 = H.zz;
     const float det = m00 * (m11 * m22 - m21 * m12) - m10 * (m01 * m22 - m21 * m02) + m20 * (m01 * m12 - m11 * m02);
     if (!(fabsf(det) > 1.19209290E-7f)) return false;
-    const float i00 = (m11 * m22 - m12 * m21) / det;
-    const float i01 = -(m10 * m22 - m12 * m20) / det;
-    const float i02 = (m10 * m21 - m11 * m20) / det;
-    const float i11 = (m00 * m22 - m02 * m20) / det;
-    const float i12 = -(m00 * m21 - m01 * m20) / det;
-    const float i22 = (m00 * m11 - m01 * m10) / det;
-    const float i10 = -(m01 * m22 - m02 * m21) / det;
-    const float i20 = (m01 * m12 - m02 * m11) / det;
-    const float i21 = -(m00 * m12 - m02 * m10) / det;
+    // the adjugate's entries (negated where the inverse negates them), each / det
+    const float num[9] = {(m11 * m22 - m12 * m21), -(m10 * m22 - m12 * m20), (m10 * m21 - m11 * m20),
+                          (m00 * m22 - m02 * m20), -(m00 * m21 - m01 * m20), (m00 * m11 - m01 * m10),
+                          -(m01 * m22 - m02 * m21), (m01 * m12 - m02 * m11), -(m00 * m12 - m02 * m10)};
+    float q[9];
+    divide_all(num, det, q);
+    const float i00 = q[0], i01 = q[1], i02 = q[2], i11 = q[3], i12 = q[4], i22 = q[5];
+    const float i10 = q[6], i20 = q[7], i21 = q[8];
     dx.x = (i00 * f.x + i10 * f.y) + i20 * f.z;
     dx.y = (i01 * f.x + i11 * f.y) + i21 * f.z;
     dx.z = (i02 * f.x + i12 * f.y) + i22 * f.z;
