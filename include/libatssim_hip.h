@@ -177,9 +177,9 @@ int lats_vbd_xpbd_clusters(lats_vbd* ctx, int which, uint32_t* csr, uint64_t cap
  * parallel and summed in the reference's order (results are identical). */
 int lats_vbd_set_lanes_per_vertex(lats_vbd* ctx, uint32_t lanes);
 int lats_vbd_get_lanes_per_vertex(lats_vbd* ctx, uint32_t* lanes);
-/* Block mapping of the colour passes (graph strategy): the cloth is cut into
- * `regions` (1..8) bands of consecutive vertex ids and block b of every
- * colour pass works on band b % regions.  With 8, each XCD updates the same
+/* Block mapping of the colour passes (graph strategy): every colour is cut
+ * into `regions` (1..8) bands of consecutive vertex ids, of equal vertex
+ * count, and block b of every colour pass works on band b % regions.  With 8, each XCD updates the same
  * band in every pass (workgroups go to the XCDs round-robin).  Results are
  * identical for every value; default 1. */
 int lats_vbd_set_regions(lats_vbd* ctx, uint32_t regions);

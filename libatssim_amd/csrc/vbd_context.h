@@ -53,7 +53,6 @@ struct lats_vbd {
     float h = 0.f, h2inv = 0.f, damp = 0.f, h_energy = 0.f;
     std::vector<uint32_t> cstart, ccount, eoff, boff;
     std::vector<lats::ColourInfo> colours;   // host copy of colours_d
-    std::vector<lats::RegionSplit> region_split;   // per colour (colour-pass block mapping)
     uint32_t regions = 1;
     uint32_t colour_block = 256;   // threads per block of the colour passes (64, 128, 256)
     std::vector<uint32_t> perm, inv;

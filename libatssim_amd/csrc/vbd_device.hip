@@ -47,44 +47,39 @@ namespace {
 constexpr int kBlock = 256;
 
 // One colour pass, G lanes per vertex (G = 1: one thread per vertex).
-// The colour is cut into `rs.n` regions of consecutive vertex ids (regions
-// of the cloth); block b works on region b % n.  Workgroups are handed to the
-// XCDs round-robin, so with n = 8 each XCD updates the same band of the cloth
-// in every colour pass and the neighbours it gathers are mostly lines its own
-// L2 already holds; n = 1 is the plain linear mapping.
-// B threads per block (64, 128 or 256): smaller blocks spread a small colour
-// pass's waves more evenly over the CUs (results are identical).
+// The colour's index range is cut into gridDim.x bands of `band` consecutive
+// indices (a colour's slots are in ascending vertex id, so a band is a band
+// of the cloth) and block (r, j) works on band r.  Workgroups are handed to
+// the XCDs round-robin by linear id r + j * gridDim.x, so with 8 bands each
+// XCD updates the same band of the cloth in every colour pass and the
+// neighbours it gathers are mostly lines its own L2 already holds; one band
+// is the plain linear mapping.  B threads per block (64, 128 or 256): smaller
+// blocks spread a small colour pass's waves more evenly over the CUs
+// (results are identical).
+// The leading scalar arguments are all the first loads (the ELL prefetch)
+// need: they arrive in SGPRs with the wave (kernel argument preloading), so
+// those loads issue before any read of the argument block returns.
 template <int G, int B>
-__global__ __launch_bounds__(B) void k_vbd_colour(VbdArgs a, ColourInfo ci, RegionSplit rs) {
-    // Every kernel argument is named here first (no instruction), so all of
-    // them arrive in one scalar round trip: the argument block is otherwise
-    // loaded only past the early exit below, after the region bounds.
-    asm volatile("" ::"s"(a.x), "s"(a.x0), "s"(a.v), "s"(a.mass), "s"(a.meta), "s"(a.ell_e), "s"(a.ell_bq),
-                 "s"(a.ell_bi), "s"(a.fp), "s"(ci.start), "s"(ci.count), "s"(ci.eoff), "s"(ci.boff), "s"(ci.ew),
-                 "s"(ci.bw), "s"(rs.n), "s"(rs.bound[kMaxRegions]));
-    const uint32_t r = blockIdx.x % rs.n, j = blockIdx.x / rs.n;
-    // the region's bounds selected from the whole table (an indexed read of
-    // rs.bound[r] is a second round trip, after rs.n)
-    uint32_t lo = rs.bound[0], hi = rs.bound[1];
-#pragma unroll
-    for (uint32_t q = 1; q < kMaxRegions; q++)
-        if (q == r) {
-            lo = rs.bound[q];
-            hi = rs.bound[q + 1];
-        }
+__global__ __launch_bounds__(B) void k_vbd_colour(const uint2* ell_e, const float4* ell_bq, const uint4* ell_bi,
+                                                  uint32_t start, uint32_t count, uint32_t eoff, uint32_t boff,
+                                                  uint32_t ew, uint32_t bw, uint32_t band, VbdArgs a) {
+    a.ell_e = ell_e;
+    a.ell_bq = ell_bq;
+    a.ell_bi = ell_bi;
+    const uint32_t lo = blockIdx.x * band, j = blockIdx.y;
+    const uint32_t hi = min(lo + band, count);
     const F3 ext = F3{a.fp->ext_cur[0], a.fp->ext_cur[1], a.fp->ext_cur[2]};
-    const uint32_t n = ci.count;
     if (G == 1) {
         const uint32_t i = lo + j * B + threadIdx.x;
         if (i >= hi) return;
-        vbd_vertex_ext(a, ci.start + i, i, n, ci.eoff, ci.boff, ext);
+        vbd_vertex_ext(a, start + i, i, count, eoff, boff, ext);
     } else {
         const uint32_t first = lo + j * (B / G) + (threadIdx.x / 64) * (64 / G);
-        if (first >= hi) return;                  // the whole wave is past the region
+        if (first >= hi) return;                  // the whole wave is past the band
         const uint32_t i = lo + j * (B / G) + threadIdx.x / G, lane = threadIdx.x % G;
         const bool valid = i < hi;
-        vbd_vertex_group<G>(a, ci.start + (valid ? i : lo), valid ? i : lo, n, ci.eoff, ci.boff, ci.ew, ci.bw, ext,
-                            lane, valid);
+        vbd_vertex_group<G>(a, start + (valid ? i : lo), valid ? i : lo, count, eoff, boff, ew, bw, ext, lane,
+                            valid);
     }
 }
 
@@ -835,44 +830,31 @@ void enqueue_energy(lats_vbd* c, const float* xb, uint32_t row, hipStream_t st =
     enqueue_energy_partials(c, xb, c->partial.p + (size_t)row * c->energy_nblk, st);
 }
 
-// Cuts every colour into `n` regions of consecutive vertex ids (a colour's
-// slots are in ascending vertex id, so a region is an index range).
-void split_regions(lats_vbd* c, uint32_t n) {
-    const uint32_t per = (c->V + n - 1) / n;
-    c->region_split.assign(c->C, RegionSplit{});
-    for (uint32_t k = 0; k < c->C; k++) {
-        const uint32_t* ids = c->perm.data() + c->colours[k].start;
-        const uint32_t cnt = c->colours[k].count;
-        RegionSplit& rs = c->region_split[k];
-        rs.n = n;
-        for (uint32_t r = 0; r <= n; r++)
-            rs.bound[r] = r == n ? cnt : (uint32_t)(std::lower_bound(ids, ids + cnt, r * per) - ids);
-    }
-    c->regions = n;
-}
+// The colour passes cut every colour into `n` bands of equal index count.
+void split_regions(lats_vbd* c, uint32_t n) { c->regions = n; }
 
 template <int B>
-void launch_colour(lats_vbd* c, const VbdArgs& a, const ColourInfo& ci, const RegionSplit& rs, uint32_t most) {
-    const uint32_t per_block = B / c->lanes;
-    const dim3 grid(std::max(1u, rs.n * ((most + per_block - 1) / per_block))), block(B);
+void launch_colour(lats_vbd* c, const VbdArgs& a, const ColourInfo& ci) {
+    const uint32_t per_block = B / c->lanes, R = c->regions;
+    const uint32_t band = (ci.count + R - 1) / R;
+    const dim3 grid(R, std::max(1u, (band + per_block - 1) / per_block)), block(B);
+#define LATS_COLOUR_ARGS a.ell_e, a.ell_bq, a.ell_bi, ci.start, ci.count, ci.eoff, ci.boff, ci.ew, ci.bw, band, a
     switch (c->lanes) {
-        case 4: hipLaunchKernelGGL((k_vbd_colour<4, B>), grid, block, 0, c->stream, a, ci, rs); break;
-        case 8: hipLaunchKernelGGL((k_vbd_colour<8, B>), grid, block, 0, c->stream, a, ci, rs); break;
-        case 16: hipLaunchKernelGGL((k_vbd_colour<16, B>), grid, block, 0, c->stream, a, ci, rs); break;
-        default: hipLaunchKernelGGL((k_vbd_colour<1, B>), grid, block, 0, c->stream, a, ci, rs); break;
+        case 4: hipLaunchKernelGGL((k_vbd_colour<4, B>), grid, block, 0, c->stream, LATS_COLOUR_ARGS); break;
+        case 8: hipLaunchKernelGGL((k_vbd_colour<8, B>), grid, block, 0, c->stream, LATS_COLOUR_ARGS); break;
+        case 16: hipLaunchKernelGGL((k_vbd_colour<16, B>), grid, block, 0, c->stream, LATS_COLOUR_ARGS); break;
+        default: hipLaunchKernelGGL((k_vbd_colour<1, B>), grid, block, 0, c->stream, LATS_COLOUR_ARGS); break;
     }
+#undef LATS_COLOUR_ARGS
 }
 
 void enqueue_colour(lats_vbd* c, float* xb, uint32_t col) {
-    const RegionSplit& rs = c->region_split[col];
-    uint32_t most = 0;
-    for (uint32_t r = 0; r < rs.n; r++) most = std::max(most, rs.bound[r + 1] - rs.bound[r]);
     const VbdArgs a = c->args(xb);
     const ColourInfo& ci = c->colours[col];
     switch (c->colour_block) {
-        case 64: launch_colour<64>(c, a, ci, rs, most); break;
-        case 128: launch_colour<128>(c, a, ci, rs, most); break;
-        default: launch_colour<256>(c, a, ci, rs, most); break;
+        case 64: launch_colour<64>(c, a, ci); break;
+        case 128: launch_colour<128>(c, a, ci); break;
+        default: launch_colour<256>(c, a, ci); break;
     }
 }
 

@@ -17,13 +17,8 @@ struct ColourInfo {
     uint32_t start, count, eoff, boff, ew, bw;
 };
 
-// A colour cut into n regions of consecutive vertex ids: region r holds the
-// colour's indices [bound[r], bound[r + 1]).
+// Most bands a colour pass cuts a colour into (one per XCD).
 constexpr uint32_t kMaxRegions = 8;
-struct RegionSplit {
-    uint32_t n;
-    uint32_t bound[kMaxRegions + 1];
-};
 
 struct FrameParams {
     const float* forces;   // per-frame external force (3 floats / frame) or null
@@ -299,6 +294,9 @@ __device__ __forceinline__ void vbd_vertex_group(const VbdArgs& a, uint32_t s, u
         pq[k] = a.ell_bq[at];
         pi[k] = a.ell_bi[at];
     }
+    // the ELL loads go out first: their addresses need no argument-block read
+    // when the caller has the ELL bases and colour geometry in registers
+    __builtin_amdgcn_sched_barrier(0);
     const F3 xk = mem.ldx(a.x, s);
     const F3 x0 = mem.ld(a.x0, s);
     const F3 v0 = mem.ld(a.v, s);
