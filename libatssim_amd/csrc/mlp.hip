@@ -524,12 +524,10 @@ __global__ __launch_bounds__(kThreads) void k_mlp_features(FeatArgs f) {
     for (uint32_t j = 0; j < bdeg; j++) {
         const uint32_t k = co.boff + j * n + i;
         const float4 q = a.ell_bq[k];
-        const uint4 id = a.ell_bi[k];
-        const uint32_t o = id.x >> 30;
-        const F3 p0 = (o == 0) ? xp : ld3(a.x, id.x & 0x3fffffffu);
-        const F3 p1 = (o == 1) ? xp : ld3(a.x, id.y);
-        const F3 p2 = (o == 2) ? xp : ld3(a.x, id.z);
-        const F3 p3 = (o == 3) ? xp : ld3(a.x, id.w);
+        const PairIds id = a.ell_bi[k];
+        const uint32_t o = id.a >> 30;
+        F3 p0, p1, p2, p3;
+        pair_points(o, xp, ld3(a.x, id.a & 0x3fffffffu), ld3(a.x, id.b), ld3(a.x, id.c), p0, p1, p2, p3);
         bending_term(q, o, p0, p1, p2, p3, a.kb, hb, fb);
     }
     const float invd = edeg ? 1.f / (float)edeg : 0.f;

@@ -44,7 +44,7 @@ struct BatchArgs {
     const uint32_t* meta;
     const uint2* ell_e;
     const float4* ell_bq;
-    const uint4* ell_bi;
+    const PairIds* ell_bi;
     const ClothInfo* cloths;
     const uint32_t* order;     // cloth ids, largest first
     const ColourInfo* colours;
@@ -196,7 +196,7 @@ struct lats_batch {
     DBuf<uint32_t> meta, order, work;
     DBuf<uint2> ell_e;
     DBuf<float4> ell_bq;
-    DBuf<uint4> ell_bi;
+    DBuf<PairIds> ell_bi;
     DBuf<ClothInfo> cloths_d;
     DBuf<ColourInfo> colours_d;
     DBuf<FrameParams> fp;

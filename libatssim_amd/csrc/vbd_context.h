@@ -63,7 +63,8 @@ struct lats_vbd {
     lats::DBuf<uint32_t> meta, perm_d, inv_d;
     lats::DBuf<uint2> ell_e, e_slots;
     lats::DBuf<float4> ell_bq, Qd;
-    lats::DBuf<uint4> ell_bi, b_slots;
+    lats::DBuf<lats::PairIds> ell_bi;
+    lats::DBuf<uint4> b_slots;
     lats::DBuf<float> rest;
     lats::DBuf<lats::FrameParams> fp;
     lats::DBuf<float> forces;

@@ -60,7 +60,7 @@ constexpr int kBlock = 256;
 // need: they arrive in SGPRs with the wave (kernel argument preloading), so
 // those loads issue before any read of the argument block returns.
 template <int G, int B>
-__global__ __launch_bounds__(B) void k_vbd_colour(const uint2* ell_e, const float4* ell_bq, const uint4* ell_bi,
+__global__ __launch_bounds__(B) void k_vbd_colour(const uint2* ell_e, const float4* ell_bq, const PairIds* ell_bi,
                                                   uint32_t start, uint32_t count, uint32_t eoff, uint32_t boff,
                                                   uint32_t ew, uint32_t bw, uint32_t band, VbdArgs a) {
     a.ell_e = ell_e;
@@ -1366,7 +1366,7 @@ int lats_vbd_create(const lats_mesh_desc* m, const lats_scene_params* prm, lats_
     auto pad = [](auto& v, auto z) { if (v.empty()) v.push_back(z); };
     pad(L.ell_e, uint2{0, 0});
     pad(L.ell_bq, float4{0, 0, 0, 0});
-    pad(L.ell_bi, uint4{0, 0, 0, 0});
+    pad(L.ell_bi, PairIds{0, 0, 0});
     pad(L.e_slots, uint2{0, 0});
     pad(L.rest, 0.f);
     pad(L.b_slots, uint4{0, 0, 0, 0});
@@ -2257,7 +2257,7 @@ int lats_vbd_device_bytes(lats_vbd* c, uint64_t* bytes) {
     uint64_t b = 0;
     b += (c->x.n + c->x0.n + c->v.n + c->mass.n + c->rest.n + c->scratch3.n + c->forces.n + c->energy.n) * 4;
     b += (c->meta.n + c->perm_d.n + c->inv_d.n) * 4;
-    b += (c->ell_e.n + c->e_slots.n) * 8 + (c->ell_bq.n + c->Qd.n + c->ell_bi.n + c->b_slots.n) * 16;
+    b += (c->ell_e.n + c->e_slots.n) * 8 + (c->ell_bq.n + c->Qd.n + c->b_slots.n) * 16 + c->ell_bi.n * 12;
     b += c->partial.n * 8;
     for (int k = 0; k < LATS_NUM_ASYNC_BUFFERS; k++) b += (c->iterbuf[k].n + c->beginbuf[k].n) * 4;
     *bytes = b;

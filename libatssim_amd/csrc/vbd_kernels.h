@@ -17,6 +17,21 @@ struct ColourInfo {
     uint32_t start, count, eoff, boff, ew, bw;
 };
 
+// One incidence of a vertex in a bending pair: the pair's other three
+// slots in pair order, and in the top two bits of `a` the vertex's own
+// position o in the pair (the vertex's row of Q comes with it, in ell_bq).
+struct PairIds {
+    uint32_t a, b, c;
+};
+// The pair's four points in pair order: the vertex itself at position o,
+// the others (g0, g1, g2) around it.
+__device__ __forceinline__ void pair_points(uint32_t o, F3 xk, F3 g0, F3 g1, F3 g2, F3& p0, F3& p1, F3& p2, F3& p3) {
+    p0 = (o == 0) ? xk : g0;
+    p1 = (o == 1) ? xk : (o == 0) ? g0 : g1;
+    p2 = (o == 2) ? xk : (o <= 1) ? g1 : g2;
+    p3 = (o == 3) ? xk : g2;
+}
+
 // Most bands a colour pass cuts a colour into (one per XCD).
 constexpr uint32_t kMaxRegions = 8;
 
@@ -36,7 +51,7 @@ struct VbdArgs {
     const uint32_t* meta;
     const uint2* ell_e;
     const float4* ell_bq;
-    const uint4* ell_bi;
+    const PairIds* ell_bi;
     const FrameParams* fp;
     float h, h2inv, ks, kb;
     float g[3];
@@ -187,12 +202,10 @@ __device__ __forceinline__ void vbd_vertex_ext(const VbdArgs& a, uint32_t s, uin
     for (uint32_t j = 0; j < bdeg; j++) {
         const uint32_t k = boff + j * n + i;
         const float4 q = a.ell_bq[k];
-        const uint4 id = a.ell_bi[k];
-        const uint32_t o = id.x >> 30;
-        const F3 p0 = (o == 0) ? xk : mem.ldx(a.x, id.x & 0x3fffffffu);
-        const F3 p1 = (o == 1) ? xk : mem.ldx(a.x, id.y);
-        const F3 p2 = (o == 2) ? xk : mem.ldx(a.x, id.z);
-        const F3 p3 = (o == 3) ? xk : mem.ldx(a.x, id.w);
+        const PairIds id = a.ell_bi[k];
+        const uint32_t o = id.a >> 30;
+        F3 p0, p1, p2, p3;
+        pair_points(o, xk, mem.ldx(a.x, id.a & 0x3fffffffu), mem.ldx(a.x, id.b), mem.ldx(a.x, id.c), p0, p1, p2, p3);
         bending_term(q, o, p0, p1, p2, p3, a.kb, hb, fb);
     }
     H.xx = H.xx + hb;
@@ -245,10 +258,6 @@ __device__ __forceinline__ void spring_term_masked(const VbdArgs& a, F3 xk, bool
     t = Sym3{on ? u.xx : 0.f, on ? u.xy : 0.f, on ? u.xz : 0.f, on ? u.yy : 0.f, on ? u.yz : 0.f, on ? u.zz : 0.f};
     tf = F3{on ? uf.x : 0.f, on ? uf.y : 0.f, on ? uf.z : 0.f};
 }
-template <class Mem>
-__device__ __forceinline__ F3 pair_point(const Mem& mem, const VbdArgs& a, F3 xk, uint32_t o, uint32_t k, uint32_t id) {
-    return (o == k) ? xk : mem.ldx(a.x, id);
-}
 
 // The same update with G lanes per vertex (lane = 0..G-1 inside an aligned
 // group of G lanes).  Lane j computes the terms of incident springs / pairs
@@ -276,7 +285,7 @@ __device__ __forceinline__ void vbd_vertex_group(const VbdArgs& a, uint32_t s, u
     // it before the next one, which serialises the prefetch rounds.
     uint2 pe[PE];
     float4 pq[PB];
-    uint4 pi[PB];
+    PairIds pi[PB];
     // (index arithmetic without branches: a zero width multiplies the index
     // by zero, so no load waits behind a branch on the width)
     const uint32_t e_on = ew != 0u, b_on = bw != 0u;
@@ -313,16 +322,12 @@ __device__ __forceinline__ void vbd_vertex_group(const VbdArgs& a, uint32_t s, u
     F3 pxo[PE];
 #pragma unroll
     for (uint32_t k = 0; k < PE; k++) pxo[k] = mem.ldx(a.x, pe[k].x);
-    F3 pp[PB][4];
+    F3 pg[PB][3];
 #pragma unroll
     for (uint32_t k = 0; k < PB; k++) {
-        const uint32_t o = pi[k].x >> 30;
-        const F3 q0 = mem.ldx(a.x, pi[k].x & 0x3fffffffu), q1 = mem.ldx(a.x, pi[k].y);
-        const F3 q2 = mem.ldx(a.x, pi[k].z), q3 = mem.ldx(a.x, pi[k].w);
-        pp[k][0] = (o == 0) ? xk : q0;
-        pp[k][1] = (o == 1) ? xk : q1;
-        pp[k][2] = (o == 2) ? xk : q2;
-        pp[k][3] = (o == 3) ? xk : q3;
+        pg[k][0] = mem.ldx(a.x, pi[k].a & 0x3fffffffu);
+        pg[k][1] = mem.ldx(a.x, pi[k].b);
+        pg[k][2] = mem.ldx(a.x, pi[k].c);
     }
     __builtin_amdgcn_sched_barrier(0);   // every gather issued before the arithmetic
 
@@ -391,7 +396,12 @@ __device__ __forceinline__ void vbd_vertex_group(const VbdArgs& a, uint32_t s, u
         if (r >= bdeg) break;
         float th = 0.f;
         F3 tf = F3{0.f, 0.f, 0.f};
-        if (r + lane < bdeg) bending_term(pq[k], pi[k].x >> 30, pp[k][0], pp[k][1], pp[k][2], pp[k][3], a.kb, th, tf);
+        if (r + lane < bdeg) {
+            const uint32_t o = pi[k].a >> 30;
+            F3 p0, p1, p2, p3;
+            pair_points(o, xk, pg[k][0], pg[k][1], pg[k][2], p0, p1, p2, p3);
+            bending_term(pq[k], o, p0, p1, p2, p3, a.kb, th, tf);
+        }
         add_pair_round(th, tf);
     }
     for (uint32_t r = PB * G; r < bdeg; r += G) {
@@ -401,10 +411,12 @@ __device__ __forceinline__ void vbd_vertex_group(const VbdArgs& a, uint32_t s, u
         if (j < bdeg) {
             const uint32_t k = boff + j * n + i;
             const float4 q = a.ell_bq[k];
-            const uint4 id = a.ell_bi[k];
-            const uint32_t o = id.x >> 30;
-            bending_term(q, o, pair_point(mem, a, xk, o, 0, id.x & 0x3fffffffu), pair_point(mem, a, xk, o, 1, id.y),
-                         pair_point(mem, a, xk, o, 2, id.z), pair_point(mem, a, xk, o, 3, id.w), a.kb, th, tf);
+            const PairIds id = a.ell_bi[k];
+            const uint32_t o = id.a >> 30;
+            F3 p0, p1, p2, p3;
+            pair_points(o, xk, mem.ldx(a.x, id.a & 0x3fffffffu), mem.ldx(a.x, id.b), mem.ldx(a.x, id.c), p0, p1, p2,
+                        p3);
+            bending_term(q, o, p0, p1, p2, p3, a.kb, th, tf);
         }
         add_pair_round(th, tf);
     }

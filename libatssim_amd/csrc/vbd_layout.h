@@ -24,7 +24,7 @@ struct ClothLayout {
     std::vector<ColourInfo> colours;   // global slots and ELL bases
     std::vector<uint2> ell_e;
     std::vector<float4> ell_bq;
-    std::vector<uint4> ell_bi;
+    std::vector<PairIds> ell_bi;
     std::vector<uint2> e_slots;        // per edge (energy)
     std::vector<float> rest;
     std::vector<uint4> b_slots;        // per bending pair (energy)
@@ -125,7 +125,7 @@ inline bool append_cloth(const lats_mesh_desc& m, ClothLayout& L, std::vector<ui
         L.colours.push_back(ColourInfo{start, n, (uint32_t)eoff, (uint32_t)boff, emax, bmax});
         L.ell_e.resize(eoff + (size_t)emax * n, uint2{0, 0});
         L.ell_bq.resize(boff + (size_t)bmax * n, float4{0, 0, 0, 0});
-        L.ell_bi.resize(boff + (size_t)bmax * n, uint4{0, 0, 0, 0});
+        L.ell_bi.resize(boff + (size_t)bmax * n, PairIds{0, 0, 0});
         for (uint32_t i = 0; i < n; i++) {
             const uint32_t vid = cl[C + 1 + (start - (uint32_t)base) + i];
             for (uint32_t j = 0, r = pe[vid]; r < pe[vid + 1]; r++, j++) {
@@ -150,7 +150,10 @@ inline bool append_cloth(const lats_mesh_desc& m, ClothLayout& L, std::vector<ui
                 const float* Q = &m.bending_Q[16 * (size_t)b + 4 * o];
                 const size_t at = boff + (size_t)j * n + i;
                 L.ell_bq[at] = float4{Q[0], Q[1], Q[2], Q[3]};
-                L.ell_bi[at] = uint4{inv[q4[0]] | (o << 30), inv[q4[1]], inv[q4[2]], inv[q4[3]]};
+                uint32_t others[3], no = 0;
+                for (uint32_t t = 0; t < 4; t++)
+                    if (t != o) others[no++] = inv[q4[t]];
+                L.ell_bi[at] = PairIds{others[0] | (o << 30), others[1], others[2]};
             }
         }
     }
