@@ -378,6 +378,8 @@ def bench_sphere(args):
         g.set_lanes_per_vertex(args.lanes)
     if args.regions:
         g.set_regions(args.regions)
+    if args.colour_block:
+        g.set_colour_block(args.colour_block)
     g.run_frames(12)
     if args.strategy == "auto":
         # three timed frames per candidate: one frame (~10 ms) is too short to
@@ -387,6 +389,8 @@ def bench_sphere(args):
             g.set_lanes_per_vertex(args.lanes)
         if args.regions:
             g.set_regions(args.regions)
+        if args.colour_block:
+            g.set_colour_block(args.colour_block)
     else:
         strategy, blocks = g.strategy()
     if W:
@@ -601,7 +605,9 @@ def bench_launch_floor(args):
     frame graph of 7 colours x 100 sweeps replays its passes), and one grid
     barrier inside a cooperative kernel (800 barriers), at the thread counts
     of the 256x256 colour passes (8,192-vertex colours x 8 lanes = 65,536
-    threads) and of 256 / 64 blocks for the barrier."""
+    threads) and of 256 / 64 blocks for the barrier; and launches that re-read
+    one 2 MiB buffer (for the FETCH_SIZE counter: does a launch find the
+    previous launch's lines in L2?)."""
     import ctypes
     from libatssim_amd import _lib
     L = _lib.lib()
@@ -617,7 +623,8 @@ def bench_launch_floor(args):
            "gather_65536_threads": floor(_lib.LATS_FLOOR_GATHER, 65536),
            "gather_8192_threads": floor(_lib.LATS_FLOOR_GATHER, 8192),
            "grid_barrier_256_blocks": floor(_lib.LATS_FLOOR_GRID_BARRIER, 256 * 256),
-           "grid_barrier_64_blocks": floor(_lib.LATS_FLOOR_GRID_BARRIER, 64 * 256)}
+           "grid_barrier_64_blocks": floor(_lib.LATS_FLOOR_GRID_BARRIER, 64 * 256),
+           "reread_2MiB_buffer": floor(_lib.LATS_FLOOR_REREAD, 131072)}
     print(json.dumps(out), flush=True)
 
 

@@ -262,7 +262,7 @@ atexit.register(close_all)
 
 
 MEASURE_PATH = os.path.join(_HERE, "lib", "liblats_measure.so")
-LATS_FLOOR_EMPTY, LATS_FLOOR_GATHER, LATS_FLOOR_GRID_BARRIER = 0, 1, 2
+LATS_FLOOR_EMPTY, LATS_FLOOR_GATHER, LATS_FLOOR_GRID_BARRIER, LATS_FLOOR_REREAD = 0, 1, 2, 3
 _MEASURE = None
 
 

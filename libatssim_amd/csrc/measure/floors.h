@@ -19,10 +19,15 @@ extern "C" {
  * dependent ELL -> position chain of a colour pass, without its math); or
  * mean microseconds per barrier of one cooperative kernel of threads / 256
  * co-resident blocks crossing `launches` grid-wide barriers (GRID_BARRIER,
- * bounded waits: LATS_ERR_TIMEOUT if one gives up). */
+ * bounded waits: LATS_ERR_TIMEOUT if one gives up); or mean microseconds
+ * per launch of a graph of kernels that each read the same buffer of
+ * `threads` float4s, block b the same lines in every launch (REREAD: under
+ * the FETCH_SIZE counter it shows whether a launch finds the previous
+ * launch's lines in L2). */
 #define LATS_FLOOR_EMPTY 0
 #define LATS_FLOOR_GATHER 1
 #define LATS_FLOOR_GRID_BARRIER 2
+#define LATS_FLOOR_REREAD 3
 int lats_time_floor(int kind, uint32_t threads, uint32_t launches, float* us);
 
 #ifdef __cplusplus

@@ -1,6 +1,7 @@
 // floors.hip -- latency floors of the GPU at the colour-pass shape: what an
 // empty kernel, a kernel doing one dependent gather, and a grid-wide barrier
-// inside one cooperative kernel cost.  Measurement support for the colour
+// inside one cooperative kernel cost; and a kernel re-reading one buffer in
+// every launch, for the L2 counters.  Measurement support for the colour
 // pass's roofline reading (a pass at 256x256 moves ~3 MB, 0.4 us of HBM time,
 // so its floor is set by these latencies, not by bandwidth).  No reference
 // counterpart.
@@ -46,6 +47,17 @@ __global__ __launch_bounds__(kB) void k_floor_gather(const uint32_t* idx, const 
 #pragma unroll
     for (uint32_t k = 0; k < kGathers; k++) s += x[idx[(size_t)k * n + t]];
     y[t] = s;
+}
+
+// Each thread reads its float4 of a buffer that the same blocks read in
+// every launch (the buffer is zero-filled, so the store never happens).
+// Under the PMC counters (FETCH_SIZE per dispatch) this shows whether a
+// launch finds the lines the previous launch read in L2.
+__global__ __launch_bounds__(kB) void k_floor_reread(const float4* x, float* y, uint32_t n) {
+    const uint32_t t = blockIdx.x * kB + threadIdx.x;
+    if (t >= n) return;
+    const float4 v = x[t];
+    if ((v.x + v.y) + (v.z + v.w) > 1e30f) y[0] = 0.f;
 }
 
 // `rounds` grid barriers over a monotonically increasing arrival counter.
@@ -99,7 +111,7 @@ struct Stream {
 
 extern "C" int lats_time_floor(int kind, uint32_t threads, uint32_t launches, float* us) {
     if (!us) return fail(LATS_ERR_INVALID_ARGUMENT, "null argument");
-    if (kind < LATS_FLOOR_EMPTY || kind > LATS_FLOOR_GRID_BARRIER)
+    if (kind < LATS_FLOOR_EMPTY || kind > LATS_FLOOR_REREAD)
         return fail(LATS_ERR_INVALID_ARGUMENT, "unknown floor kind");
     if (threads == 0 || threads > (1u << 24) || launches == 0 || launches > 100000)
         return fail(LATS_ERR_INVALID_ARGUMENT, "threads must be in 1..2^24 and launches in 1..100000");
@@ -156,12 +168,19 @@ extern "C" int lats_time_floor(int kind, uint32_t threads, uint32_t launches, fl
         FCHK(hipMemset(x.p, 0, (size_t)nx * 4));
         FCHK(hipMalloc(&y.p, (size_t)threads * 4));
     }
+    if (kind == LATS_FLOOR_REREAD) {
+        FCHK(hipMalloc(&x.p, (size_t)threads * 16));
+        FCHK(hipMemset(x.p, 0, (size_t)threads * 16));
+    }
     // one graph of `launches` back-to-back kernels, as a frame graph runs its colour passes
     hipGraph_t graph = nullptr;
     FCHK(hipStreamBeginCapture(st.s, hipStreamCaptureModeThreadLocal));
     for (uint32_t l = 0; l < launches; l++) {
         if (kind == LATS_FLOOR_EMPTY)
             hipLaunchKernelGGL(k_floor_empty, dim3(blocks), dim3(kB), 0, st.s, static_cast<float*>(p.p));
+        else if (kind == LATS_FLOOR_REREAD)
+            hipLaunchKernelGGL(k_floor_reread, dim3(blocks), dim3(kB), 0, st.s, static_cast<const float4*>(x.p),
+                               static_cast<float*>(p.p), threads);
         else
             hipLaunchKernelGGL(k_floor_gather, dim3(blocks), dim3(kB), 0, st.s, static_cast<const uint32_t*>(idx.p),
                                static_cast<const float*>(x.p), static_cast<float*>(y.p), threads);

@@ -21,7 +21,9 @@ def test_floor_kinds(gpu_available):
     empty = _floor(_lib.LATS_FLOOR_EMPTY, 65536)
     gather = _floor(_lib.LATS_FLOOR_GATHER, 65536)
     barrier = _floor(_lib.LATS_FLOOR_GRID_BARRIER, 64 * 256)
+    reread = _floor(_lib.LATS_FLOOR_REREAD, 131072)
     assert 0.0 < empty < 1000.0
+    assert 0.0 < reread < 1000.0
     assert gather >= 0.5 * empty
     assert 0.0 < barrier < 10000.0
 
@@ -29,5 +31,6 @@ def test_floor_kinds(gpu_available):
 def test_floor_rejects(gpu_available):
     us = ctypes.c_float()
     assert _lib.measure_lib().lats_time_floor(7, 64, 10, ctypes.byref(us)) == 1
+    assert _lib.measure_lib().lats_time_floor(4, 64, 10, ctypes.byref(us)) == 1
     assert _lib.measure_lib().lats_time_floor(_lib.LATS_FLOOR_EMPTY, 0, 10, ctypes.byref(us)) == 1
     assert _lib.measure_lib().lats_time_floor(_lib.LATS_FLOOR_GRID_BARRIER, 1 << 24, 10, ctypes.byref(us)) == 1
