@@ -605,9 +605,10 @@ def bench_launch_floor(args):
     frame graph of 7 colours x 100 sweeps replays its passes), and one grid
     barrier inside a cooperative kernel (800 barriers), at the thread counts
     of the 256x256 colour passes (8,192-vertex colours x 8 lanes = 65,536
-    threads) and of 256 / 64 blocks for the barrier; and launches that re-read
+    threads) and of 256 / 64 blocks for the barrier; launches that re-read
     one 2 MiB buffer (for the FETCH_SIZE counter: does a launch find the
-    previous launch's lines in L2?)."""
+    previous launch's lines in L2?); and the hand-off between neighbouring
+    blocks of one persistent kernel (per phase, with and without fences)."""
     import ctypes
     from libatssim_amd import _lib
     L = _lib.lib()
@@ -624,7 +625,10 @@ def bench_launch_floor(args):
            "gather_8192_threads": floor(_lib.LATS_FLOOR_GATHER, 8192),
            "grid_barrier_256_blocks": floor(_lib.LATS_FLOOR_GRID_BARRIER, 256 * 256),
            "grid_barrier_64_blocks": floor(_lib.LATS_FLOOR_GRID_BARRIER, 64 * 256),
-           "reread_2MiB_buffer": floor(_lib.LATS_FLOOR_REREAD, 131072)}
+           "reread_2MiB_buffer": floor(_lib.LATS_FLOOR_REREAD, 131072),
+           "handoff_fenced_256_blocks": floor(_lib.LATS_FLOOR_HANDOFF_FENCED, 256 * 256),
+           "handoff_relaxed_256_blocks": floor(_lib.LATS_FLOOR_HANDOFF_RELAXED, 256 * 256),
+           "handoff_relaxed_same_xcd_256_blocks": floor(_lib.LATS_FLOOR_HANDOFF_XCD, 256 * 256)}
     print(json.dumps(out), flush=True)
 
 

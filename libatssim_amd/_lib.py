@@ -263,6 +263,7 @@ atexit.register(close_all)
 
 MEASURE_PATH = os.path.join(_HERE, "lib", "liblats_measure.so")
 LATS_FLOOR_EMPTY, LATS_FLOOR_GATHER, LATS_FLOOR_GRID_BARRIER, LATS_FLOOR_REREAD = 0, 1, 2, 3
+LATS_FLOOR_HANDOFF_FENCED, LATS_FLOOR_HANDOFF_RELAXED, LATS_FLOOR_HANDOFF_XCD = 4, 5, 6
 _MEASURE = None
 
 

@@ -23,11 +23,19 @@ extern "C" {
  * per launch of a graph of kernels that each read the same buffer of
  * `threads` float4s, block b the same lines in every launch (REREAD: under
  * the FETCH_SIZE counter it shows whether a launch finds the previous
- * launch's lines in L2). */
+ * launch's lines in L2); or mean microseconds per phase of one cooperative
+ * kernel of threads / 256 co-resident blocks in which block b starts phase q
+ * once its two neighbours published q - 1 (HANDOFF_*: neighbours one block
+ * apart, on other XCDs, with agent-scope acquire / release fences around the
+ * hand-off or with relaxed device-scope accesses only; HANDOFF_XCD:
+ * neighbours eight blocks apart, on the same XCD, relaxed; bounded waits). */
 #define LATS_FLOOR_EMPTY 0
 #define LATS_FLOOR_GATHER 1
 #define LATS_FLOOR_GRID_BARRIER 2
 #define LATS_FLOOR_REREAD 3
+#define LATS_FLOOR_HANDOFF_FENCED 4
+#define LATS_FLOOR_HANDOFF_RELAXED 5
+#define LATS_FLOOR_HANDOFF_XCD 6
 int lats_time_floor(int kind, uint32_t threads, uint32_t launches, float* us);
 
 #ifdef __cplusplus

@@ -1,7 +1,8 @@
 // floors.hip -- latency floors of the GPU at the colour-pass shape: what an
 // empty kernel, a kernel doing one dependent gather, and a grid-wide barrier
-// inside one cooperative kernel cost; and a kernel re-reading one buffer in
-// every launch, for the L2 counters.  Measurement support for the colour
+// inside one cooperative kernel cost, what a hand-off between neighbouring
+// blocks of one persistent kernel costs, with and without fences; and a
+// kernel re-reading one buffer in every launch, for the L2 counters.  Measurement support for the colour
 // pass's roofline reading (a pass at 256x256 moves ~3 MB, 0.4 us of HBM time,
 // so its floor is set by these latencies, not by bandwidth).  No reference
 // counterpart.
@@ -60,6 +61,43 @@ __global__ __launch_bounds__(kB) void k_floor_reread(const float4* x, float* y, 
     if ((v.x + v.y) + (v.z + v.w) > 1e30f) y[0] = 0.f;
 }
 
+// `rounds` neighbour hand-offs: block b starts phase q once blocks b - d and
+// b + d (mod the grid) have published q - 1, then publishes q (the shape of a
+// persistent colour pass that synchronises with its neighbouring tiles
+// only).  fenced: an agent-scope acquire fence after the wait and a release
+// fence before the publish (the C++ memory model's hand-off; on this GPU
+// they write back and invalidate L2); otherwise relaxed device-scope
+// accesses only.  Blocks are co-resident (cooperative launch) and every wait
+// is bounded: a block that gives up records it and every block then leaves.
+__global__ __launch_bounds__(kB) void k_floor_handoff(uint32_t* prog, uint32_t* timed_out, uint32_t rounds,
+                                                      uint32_t d, int fenced) {
+    __shared__ uint32_t s_stop;
+    const uint32_t n = gridDim.x, b = blockIdx.x;
+    const uint32_t lo = (b + n - d % n) % n, hi = (b + d) % n;
+    for (uint32_t q = 1; q <= rounds; q++) {
+        if (threadIdx.x == 0) {
+            uint32_t polls = 0;
+            while (__hip_atomic_load(prog + lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < q - 1 ||
+                   __hip_atomic_load(prog + hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < q - 1) {
+                if (++polls > kSpinLimit ||
+                    __hip_atomic_load(timed_out, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
+                    atomicOr(timed_out, 1u);
+                    break;
+                }
+            }
+            if (fenced) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            s_stop = __hip_atomic_load(timed_out, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        __syncthreads();
+        if (s_stop) return;   // the whole block (thread 0 decided for it)
+        if (threadIdx.x == 0) {
+            if (fenced) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            __hip_atomic_store(prog + b, q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        __syncthreads();
+    }
+}
+
 // `rounds` grid barriers over a monotonically increasing arrival counter.
 // The blocks are co-resident (cooperative launch); every wait is bounded
 // anyway: after kSpinLimit polls the block records the failure and leaves
@@ -111,7 +149,7 @@ struct Stream {
 
 extern "C" int lats_time_floor(int kind, uint32_t threads, uint32_t launches, float* us) {
     if (!us) return fail(LATS_ERR_INVALID_ARGUMENT, "null argument");
-    if (kind < LATS_FLOOR_EMPTY || kind > LATS_FLOOR_REREAD)
+    if (kind < LATS_FLOOR_EMPTY || kind > LATS_FLOOR_HANDOFF_XCD)
         return fail(LATS_ERR_INVALID_ARGUMENT, "unknown floor kind");
     if (threads == 0 || threads > (1u << 24) || launches == 0 || launches > 100000)
         return fail(LATS_ERR_INVALID_ARGUMENT, "threads must be in 1..2^24 and launches in 1..100000");
@@ -152,6 +190,38 @@ extern "C" int lats_time_floor(int kind, uint32_t threads, uint32_t launches, fl
         uint32_t to = 0;
         FCHK(hipMemcpy(&to, flag, 4, hipMemcpyDeviceToHost));
         if (to) return fail(LATS_ERR_TIMEOUT, "grid barrier wait exceeded its poll limit");
+        *us = best * 1e3f / (float)launches;
+        return LATS_OK;
+    }
+    if (kind == LATS_FLOOR_HANDOFF_FENCED || kind == LATS_FLOOR_HANDOFF_RELAXED || kind == LATS_FLOOR_HANDOFF_XCD) {
+        int dev = 0, per_cu = 0;
+        hipDeviceProp_t prop;
+        FCHK(hipGetDevice(&dev));
+        FCHK(hipGetDeviceProperties(&prop, dev));
+        FCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_floor_handoff, kB, 0));
+        if ((uint64_t)blocks > (uint64_t)per_cu * prop.multiProcessorCount)
+            return fail(LATS_ERR_INVALID_ARGUMENT, "more blocks than can be co-resident");
+        FCHK(hipMalloc(&bar.p, 4 * ((size_t)blocks + 1)));
+        uint32_t* prog = static_cast<uint32_t*>(bar.p);
+        uint32_t* flag = prog + blocks;
+        uint32_t rounds = launches;
+        // neighbours one block apart sit on different XCDs (workgroups go to
+        // the XCDs round-robin); eight apart, on the same one
+        uint32_t d = kind == LATS_FLOOR_HANDOFF_XCD ? 8u : 1u;
+        int fenced = kind == LATS_FLOOR_HANDOFF_FENCED;
+        void* kargs[] = {&prog, &flag, &rounds, &d, &fenced};
+        float best = 0.f;
+        for (int rep = 0; rep < 2; rep++) {   // the first launch warms up; the second is timed
+            FCHK(hipMemsetAsync(bar.p, 0, 4 * ((size_t)blocks + 1), st.s));
+            FCHK(hipEventRecord(e0.e, st.s));
+            FCHK(hipLaunchCooperativeKernel((const void*)k_floor_handoff, dim3(blocks), dim3(kB), kargs, 0, st.s));
+            FCHK(hipEventRecord(e1.e, st.s));
+            FCHK(hipStreamSynchronize(st.s));
+            FCHK(hipEventElapsedTime(&best, e0.e, e1.e));
+        }
+        uint32_t to = 0;
+        FCHK(hipMemcpy(&to, flag, 4, hipMemcpyDeviceToHost));
+        if (to) return fail(LATS_ERR_TIMEOUT, "hand-off wait exceeded its poll limit");
         *us = best * 1e3f / (float)launches;
         return LATS_OK;
     }

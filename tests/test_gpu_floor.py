@@ -22,6 +22,8 @@ def test_floor_kinds(gpu_available):
     gather = _floor(_lib.LATS_FLOOR_GATHER, 65536)
     barrier = _floor(_lib.LATS_FLOOR_GRID_BARRIER, 64 * 256)
     reread = _floor(_lib.LATS_FLOOR_REREAD, 131072)
+    for kind in (_lib.LATS_FLOOR_HANDOFF_FENCED, _lib.LATS_FLOOR_HANDOFF_RELAXED, _lib.LATS_FLOOR_HANDOFF_XCD):
+        assert 0.0 < _floor(kind, 64 * 256) < 10000.0
     assert 0.0 < empty < 1000.0
     assert 0.0 < reread < 1000.0
     assert gather >= 0.5 * empty
@@ -31,6 +33,6 @@ def test_floor_kinds(gpu_available):
 def test_floor_rejects(gpu_available):
     us = ctypes.c_float()
     assert _lib.measure_lib().lats_time_floor(7, 64, 10, ctypes.byref(us)) == 1
-    assert _lib.measure_lib().lats_time_floor(4, 64, 10, ctypes.byref(us)) == 1
+    assert _lib.measure_lib().lats_time_floor(-1, 64, 10, ctypes.byref(us)) == 1
     assert _lib.measure_lib().lats_time_floor(_lib.LATS_FLOOR_EMPTY, 0, 10, ctypes.byref(us)) == 1
     assert _lib.measure_lib().lats_time_floor(_lib.LATS_FLOOR_GRID_BARRIER, 1 << 24, 10, ctypes.byref(us)) == 1
