@@ -79,7 +79,7 @@ __device__ __forceinline__ void batch_cloth(const BatchArgs& p, const ClothInfo&
             const ColourInfo co = p.colours[ci.cbase + c];
             if (G == 1) {
                 for (uint32_t i = tid; i < co.count; i += nt)
-                    vbd_vertex_ext(a, co.start + i, i, co.count, co.eoff, co.boff, ext, mem);
+                    vbd_vertex_ext(a, co.start + i, i, co.count, co.eoff, co.boff, co.ew, co.bw, ext, mem);
             } else {
                 // groups of G lanes; a wave leaves the colour together
                 for (uint32_t i0 = 0; i0 < co.count; i0 += nt / G) {

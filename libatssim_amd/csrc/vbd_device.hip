@@ -72,7 +72,7 @@ __global__ __launch_bounds__(B) void k_vbd_colour(const uint2* ell_e, const floa
     if (G == 1) {
         const uint32_t i = lo + j * B + threadIdx.x;
         if (i >= hi) return;
-        vbd_vertex_ext(a, start + i, i, count, eoff, boff, ext);
+        vbd_vertex_ext(a, start + i, i, count, eoff, boff, ew, bw, ext);
     } else {
         const uint32_t first = lo + j * (B / G) + (threadIdx.x / 64) * (64 / G);
         if (first >= hi) return;                  // the whole wave is past the band
@@ -348,7 +348,7 @@ __global__ __launch_bounds__(kBlock) void k_vbd_persistent(PersistArgs p) {
                 const ColourInfo ci = p.colours[c];
                 if (G == 1) {
                     for (uint32_t i = t0; i < ci.count; i += stride)
-                        vbd_vertex_ext(p.a, ci.start + i, i, ci.count, ci.eoff, ci.boff, ext);
+                        vbd_vertex_ext(p.a, ci.start + i, i, ci.count, ci.eoff, ci.boff, ci.ew, ci.bw, ext);
                 } else {
                     // a whole wave moves through the colour together
                     for (uint32_t i = t0 / G; i - (i % (64 / G)) < ci.count; i += stride / G) {
@@ -479,7 +479,7 @@ __global__ __launch_bounds__(kBlock) void k_vbd_flow(FlowArgs p) {
                 const uint2 r = p.trange[t * C + c];
                 if (G == 1) {
                     for (uint32_t i = r.x + tid; i < r.y; i += kBlock)
-                        vbd_vertex_ext<Mem>(p.a, ci.start + i, i, ci.count, ci.eoff, ci.boff, ext);
+                        vbd_vertex_ext<Mem>(p.a, ci.start + i, i, ci.count, ci.eoff, ci.boff, ci.ew, ci.bw, ext);
                 } else {
                     // groups of G lanes; a wave leaves together
                     for (uint32_t i0 = r.x; i0 < r.y; i0 += kBlock / G) {

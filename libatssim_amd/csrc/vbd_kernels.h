@@ -165,14 +165,64 @@ struct LdsMem {
     }
 };
 
-// One VBD vertex update: VBD::compute_inertia + compute_stretch_mass_spring +
-// compute_bending_quadratic (xpbd_constraints.h:4027-4205) summed as
-// Hf = inertia; Hf += sum(stretch); Hf += sum(bending)
-// (example2_async_iter.cpp:939-992), then the guarded 3x3 Newton step of
-// vbd_step (:993-1014).  i = index inside the colour, n = colour size.
+// One incident spring / bending pair term, zero for lanes past the degree.
+__device__ __forceinline__ void spring_term_of(const VbdArgs& a, F3 xk, bool on, uint2 e, F3 xo, Sym3& t, F3& tf) {
+    t = sym_zero();
+    tf = F3{0.f, 0.f, 0.f};
+    if (on) stretch_values(xk, xo, __uint_as_float(e.y), a.ks, t, tf);
+}
+// The same, computed on every lane and masked afterwards: a lane past the
+// degree works on a valid placeholder entry (finite: l >= epsilon).  Used
+// where the neighbour was gathered ahead: computing the term unconditionally
+// keeps that gather out of a branch, so it is issued with the others.
+__device__ __forceinline__ void spring_term_masked(const VbdArgs& a, F3 xk, bool on, uint2 e, F3 xo, Sym3& t,
+                                                   F3& tf) {
+    Sym3 u;
+    F3 uf;
+    stretch_values(xk, xo, __uint_as_float(e.y), a.ks, u, uf);
+    t = Sym3{on ? u.xx : 0.f, on ? u.xy : 0.f, on ? u.xz : 0.f, on ? u.yy : 0.f, on ? u.yz : 0.f, on ? u.zz : 0.f};
+    tf = F3{on ? uf.x : 0.f, on ? uf.y : 0.f, on ? uf.z : 0.f};
+}
+
+// One VBD vertex update on one lane: VBD::compute_inertia +
+// compute_stretch_mass_spring + compute_bending_quadratic
+// (xpbd_constraints.h:4027-4205) summed as Hf = inertia; Hf += sum(stretch);
+// Hf += sum(bending) (example2_async_iter.cpp:939-992), then the guarded 3x3
+// Newton step of vbd_step (:993-1014).  i = index inside the colour, n =
+// colour size, ew / bw = the colour's ELL widths.
+// The incidences are loaded ahead of their use, in chunks: the first chunk
+// of kOneChunk spring entries and of kOneChunk bending entries together with
+// the vertex's own state, every later chunk in one go, each from rows
+// clamped to the colour's width (padding entries are valid slots), so a
+// vertex waits for about (edeg + bdeg) / kOneChunk ELL round trips rather
+// than one per incidence.  The sums run in incidence order as in the
+// reference; a spring past the degree adds a +0 term (a sum that starts at
+// +0 never becomes -0, so +0 changes nothing), a pair past the degree adds
+// nothing.
+constexpr uint32_t kOneChunk = 4;
 template <class Mem = PlainMem>
 __device__ __forceinline__ void vbd_vertex_ext(const VbdArgs& a, uint32_t s, uint32_t i, uint32_t n, uint32_t eoff,
-                                               uint32_t boff, F3 ext, Mem mem = Mem{}) {
+                                               uint32_t boff, uint32_t ew, uint32_t bw, F3 ext, Mem mem = Mem{}) {
+    const uint32_t e_on = ew != 0u, b_on = bw != 0u;
+    uint2 pe[kOneChunk];
+    float4 pq[kOneChunk];
+    PairIds pi[kOneChunk];
+    auto load_springs = [&](uint32_t r0) {
+#pragma unroll
+        for (uint32_t k = 0; k < kOneChunk; k++)
+            pe[k] = a.ell_e[(eoff + (r0 + k < ew ? r0 + k : ew - 1) * n + i) * e_on];
+    };
+    auto load_pairs = [&](uint32_t r0) {
+#pragma unroll
+        for (uint32_t k = 0; k < kOneChunk; k++) {
+            const uint32_t at = (boff + (r0 + k < bw ? r0 + k : bw - 1) * n + i) * b_on;
+            pq[k] = a.ell_bq[at];
+            pi[k] = a.ell_bi[at];
+        }
+    };
+    load_springs(0);
+    load_pairs(0);
+    __builtin_amdgcn_sched_barrier(0);   // the ELL loads go out first
     const F3 xk = mem.ldx(a.x, s);
     const F3 x0 = mem.ld(a.x0, s);
     const F3 v0 = mem.ld(a.v, s);
@@ -188,10 +238,19 @@ __device__ __forceinline__ void vbd_vertex_ext(const VbdArgs& a, uint32_t s, uin
     // stretch: local sum from zero, then added to Hf
     Sym3 S = sym_zero();
     F3 fs = F3{0.f, 0.f, 0.f};
-    for (uint32_t j = 0; j < edeg; j++) {
-        const uint2 e = a.ell_e[eoff + j * n + i];
-        const F3 xo = mem.ldx(a.x, e.x);
-        stretch_term(xk, xo, __uint_as_float(e.y), a.ks, S, fs);
+    for (uint32_t r0 = 0; r0 < edeg; r0 += kOneChunk) {
+        if (r0) load_springs(r0);
+        F3 xo[kOneChunk];
+#pragma unroll
+        for (uint32_t k = 0; k < kOneChunk; k++) xo[k] = mem.ldx(a.x, pe[k].x);
+#pragma unroll
+        for (uint32_t k = 0; k < kOneChunk; k++) {
+            Sym3 t;
+            F3 tf;
+            spring_term_masked(a, xk, r0 + k < edeg, pe[k], xo[k], t, tf);
+            sym_add(S, t);
+            fs = add3(fs, tf);
+        }
     }
     sym_add(H, S);
     f = add3(f, fs);
@@ -199,14 +258,24 @@ __device__ __forceinline__ void vbd_vertex_ext(const VbdArgs& a, uint32_t s, uin
     // bending: diagonal Hessian only (k * Q[o][o] * I)
     float hb = 0.f;
     F3 fb = F3{0.f, 0.f, 0.f};
-    for (uint32_t j = 0; j < bdeg; j++) {
-        const uint32_t k = boff + j * n + i;
-        const float4 q = a.ell_bq[k];
-        const PairIds id = a.ell_bi[k];
-        const uint32_t o = id.a >> 30;
-        F3 p0, p1, p2, p3;
-        pair_points(o, xk, mem.ldx(a.x, id.a & 0x3fffffffu), mem.ldx(a.x, id.b), mem.ldx(a.x, id.c), p0, p1, p2, p3);
-        bending_term(q, o, p0, p1, p2, p3, a.kb, hb, fb);
+    for (uint32_t r0 = 0; r0 < bdeg; r0 += kOneChunk) {
+        if (r0) load_pairs(r0);
+        F3 g[kOneChunk][3];
+#pragma unroll
+        for (uint32_t k = 0; k < kOneChunk; k++) {
+            g[k][0] = mem.ldx(a.x, pi[k].a & 0x3fffffffu);
+            g[k][1] = mem.ldx(a.x, pi[k].b);
+            g[k][2] = mem.ldx(a.x, pi[k].c);
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < kOneChunk; k++) {
+            if (r0 + k < bdeg) {
+                const uint32_t o = pi[k].a >> 30;
+                F3 p0, p1, p2, p3;
+                pair_points(o, xk, g[k][0], g[k][1], g[k][2], p0, p1, p2, p3);
+                bending_term(pq[k], o, p0, p1, p2, p3, a.kb, hb, fb);
+            }
+        }
     }
     H.xx = H.xx + hb;
     H.yy = H.yy + hb;
@@ -237,26 +306,6 @@ __device__ __forceinline__ void vbd_vertex_ext(const VbdArgs& a, uint32_t s, uin
 
     F3 dx;
     if (newton_step(H, f, dx)) mem.stx(a.x, s, add3(xk, dx));
-}
-
-
-// One incident spring / bending pair term, zero for lanes past the degree.
-__device__ __forceinline__ void spring_term_of(const VbdArgs& a, F3 xk, bool on, uint2 e, F3 xo, Sym3& t, F3& tf) {
-    t = sym_zero();
-    tf = F3{0.f, 0.f, 0.f};
-    if (on) stretch_values(xk, xo, __uint_as_float(e.y), a.ks, t, tf);
-}
-// The same, computed on every lane and masked afterwards: a lane past the
-// degree works on a valid placeholder entry (finite: l >= epsilon).  Used
-// where the neighbour was gathered ahead: computing the term unconditionally
-// keeps that gather out of a branch, so it is issued with the others.
-__device__ __forceinline__ void spring_term_masked(const VbdArgs& a, F3 xk, bool on, uint2 e, F3 xo, Sym3& t,
-                                                   F3& tf) {
-    Sym3 u;
-    F3 uf;
-    stretch_values(xk, xo, __uint_as_float(e.y), a.ks, u, uf);
-    t = Sym3{on ? u.xx : 0.f, on ? u.xy : 0.f, on ? u.xz : 0.f, on ? u.yy : 0.f, on ? u.yz : 0.f, on ? u.zz : 0.f};
-    tf = F3{on ? uf.x : 0.f, on ? uf.y : 0.f, on ? uf.z : 0.f};
 }
 
 // The same update with G lanes per vertex (lane = 0..G-1 inside an aligned
