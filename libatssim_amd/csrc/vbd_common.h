@@ -113,7 +113,8 @@ LATS_HD void inertia_term(F3 xk, F3 x0, F3 v0, float m, uint32_t pinned, const f
 // One incident spring.  diff = x_other - x_self equals +/- the reference's
 // x[e1] - x[e0] exactly, and the force on the vertex is (k * diff/l) * C in
 // both orientations (IEEE negation commutes with rounding).
-LATS_D void stretch_term(F3 xk, F3 xo, float L, float k, Sym3& S, F3& fs) {
+// The values one incident spring adds to S and f.
+LATS_D void stretch_parts(F3 xk, F3 xo, float L, float k, Sym3& t, F3& tf) {
     const F3 diff = sub3(xo, xk);
     const float len = sqrtf(dot3(diff, diff));
     const float eps = 1.19209290E-7f;
@@ -124,42 +125,53 @@ LATS_D void stretch_term(F3 xk, F3 xo, float L, float k, Sym3& S, F3& fs) {
     float q[4];
     divide_all(num, l, q);
     const F3 dir = F3{q[0], q[1], q[2]};
-    const F3 f = F3{(k * dir.x) * C, (k * dir.y) * C, (k * dir.z) * C};
+    tf = F3{(k * dir.x) * C, (k * dir.y) * C, (k * dir.z) * C};
     const float xinv = q[3];
     const float xsq = xinv * xinv;
     const float s1 = k * xsq;
-    const float t = 1.f - L * xinv;
-    const float s2 = k * ((t < 0.0f) ? 0.0f : t);
+    const float r = 1.f - L * xinv;
+    const float s2 = k * ((r < 0.0f) ? 0.0f : r);
     const float dxx = diff.x * diff.x, dxy = diff.x * diff.y, dxz = diff.x * diff.z;
     const float dyy = diff.y * diff.y, dyz = diff.y * diff.z, dzz = diff.z * diff.z;
-    S.xx = S.xx + (s1 * dxx + s2 * (1.f - xsq * dxx));
-    S.xy = S.xy + (s1 * dxy + s2 * (0.f - xsq * dxy));
-    S.xz = S.xz + (s1 * dxz + s2 * (0.f - xsq * dxz));
-    S.yy = S.yy + (s1 * dyy + s2 * (1.f - xsq * dyy));
-    S.yz = S.yz + (s1 * dyz + s2 * (0.f - xsq * dyz));
-    S.zz = S.zz + (s1 * dzz + s2 * (1.f - xsq * dzz));
-    fs = add3(fs, f);
+    t.xx = s1 * dxx + s2 * (1.f - xsq * dxx);
+    t.xy = s1 * dxy + s2 * (0.f - xsq * dxy);
+    t.xz = s1 * dxz + s2 * (0.f - xsq * dxz);
+    t.yy = s1 * dyy + s2 * (1.f - xsq * dyy);
+    t.yz = s1 * dyz + s2 * (0.f - xsq * dyz);
+    t.zz = s1 * dzz + s2 * (1.f - xsq * dzz);
 }
-
-// The same spring term on its own (equal to stretch_term from a zero sum).
-LATS_D void stretch_values(F3 xk, F3 xo, float L, float k, Sym3& he, F3& f) {
-    he = sym_zero();
-    f = F3{0.f, 0.f, 0.f};
-    stretch_term(xk, xo, L, k, he, f);
+LATS_D void stretch_term(F3 xk, F3 xo, float L, float k, Sym3& S, F3& fs) {
+    Sym3 t;
+    F3 tf;
+    stretch_parts(xk, xo, L, k, t, tf);
+    sym_add(S, t);
+    fs = add3(fs, tf);
 }
 
 // One incident bending pair: f = k * (-(sum_j Q[o][j] x_j)), H += k Q[o][o] I.
 // q = row o of Q in pair-local order (e0, e1, opp0, opp1).
-LATS_HD void bending_term(float4 q, uint32_t o, F3 p0, F3 p1, F3 p2, F3 p3, float k, float& hb, F3& fb) {
+LATS_HD F3 bending_force(float4 q, F3 p0, F3 p1, F3 p2, F3 p3, float k) {
     F3 f = F3{0.f, 0.f, 0.f};
     f = sub3(f, scale3(q.x, p0));
     f = sub3(f, scale3(q.y, p1));
     f = sub3(f, scale3(q.z, p2));
     f = sub3(f, scale3(q.w, p3));
-    f = scale3(k, f);
+    return scale3(k, f);
+}
+// The pair's terms added to the sums hb and fb.
+LATS_HD void bending_term(float4 q, uint32_t o, F3 p0, F3 p1, F3 p2, F3 p3, float k, float& hb, F3& fb) {
+    const F3 f = bending_force(q, p0, p1, p2, p3, k);
     const float qoo = (o == 0) ? q.x : (o == 1) ? q.y : (o == 2) ? q.z : q.w;
     hb = hb + k * qoo;
     fb = add3(fb, f);
+}
+// The same terms on their own (th, tf), with Q[o][o] selected without
+// branches.  (Measured: the lane-group update is faster with the selects,
+// the one-lane update with the compiler's branches of bending_term.)
+LATS_HD void bending_parts(float4 q, uint32_t o, F3 p0, F3 p1, F3 p2, F3 p3, float k, float& th, F3& tf) {
+    tf = bending_force(q, p0, p1, p2, p3, k);
+    const float q01 = (o & 1u) ? q.y : q.x, q23 = (o & 1u) ? q.w : q.z;
+    th = k * ((o & 2u) ? q23 : q01);
 }
 
 // vbd_step: det(H) (glm), |det| > Epsilon ? dx = adj(H)/det * f : no move.

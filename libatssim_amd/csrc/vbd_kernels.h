@@ -166,10 +166,13 @@ struct LdsMem {
 };
 
 // One incident spring / bending pair term, zero for lanes past the degree.
+// The terms are the values the reference adds, taken on their own; every sum
+// they enter starts at +0, and such a sum (never -0 in round-to-nearest) is
+// unchanged by a +0 or a -0 term, so the sums are the reference's.
 __device__ __forceinline__ void spring_term_of(const VbdArgs& a, F3 xk, bool on, uint2 e, F3 xo, Sym3& t, F3& tf) {
     t = sym_zero();
     tf = F3{0.f, 0.f, 0.f};
-    if (on) stretch_values(xk, xo, __uint_as_float(e.y), a.ks, t, tf);
+    if (on) stretch_parts(xk, xo, __uint_as_float(e.y), a.ks, t, tf);
 }
 // The same, computed on every lane and masked afterwards: a lane past the
 // degree works on a valid placeholder entry (finite: l >= epsilon).  Used
@@ -179,7 +182,7 @@ __device__ __forceinline__ void spring_term_masked(const VbdArgs& a, F3 xk, bool
                                                    F3& tf) {
     Sym3 u;
     F3 uf;
-    stretch_values(xk, xo, __uint_as_float(e.y), a.ks, u, uf);
+    stretch_parts(xk, xo, __uint_as_float(e.y), a.ks, u, uf);
     t = Sym3{on ? u.xx : 0.f, on ? u.xy : 0.f, on ? u.xz : 0.f, on ? u.yy : 0.f, on ? u.yz : 0.f, on ? u.zz : 0.f};
     tf = F3{on ? uf.x : 0.f, on ? uf.y : 0.f, on ? uf.z : 0.f};
 }
@@ -449,7 +452,7 @@ __device__ __forceinline__ void vbd_vertex_group(const VbdArgs& a, uint32_t s, u
             const uint32_t o = pi[k].a >> 30;
             F3 p0, p1, p2, p3;
             pair_points(o, xk, pg[k][0], pg[k][1], pg[k][2], p0, p1, p2, p3);
-            bending_term(pq[k], o, p0, p1, p2, p3, a.kb, th, tf);
+            bending_parts(pq[k], o, p0, p1, p2, p3, a.kb, th, tf);
         }
         add_pair_round(th, tf);
     }
@@ -465,7 +468,7 @@ __device__ __forceinline__ void vbd_vertex_group(const VbdArgs& a, uint32_t s, u
             F3 p0, p1, p2, p3;
             pair_points(o, xk, mem.ldx(a.x, id.a & 0x3fffffffu), mem.ldx(a.x, id.b), mem.ldx(a.x, id.c), p0, p1, p2,
                         p3);
-            bending_term(q, o, p0, p1, p2, p3, a.kb, th, tf);
+            bending_parts(q, o, p0, p1, p2, p3, a.kb, th, tf);
         }
         add_pair_round(th, tf);
     }
