@@ -14,7 +14,10 @@ batch_c2     512 independent grid cloths, side n ~ U{16..128} (~3.2 M vertices),
 sphere_c4    (sphere_scene) 1024x1024 horizontal grid (2 mm spacing, 2.05 m wide, no pins)
              10 cm above a fixed sphere of radius 0.3 m (icosphere, 20,480
              faces); contact thickness 2 mm, k_contact 1e5, detection radius
-             2.5 cm (one frame of fall at impact speed), hash cell 5 cm;
+             2.5 cm (one frame of fall at impact speed), hash cell 2x the
+             edge length (4 mm; every face is entered in each cell its
+             bounding box grown by the radius touches, so one cell lookup
+             finds every face within the radius);
              50 iterations/frame, 120 frames.  Warm-start training drops
              (sphere_train_scenario): 384x384 cloths of the same spacing,
              dropped from 6-14 cm, centre offset up to +-10 cm.
@@ -144,7 +147,7 @@ def sphere_scene(n=1024, spacing=0.002, radius=0.3, drop=0.1, level=5, seed=2025
     x, f = horizontal_grid(n, spacing, radius + drop, seed)
     cloth = Cloth(x, f, np.float32(1e-3), np.zeros(n * n, np.uint8))
     ox, of = make_icosphere(level, radius)
-    contact = dict(thickness=0.002, stiffness=1e5, radius=0.025, cell=0.05)
+    contact = dict(thickness=0.002, stiffness=1e5, radius=0.025, cell=2.0 * spacing)
     return cloth, Params(constraint_iter_count=iters), (ox, of), contact
 
 

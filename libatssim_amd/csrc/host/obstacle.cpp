@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <map>
+#include <vector>
 
 namespace lats {
 
@@ -44,7 +45,11 @@ bool build_obstacle_hash(const float* x, uint32_t nv, const int32_t* f, uint32_t
         if (h.dims[d] >= 1024) { err = "obstacle spans more than 1023 cells per axis: use a larger cell"; return false; }
     }
     auto coord = [&](float p, int d) { return (int)std::floor((p - h.origin[d]) * h.inv_cell); };
-    std::map<uint32_t, std::vector<uint32_t>> cells;
+    // every face goes to each cell its bounding box grown by the radius
+    // touches, so the cell of a point holds every face within the radius
+    // of it: counted on a dense grid over the grown bounds, then filled in
+    // face order (two passes, no per-cell containers)
+    std::vector<int> range(6 * (size_t)nf);
     for (uint32_t k = 0; k < nf; k++) {
         float a[3] = {3.4e38f, 3.4e38f, 3.4e38f}, b[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
         for (int j = 0; j < 3; j++)
@@ -52,35 +57,59 @@ bool build_obstacle_hash(const float* x, uint32_t nv, const int32_t* f, uint32_t
                 a[d] = std::min(a[d], x[3 * (size_t)f[3 * k + j] + d]);
                 b[d] = std::max(b[d], x[3 * (size_t)f[3 * k + j] + d]);
             }
-        int c0[3], c1[3];
         for (int d = 0; d < 3; d++) {
-            c0[d] = std::max(0, coord(a[d] - grow, d));
-            c1[d] = std::min(h.dims[d] - 1, coord(b[d] + grow, d));
+            range[6 * (size_t)k + d] = std::max(0, coord(a[d] - grow, d));
+            range[6 * (size_t)k + 3 + d] = std::min(h.dims[d] - 1, coord(b[d] + grow, d));
         }
-        for (int ix = c0[0]; ix <= c1[0]; ix++)
-            for (int iy = c0[1]; iy <= c1[1]; iy++)
-                for (int iz = c0[2]; iz <= c1[2]; iz++) cells[cell_key(ix, iy, iz)].push_back(k);
     }
-    for (uint32_t T = std::max<uint32_t>(64, 4 * (uint32_t)cells.size());; T *= 2) {
+    const size_t ncell = (size_t)h.dims[0] * h.dims[1] * h.dims[2];
+    if (ncell > (size_t)1 << 28) { err = "obstacle grid too fine: use a larger cell"; return false; }
+    auto dense = [&](int ix, int iy, int iz) { return ((size_t)ix * h.dims[1] + iy) * h.dims[2] + iz; };
+    auto each_cell = [&](uint32_t k, auto&& fn) {
+        const int* r = &range[6 * (size_t)k];
+        for (int ix = r[0]; ix <= r[3]; ix++)
+            for (int iy = r[1]; iy <= r[4]; iy++)
+                for (int iz = r[2]; iz <= r[5]; iz++) fn(ix, iy, iz);
+    };
+    std::vector<uint32_t> cnt(ncell + 1, 0u);
+    uint64_t total = 0;
+    for (uint32_t k = 0; k < nf; k++)
+        each_cell(k, [&](int ix, int iy, int iz) {
+            cnt[dense(ix, iy, iz)]++;
+            total++;
+        });
+    if (total >= (1ull << 32)) { err = "obstacle hash too large: use a larger cell"; return false; }
+    std::vector<uint32_t> off(ncell + 1, 0u);
+    uint32_t occupied = 0;
+    for (size_t c = 0; c < ncell; c++) {
+        off[c + 1] = off[c] + cnt[c];
+        occupied += cnt[c] != 0u;
+    }
+    std::vector<uint32_t> faces(total), fill(off.begin(), off.end() - 1);
+    for (uint32_t k = 0; k < nf; k++) each_cell(k, [&](int ix, int iy, int iz) { faces[fill[dense(ix, iy, iz)]++] = k; });
+    for (uint32_t T = std::max<uint32_t>(64, 4 * occupied);; T *= 2) {
         h.table_size = T;
         h.belongs.assign(T, 0xffffffffu);
         h.start.assign(T, 0);
         h.count.assign(T, 0);
-        h.faces.clear();
         bool ok = true;
-        for (const auto& kv : cells) {
-            const uint32_t key = kv.first;
-            const int ix = (int)(key >> 20), iy = (int)((key >> 10) & 1023u), iz = (int)(key & 1023u);
-            uint32_t s = hash_coord(ix, iy, iz, T);
-            int j = 0;
-            while (j < 32 && h.belongs[s] != 0xffffffffu) { s = (s + 1) % T; j++; }
-            if (j == 32) { ok = false; break; }
-            h.belongs[s] = key;
-            h.start[s] = (uint32_t)h.faces.size();
-            h.count[s] = (uint32_t)kv.second.size();
-            h.faces.insert(h.faces.end(), kv.second.begin(), kv.second.end());
+        for (int ix = 0; ix < h.dims[0] && ok; ix++)
+            for (int iy = 0; iy < h.dims[1] && ok; iy++)
+                for (int iz = 0; iz < h.dims[2]; iz++) {
+                    const size_t c = dense(ix, iy, iz);
+                    if (!cnt[c]) continue;
+                    uint32_t s = hash_coord(ix, iy, iz, T);
+                    int j = 0;
+                    while (j < 32 && h.belongs[s] != 0xffffffffu) { s = (s + 1) % T; j++; }
+                    if (j == 32) { ok = false; break; }
+                    h.belongs[s] = cell_key(ix, iy, iz);
+                    h.start[s] = off[c];
+                    h.count[s] = cnt[c];
+                }
+        if (ok) {
+            h.faces.swap(faces);
+            return true;
         }
-        if (ok) return true;
         if (T > (1u << 28)) { err = "obstacle hash does not fit"; return false; }
     }
 }

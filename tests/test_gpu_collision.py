@@ -111,18 +111,20 @@ def test_sphere_c4_full_size(gpu_available):
     assert r.min() > 0.3 - 0.01
 
 
-def test_contacts_match_oracle_dense_cells(gpu_available):
-    """The sphere_c4 sphere and contact parameters (level-5 icosphere, 5 cm
-    cells, 2.5 cm query radius: ~100 faces per cell, so all 8 lanes of the
-    detection kernel find candidates and their nearest lists are merged) on a
-    96x96 piece of the cloth dropped 1.5 cm: contact sets and positions equal
-    the oracle's brute-force query every frame."""
-    _print_case("96x96 cloth on the sphere_c4 sphere vs oracle, 6 frames")
+@pytest.mark.parametrize("cell", [0.05, 0.004])
+def test_contacts_match_oracle_dense_cells(gpu_available, cell):
+    """The sphere_c4 sphere and contact parameters (level-5 icosphere, 2.5 cm
+    query radius) on a 96x96 piece of the cloth dropped 1.5 cm, with 5 cm
+    cells (~100 faces per cell, so all 8 lanes of the detection kernel find
+    candidates and their nearest lists are merged) and with sphere_c4's own
+    4 mm cells (2x the edge length): contact sets and positions equal the
+    oracle's brute-force query every frame."""
+    _print_case(f"96x96 cloth on the sphere_c4 sphere vs oracle, {cell * 100:.1f} cm cells, 6 frames")
     n, spacing, radius = 96, 0.002, 0.3
     x, f = configs.horizontal_grid(n, spacing, radius + 0.015, seed=91)
     cloth = Cloth(x, f, np.float32(1e-3), np.zeros(n * n, np.uint8))
     ox, of = make_icosphere(5, radius)
-    ct = dict(thickness=0.002, stiffness=1e5, radius=0.025, cell=0.05)
+    ct = dict(thickness=0.002, stiffness=1e5, radius=0.025, cell=cell)
     params = Params(constraint_iter_count=20)
     g = GpuVBD(cloth, params)
     g.set_lanes_per_vertex(4)
