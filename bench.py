@@ -300,11 +300,12 @@ def bench_warm_start(args, cloth, params, wind, plain_ms, lanes=8, regions=1, co
       warm-started solve gets there; a deployed solver does not know that
       energy.
     deployable: the target-free stopping rule (energy decrease per sweep <=
-      tol * |E|) with tol calibrated on two training scenarios; the plain
-      energy is computed only to report how many frames reach it."""
+      tol * |E|) with tol and the check schedule calibrated on four training
+      scenarios; the plain energy is computed only to report how many frames
+      reach it."""
     import numpy as np
     from libatssim_amd import configs
-    from libatssim_amd.warm_start import WarmStartMLP, calibrate_stop_rule, evaluate, evaluate_rule, train_recipe
+    from libatssim_amd.warm_start import WarmStartMLP, calibrate_stop_schedule, evaluate, evaluate_rule, train_recipe
     t0 = time.perf_counter()
     scen = [configs.warm_c3_scenario(k, n=256, frames=args.warm_frames) for k in range(args.warm_scenarios)]
     m = WarmStartMLP(2024)
@@ -319,8 +320,10 @@ def bench_warm_start(args, cloth, params, wind, plain_ms, lanes=8, regions=1, co
     train_recipe(m, scen, params, args.warm_frames, plain_steps=steps, guided_rounds=rounds, guided_steps=steps,
                  record_every=args.warm_record_every)
     cal_frames = min(30, args.warm_frames)
-    tol, tried = calibrate_stop_rule(m, [configs.warm_c3_scenario(k, n=256, frames=cal_frames) for k in range(2)],
-                                     params, cal_frames, lanes=lanes, regions=regions, colour_block=colour_block)
+    cal_scen = min(4, args.warm_scenarios)
+    cal = calibrate_stop_schedule(m, [configs.warm_c3_scenario(k, n=256, frames=cal_frames) for k in range(cal_scen)],
+                                  params, cal_frames, lanes=lanes, regions=regions, colour_block=colour_block)
+    tol, fc, ce = cal["tol"], cal["first_check"], cal["check_every"]
     train_s = time.perf_counter() - t0
     full_wind = configs.grid_c1(frames=60)[3]
     frames = len(full_wind)
@@ -340,11 +343,17 @@ def bench_warm_start(args, cloth, params, wind, plain_ms, lanes=8, regions=1, co
                               colour_block=colour_block))
     oracle["stop"] = ("first check at or below the plain 100-iteration energy of the same frame start, computed "
                       "beforehand on a second context (not known to a deployed solver)")
-    deploy = summary(evaluate_rule(m, cloth, params, full_wind, frames, tol, lanes=lanes, regions=regions,
-                                   colour_block=colour_block))
+    deploy = summary(evaluate_rule(m, cloth, params, full_wind, frames, tol, first_check=fc, check_every=ce,
+                                   lanes=lanes, regions=regions, colour_block=colour_block))
     deploy["stop"] = "energy decrease per sweep <= tol * |E| since the previous check (no reference solve)"
     deploy["tol"] = tol
-    deploy["calibration"] = {"scenarios": 2, "frames": cal_frames, "candidates": tried}
+    deploy["first_check"] = fc
+    deploy["check_every"] = ce
+    deploy["calibration"] = {"scenarios": cal_scen, "frames": cal_frames,
+                             "rule": "fastest (schedule, tol) within 1 point of the best share of frames reaching "
+                                     "the plain energy, on training scenarios",
+                             "candidates": [{k: (round(v, 4) if isinstance(v, float) and k != "tol" else v)
+                                             for k, v in t.items()} for t in cal["tried"]]}
     fwd_ms, step_ms = m.time_kernels(cloth.num_verts, 20)   # after the evaluation: timing steps train
     return {
         "ms_per_frame_plain": round(plain_ms, 4),
