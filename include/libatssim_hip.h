@@ -152,11 +152,33 @@ typedef struct lats_vbd lats_vbd;
 #define LATS_STRATEGY_PERSISTENT 2  /* one cooperative kernel per frame, grid barrier per colour */
 #define LATS_STRATEGY_FLOW 3        /* one cooperative kernel per run of frames, tiles of vertices hand
                                        each colour pass on to their neighbouring tiles (no grid barrier) */
+#define LATS_STRATEGY_PATCH 4       /* several colour passes per launch: each workgroup holds one patch of
+                                       the cloth and its halo in LDS and recomputes the halo's updates
+                                       (identical values), frame captured in a hipGraph */
 
 /* Creates the context on the current HIP device; uploads everything. */
 int lats_vbd_create(const lats_mesh_desc* mesh, const lats_scene_params* params, lats_vbd** out);
 int lats_vbd_destroy(lats_vbd* ctx);
 int lats_vbd_set_strategy(lats_vbd* ctx, int strategy);
+/* Patch strategy geometry.  patches: a multiple of 8 (0: about 256 vertices
+ * per patch, 64..1024 patches); colours_per_launch: colour passes one launch
+ * runs (0 or >= the colour count: a whole sweep per launch); threads per
+ * workgroup: 256, 512 or 1024.  Lanes per vertex come from
+ * lats_vbd_set_lanes_per_vertex (4, 8, 16; 1 runs as 8).  The layout is
+ * built on first use; a region that does not fit one workgroup's LDS gives
+ * LATS_ERR_UNSUPPORTED.  No counterpart in the reference (its colour loop
+ * dispatches one pass per colour, example3_hybrid_cpu_gpu.cpp:1217-1348). */
+typedef struct lats_patch_info {
+    uint32_t patches, colours_per_launch, launches_per_sweep, threads, lanes;
+    uint32_t max_region;                /* largest region of one workgroup (LDS slots, 16 B each) */
+    uint64_t updates_per_sweep;         /* vertex updates of one sweep, halo updates included */
+    uint64_t region_slots_per_sweep;    /* positions staged into LDS per sweep */
+    uint64_t ell_entries_per_sweep;     /* local ELL entries (springs + pairs) read per sweep */
+    double build_ms;                    /* host time of the last layout build */
+} lats_patch_info;
+int lats_vbd_set_patches(lats_vbd* ctx, uint32_t patches, uint32_t colours_per_launch, uint32_t threads);
+/* Builds the layout if needed, then describes it. */
+int lats_vbd_get_patches(lats_vbd* ctx, lats_patch_info* info);
 /* Solver of physics_step / run_frames / dispatch: VBD (default) or XPBD
  * (CpuSolver::physics_step_xpbd, example2_async_iter.cpp:1017-1081: predict,
  * multipliers reset, per iteration every spring cluster then every bending

@@ -33,6 +33,7 @@ LATS_STRATEGY_AUTO = 0
 LATS_STRATEGY_GRAPH = 1
 LATS_STRATEGY_PERSISTENT = 2
 LATS_STRATEGY_FLOW = 3
+LATS_STRATEGY_PATCH = 4
 
 LATS_FN_ADDITIONAL_ROOT = 0
 LATS_FN_ADDITIONAL_TERMINAL = 1
@@ -104,6 +105,21 @@ class LaunchParam(ctypes.Structure):
     ]
 
 
+class PatchInfo(ctypes.Structure):
+    _fields_ = [
+        ("patches", ctypes.c_uint32),
+        ("colours_per_launch", ctypes.c_uint32),
+        ("launches_per_sweep", ctypes.c_uint32),
+        ("threads", ctypes.c_uint32),
+        ("lanes", ctypes.c_uint32),
+        ("max_region", ctypes.c_uint32),
+        ("updates_per_sweep", ctypes.c_uint64),
+        ("region_slots_per_sweep", ctypes.c_uint64),
+        ("ell_entries_per_sweep", ctypes.c_uint64),
+        ("build_ms", ctypes.c_double),
+    ]
+
+
 # name -> (restype, argtypes); every symbol declared in include/libatssim_hip.h
 _VP = ctypes.c_void_p
 _U32 = ctypes.c_uint32
@@ -132,6 +148,8 @@ SIGNATURES = {
     "lats_vbd_set_persistent_blocks": (_I, [_VP, _U32]),
     "lats_vbd_set_lanes_per_vertex": (_I, [_VP, _U32]),
     "lats_vbd_set_regions": (_I, [_VP, _U32]),
+    "lats_vbd_set_patches": (_I, [_VP, _U32, _U32, _U32]),
+    "lats_vbd_get_patches": (_I, [_VP, ctypes.POINTER(PatchInfo)]),
     "lats_vbd_set_solver": (_I, [_VP, _I]),
     "lats_vbd_get_solver": (_I, [_VP, ctypes.POINTER(_I)]),
     "lats_vbd_xpbd_clusters": (_I, [_VP, _I, _VP, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)]),

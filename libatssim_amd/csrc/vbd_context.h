@@ -9,6 +9,7 @@
 #include "../../include/libatssim_hip.h"
 #include "host/obstacle.h"
 #include "vbd_kernels.h"
+#include "vbd_patch.h"
 
 namespace lats {
 
@@ -114,6 +115,22 @@ struct lats_vbd {
     lats::DBuf<uint2> flow_range;
     lats::DBuf<uint32_t> flow_nb_off, flow_nb, flow_progress, flow_error;
 
+    // patch strategy: layout built on first use for (patches, colours per
+    // launch); positions alternate between x and xalt from launch to launch
+    uint32_t patch_target = 0;        // patches requested (0 = derived from the vertex count)
+    uint32_t patch_colours = 0;       // colours per launch requested (0 = every colour)
+    uint32_t patch_threads = 1024;    // threads per workgroup (256, 512, 1024)
+    uint32_t patch_built_p = 0, patch_built_f = 0;   // built layout (0 = none)
+    std::vector<uint32_t> pt_group_lds;   // per launch of a sweep: largest region in LDS slots
+    lats::PatchStats patch_stats;
+    lats::DBuf<lats::PatchHead> pt_heads;
+    lats::DBuf<lats::PatchStep> pt_steps;
+    lats::DBuf<uint32_t> pt_region;
+    lats::DBuf<uint2> pt_entries, pt_ell_e;
+    lats::DBuf<float4> pt_ell_bq;
+    lats::DBuf<lats::PairIds> pt_ell_bi;
+    lats::DBuf<float> xalt;
+
     // graph per (record) flag; graphs of k colour sweeps (lats_vbd_iterate)
     hipGraphExec_t graph[2] = {nullptr, nullptr};
     std::map<uint32_t, hipGraphExec_t> sweep_graphs;
@@ -190,4 +207,12 @@ int energies_of(lats_vbd* c, const float* const* xs, uint32_t n, float* out);
 // (n <= kHostEnergies).  The energy scratch rows are c's: no other energy
 // evaluation of c may run until they have landed.
 int energies_enqueue(lats_vbd* c, const float* const* xs, uint32_t n, hipStream_t st = nullptr);
+// Patch strategy (vbd_patch.hip): builds the layout for the requested
+// patches / colours per launch unless it is built already (host waits; not
+// inside a capture); launches per colour sweep; one launch of group g.
+int patch_prepare(lats_vbd* c);
+uint32_t patch_launches_per_sweep(const lats_vbd* c);
+uint32_t patch_lanes(const lats_vbd* c);
+uint32_t patch_block(const lats_vbd* c);
+void enqueue_patch_launch(lats_vbd* c, uint32_t g, const float* xin, float* xout);
 }  // namespace lats

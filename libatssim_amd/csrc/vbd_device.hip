@@ -1027,6 +1027,30 @@ int build_xpbd(lats_vbd* c) {
     return LATS_OK;
 }
 
+// Colour sweeps with the patch strategy: launches alternate between x and
+// xalt (each reads one and writes the other), the result ends in x.  With
+// `record`, the energy rows 0..sweeps of the frame are written as well.
+void enqueue_patch_sweeps(lats_vbd* c, uint32_t sweeps, bool record) {
+    float* cur = c->x.p;
+    float* alt = c->xalt.p;
+    const uint32_t L = patch_launches_per_sweep(c);
+    for (uint32_t it = 0; it < sweeps; it++) {
+        if (record && it == 0) enqueue_energy(c, cur, 0);
+        for (uint32_t g = 0; g < L; g++) {
+            enqueue_patch_launch(c, g, cur, alt);
+            std::swap(cur, alt);
+        }
+        if (record) enqueue_energy(c, cur, it + 1);
+    }
+    if (cur != c->x.p)
+        hipLaunchKernelGGL(k_copy_to_2, dim3(nblocks(3 * (uint64_t)c->V)), dim3(kBlock), 0, c->stream,
+                           (const float*)cur, c->x.p, (float*)nullptr, 3 * c->V);
+}
+
+bool uses_patches(const lats_vbd* c) {
+    return c->solver == LATS_SOLVER_VBD && c->strategy == LATS_STRATEGY_PATCH;
+}
+
 // The whole frame as a kernel sequence (captured into a graph by the caller).
 void enqueue_frame(lats_vbd* c, bool record) {
     if (c->solver == LATS_SOLVER_XPBD) return enqueue_xpbd_frame(c, record);
@@ -1034,6 +1058,11 @@ void enqueue_frame(lats_vbd* c, bool record) {
     for (uint32_t sub = 0; sub < nsub; sub++) {
         enqueue_predict(c);
         enqueue_detect(c);
+        if (uses_patches(c)) {
+            enqueue_patch_sweeps(c, c->prm.constraint_iter_count, record);
+            enqueue_update(c, sub + 1 == nsub);
+            continue;
+        }
         for (uint32_t it = 0; it < c->prm.constraint_iter_count; it++) {
             if (record && it == 0) enqueue_energy(c, c->x.p, 0);
             for (uint32_t col = 0; col < c->C; col++) enqueue_colour(c, c->x.p, col);
@@ -1245,6 +1274,10 @@ int flow_check(lats_vbd* c) {
 }
 
 int get_graph(lats_vbd* c, bool record, hipGraphExec_t* out) {
+    if (uses_patches(c)) {
+        int rc = patch_prepare(c);
+        if (rc) return rc;
+    }
     if (record) {
         int rc = ensure_energy_rows(c, c->prm.constraint_iter_count + 1);
         if (rc) return rc;
@@ -1427,11 +1460,44 @@ int lats_vbd_destroy(lats_vbd* c) {
 
 int lats_vbd_set_strategy(lats_vbd* c, int strategy) {
     if (!c) return fail(LATS_ERR_INVALID_ARGUMENT, "null context");
-    if (strategy < LATS_STRATEGY_AUTO || strategy > LATS_STRATEGY_FLOW)
+    if (strategy < LATS_STRATEGY_AUTO || strategy > LATS_STRATEGY_PATCH)
         return fail(LATS_ERR_INVALID_ARGUMENT, "unknown strategy");
     if ((strategy == LATS_STRATEGY_PERSISTENT || strategy == LATS_STRATEGY_FLOW) && c->prm.num_substep != 1)
         return fail(LATS_ERR_UNSUPPORTED, "persistent and flow strategies run one substep per frame");
+    if (strategy != c->strategy) c->drop_graphs();   // captured frames hold the strategy's kernels
     c->strategy = strategy;
+    return LATS_OK;
+}
+
+int lats_vbd_set_patches(lats_vbd* c, uint32_t patches, uint32_t colours_per_launch, uint32_t threads) {
+    if (!c) return fail(LATS_ERR_INVALID_ARGUMENT, "null context");
+    if (patches % kXcds) return fail(LATS_ERR_INVALID_ARGUMENT, "patches must be a multiple of 8 (0: derived)");
+    if (patches > c->V) return fail(LATS_ERR_INVALID_ARGUMENT, "more patches than vertices");
+    if (threads != 256 && threads != 512 && threads != 1024)
+        return fail(LATS_ERR_INVALID_ARGUMENT, "patch workgroups must have 256, 512 or 1024 threads");
+    if (patches != c->patch_target || colours_per_launch != c->patch_colours || threads != c->patch_threads)
+        c->drop_graphs();
+    c->patch_target = patches;
+    c->patch_colours = colours_per_launch;
+    c->patch_threads = threads;
+    return LATS_OK;
+}
+
+int lats_vbd_get_patches(lats_vbd* c, lats_patch_info* info) {
+    if (!c || !info) return fail(LATS_ERR_INVALID_ARGUMENT, "null argument");
+    int rc = patch_prepare(c);
+    if (rc) return rc;
+    const PatchStats& st = c->patch_stats;
+    info->patches = st.patches;
+    info->colours_per_launch = st.colours_per_launch;
+    info->launches_per_sweep = st.launches_per_sweep;
+    info->threads = patch_block(c);
+    info->lanes = patch_lanes(c);
+    info->max_region = st.max_region;
+    info->updates_per_sweep = st.updates_per_sweep;
+    info->region_slots_per_sweep = st.region_slots;
+    info->ell_entries_per_sweep = st.ell_entries;
+    info->build_ms = st.build_ms;
     return LATS_OK;
 }
 
@@ -2114,14 +2180,21 @@ int lats_vbd_iterate(lats_vbd* c, uint32_t sweeps) {
     if (!c) return fail(LATS_ERR_INVALID_ARGUMENT, "null context");
     if (c->solver != LATS_SOLVER_VBD) return fail(LATS_ERR_UNSUPPORTED, "colour sweeps belong to the VBD solver");
     if (sweeps == 0) return LATS_OK;
+    if (uses_patches(c)) {
+        int rc = patch_prepare(c);
+        if (rc) return rc;
+    }
     // one graph per distinct sweep count (a handful in practice)
     auto it = c->sweep_graphs.find(sweeps);
     if (it == c->sweep_graphs.end()) {
         if (c->sweep_graphs.size() >= 16) c->drop_graphs();
         hipGraph_t graph;
         HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        for (uint32_t k = 0; k < sweeps; k++)
-            for (uint32_t col = 0; col < c->C; col++) enqueue_colour(c, c->x.p, col);
+        if (uses_patches(c))
+            enqueue_patch_sweeps(c, sweeps, false);
+        else
+            for (uint32_t k = 0; k < sweeps; k++)
+                for (uint32_t col = 0; col < c->C; col++) enqueue_colour(c, c->x.p, col);
         HIPCHK(hipStreamEndCapture(c->stream, &graph));
         hipGraphExec_t exec = nullptr;
         hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);

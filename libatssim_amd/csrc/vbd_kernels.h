@@ -323,50 +323,92 @@ __device__ __forceinline__ void vbd_vertex_ext(const VbdArgs& a, uint32_t s, uin
 // The ELL entries of the first PE spring rounds and PB pair rounds are loaded
 // before the vertex's degrees arrive (padding entries are valid slots), so
 // the neighbour gathers follow after one memory round trip instead of two.
-template <int G, class Mem = PlainMem>
-__device__ __forceinline__ void vbd_vertex_group(const VbdArgs& a, uint32_t s, uint32_t i, uint32_t n, uint32_t eoff,
-                                                 uint32_t boff, uint32_t ew, uint32_t bw, F3 ext, uint32_t lane,
-                                                 bool valid, Mem mem = Mem{}) {
-    constexpr uint32_t PE = (8 + G - 1) / G;    // spring rounds prefetched (degrees <= 8)
-    constexpr uint32_t PB = (16 + G - 1) / G;   // pair rounds prefetched (degrees <= 16)
-    // Every load below is unconditional, from an address that is always
-    // valid: ELL rows past the colour's width are clamped to its last row
-    // (a colour without springs / pairs reads entry 0), and padding entries
-    // hold slot 0.  Values a lane does not own are masked where they are
-    // used.  A conditional load (cond ? load : 0) makes the compiler wait for
-    // it before the next one, which serialises the prefetch rounds.
+// The loads of one vertex update with G lanes per vertex that come from
+// memory the update does not write: the ELL entries of the first PE spring
+// rounds and PB pair rounds, and the vertex's own frame state.  Split from
+// the arithmetic (group_finish) so a caller can issue them one update ahead.
+template <int G>
+struct GroupLoads {
+    static constexpr uint32_t PE = (8 + G - 1) / G;    // spring rounds prefetched (degrees <= 8)
+    static constexpr uint32_t PB = (16 + G - 1) / G;   // pair rounds prefetched (degrees <= 16)
     uint2 pe[PE];
     float4 pq[PB];
     PairIds pi[PB];
+    F3 x0, v0;
+    float m;
+    uint32_t meta;
+};
+
+// Every load below is unconditional, from an address that is always
+// valid: ELL rows past the colour's width are clamped to its last row
+// (a colour without springs / pairs reads entry 0), and padding entries
+// hold slot 0.  Values a lane does not own are masked where they are
+// used.  A conditional load (cond ? load : 0) makes the compiler wait for
+// it before the next one, which serialises the prefetch rounds.
+template <int G>
+__device__ __forceinline__ void group_load_ell(const VbdArgs& a, uint32_t i, uint32_t n, uint32_t eoff, uint32_t boff,
+                                               uint32_t ew, uint32_t bw, uint32_t lane, GroupLoads<G>& L) {
     // (index arithmetic without branches: a zero width multiplies the index
     // by zero, so no load waits behind a branch on the width)
     const uint32_t e_on = ew != 0u, b_on = bw != 0u;
 #pragma unroll
-    for (uint32_t k = 0; k < PE; k++) {
+    for (uint32_t k = 0; k < GroupLoads<G>::PE; k++) {
         const uint32_t j = k * G + lane;
         const uint32_t row = j < ew ? j : ew - 1;
-        pe[k] = a.ell_e[(eoff + row * n + i) * e_on];
+        L.pe[k] = a.ell_e[(eoff + row * n + i) * e_on];
     }
 #pragma unroll
-    for (uint32_t k = 0; k < PB; k++) {
+    for (uint32_t k = 0; k < GroupLoads<G>::PB; k++) {
         const uint32_t j = k * G + lane;
         const uint32_t row = j < bw ? j : bw - 1;
         const uint32_t at = (boff + row * n + i) * b_on;
-        pq[k] = a.ell_bq[at];
-        pi[k] = a.ell_bi[at];
+        L.pq[k] = a.ell_bq[at];
+        L.pi[k] = a.ell_bi[at];
     }
+}
+template <int G, class Mem>
+__device__ __forceinline__ void group_load_own(const VbdArgs& a, uint32_t s, GroupLoads<G>& L, Mem mem) {
+    L.x0 = mem.ld(a.x0, s);
+    L.v0 = mem.ld(a.v, s);
+    L.m = a.mass[s];
+    L.meta = a.meta[s];
+}
+
+template <int G, class Mem>
+__device__ __forceinline__ void group_finish(const VbdArgs& a, const GroupLoads<G>& L, F3 xk, uint32_t s, uint32_t sx,
+                                             uint32_t i, uint32_t n, uint32_t eoff, uint32_t boff, F3 ext,
+                                             uint32_t lane, bool valid, Mem mem);
+
+template <int G, class Mem = PlainMem>
+__device__ __forceinline__ void vbd_vertex_group(const VbdArgs& a, uint32_t s, uint32_t i, uint32_t n, uint32_t eoff,
+                                                 uint32_t boff, uint32_t ew, uint32_t bw, F3 ext, uint32_t lane,
+                                                 bool valid, Mem mem = Mem{}) {
+    GroupLoads<G> L;
+    group_load_ell<G>(a, i, n, eoff, boff, ew, bw, lane, L);
     // the ELL loads go out first: their addresses need no argument-block read
     // when the caller has the ELL bases and colour geometry in registers
     __builtin_amdgcn_sched_barrier(0);
     const F3 xk = mem.ldx(a.x, s);
-    const F3 x0 = mem.ld(a.x0, s);
-    const F3 v0 = mem.ld(a.v, s);
-    const float m = a.mass[s];
-    const uint32_t meta_raw = a.meta[s];
+    group_load_own<G>(a, s, L, mem);
     // every load above is issued before any gather below (the scheduler
     // would otherwise issue later rounds' ELL loads after the first wait)
     __builtin_amdgcn_sched_barrier(0);
-    const uint32_t meta = valid ? meta_raw : 0u;
+    group_finish<G>(a, L, xk, s, s, i, n, eoff, boff, ext, lane, valid, mem);
+}
+
+// The gathers and the arithmetic of the update: s = the vertex's slot (own
+// state, contacts), sx = where its position is read and written through mem.
+template <int G, class Mem>
+__device__ __forceinline__ void group_finish(const VbdArgs& a, const GroupLoads<G>& L, F3 xk, uint32_t s, uint32_t sx,
+                                             uint32_t i, uint32_t n, uint32_t eoff, uint32_t boff, F3 ext,
+                                             uint32_t lane, bool valid, Mem mem) {
+    constexpr uint32_t PE = GroupLoads<G>::PE, PB = GroupLoads<G>::PB;
+    const uint2* pe = L.pe;
+    const float4* pq = L.pq;
+    const PairIds* pi = L.pi;
+    const F3 x0 = L.x0, v0 = L.v0;
+    const float m = L.m;
+    const uint32_t meta = valid ? L.meta : 0u;
     const uint32_t edeg = meta & 0xffu, bdeg = (meta >> 8) & 0xffu;
 
     // neighbour gathers of the prefetched rounds (lanes past the degree load
@@ -493,12 +535,12 @@ __device__ __forceinline__ void vbd_vertex_group(const VbdArgs& a, uint32_t s, u
                                    {H.xz + Hc[2][0], H.yz + Hc[2][1], H.zz + Hc[2][2]}};
             f = add3(f, fc);
             F3 dx;
-            if (newton_step_full(M, f, dx)) mem.stx(a.x, s, add3(xk, dx));
+            if (newton_step_full(M, f, dx)) mem.stx(a.x, sx, add3(xk, dx));
             return;
         }
     }
     F3 dx;
-    if (newton_step(H, f, dx)) mem.stx(a.x, s, add3(xk, dx));
+    if (newton_step(H, f, dx)) mem.stx(a.x, sx, add3(xk, dx));
 }
 
 }  // namespace lats

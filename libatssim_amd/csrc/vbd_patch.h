@@ -1,0 +1,92 @@
+// vbd_patch.h -- the patch strategy: several colour passes in one launch.
+//
+// The cloth's slots are split into P patches (tiles of the rest shape).  A
+// launch runs a group of F consecutive colours; workgroup p owns patch p.  It
+// copies the positions of its region (its patch, the halo it must update
+// and the vertices those updates read) from the input buffer into LDS, runs
+// the F colour steps on LDS with a workgroup barrier between steps, and
+// writes the positions of the slots it owns to the output buffer.
+//
+// A colour step of patch p updates every vertex of that colour that a later
+// step of the group reads (the halo) as well as its own: in Gauss-Seidel
+// order a vertex's update reads its neighbours' latest values, and a
+// neighbour of an earlier colour of the group is updated in this launch, so
+// the patch computes that update itself, from the same inputs as the owner
+// does, with the same arithmetic: the values are identical.  Reads come from
+// the input buffer only and writes go to the output buffer only, so no
+// workgroup ever sees another's writes of the same launch; the next launch
+// swaps the buffers.  Every slot is owned by exactly one patch, so the output
+// buffer holds every position after the launch.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "vbd_kernels.h"
+
+struct lats_vbd;
+
+namespace lats {
+
+// Most colour steps one launch runs.
+constexpr uint32_t kMaxPatchSteps = 16;
+// Workgroups go to the 8 XCDs round-robin; patches are numbered in rows of
+// the tile grid, so workgroup b runs patch (b % 8) * (P / 8) + b / 8 and each
+// XCD works on one band of neighbouring patches (halo reads of neighbours
+// then hit the XCD's own L2).  P is a multiple of 8.
+constexpr uint32_t kXcds = 8;
+
+// One colour step of one patch: its updates are entries [off, off + n) of
+// the entry list ({slot, LDS index}), with a local ELL (ELL entries hold LDS
+// indices) at bases eoff / boff and widths ew / bw.
+struct PatchStep {
+    uint32_t off, n, eoff, boff, ew, bw;
+};
+// One patch of one group: its region is region[roff, roff + rsize) (global
+// slots; LDS index = position in that list), of which the first `owned` are
+// the slots the patch owns and the first `nstate` (owned, then the halo it
+// updates) have their frame state staged in LDS; its steps are
+// steps[step_off, step_off + nsteps).
+struct PatchHead {
+    uint32_t roff, rsize, owned, nstate, nsteps, step_off, pad0, pad1;
+};
+// LDS bytes of one workgroup: positions of the region, frame state of the
+// first nstate slots (x_start and mass, v and the meta bits).
+inline uint32_t patch_lds_bytes(uint32_t rsize, uint32_t nstate) { return 16u * (rsize + 2u * nstate); }
+
+struct PatchArgs {
+    const PatchHead* heads;   // the group's P heads
+    const PatchStep* steps;
+    const uint32_t* region;
+    const uint2* entries;
+    const float* xin;
+    float* xout;
+    uint32_t per_xcd;         // P / 8
+};
+
+// Positions held in LDS as float4 (16-byte aligned single accesses).
+struct PatchMem {
+    static constexpr bool kStaged = true;
+    float4* lds;
+    static __device__ __forceinline__ F3 ld(const float* p, uint32_t s) { return ld3(p, s); }
+    static __device__ __forceinline__ void st(float* p, uint32_t s, F3 a) { st3(p, s, a); }
+    __device__ __forceinline__ F3 ldx(const float*, uint32_t r) const {
+        const float4 v = lds[r];
+        return F3{v.x, v.y, v.z};
+    }
+    __device__ __forceinline__ void stx(float*, uint32_t r, F3 a) const { lds[r] = float4{a.x, a.y, a.z, 0.f}; }
+};
+
+// Host-side summary of a built layout.
+struct PatchStats {
+    uint32_t patches = 0, colours_per_launch = 0, launches_per_sweep = 0;
+    uint32_t max_region = 0;          // largest region (LDS slots)
+    uint64_t updates_per_sweep = 0;   // vertex updates of one sweep, halos included
+    uint64_t region_slots = 0;        // sum of region sizes over a sweep's launches
+    uint64_t ell_entries = 0;         // local ELL entries (springs + pairs) of a sweep
+    double build_ms = 0.0;
+};
+
+}  // namespace lats

@@ -1,0 +1,324 @@
+// vbd_patch.hip -- the patch strategy of the VBD colour loop (see
+// vbd_patch.h): F colour passes per launch, each workgroup on one patch of
+// the cloth held in LDS, halo updates recomputed locally.
+//
+// The per-vertex arithmetic is vbd_vertex_group's (group_load_ell /
+// group_load_own / group_finish), so every update is bit-identical to the
+// graph strategy's, which follows the reference's FP32 order
+// (xpbd_constraints.h:4027-4205, example2_async_iter.cpp:939-1014).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/libatssim_hip.h"
+#include "host/errors.h"
+#include "vbd_context.h"
+#include "vbd_kernels.h"
+#include "vbd_patch.h"
+#include "vbd_patch_build.h"
+
+using namespace lats;
+
+namespace {
+
+int fail(int code, const std::string& msg) {
+    lats::set_error(msg);
+    return code;
+}
+
+#define HIPCHK(call)                                                                            \
+    do {                                                                                        \
+        hipError_t e_ = (call);                                                                 \
+        if (e_ != hipSuccess)                                                                   \
+            return fail(LATS_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));       \
+    } while (0)
+
+// Uniform tables (heads, steps) are read through the constant address space:
+// scalar loads, which no workgroup barrier orders behind the vector loads in
+// flight (a barrier's fences make the compiler treat later global loads as
+// clobbered and issue them as vector loads, which wait on every load before).
+template <class T>
+using Uniform = const __attribute__((address_space(4))) T*;
+__device__ __forceinline__ PatchHead head_at(Uniform<PatchHead> p, uint32_t j) {
+    return PatchHead{p[j].roff, p[j].rsize, p[j].owned, p[j].nstate, p[j].nsteps, p[j].step_off, 0u, 0u};
+}
+__device__ __forceinline__ PatchStep step_at(Uniform<PatchStep> p, uint32_t j) {
+    return PatchStep{p[j].off, p[j].n, p[j].eoff, p[j].boff, p[j].ew, p[j].bw};
+}
+
+// One launch: workgroup b runs patch (b % 8) * per_xcd + b / 8 through the
+// group's colour steps.  B threads, G lanes per vertex update, so B / G
+// updates per round; a step takes ceil(n / (B / G)) rounds.
+// LDS: positions of the region (rsize float4), then the frame state of the
+// first nstate slots (x_start | mass, v | meta bits).  The only global loads
+// after the region are a round's update entries and ELL entries, which no
+// step writes: those of the next round are issued before the current round's
+// gathers and arithmetic, so their latency overlaps them.
+template <int G, int B>
+__global__ __launch_bounds__(B) void k_vbd_patch(PatchArgs k, VbdArgs a) {
+    extern __shared__ float4 s_lds[];
+    const uint32_t b = blockIdx.x;
+    const uint32_t p = (b % kXcds) * k.per_xcd + b / kXcds;
+    const PatchHead hd = head_at((Uniform<PatchHead>)k.heads, p);
+    const Uniform<PatchStep> steps = (Uniform<PatchStep>)k.steps + hd.step_off;
+    float4* const s_pos = s_lds;
+    float4* const s_x0m = s_lds + hd.rsize;
+    float4* const s_vm = s_x0m + hd.nstate;
+    const PatchMem mem{s_pos};
+    constexpr uint32_t S = B / G;   // updates per round
+    const uint32_t lane = threadIdx.x % G, gi = threadIdx.x / G;
+    const uint32_t wave_first = (threadIdx.x / 64) * (64 / G);   // first update index of this wave in a round
+
+    auto next_step = [&](uint32_t j) {
+        while (j < hd.nsteps && steps[j].n == 0) j++;
+        return j;
+    };
+    // The global loads of one round; lanes past the step's end load entry 0.
+    auto load = [&](const PatchStep& st, uint32_t base, GroupLoads<G>& L, uint2& ent) {
+        const uint32_t i = base + gi;
+        const uint32_t ii = i < st.n ? i : 0u;
+        ent = k.entries[st.off + ii];
+        group_load_ell<G>(a, ii, st.n, st.eoff, st.boff, st.ew, st.bw, lane, L);
+    };
+
+    // the first round's loads, then the region (they overlap).  Every round
+    // load is unconditional, from valid addresses (a step's entry `off` is
+    // valid even when the step is empty): a load on one path only would make
+    // the compiler wait for every load in flight where the paths join.
+    uint32_t j = next_step(0), base = 0;
+    PatchStep st = step_at(steps, j < hd.nsteps ? j : 0u);
+    // two register sets for the rounds' loads, used alternately (a copy from
+    // one to the other would wait for the loads in flight)
+    GroupLoads<G> LA, LB;
+    uint2 eA, eB;
+    load(st, 0u, LA, eA);
+    {
+        // RU slots per thread per pass, every load of a pass issued before
+        // its stores (addresses clamped into the region, stores masked)
+        constexpr uint32_t RU = B >= 1024 ? 2u : (B >= 512 ? 4u : 8u);
+        for (uint32_t r0 = 0; r0 < hd.rsize; r0 += RU * B) {
+            uint32_t sl[RU];
+#pragma unroll
+            for (uint32_t u = 0; u < RU; u++) {
+                const uint32_t r = r0 + u * B + threadIdx.x;
+                sl[u] = k.region[hd.roff + (r < hd.rsize ? r : hd.rsize - 1u)];
+            }
+            F3 xp[RU], x0[RU], v0[RU];
+            float m[RU];
+            uint32_t mt[RU];
+#pragma unroll
+            for (uint32_t u = 0; u < RU; u++) {
+                xp[u] = ld3(k.xin, sl[u]);
+                x0[u] = ld3(a.x0, sl[u]);
+                v0[u] = ld3(a.v, sl[u]);
+                m[u] = a.mass[sl[u]];
+                mt[u] = a.meta[sl[u]];
+            }
+            // (every value used here, so no load sinks into a masked store
+            // below and waits there)
+#pragma unroll
+            for (uint32_t u = 0; u < RU; u++)
+                asm volatile("" ::"v"(xp[u].x), "v"(xp[u].y), "v"(xp[u].z), "v"(x0[u].x), "v"(x0[u].y), "v"(x0[u].z),
+                             "v"(v0[u].x), "v"(v0[u].y), "v"(v0[u].z), "v"(m[u]), "v"(mt[u]));
+#pragma unroll
+            for (uint32_t u = 0; u < RU; u++) {
+                const uint32_t r = r0 + u * B + threadIdx.x;
+                if (r < hd.rsize) s_pos[r] = float4{xp[u].x, xp[u].y, xp[u].z, 0.f};
+                if (r < hd.nstate) {
+                    s_x0m[r] = float4{x0[u].x, x0[u].y, x0[u].z, m[u]};
+                    s_vm[r] = float4{v0[u].x, v0[u].y, v0[u].z, __uint_as_float(mt[u])};
+                }
+            }
+        }
+    }
+    const F3 ext = F3{a.fp->ext_cur[0], a.fp->ext_cur[1], a.fp->ext_cur[2]};
+    __syncthreads();   // region in LDS
+    // One round: issue the next round's loads into (nxt, enxt), update the
+    // vertices of the current one from (cur, ecur); false after the last.
+    auto round = [&](GroupLoads<G>& cur, uint2& ecur, GroupLoads<G>& nxt, uint2& enxt) {
+        uint32_t j2 = j, base2 = base + S;
+        PatchStep st2 = st;
+        if (base2 >= st.n) {
+            base2 = 0;
+            j2 = next_step(j + 1);
+            if (j2 < hd.nsteps) st2 = step_at(steps, j2);
+        }
+        load(st2, base2, nxt, enxt);   // (after the last round: this round's first, unused)
+        if (base + wave_first < st.n) {   // (the whole wave is past the step's end otherwise)
+            const uint32_t i = base + gi;
+            const bool valid = i < st.n;
+            const float4 xm = s_x0m[ecur.y], vm = s_vm[ecur.y];
+            cur.x0 = F3{xm.x, xm.y, xm.z};
+            cur.m = xm.w;
+            cur.v0 = F3{vm.x, vm.y, vm.z};
+            cur.meta = __float_as_uint(vm.w);
+            const F3 xk = mem.ldx(nullptr, ecur.y);
+            group_finish<G>(a, cur, xk, ecur.x, ecur.y, valid ? i : 0u, st.n, st.eoff, st.boff, ext, lane, valid,
+                            mem);
+        }
+        if (j2 != j) __syncthreads();   // the step's LDS writes before the next step's reads
+        st = st2;
+        j = j2;
+        base = base2;
+        return j < hd.nsteps;
+    };
+    while (j < hd.nsteps) {
+        if (!round(LA, eA, LB, eB)) break;
+        if (!round(LB, eB, LA, eA)) break;
+    }
+    for (uint32_t r = threadIdx.x; r < hd.owned; r += B) {
+        const float4 v = s_pos[r];
+        st3(k.xout, k.region[hd.roff + r], F3{v.x, v.y, v.z});
+    }
+}
+
+template <int G>
+const void* patch_kernel_b(uint32_t threads) {
+    switch (threads) {
+        case 256: return (const void*)k_vbd_patch<G, 256>;
+        case 512: return (const void*)k_vbd_patch<G, 512>;
+        default:
+            if constexpr (G == 4) return (const void*)k_vbd_patch<G, 512>;
+            else return (const void*)k_vbd_patch<G, 1024>;
+    }
+}
+const void* patch_kernel(uint32_t lanes, uint32_t threads) {
+    switch (lanes) {
+        case 4: return patch_kernel_b<4>(threads);
+        case 16: return patch_kernel_b<16>(threads);
+        default: return patch_kernel_b<8>(threads);
+    }
+}
+
+}  // namespace
+
+namespace lats {
+
+uint32_t patch_lanes(const lats_vbd* c) { return (c->lanes == 4 || c->lanes == 16) ? c->lanes : 8u; }
+// Threads per workgroup of the launches (4 lanes at 1024 threads would spill
+// registers: 512 instead).
+uint32_t patch_block(const lats_vbd* c) {
+    return (patch_lanes(c) == 4 && c->patch_threads == 1024) ? 512u : c->patch_threads;
+}
+
+// Builds the patch layout of context c for P patches and F colours per
+// launch from the context's device arrays.
+int patch_build(lats_vbd* c, uint32_t P, uint32_t F) {
+    const auto t0 = std::chrono::steady_clock::now();
+    const uint32_t V = c->V;
+    std::vector<uint32_t> meta(V);
+    std::vector<float> x(3 * (size_t)V);
+    std::vector<uint2> ell_e(c->ell_e.n);
+    std::vector<float4> ell_bq(c->ell_bq.n);
+    std::vector<PairIds> ell_bi(c->ell_bi.n);
+    HIPCHK(hipMemcpyAsync(meta.data(), c->meta.p, 4 * (size_t)V, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(x.data(), c->x.p, 12 * (size_t)V, hipMemcpyDeviceToHost, c->stream));
+    if (!ell_e.empty())
+        HIPCHK(hipMemcpyAsync(ell_e.data(), c->ell_e.p, ell_e.size() * sizeof(uint2), hipMemcpyDeviceToHost, c->stream));
+    if (!ell_bq.empty())
+        HIPCHK(hipMemcpyAsync(ell_bq.data(), c->ell_bq.p, ell_bq.size() * sizeof(float4), hipMemcpyDeviceToHost,
+                              c->stream));
+    if (!ell_bi.empty())
+        HIPCHK(hipMemcpyAsync(ell_bi.data(), c->ell_bi.p, ell_bi.size() * sizeof(PairIds), hipMemcpyDeviceToHost,
+                              c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    int lds_limit = 0;
+    HIPCHK(hipDeviceGetAttribute(&lds_limit, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device));
+    PatchSource src;
+    src.V = V;
+    src.C = c->C;
+    src.colours = c->colours.data();
+    src.meta = meta.data();
+    src.x = x.data();
+    src.ell_e = ell_e.data();
+    src.ell_bq = ell_bq.data();
+    src.ell_bi = ell_bi.data();
+    PatchHost h;
+    std::string err;
+    if (!patch_build_host(src, P, F, (uint32_t)lds_limit, h, err))
+        return fail(err.rfind("patch", 0) == 0 ? LATS_ERR_UNSUPPORTED : LATS_ERR_INVALID_ARGUMENT, err);
+    HIPCHK(c->pt_heads.upload(h.heads));
+    HIPCHK(c->pt_steps.upload(h.steps));
+    HIPCHK(c->pt_region.upload(h.region));
+    HIPCHK(c->pt_entries.upload(h.entries));
+    HIPCHK(c->pt_ell_e.upload(h.ell_e));
+    HIPCHK(c->pt_ell_bq.upload(h.ell_bq));
+    HIPCHK(c->pt_ell_bi.upload(h.ell_bi));
+    if (c->xalt.n != 3 * (size_t)V) HIPCHK(c->xalt.alloc(3 * (size_t)V));
+    c->pt_group_lds = h.group_lds;
+    h.stats.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    c->patch_stats = h.stats;
+    c->patch_built_p = h.stats.patches;
+    c->patch_built_f = h.stats.colours_per_launch;
+    return LATS_OK;
+}
+
+// The patch count used when none is set: about 256 slots per patch, at
+// least 64 and at most 1024 patches, a multiple of 8.
+uint32_t patch_default_count(const lats_vbd* c) {
+    uint32_t P = std::max(64u, std::min(1024u, c->V / 256u));
+    P = std::max(8u, P / 8u * 8u);
+    while (P > 8 && P > c->V) P -= 8;
+    return P;
+}
+
+int patch_prepare(lats_vbd* c) {
+    if (c->V < 8) return fail(LATS_ERR_UNSUPPORTED, "patch strategy needs at least 8 vertices");
+    const uint32_t P = c->patch_target ? c->patch_target : patch_default_count(c);
+    uint32_t F = c->patch_colours ? std::min(c->patch_colours, c->C) : c->C;
+    F = std::min(F, kMaxPatchSteps);
+    if (c->patch_built_p == P && c->patch_built_f == F) return LATS_OK;
+    c->drop_graphs();
+    c->patch_built_p = c->patch_built_f = 0;
+    int rc = patch_build(c, P, F);
+    if (rc) return rc;
+    // dynamic LDS above the default limit must be allowed per kernel
+    const uint32_t most = *std::max_element(c->pt_group_lds.begin(), c->pt_group_lds.end());
+    for (uint32_t lanes : {4u, 8u, 16u})
+        for (uint32_t threads : {256u, 512u, 1024u})
+            HIPCHK(hipFuncSetAttribute(patch_kernel(lanes, threads), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)most));
+    return LATS_OK;
+}
+
+uint32_t patch_launches_per_sweep(const lats_vbd* c) { return (uint32_t)c->pt_group_lds.size(); }
+
+// Launch of group g reading positions from xin and writing them to xout.
+void enqueue_patch_launch(lats_vbd* c, uint32_t g, const float* xin, float* xout) {
+    const uint32_t P = c->patch_built_p;
+    PatchArgs k;
+    k.heads = c->pt_heads.p + (size_t)g * P;
+    k.steps = c->pt_steps.p;
+    k.region = c->pt_region.p;
+    k.entries = c->pt_entries.p;
+    k.xin = xin;
+    k.xout = xout;
+    k.per_xcd = P / kXcds;
+    VbdArgs a = c->args(nullptr);
+    a.ell_e = c->pt_ell_e.p;
+    a.ell_bq = c->pt_ell_bq.p;
+    a.ell_bi = c->pt_ell_bi.p;
+    const size_t shm = c->pt_group_lds[g];
+    const uint32_t threads = patch_block(c);
+    const dim3 grid(P), block(threads);
+#define LATS_PATCH(Gv, Bv) hipLaunchKernelGGL((k_vbd_patch<Gv, Bv>), grid, block, shm, c->stream, k, a)
+    switch (patch_lanes(c) * 10000 + threads) {
+        case 40256: LATS_PATCH(4, 256); break;
+        case 40512: LATS_PATCH(4, 512); break;
+        case 80256: LATS_PATCH(8, 256); break;
+        case 80512: LATS_PATCH(8, 512); break;
+        case 160256: LATS_PATCH(16, 256); break;
+        case 160512: LATS_PATCH(16, 512); break;
+        case 161024: LATS_PATCH(16, 1024); break;
+        default: LATS_PATCH(8, 1024); break;
+    }
+#undef LATS_PATCH
+}
+
+}  // namespace lats

@@ -269,6 +269,18 @@ class GpuVBD:
         """Bands of the colour-pass block mapping (1..8, results identical)."""
         check(lib().lats_vbd_set_regions(self._h, int(regions)))
 
+    def set_patches(self, patches=0, colours_per_launch=0, threads=1024):
+        """Patch strategy geometry: patches (multiple of 8, 0 = derived),
+        colour passes per launch (0 = a whole sweep), threads per workgroup
+        (256, 512, 1024).  Results are identical for every choice."""
+        check(lib().lats_vbd_set_patches(self._h, int(patches), int(colours_per_launch), int(threads)))
+
+    def patch_info(self):
+        """The patch layout in use (built on first call): dict of lats_patch_info."""
+        info = _lib.PatchInfo()
+        check(lib().lats_vbd_get_patches(self._h, ctypes.byref(info)))
+        return {name: getattr(info, name) for name, _ in info._fields_}
+
     def set_colour_block(self, threads):
         check(lib().lats_vbd_set_colour_block(self._h, int(threads)))
 

@@ -56,3 +56,13 @@ def test_waiting_events_at_depth(gpu_available):
     print("[case] host/GPU waiting events, depth 60 x 100 loops", flush=True)
     r = _run("test_waiting_events", timeout=300)
     assert r.returncode == 0 and "PASS" in r.stdout, r.stdout + r.stderr
+
+
+def test_patch_layout_bounds_and_gauss_seidel_order():
+    """Patch strategy layout (CPU): every index a patch launch computes is in
+    bounds for 4/8/16 lanes and 256/512/1024 threads, every patch and slot is
+    covered once per launch, and emulated launches reproduce the plain
+    colour-by-colour sweep bit for bit (grids of even, odd and narrow sizes,
+    a mesh with holes, the 256x256 cloth)."""
+    r = _run("test_patch_layout")
+    assert r.returncode == 0 and "PASS" in r.stdout, r.stdout + r.stderr
