@@ -31,6 +31,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 METRIC = "VBD vertex-updates/s (256² cloth); ms/frame to fixed residual, learned vs plain"
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_vbd_colour_grid256_traffic.txt")
+PATCH_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_vbd_patch_grid256_traffic.txt")
 
 
 def read_traffic(path, key="hbm_bytes_per_launch"):
@@ -127,7 +128,10 @@ def main():
                                                                "xpbd_c1"],
                     help="grid_c1 (default, the metric's workload), batch_c2 (512 cloths) or "
                          "copy_calibration (a plain 256 MiB device copy, the reference for HBM counters)")
-    ap.add_argument("--strategy", default="auto", choices=["auto", "graph", "persistent", "flow"])
+    ap.add_argument("--strategy", default="auto", choices=["auto", "graph", "persistent", "flow", "patch"])
+    ap.add_argument("--patches", type=int, default=0, help="patch strategy: patches (multiple of 8; 0 = derived)")
+    ap.add_argument("--patch-colours", type=int, default=0, help="patch strategy: colours per launch (0 = all)")
+    ap.add_argument("--patch-threads", type=int, default=1024, help="patch strategy: threads per workgroup")
     ap.add_argument("--tiles", type=int, default=0, help="flow strategy: vertex tiles (0 = default)")
     ap.add_argument("--lanes", type=int, default=0, help="lanes per vertex (1, 4, 8, 16); 0 = autotuned")
     ap.add_argument("--regions", type=int, default=0, help="colour-pass cloth bands (1..8); 0 = autotuned")
@@ -169,7 +173,9 @@ def main():
         strategy, blocks = g.autotune(frames=2)
     else:
         strategy = {"graph": _lib.LATS_STRATEGY_GRAPH, "persistent": _lib.LATS_STRATEGY_PERSISTENT,
-                    "flow": _lib.LATS_STRATEGY_FLOW}[args.strategy]
+                    "flow": _lib.LATS_STRATEGY_FLOW, "patch": _lib.LATS_STRATEGY_PATCH}[args.strategy]
+        if args.strategy == "patch":
+            g.set_patches(args.patches, args.patch_colours, args.patch_threads)
         g.set_strategy(strategy, persistent_blocks=args.tiles if args.strategy == "flow" else None)
         blocks = g.strategy()[1]
     if args.lanes:
@@ -198,6 +204,14 @@ def main():
         kernel = "k_vbd_persistent" if strategy == _lib.LATS_STRATEGY_PERSISTENT else "k_vbd_flow"
         mean_ms = gpu_ms
         bytes_per_launch = b_sweep * iters * K
+    elif strategy == _lib.LATS_STRATEGY_PATCH:
+        # launches_per_sweep patch launches per sweep, each running its
+        # group's colours: the timed region's HIP events over K frames
+        # (slightly overstated by one predict and one update per frame)
+        kernel = "k_vbd_patch"
+        per_sweep = g.patch_info()["launches_per_sweep"]
+        mean_ms = gpu_ms / (K * iters * per_sweep)
+        bytes_per_launch = b_sweep / per_sweep
     else:
         # The timed region's own HIP events (on the solver's stream, around
         # the K frame graphs): K x iterations x colours colour launches plus
@@ -209,8 +223,13 @@ def main():
         graph_ms, launches, kvu = g.time_colour_kernels(20)
         bytes_per_launch = b_sweep / t.num_clusters
     achieved = bytes_per_launch / (mean_ms * 1e-3) / 1e9
-    traffic = (read_colour_traffic(TRAFFIC_FILE, g.lanes_per_vertex(), g.regions(), g.colour_block())
-               if kernel == "k_vbd_colour" else None)
+    traffic = None
+    if kernel == "k_vbd_colour":
+        traffic = read_colour_traffic(TRAFFIC_FILE, g.lanes_per_vertex(), g.regions(), g.colour_block())
+    elif kernel == "k_vbd_patch":
+        pi = g.patch_info()
+        traffic = read_traffic(PATCH_TRAFFIC_FILE, f"hbm_bytes_per_launch_patch {pi['patches']} "
+                                                   f"{pi['colours_per_launch'] % t.num_clusters} {pi['threads']} {pi['lanes']}")
     roofline = {
         "bound": "hbm",
         "achieved": round(achieved, 2),
@@ -243,8 +262,8 @@ def main():
                         "top row pinned, seeded wind, 100 VBD iterations/frame, one frame per step",
             "colours": t.num_clusters,
             "iterations": iters,
-            "strategy": {_lib.LATS_STRATEGY_PERSISTENT: "persistent", _lib.LATS_STRATEGY_FLOW: "flow"}.get(
-                strategy, "graph"),
+            "strategy": {_lib.LATS_STRATEGY_PERSISTENT: "persistent", _lib.LATS_STRATEGY_FLOW: "flow",
+                         _lib.LATS_STRATEGY_PATCH: "patch"}.get(strategy, "graph"),
             "persistent_blocks_or_tiles": blocks,
             "lanes_per_vertex": g.lanes_per_vertex(),
             "regions": g.regions(),
@@ -253,6 +272,8 @@ def main():
         },
         "roofline": roofline,
     }
+    if strategy == _lib.LATS_STRATEGY_PATCH:
+        out["config"]["patch"] = {k: v for k, v in g.patch_info().items() if k != "build_ms"}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(cloth, params, wind)
     if not args.no_warm_start:

@@ -1647,7 +1647,11 @@ int lats_vbd_set_persistent_blocks(lats_vbd* c, uint32_t blocks) {
 int lats_vbd_get_strategy(lats_vbd* c, int* strategy, uint32_t* blocks) {
     if (!c) return fail(LATS_ERR_INVALID_ARGUMENT, "null context");
     if (strategy) *strategy = c->strategy;
-    if (blocks) {
+    if (blocks && c->strategy == LATS_STRATEGY_PATCH) {
+        int rc = patch_prepare(c);
+        if (rc) return rc;
+        *blocks = c->patch_built_p;
+    } else if (blocks) {
         uint32_t g = 0;
         int rc = persistent_grid(c, &g);
         if (rc) return rc;
@@ -1711,6 +1715,18 @@ int lats_vbd_autotune(lats_vbd* c, uint32_t frames, int* chosen, uint32_t* chose
             if (tiles <= fcap && tiles * 32u <= c->V) cands.push_back({LATS_STRATEGY_FLOW, tiles, lanes});
     }
     c->lanes = keep_lanes;
+    // patch strategy (several colour passes per launch): measured to pay on
+    // cloths of up to ~10^4-10^5 vertices, where a colour pass is mostly
+    // launch overhead; candidates {patches, colours per launch (0 = all),
+    // threads}, 8 lanes per vertex
+    const uint32_t keep_pt = c->patch_target, keep_pf = c->patch_colours, keep_pb = c->patch_threads;
+    if (c->V >= 64 && c->V <= (1u << 18)) {
+        auto round8 = [](uint32_t n) { return std::max(8u, std::min(4096u, n / 8u * 8u)); };
+        const uint32_t pa = round8(c->V / 256u), pb = round8(c->V / 64u);
+        for (const Cand& k : {Cand{LATS_STRATEGY_PATCH, pa, 8, 0, 1024}, Cand{LATS_STRATEGY_PATCH, pa, 8, 0, 512},
+                              Cand{LATS_STRATEGY_PATCH, pb, 8, 0, 256}, Cand{LATS_STRATEGY_PATCH, pa, 8, 4, 1024}})
+            if (k.blocks <= c->V) cands.push_back(k);
+    }
     float best = 3.4e38f;
     Cand win = cands[0];
     const int keep_strategy = c->strategy;
@@ -1723,6 +1739,33 @@ int lats_vbd_autotune(lats_vbd* c, uint32_t frames, int* chosen, uint32_t* chose
             c->drop_graphs();
         }
     };
+    // a candidate's settings (patch candidates carry patches / colours per
+    // launch / threads in blocks / regions / threads)
+    auto apply = [&](const Cand& k) {
+        if (c->strategy != k.strategy) c->drop_graphs();
+        c->strategy = k.strategy;
+        if (k.strategy == LATS_STRATEGY_PATCH) {
+            if (c->lanes != k.lanes || c->patch_target != k.blocks || c->patch_colours != k.regions ||
+                c->patch_threads != k.threads)
+                c->drop_graphs();
+            c->lanes = k.lanes;
+            c->patch_target = k.blocks;
+            c->patch_colours = k.regions;
+            c->patch_threads = k.threads;
+        } else {
+            c->persist_blocks = k.blocks;
+            use(k.lanes, k.regions, k.threads);
+        }
+    };
+    auto restore_settings = [&]() {
+        if (c->strategy != keep_strategy) c->drop_graphs();
+        c->strategy = keep_strategy;
+        c->persist_blocks = keep_blocks;
+        c->patch_target = keep_pt;
+        c->patch_colours = keep_pf;
+        c->patch_threads = keep_pb;
+        use(keep_lanes, keep_regions, keep_threads);
+    };
     // Two passes over the candidates, each candidate's best time counts:
     // the first timings after a context is created can be several per cent
     // off (clocks and caches settling), enough to swap candidates that close.
@@ -1730,15 +1773,17 @@ int lats_vbd_autotune(lats_vbd* c, uint32_t frames, int* chosen, uint32_t* chose
     for (int pass = 0; pass < 2; pass++) {
         for (size_t i = 0; i < cands.size(); i++) {
             const Cand& k = cands[i];
-            c->strategy = k.strategy;
-            c->persist_blocks = k.blocks;
-            use(k.lanes, k.regions, k.threads);
+            apply(k);
             int rc = lats_vbd_run_frames(c, 1, nullptr);   // warm-up / capture
             if (!rc) rc = lats_vbd_run_frames(c, frames, nullptr);
+            if (rc && k.strategy == LATS_STRATEGY_PATCH && rc == LATS_ERR_UNSUPPORTED) {
+                // (a layout that does not fit LDS is no candidate)
+                rc = restore();
+                if (rc) return rc;
+                continue;
+            }
             if (rc) {
-                c->strategy = keep_strategy;
-                c->persist_blocks = keep_blocks;
-                use(keep_lanes, keep_regions, keep_threads);
+                restore_settings();
                 restore();
                 return rc;
             }
@@ -1758,9 +1803,8 @@ int lats_vbd_autotune(lats_vbd* c, uint32_t frames, int* chosen, uint32_t* chose
                         (float)k.threads})
             c->tune_results.push_back(v);
     }
-    c->strategy = win.strategy;
-    c->persist_blocks = win.blocks;
-    use(win.lanes, win.regions, win.threads);
+    restore_settings();
+    apply(win);
     if (chosen) *chosen = win.strategy;
     if (chosen_blocks) *chosen_blocks = win.blocks;
     return LATS_OK;

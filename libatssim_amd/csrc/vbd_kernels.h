@@ -379,6 +379,36 @@ __device__ __forceinline__ void group_finish(const VbdArgs& a, const GroupLoads<
                                              uint32_t i, uint32_t n, uint32_t eoff, uint32_t boff, F3 ext,
                                              uint32_t lane, bool valid, Mem mem);
 
+// The end of a vertex update once H and f hold the inertia, stretch and
+// bending sums: obstacle contacts of slot s (their local sum from zero, then
+// added; the contact Hessian is not exactly symmetric, so the full 3x3 is
+// solved), then the guarded Newton step; the new position goes to sx.
+template <class Mem>
+__device__ __forceinline__ void vertex_solve(const VbdArgs& a, const Sym3& H, F3 f, F3 xk, uint32_t s, uint32_t sx,
+                                             Mem mem) {
+    if (a.ccount) {
+        const uint32_t nc = a.ccount[s];
+        if (nc) {
+            float Hc[3][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
+            F3 fc = F3{0.f, 0.f, 0.f};
+            for (uint32_t k = 0; k < nc; k++) {
+                const float4 q = a.ct_xs[k * a.nslots + s];
+                const float4 nn = a.ct_n[k * a.nslots + s];
+                contact_term(xk, F3{q.x, q.y, q.z}, F3{nn.x, nn.y, nn.z}, q.w, a.thickness, Hc, fc);
+            }
+            const float M[3][3] = {{H.xx + Hc[0][0], H.xy + Hc[0][1], H.xz + Hc[0][2]},
+                                   {H.xy + Hc[1][0], H.yy + Hc[1][1], H.yz + Hc[1][2]},
+                                   {H.xz + Hc[2][0], H.yz + Hc[2][1], H.zz + Hc[2][2]}};
+            f = add3(f, fc);
+            F3 dx;
+            if (newton_step_full(M, f, dx)) mem.stx(a.x, sx, add3(xk, dx));
+            return;
+        }
+    }
+    F3 dx;
+    if (newton_step(H, f, dx)) mem.stx(a.x, sx, add3(xk, dx));
+}
+
 template <int G, class Mem = PlainMem>
 __device__ __forceinline__ void vbd_vertex_group(const VbdArgs& a, uint32_t s, uint32_t i, uint32_t n, uint32_t eoff,
                                                  uint32_t boff, uint32_t ew, uint32_t bw, F3 ext, uint32_t lane,
@@ -519,28 +549,7 @@ __device__ __forceinline__ void group_finish(const VbdArgs& a, const GroupLoads<
     H.zz = H.zz + hb;
     f = add3(f, fb);
     if (!valid || lane != 0) return;
-
-    if (a.ccount) {
-        const uint32_t nc = a.ccount[s];
-        if (nc) {
-            float Hc[3][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
-            F3 fc = F3{0.f, 0.f, 0.f};
-            for (uint32_t k = 0; k < nc; k++) {
-                const float4 q = a.ct_xs[k * a.nslots + s];
-                const float4 nn = a.ct_n[k * a.nslots + s];
-                contact_term(xk, F3{q.x, q.y, q.z}, F3{nn.x, nn.y, nn.z}, q.w, a.thickness, Hc, fc);
-            }
-            const float M[3][3] = {{H.xx + Hc[0][0], H.xy + Hc[0][1], H.xz + Hc[0][2]},
-                                   {H.xy + Hc[1][0], H.yy + Hc[1][1], H.yz + Hc[1][2]},
-                                   {H.xz + Hc[2][0], H.yz + Hc[2][1], H.zz + Hc[2][2]}};
-            f = add3(f, fc);
-            F3 dx;
-            if (newton_step_full(M, f, dx)) mem.stx(a.x, sx, add3(xk, dx));
-            return;
-        }
-    }
-    F3 dx;
-    if (newton_step(H, f, dx)) mem.stx(a.x, sx, add3(xk, dx));
+    vertex_solve(a, H, f, xk, s, sx, mem);
 }
 
 }  // namespace lats

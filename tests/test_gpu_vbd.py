@@ -296,7 +296,8 @@ def test_persistent_c1_and_autotune(gpu_available):
     b = GpuVBD(cloth, params)
     x_before = b.state()[0]
     st, nb = b.autotune(frames=1)
-    assert st in (_lib.LATS_STRATEGY_GRAPH, _lib.LATS_STRATEGY_PERSISTENT, _lib.LATS_STRATEGY_FLOW)
+    assert st in (_lib.LATS_STRATEGY_GRAPH, _lib.LATS_STRATEGY_PERSISTENT, _lib.LATS_STRATEGY_FLOW,
+                  _lib.LATS_STRATEGY_PATCH)
     assert np.array_equal(b.state()[0], x_before)          # autotune restores the state
     # the reported candidates: the kept configuration is the fastest one
     res = b.autotune_results()
@@ -305,6 +306,11 @@ def test_persistent_c1_and_autotune(gpu_available):
     assert int(win[0]) == st and int(win[2]) == b.lanes_per_vertex()
     if st == _lib.LATS_STRATEGY_GRAPH:
         assert int(win[3]) == b.regions() and int(win[5]) == b.colour_block()
+    if st == _lib.LATS_STRATEGY_PATCH:
+        info = b.patch_info()
+        assert int(win[1]) == info["patches"] == nb and int(win[5]) == info["threads"]
+    # patch candidates are timed on this cloth (65,536 vertices)
+    assert (res[:, 0] == _lib.LATS_STRATEGY_PATCH).sum() >= 1
     b.set_strategy(_lib.LATS_STRATEGY_PERSISTENT)
     a.run_frames(3, wind)
     b.run_frames(3, wind)
