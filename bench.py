@@ -16,11 +16,18 @@ Also reported:
   cpu_baseline  the CPU oracle (C++ restatement of the reference's CPU VBD,
                 per-colour parallel passes) on this host's cores, over a bounded
                 sample of the same workload (whole frames, >= 10 s of work).
+  warm_start    the learned warm start on the same cloth (trained in the run).
+  other_workloads
+                short runs of the other configured workloads -- batch_c2 (512
+                cloths), sphere_c4 (1024x1024 on a sphere), xpbd_c1 (XPBD on the
+                256x256 cloth) -- each this script in a child process, its line
+                summarised (--no-other-workloads skips them).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
 import argparse
 import json
+import subprocess
 import os
 import sys
 import time
@@ -138,6 +145,8 @@ def main():
     ap.add_argument("--colour-block", type=int, default=0, help="colour-pass block threads (64, 128, 256); "
                                                                  "0 = autotuned")
     ap.add_argument("--no-warm-start", action="store_true", help="skip the learned warm-start measurement")
+    ap.add_argument("--no-other-workloads", action="store_true",
+                    help="grid_c1 only: skip the short batch_c2 / sphere_c4 / xpbd_c1 runs reported beside it")
     ap.add_argument("--warm-scenarios", type=int, default=8, help="training scenarios of the warm start")
     ap.add_argument("--warm-frames", type=int, default=60, help="frames per training scenario")
     ap.add_argument("--warm-record-every", type=int, default=1, help="record every k-th training frame")
@@ -147,6 +156,13 @@ def main():
         print("bench.py measures one GPU: the VBD hot path runs on a single MI355X "
               "(multi-GPU sharding is a design note only); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+
+    # the other workloads run first, each in a child process, before this
+    # process uses the GPU (a device shared with a live context refuses
+    # cooperative launches, which the children's autotuners time)
+    others = None
+    if args.workload == "grid_c1" and not args.no_other_workloads:
+        others = other_workloads()
 
     from libatssim_amd import GpuVBD, configs, _lib
 
@@ -280,7 +296,45 @@ def main():
         out["warm_start"] = bench_warm_start(args, cloth, params, wind, plain_ms=gpu_ms / K,
                                              lanes=g.lanes_per_vertex(), regions=g.regions(),
                                              colour_block=g.colour_block())
+    if others is not None:
+        out["other_workloads"] = others
     print(json.dumps(out), flush=True)
+
+
+# The other configured workloads, each a short run of this script in a child
+# process (its own GPU context), summarised beside the grid_c1 line.
+OTHER_WORKLOADS = [
+    ("batch_c2", ["--workload", "batch_c2", "--steps", "5", "--warmup", "1", "--no-cpu-baseline"]),
+    ("sphere_c4", ["--workload", "sphere_c4", "--steps", "5", "--warmup", "1", "--no-warm-start"]),
+    ("xpbd_c1", ["--workload", "xpbd_c1", "--steps", "5", "--warmup", "1", "--no-cpu-baseline"]),
+]
+
+
+def other_workloads():
+    res = {}
+    for name, extra in OTHER_WORKLOADS:
+        cmd = [sys.executable, os.path.abspath(__file__)] + extra
+        print(f"[bench] {name}: python3 bench.py {' '.join(extra)}", file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        try:
+            r = subprocess.run(cmd, capture_output=True, text=True, timeout=240)
+            line = [x for x in r.stdout.splitlines() if x.startswith("{")]
+            if r.returncode != 0 or not line:
+                res[name] = {"error": f"exit {r.returncode}: {r.stderr[-300:]}"}
+                continue
+            d = json.loads(line[-1])
+        except (subprocess.TimeoutExpired, ValueError) as e:
+            res[name] = {"error": str(e)[:300]}
+            continue
+        keep = {k: d[k] for k in ("metric", "value", "unit", "ms_per_step", "steps", "warmup") if k in d}
+        keep["command"] = "python3 bench.py " + " ".join(extra)
+        keep["seconds"] = round(time.perf_counter() - t0, 1)
+        print(f"[bench] {name}: {keep['seconds']} s", file=sys.stderr, flush=True)
+        keep["config"] = {k: v for k, v in d.get("config", {}).items()}
+        if "roofline" in d:
+            keep["roofline"] = d["roofline"]
+        res[name] = keep
+    return res
 
 
 BF16_DENSE_PEAK_TFLOPS = 2500.0
@@ -542,7 +596,11 @@ def bench_batch(args):
                    "lds_staged_cloths": sum(c.topology.num_verts <= b.lds_slots()[0] for c in cloths),
                    "gpu_ms_per_step": gpu_ms / K},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": "k_vbd_batch"},
+                     "frac": round(achieved / HBM_PEAK_GBS, 5),
+                     # one launch runs a frame (100 sweeps of every cloth): PMC bytes of one launch
+                     "traffic": read_traffic(os.path.join(ROOT, "profiles", "k_vbd_batch_c2_profile.txt")),
+                     "kernel": "k_vbd_batch", "mean_launch_us": round(gpu_ms / K * 1e3, 3),
+                     "algorithmic_bytes_per_launch": round(nb_sweep * iters, 1)},
     }
     print(json.dumps(out), flush=True)
 

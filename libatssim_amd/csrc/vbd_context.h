@@ -97,6 +97,7 @@ struct lats_vbd {
     uint32_t lanes = 1;               // lanes per vertex (1, 4, 8, 16)
     uint32_t persist_max_lanes = 0;
     uint32_t persist_max_blocks = 0;  // co-residency limit (queried on first use)
+    bool cooperative_refused = false; // the last cooperative launch was refused by the runtime (nothing enqueued)
 
     // XPBD solver (built on first use): constraints in cluster order, their
     // multipliers, cluster bounds, 1/m per slot

@@ -1130,7 +1130,12 @@ int run_persistent(lats_vbd* c, uint32_t nframes) {
         pa.nframes = std::min(chunk, nframes - done);
         hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c->stream, c->barrier.p, 2u);
         void* kargs[] = {&pa};
-        HIPCHK(hipLaunchCooperativeKernel(persistent_kernel(c->lanes), dim3(g), dim3(kBlock), kargs, 0, c->stream));
+        const hipError_t e = hipLaunchCooperativeKernel(persistent_kernel(c->lanes), dim3(g), dim3(kBlock), kargs, 0,
+                                                        c->stream);
+        if (e != hipSuccess) {
+            c->cooperative_refused = true;
+            return fail(LATS_ERR_HIP, std::string("hipLaunchCooperativeKernel (persistent): ") + hipGetErrorString(e));
+        }
         done += pa.nframes;
     }
     return LATS_OK;
@@ -1247,8 +1252,12 @@ int run_flow(lats_vbd* c, uint32_t nframes) {
         }
         fa.base = c->flow_base;
         void* kargs[] = {&fa};
-        HIPCHK(hipLaunchCooperativeKernel(flow_kernel(c->lanes), dim3(c->flow_tiles), dim3(kBlock), kargs, 0,
-                                          c->stream));
+        const hipError_t e = hipLaunchCooperativeKernel(flow_kernel(c->lanes), dim3(c->flow_tiles), dim3(kBlock), kargs,
+                                                        0, c->stream);
+        if (e != hipSuccess) {
+            c->cooperative_refused = true;
+            return fail(LATS_ERR_HIP, std::string("hipLaunchCooperativeKernel (flow): ") + hipGetErrorString(e));
+        }
         hipLaunchKernelGGL(k_flow_finish, dim3(1), dim3(64), 0, c->stream, c->fp.p, fa.nframes);
         c->flow_base += (uint32_t)(fa.nframes * per_frame);
         done += fa.nframes;
@@ -1774,10 +1783,21 @@ int lats_vbd_autotune(lats_vbd* c, uint32_t frames, int* chosen, uint32_t* chose
         for (size_t i = 0; i < cands.size(); i++) {
             const Cand& k = cands[i];
             apply(k);
+            c->cooperative_refused = false;
             int rc = lats_vbd_run_frames(c, 1, nullptr);   // warm-up / capture
             if (!rc) rc = lats_vbd_run_frames(c, frames, nullptr);
             if (rc && k.strategy == LATS_STRATEGY_PATCH && rc == LATS_ERR_UNSUPPORTED) {
                 // (a layout that does not fit LDS is no candidate)
+                rc = restore();
+                if (rc) return rc;
+                continue;
+            }
+            if (rc == LATS_ERR_HIP && (k.strategy == LATS_STRATEGY_PERSISTENT || k.strategy == LATS_STRATEGY_FLOW) &&
+                c->cooperative_refused) {
+                // the runtime refused the cooperative launch itself (nothing
+                // was enqueued; e.g. the device is shared with another
+                // process): no candidate here
+                c->cooperative_refused = false;
                 rc = restore();
                 if (rc) return rc;
                 continue;
