@@ -60,6 +60,7 @@ def lib():
         L.orc_max_threads.restype = ctypes.c_int
         L.orc_sim_set_obstacle.argtypes = [ctypes.c_void_p, ctypes.c_uint32, _f32p, ctypes.c_uint32, _i32p,
                                            ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_int]
+        L.orc_sim_set_contact_prefilter.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.orc_sim_contacts.restype = ctypes.c_uint32
         L.orc_sim_contacts.argtypes = [ctypes.c_void_p, _u32p, _i32p, ctypes.c_uint32]
         L.orc_sim_async_begin.argtypes = [ctypes.c_void_p]
@@ -218,6 +219,10 @@ class Sim:
         x, f = self._obs
         lib().orc_sim_set_obstacle(self._h, len(x), _ptr(x, _f32p), len(f), _ptr(f, _i32p), float(thickness),
                                    float(stiffness), float(radius), int(max_contacts))
+
+    def set_contact_prefilter(self, on):
+        """The exact prefilter of the contact query (default on; off = every face)."""
+        lib().orc_sim_set_contact_prefilter(self._h, int(bool(on)))
 
     def contacts(self, max_contacts=4):
         """(counts[V], face ids[V, max_contacts] ascending, -1 padded)."""

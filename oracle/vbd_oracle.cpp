@@ -251,6 +251,11 @@ template <typename T> struct Sim {
     int max_contacts = 0;
     struct Contact { int fid; V3<T> xs, n; };
     std::vector<std::vector<Contact>> contacts;  // per vertex, ascending fid
+    // exact prefilter of the face loop (same contacts, see detect_contacts):
+    // per face centroid and circumradius about it, the obstacle's box (FP64)
+    bool prefilter = true;
+    std::vector<double> obs_c, obs_r;
+    double obs_lo[3] = {0, 0, 0}, obs_hi[3] = {0, 0, 0};
     // asynchronous iteration: 32 iterate + 32 begin position buffers,
     // indexed modulo 32 (example2_async_iter.cpp:1147-1157)
     std::vector<V3<T>> iter_buf[32], begin_buf[32];
@@ -319,16 +324,39 @@ template <typename T> static bool template_vf(const V3<T> v[4], T min_distance, 
 
 // Contact detection after predict: every obstacle face within `radius` of
 // the predicted position (template_vf), the max_contacts nearest by
-// (distance, face id), kept in ascending face id.  Brute force over faces.
+// (distance, face id), kept in ascending face id.  Brute force over faces,
+// with an exact prefilter: template_vf's closest point lies on the triangle
+// (the edge fallback clamps to the edge; the projection is used only with
+// |b0| + |b1| + |b2| <= 1 + 1e-8), so a face it accepts has its centroid
+// within radius + circumradius of the vertex (triangle inequality); faces
+// farther than that plus a slack of 1 % (far above FP32 rounding of the
+// distance) are skipped, as is every face for a vertex farther than the
+// radius plus the slack from the obstacle's box.  The skipped faces are ones
+// template_vf rejects, so the contacts are those of the plain loop (a CPU
+// test compares both on random scenes).
 template <typename T> static void detect_contacts(Sim<T>& s) {
     const long nv = (long)s.x.size();
     const int nf = (int)s.obs_f.size() / 3;
     s.contacts.assign(nv, {});
+    double rmax = 0.0;
+    for (int f = 0; f < nf; f++) rmax = std::max(rmax, s.obs_r[f]);
+    const double rad = (double)s.radius, box_reach = 1.01 * (rad + rmax) + 1e-6;
 #pragma omp parallel for schedule(dynamic, 64) num_threads(s.nthreads)
     for (long vid = 0; vid < nv; vid++) {
         struct Cand { T d; int fid; T b[3]; };
         std::vector<Cand> keep;
+        const double p[3] = {(double)s.x[vid].x, (double)s.x[vid].y, (double)s.x[vid].z};
+        if (s.prefilter) {
+            bool near = true;
+            for (int d = 0; d < 3; d++) near = near && p[d] >= s.obs_lo[d] - box_reach && p[d] <= s.obs_hi[d] + box_reach;
+            if (!near) continue;
+        }
         for (int f = 0; f < nf; f++) {
+            if (s.prefilter) {
+                const double dx = p[0] - s.obs_c[3 * f], dy = p[1] - s.obs_c[3 * f + 1], dz = p[2] - s.obs_c[3 * f + 2];
+                const double reach = 1.01 * (rad + s.obs_r[f]) + 1e-6;
+                if (dx * dx + dy * dy + dz * dz > reach * reach) continue;
+            }
             const V3<T> v[4] = {s.x[vid], s.obs_x[s.obs_f[3 * f]], s.obs_x[s.obs_f[3 * f + 1]], s.obs_x[s.obs_f[3 * f + 2]]};
             T d, b[3];
             if (!template_vf(v, s.radius, d, b)) continue;
@@ -972,6 +1000,30 @@ static void set_obstacle(Sim<T>& s, u32 nv, const float* x, u32 nf, const int* f
     s.stiffness = T(stiffness);
     s.radius = T(radius);
     s.max_contacts = max_contacts;
+    s.obs_c.assign(3 * (size_t)nf, 0.0);
+    s.obs_r.assign(nf, 0.0);
+    for (int d = 0; d < 3; d++) {
+        s.obs_lo[d] = 1e300;
+        s.obs_hi[d] = -1e300;
+    }
+    for (u32 i = 0; i < nv; i++)
+        for (int d = 0; d < 3; d++) {
+            s.obs_lo[d] = std::min(s.obs_lo[d], (double)x[3 * i + d]);
+            s.obs_hi[d] = std::max(s.obs_hi[d], (double)x[3 * i + d]);
+        }
+    for (u32 k = 0; k < nf; k++) {
+        double c[3];
+        for (int d = 0; d < 3; d++)
+            c[d] = ((double)x[3 * f[3 * k] + d] + (double)x[3 * f[3 * k + 1] + d] + (double)x[3 * f[3 * k + 2] + d]) / 3.0;
+        double r2 = 0.0;
+        for (int q = 0; q < 3; q++) {
+            double e2 = 0.0;
+            for (int d = 0; d < 3; d++) e2 += ((double)x[3 * f[3 * k + q] + d] - c[d]) * ((double)x[3 * f[3 * k + q] + d] - c[d]);
+            r2 = std::max(r2, e2);
+        }
+        for (int d = 0; d < 3; d++) s.obs_c[3 * k + d] = c[d];
+        s.obs_r[k] = std::sqrt(r2);
+    }
 }
 
 template <typename T> static u32 contact_list(const Sim<T>& s, u32* counts, int* fids, u32 cap) {
@@ -1118,6 +1170,12 @@ void orc_sim_set_obstacle(orc_sim* s, u32 nv, const float* x, u32 nf, const int*
                           float radius, int max_contacts) {
     if (s->fp64) set_obstacle(s->s64, nv, x, nf, f, thickness, stiffness, radius, max_contacts);
     else set_obstacle(s->s32, nv, x, nf, f, thickness, stiffness, radius, max_contacts);
+}
+// The exact prefilter of the contact query on (1, default) or off (0: the
+// plain loop over every face).
+void orc_sim_set_contact_prefilter(orc_sim* s, int on) {
+    s->s32.prefilter = on != 0;
+    s->s64.prefilter = on != 0;
 }
 // Contacts of the last frame: per vertex count, then (fid) lists; returns total.
 u32 orc_sim_contacts(const orc_sim* s, u32* counts, int* fids, u32 cap) {

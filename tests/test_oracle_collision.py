@@ -116,3 +116,42 @@ def test_contact_step_closed_form():
     got, _ = s.state()
     # one iteration, then update_velocity leaves x as the iterate
     assert np.allclose(got[-1], want, rtol=0, atol=1e-12), (got[-1], want)
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_contact_prefilter_is_exact(seed):
+    """The oracle's contact query skips faces by a centroid / circumradius
+    bound and vertices far from the obstacle's box; over frames of a cloth
+    dropped onto a sphere (offset and height from the seed; query radius of
+    several edge lengths, so vertices see many candidate faces and many are
+    skipped near the bound) the contacts and positions equal those of the
+    plain loop over every face, bit for bit."""
+    from libatssim_amd import configs, Params
+    from libatssim_amd.vbd import make_icosphere
+    rng = np.random.default_rng(seed)
+    n, spacing, radius = 20, 0.004, 0.03
+    x, f = configs.horizontal_grid(n, spacing, radius + 0.004 + 0.004 * rng.random(), seed=seed)
+    x = (x + np.float32([0.02 * rng.random() - 0.01, 0.0, 0.02 * rng.random() - 0.01])).astype(np.float32)
+    ox, of = make_icosphere(3, radius)
+    cloth = Cloth(x, f, np.float32(1e-3), np.zeros(n * n, np.uint8))
+    params = Params(constraint_iter_count=10)
+    sims = []
+    for pre in (True, False):
+        mesh = O.Mesh(cloth.x, cloth.faces)
+        s = O.Sim(mesh, cloth.mass, cloth.fixed, implicit_dt=params.implicit_dt, iter_count=10,
+                  k_stretch=params.stiffness_stretch_spring, k_bending=params.stiffness_quadratic_bending,
+                  damping=params.damping_cloth)
+        s._mesh_ref = mesh
+        s.set_obstacle(ox, of, 0.001, 1e5, 0.01)
+        s.set_contact_prefilter(pre)
+        sims.append(s)
+    touched = 0
+    for _ in range(8):
+        for s in sims:
+            s.step()
+        ca, fa = sims[0].contacts()
+        cb, fb = sims[1].contacts()
+        assert np.array_equal(ca, cb) and np.array_equal(fa, fb)
+        assert np.array_equal(sims[0].state()[0], sims[1].state()[0])
+        touched += int((ca > 0).sum())
+    assert touched > 0
