@@ -20,7 +20,9 @@ Also reported:
   other_workloads
                 short runs of the other configured workloads -- batch_c2 (512
                 cloths), sphere_c4 (1024x1024 on a sphere), xpbd_c1 (XPBD on the
-                256x256 cloth) -- each this script in a child process, its line
+                256x256 cloth) -- and of square8k (the reference's own example
+                mesh, ms per iteration beside the reference's published task
+                costs), each this script in a child process, its line
                 summarised (--no-other-workloads skips them).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
@@ -132,7 +134,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", default="grid_c1", choices=["grid_c1", "batch_c2", "sphere_c4", "copy_calibration",
                                                                "mlp_kernels", "comm_matrix", "launch_floor",
-                                                               "xpbd_c1"],
+                                                               "xpbd_c1", "square8k"],
                     help="grid_c1 (default, the metric's workload), batch_c2 (512 cloths) or "
                          "copy_calibration (a plain 256 MiB device copy, the reference for HBM counters)")
     ap.add_argument("--strategy", default="auto", choices=["auto", "graph", "persistent", "flow", "patch"])
@@ -180,6 +182,8 @@ def main():
         return bench_launch_floor(args)
     if args.workload == "xpbd_c1":
         return bench_xpbd(args)
+    if args.workload == "square8k":
+        return bench_square8k(args)
 
     K, W = args.steps, args.warmup
     cloth, params, _, wind = configs.grid_c1(frames=W + K)
@@ -307,6 +311,7 @@ OTHER_WORKLOADS = [
     ("batch_c2", ["--workload", "batch_c2", "--steps", "5", "--warmup", "1", "--no-cpu-baseline"]),
     ("sphere_c4", ["--workload", "sphere_c4", "--steps", "5", "--warmup", "1", "--no-warm-start"]),
     ("xpbd_c1", ["--workload", "xpbd_c1", "--steps", "5", "--warmup", "1", "--no-cpu-baseline"]),
+    ("square8k", ["--workload", "square8k", "--steps", "20", "--warmup", "3"]),
 ]
 
 
@@ -326,7 +331,9 @@ def other_workloads():
         except (subprocess.TimeoutExpired, ValueError) as e:
             res[name] = {"error": str(e)[:300]}
             continue
-        keep = {k: d[k] for k in ("metric", "value", "unit", "ms_per_step", "steps", "warmup") if k in d}
+        keep = {k: d[k] for k in ("metric", "value", "unit", "ms_per_step", "steps", "warmup", "ms_per_iteration",
+                                  "reference_published_ms_per_iteration", "speedup_vs_reference_published")
+                if k in d}
         keep["command"] = "python3 bench.py " + " ".join(extra)
         keep["seconds"] = round(time.perf_counter() - t0, 1)
         print(f"[bench] {name}: {keep['seconds']} s", file=sys.stderr, flush=True)
@@ -601,6 +608,54 @@ def bench_batch(args):
                      "traffic": read_traffic(os.path.join(ROOT, "profiles", "k_vbd_batch_c2_profile.txt")),
                      "kernel": "k_vbd_batch", "mean_launch_us": round(gpu_ms / K * 1e3, 3),
                      "algorithmic_bytes_per_launch": round(nb_sweep * iters, 1)},
+    }
+    print(json.dumps(out), flush=True)
+
+
+# The reference's published per-iteration cost of its own example on its own
+# mesh (square8K.obj, 8,282 vertices, 10 colours): the sum over the 10
+# colour clusters of the pre-profiled VBD task costs, Apple GPU (Metal) and
+# Apple CPU (12 threads), examples/example3_hybrid_cpu_gpu.cpp:2297-2306.
+REF_SQUARE8K_MS_PER_ITER = {"gpu_metal": 0.6527, "cpu_12_threads": 0.7306}
+
+
+def bench_square8k(args):
+    """The reference's own example: square8K.obj with example2's parameters
+    (pins, mass 0.01, k_s 1e4, k_b 2.5e-3, damping 2, 100 iterations per
+    frame), autotuned; ms per VBD iteration beside the reference's published
+    per-iteration task costs of the same mesh."""
+    import numpy as np
+    from libatssim_amd import GpuVBD, configs, _lib
+    d = np.load(os.path.join(ROOT, "tests", "golden", "square8k_mesh.npz"))
+    cloth = configs.square8k_cloth(d["x"], d["faces"])
+    params = configs.square8k_params()
+    t = cloth.topology
+    g = GpuVBD(cloth, params)
+    strategy, blocks = g.autotune(frames=3)
+    K, W = args.steps, args.warmup
+    if W:
+        g.run_frames(W)
+    g.synchronize()
+    t0 = time.perf_counter()
+    g.run_frames(K)
+    wall = time.perf_counter() - t0
+    gpu_ms = g.last_elapsed_ms()
+    iters = params.constraint_iter_count
+    ms_iter = gpu_ms / K / iters
+    out = {
+        "metric": "VBD vertex-updates/s (square8K.obj, the reference's example mesh)",
+        "value": t.num_verts * iters * K / wall, "unit": "vertex-updates/s", "n_gpus": 1, "steps": K, "warmup": W,
+        "ms_per_step": wall * 1e3 / K, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "the reference's square8K.obj (as committed under tests/golden)",
+        "config": {"workload": f"square8k: {t.num_verts} verts, {t.num_clusters} colours, {iters} iterations/frame, "
+                               "example2 parameters",
+                   "strategy": {_lib.LATS_STRATEGY_PERSISTENT: "persistent", _lib.LATS_STRATEGY_FLOW: "flow",
+                                _lib.LATS_STRATEGY_PATCH: "patch"}.get(strategy, "graph"),
+                   "blocks_or_patches": blocks, "lanes_per_vertex": g.lanes_per_vertex(),
+                   "gpu_ms_per_step": gpu_ms / K},
+        "ms_per_iteration": round(ms_iter, 5),
+        "reference_published_ms_per_iteration": REF_SQUARE8K_MS_PER_ITER,
+        "speedup_vs_reference_published": {k: round(v / ms_iter, 1) for k, v in REF_SQUARE8K_MS_PER_ITER.items()},
     }
     print(json.dumps(out), flush=True)
 
