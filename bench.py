@@ -54,16 +54,28 @@ def read_traffic(path, key="hbm_bytes_per_launch"):
 
 
 def read_colour_traffic(path, lanes, regions, threads):
-    """HBM bytes per colour launch from a committed PMC summary, if it was
-    measured at this lanes / bands / block-threads configuration (its
-    `hbm_bytes_per_launch L R T: N` lines), else None."""
+    """HBM bytes per colour launch from a committed PMC summary (its
+    `hbm_bytes_per_launch L R T: N` lines): measured at this lanes / bands /
+    block-threads configuration, else at the same lanes and bands with
+    another block size (a lane's loads do not depend on the block size),
+    else None."""
+    b = read_colour_traffic_shape(path, lanes, regions, threads)
+    return b[0] if b else None
+
+
+def read_colour_traffic_shape(path, lanes, regions, threads):
+    """(bytes, "L R T" shape measured) for read_colour_traffic, or None."""
     if not os.path.exists(path):
         return None
-    key = f"hbm_bytes_per_launch {lanes} {regions} {threads}:"
+    exact = f"hbm_bytes_per_launch {lanes} {regions} {threads}:"
+    near = f"hbm_bytes_per_launch {lanes} {regions} "
+    fallback = None
     for line in open(path):
-        if line.startswith(key):
-            return float(line.split(":", 1)[1])
-    return None
+        if line.startswith(exact):
+            return float(line.split(":", 1)[1]), f"{lanes} {regions} {threads}"
+        if fallback is None and line.startswith(near):
+            fallback = float(line.split(":", 1)[1]), line[len("hbm_bytes_per_launch "):].split(":", 1)[0]
+    return fallback
 
 
 def algorithmic_bytes_per_sweep(nv, ne, nb):
@@ -511,6 +523,9 @@ def bench_sphere(args):
                      "frac": round(achieved / HBM_PEAK_GBS, 5),
                      "traffic": read_colour_traffic(os.path.join(ROOT, "profiles", "k_vbd_colour_sphere1024_traffic.txt"),
                                                     g.lanes_per_vertex(), g.regions(), g.colour_block()),
+                     "traffic_measured_at_lanes_bands_threads": (read_colour_traffic_shape(
+                         os.path.join(ROOT, "profiles", "k_vbd_colour_sphere1024_traffic.txt"),
+                         g.lanes_per_vertex(), g.regions(), g.colour_block()) or (None, None))[1],
                      "kernel": "k_vbd_colour", "mean_launch_us": round(mean_ms * 1e3, 3),
                      "algorithmic_bytes_per_launch": round(b_sweep / t.num_clusters, 1)},
     }
