@@ -646,7 +646,17 @@ def bench_square8k(args):
     params = configs.square8k_params()
     t = cloth.topology
     g = GpuVBD(cloth, params)
-    strategy, blocks = g.autotune(frames=3)
+    if args.strategy == "auto":
+        strategy, blocks = g.autotune(frames=3)
+    else:
+        # a fixed strategy (no autotune: e.g. under a profiler)
+        if args.strategy == "patch":
+            g.set_patches(args.patches, args.patch_colours, args.patch_threads)
+        if args.lanes:
+            g.set_lanes_per_vertex(args.lanes)
+        g.set_strategy({"graph": _lib.LATS_STRATEGY_GRAPH, "persistent": _lib.LATS_STRATEGY_PERSISTENT,
+                        "flow": _lib.LATS_STRATEGY_FLOW, "patch": _lib.LATS_STRATEGY_PATCH}[args.strategy])
+        strategy, blocks = g.strategy()
     K, W = args.steps, args.warmup
     if W:
         g.run_frames(W)
