@@ -122,11 +122,18 @@ struct lats_vbd {
     uint32_t patch_colours = 0;       // colours per launch requested (0 = every colour)
     uint32_t patch_threads = 1024;    // threads per workgroup (256, 512, 1024)
     uint32_t patch_built_p = 0, patch_built_f = 0;   // built layout (0 = none)
-    std::vector<uint32_t> pt_group_lds;   // per launch of a sweep: largest region in LDS slots
+    std::vector<uint32_t> pt_group_lds;   // per launch of a sweep: most LDS bytes of one workgroup
+    // per launch of a sweep: first region entry, region stride, first colour
+    // (launch g runs colours [pt_group_c0[g], pt_group_c0[g + 1]))
+    std::vector<uint32_t> pt_group_roff, pt_group_rstride, pt_group_c0;
+    std::vector<uint32_t> pt_group_nsmax;   // per launch of a sweep: most slots with frame state in a region
     lats::PatchStats patch_stats;
     lats::DBuf<lats::PatchHead> pt_heads;
     lats::DBuf<lats::PatchStep> pt_steps;
     lats::DBuf<uint32_t> pt_region;
+    // frame state of every region entry, in region order (x_start | mass,
+    // v | meta bits): gathered at the start of every run of sweeps
+    lats::DBuf<float4> pt_state;
     lats::DBuf<uint2> pt_entries, pt_ell_e;
     lats::DBuf<float4> pt_ell_bq;
     lats::DBuf<lats::PairIds> pt_ell_bi;
@@ -216,4 +223,5 @@ uint32_t patch_launches_per_sweep(const lats_vbd* c);
 uint32_t patch_lanes(const lats_vbd* c);
 uint32_t patch_block(const lats_vbd* c);
 void enqueue_patch_launch(lats_vbd* c, uint32_t g, const float* xin, float* xout);
+void enqueue_patch_stage(lats_vbd* c);
 }  // namespace lats

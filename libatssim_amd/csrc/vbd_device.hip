@@ -1027,10 +1027,12 @@ int build_xpbd(lats_vbd* c) {
     return LATS_OK;
 }
 
-// Colour sweeps with the patch strategy: launches alternate between x and
-// xalt (each reads one and writes the other), the result ends in x.  With
-// `record`, the energy rows 0..sweeps of the frame are written as well.
+// Colour sweeps with the patch strategy: the frame state is staged in region
+// order, then launches alternate between x and xalt (each reads one and
+// writes the other), the result ends in x.  With `record`, the energy rows
+// 0..sweeps of the frame are written as well.
 void enqueue_patch_sweeps(lats_vbd* c, uint32_t sweeps, bool record) {
+    enqueue_patch_stage(c);
     float* cur = c->x.p;
     float* alt = c->xalt.p;
     const uint32_t L = patch_launches_per_sweep(c);
@@ -1733,7 +1735,8 @@ int lats_vbd_autotune(lats_vbd* c, uint32_t frames, int* chosen, uint32_t* chose
         auto round8 = [](uint32_t n) { return std::max(8u, std::min(4096u, n / 8u * 8u)); };
         const uint32_t pa = round8(c->V / 256u), pb = round8(c->V / 64u);
         for (const Cand& k : {Cand{LATS_STRATEGY_PATCH, pa, 8, 0, 1024}, Cand{LATS_STRATEGY_PATCH, pa, 8, 0, 512},
-                              Cand{LATS_STRATEGY_PATCH, pb, 8, 0, 256}, Cand{LATS_STRATEGY_PATCH, pa, 8, 4, 1024}})
+                              Cand{LATS_STRATEGY_PATCH, pb, 8, 0, 256}, Cand{LATS_STRATEGY_PATCH, pb, 8, 0, 512},
+                              Cand{LATS_STRATEGY_PATCH, pa, 8, 4, 1024}, Cand{LATS_STRATEGY_PATCH, pa, 8, 4, 512}})
             if (k.blocks <= c->V) cands.push_back(k);
     }
     float best = 3.4e38f;

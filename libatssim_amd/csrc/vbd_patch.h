@@ -48,22 +48,33 @@ struct PatchStep {
 // slots; LDS index = position in that list), of which the first `owned` are
 // the slots the patch owns and the first `nstate` (owned, then the halo it
 // updates) have their frame state staged in LDS; its steps are
-// steps[step_off, step_off + nsteps).
+// steps[step_off, step_off + nsteps).  Regions and steps lie at fixed strides
+// per group (region[roff + rsize, ...) up to the stride repeats slot
+// region[roff]), so a launch needs no head to address them; roff and
+// step_off restate those addresses for host code.
 struct PatchHead {
     uint32_t roff, rsize, owned, nstate, nsteps, step_off, pad0, pad1;
 };
 // LDS bytes of one workgroup: positions of the region, frame state of the
-// first nstate slots (x_start and mass, v and the meta bits).
-inline uint32_t patch_lds_bytes(uint32_t rsize, uint32_t nstate) { return 16u * (rsize + 2u * nstate); }
+// first nstate slots (x_start and mass, v and the meta bits), global slots of
+// the first `owned` (where the launch writes its results).
+inline uint32_t patch_lds_bytes(uint32_t rsize, uint32_t nstate, uint32_t owned) {
+    return 16u * (rsize + 2u * nstate) + 4u * owned;
+}
 
 struct PatchArgs {
     const PatchHead* heads;   // the group's P heads
-    const PatchStep* steps;
-    const uint32_t* region;
+    const PatchStep* steps;   // the group's steps: patch p's at p * sstride
+    const uint32_t* region;   // the group's regions: patch p's at p * rstride
     const uint2* entries;
+    const float4* state;      // the group's frame state in region order (two per entry)
     const float* xin;
     float* xout;
     uint32_t per_xcd;         // P / 8
+    uint32_t rstride;         // region stride (the group's largest region)
+    uint32_t nsmax;           // most slots with frame state in one region of the group
+    uint32_t nsteps;          // colour steps of the group
+    uint32_t sstride;         // step records per patch
 };
 
 // Positions held in LDS as float4 (16-byte aligned single accesses).

@@ -56,27 +56,35 @@ __device__ __forceinline__ PatchStep step_at(Uniform<PatchStep> p, uint32_t j) {
 // group's colour steps.  B threads, G lanes per vertex update, so B / G
 // updates per round; a step takes ceil(n / (B / G)) rounds.
 // LDS: positions of the region (rsize float4), then the frame state of the
-// first nstate slots (x_start | mass, v | meta bits).  The only global loads
-// after the region are a round's update entries and ELL entries, which no
-// step writes: those of the next round are issued before the current round's
-// gathers and arithmetic, so their latency overlaps them.
+// first nstate slots (x_start | mass, v | meta bits), then the global slots of
+// the first `owned` (the write-back reads them there, not from the region
+// list in memory again).  The only global loads after the region are a
+// round's update entries and ELL entries, which no step writes: those of the
+// next round are issued before the current round's gathers and arithmetic,
+// so their latency overlaps them.
 template <int G, int B>
 __global__ __launch_bounds__(B) void k_vbd_patch(PatchArgs k, VbdArgs a) {
     extern __shared__ float4 s_lds[];
     const uint32_t b = blockIdx.x;
     const uint32_t p = (b % kXcds) * k.per_xcd + b / kXcds;
+    // the head's sizes arrive while the region and the first round load:
+    // their addresses follow from p and the launch's strides alone
     const PatchHead hd = head_at((Uniform<PatchHead>)k.heads, p);
-    const Uniform<PatchStep> steps = (Uniform<PatchStep>)k.steps + hd.step_off;
+    const Uniform<PatchStep> steps = (Uniform<PatchStep>)k.steps + p * k.sstride;
+    const uint32_t* const region = k.region + p * k.rstride;
+    const float4* const state = k.state + 2u * p * k.rstride;
+    const uint32_t nsteps = k.nsteps;
     float4* const s_pos = s_lds;
     float4* const s_x0m = s_lds + hd.rsize;
     float4* const s_vm = s_x0m + hd.nstate;
+    uint32_t* const s_slot = reinterpret_cast<uint32_t*>(s_vm + hd.nstate);
     const PatchMem mem{s_pos};
     constexpr uint32_t S = B / G;   // updates per round
     const uint32_t lane = threadIdx.x % G, gi = threadIdx.x / G;
     const uint32_t wave_first = (threadIdx.x / 64) * (64 / G);   // first update index of this wave in a round
 
     auto next_step = [&](uint32_t j) {
-        while (j < hd.nsteps && steps[j].n == 0) j++;
+        while (j < nsteps && steps[j].n == 0) j++;
         return j;
     };
     // The global loads of one round; lanes past the step's end load entry 0.
@@ -92,7 +100,7 @@ __global__ __launch_bounds__(B) void k_vbd_patch(PatchArgs k, VbdArgs a) {
     // valid even when the step is empty): a load on one path only would make
     // the compiler wait for every load in flight where the paths join.
     uint32_t j = next_step(0), base = 0;
-    PatchStep st = step_at(steps, j < hd.nsteps ? j : 0u);
+    PatchStep st = step_at(steps, j < nsteps ? j : 0u);
     // two register sets for the rounds' loads, used alternately (a copy from
     // one to the other would wait for the loads in flight)
     GroupLoads<G> LA, LB;
@@ -100,39 +108,40 @@ __global__ __launch_bounds__(B) void k_vbd_patch(PatchArgs k, VbdArgs a) {
     load(st, 0u, LA, eA);
     {
         // RU slots per thread per pass, every load of a pass issued before
-        // its stores (addresses clamped into the region, stores masked)
+        // its stores (addresses clamped into the padded region, stores masked
+        // by the patch's own sizes)
         constexpr uint32_t RU = B >= 1024 ? 2u : (B >= 512 ? 4u : 8u);
-        for (uint32_t r0 = 0; r0 < hd.rsize; r0 += RU * B) {
+        // (the frame state comes in region order, so its loads need no
+        // slot index: only the positions wait for one)
+        const uint32_t rs = k.rstride, ns = k.nsmax;
+        for (uint32_t r0 = 0; r0 < rs; r0 += RU * B) {
             uint32_t sl[RU];
+            float4 xm[RU], vm[RU];
 #pragma unroll
             for (uint32_t u = 0; u < RU; u++) {
                 const uint32_t r = r0 + u * B + threadIdx.x;
-                sl[u] = k.region[hd.roff + (r < hd.rsize ? r : hd.rsize - 1u)];
+                sl[u] = region[r < rs ? r : rs - 1u];
+                const uint32_t rn = r < ns ? r : ns - 1u;
+                xm[u] = state[2u * rn];
+                vm[u] = state[2u * rn + 1u];
             }
-            F3 xp[RU], x0[RU], v0[RU];
-            float m[RU];
-            uint32_t mt[RU];
+            F3 xp[RU];
 #pragma unroll
-            for (uint32_t u = 0; u < RU; u++) {
-                xp[u] = ld3(k.xin, sl[u]);
-                x0[u] = ld3(a.x0, sl[u]);
-                v0[u] = ld3(a.v, sl[u]);
-                m[u] = a.mass[sl[u]];
-                mt[u] = a.meta[sl[u]];
-            }
+            for (uint32_t u = 0; u < RU; u++) xp[u] = ld3(k.xin, sl[u]);
             // (every value used here, so no load sinks into a masked store
             // below and waits there)
 #pragma unroll
             for (uint32_t u = 0; u < RU; u++)
-                asm volatile("" ::"v"(xp[u].x), "v"(xp[u].y), "v"(xp[u].z), "v"(x0[u].x), "v"(x0[u].y), "v"(x0[u].z),
-                             "v"(v0[u].x), "v"(v0[u].y), "v"(v0[u].z), "v"(m[u]), "v"(mt[u]));
+                asm volatile("" ::"v"(xp[u].x), "v"(xp[u].y), "v"(xp[u].z), "v"(xm[u].x), "v"(xm[u].y), "v"(xm[u].z),
+                             "v"(xm[u].w), "v"(vm[u].x), "v"(vm[u].y), "v"(vm[u].z), "v"(vm[u].w));
 #pragma unroll
             for (uint32_t u = 0; u < RU; u++) {
                 const uint32_t r = r0 + u * B + threadIdx.x;
                 if (r < hd.rsize) s_pos[r] = float4{xp[u].x, xp[u].y, xp[u].z, 0.f};
+                if (r < hd.owned) s_slot[r] = sl[u];
                 if (r < hd.nstate) {
-                    s_x0m[r] = float4{x0[u].x, x0[u].y, x0[u].z, m[u]};
-                    s_vm[r] = float4{v0[u].x, v0[u].y, v0[u].z, __uint_as_float(mt[u])};
+                    s_x0m[r] = xm[u];
+                    s_vm[r] = vm[u];
                 }
             }
         }
@@ -147,7 +156,7 @@ __global__ __launch_bounds__(B) void k_vbd_patch(PatchArgs k, VbdArgs a) {
         if (base2 >= st.n) {
             base2 = 0;
             j2 = next_step(j + 1);
-            if (j2 < hd.nsteps) st2 = step_at(steps, j2);
+            if (j2 < nsteps) st2 = step_at(steps, j2);
         }
         load(st2, base2, nxt, enxt);   // (after the last round: this round's first, unused)
         if (base + wave_first < st.n) {   // (the whole wave is past the step's end otherwise)
@@ -166,16 +175,29 @@ __global__ __launch_bounds__(B) void k_vbd_patch(PatchArgs k, VbdArgs a) {
         st = st2;
         j = j2;
         base = base2;
-        return j < hd.nsteps;
+        return j < nsteps;
     };
-    while (j < hd.nsteps) {
+    while (j < nsteps) {
         if (!round(LA, eA, LB, eB)) break;
         if (!round(LB, eB, LA, eA)) break;
     }
     for (uint32_t r = threadIdx.x; r < hd.owned; r += B) {
         const float4 v = s_pos[r];
-        st3(k.xout, k.region[hd.roff + r], F3{v.x, v.y, v.z});
+        st3(k.xout, s_slot[r], F3{v.x, v.y, v.z});
     }
+}
+
+// The frame state of region entry e (x_start | mass, v | meta bits) into
+// st[2e], st[2e + 1], for the n entries of every group's regions.
+__global__ __launch_bounds__(256) void k_patch_stage(const uint32_t* region, uint32_t n, const float* x0,
+                                                    const float* v, const float* mass, const uint32_t* meta,
+                                                    float4* st) {
+    const uint32_t e = blockIdx.x * 256u + threadIdx.x;
+    if (e >= n) return;
+    const uint32_t s = region[e];
+    const F3 a = ld3(x0, s), w = ld3(v, s);
+    st[2u * e] = float4{a.x, a.y, a.z, mass[s]};
+    st[2u * e + 1u] = float4{w.x, w.y, w.z, __uint_as_float(meta[s])};
 }
 
 template <int G>
@@ -246,12 +268,17 @@ int patch_build(lats_vbd* c, uint32_t P, uint32_t F) {
     HIPCHK(c->pt_heads.upload(h.heads));
     HIPCHK(c->pt_steps.upload(h.steps));
     HIPCHK(c->pt_region.upload(h.region));
+    if (c->pt_state.n != 2 * h.region.size()) HIPCHK(c->pt_state.alloc(2 * h.region.size()));
     HIPCHK(c->pt_entries.upload(h.entries));
     HIPCHK(c->pt_ell_e.upload(h.ell_e));
     HIPCHK(c->pt_ell_bq.upload(h.ell_bq));
     HIPCHK(c->pt_ell_bi.upload(h.ell_bi));
     if (c->xalt.n != 3 * (size_t)V) HIPCHK(c->xalt.alloc(3 * (size_t)V));
     c->pt_group_lds = h.group_lds;
+    c->pt_group_roff = h.group_roff;
+    c->pt_group_rstride = h.group_rstride;
+    c->pt_group_c0 = h.group_c0;
+    c->pt_group_nsmax = h.group_nsmax;
     h.stats.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     c->patch_stats = h.stats;
     c->patch_built_p = h.stats.patches;
@@ -294,12 +321,17 @@ void enqueue_patch_launch(lats_vbd* c, uint32_t g, const float* xin, float* xout
     const uint32_t P = c->patch_built_p;
     PatchArgs k;
     k.heads = c->pt_heads.p + (size_t)g * P;
-    k.steps = c->pt_steps.p;
-    k.region = c->pt_region.p;
+    k.steps = c->pt_steps.p + (size_t)g * P * c->patch_built_f;
+    k.region = c->pt_region.p + c->pt_group_roff[g];
     k.entries = c->pt_entries.p;
+    k.state = c->pt_state.p + 2 * (size_t)c->pt_group_roff[g];
     k.xin = xin;
     k.xout = xout;
     k.per_xcd = P / kXcds;
+    k.rstride = c->pt_group_rstride[g];
+    k.nsmax = c->pt_group_nsmax[g];
+    k.nsteps = c->pt_group_c0[g + 1] - c->pt_group_c0[g];
+    k.sstride = c->patch_built_f;
     VbdArgs a = c->args(nullptr);
     a.ell_e = c->pt_ell_e.p;
     a.ell_bq = c->pt_ell_bq.p;
@@ -319,6 +351,14 @@ void enqueue_patch_launch(lats_vbd* c, uint32_t g, const float* xin, float* xout
         default: LATS_PATCH(8, 1024); break;
     }
 #undef LATS_PATCH
+}
+
+// The frame state of every region entry, for the patch launches that follow
+// (x_start and v stay fixed while a frame's sweeps run).
+void enqueue_patch_stage(lats_vbd* c) {
+    const uint32_t n = (uint32_t)c->pt_region.n;
+    hipLaunchKernelGGL(k_patch_stage, dim3((n + 255u) / 256u), dim3(256), 0, c->stream, c->pt_region.p, n, c->x0.p,
+                       c->v.p, c->mass.p, c->meta.p, c->pt_state.p);
 }
 
 }  // namespace lats

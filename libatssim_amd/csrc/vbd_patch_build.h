@@ -31,13 +31,19 @@ struct PatchSource {
 // The patch layout, host copies of the arrays the patch launches read.
 struct PatchHost {
     std::vector<PatchHead> heads;     // group g's P heads at [g * P, (g + 1) * P)
-    std::vector<PatchStep> steps;
-    std::vector<uint32_t> region;
+    std::vector<PatchStep> steps;     // patch q of group g: F records at (g * P + q) * F
+    std::vector<uint32_t> region;     // patch q of group g: group_roff[g] + q * group_rstride[g]
     std::vector<uint2> entries, ell_e;
     std::vector<float4> ell_bq;
     std::vector<PairIds> ell_bi;
     std::vector<uint32_t> group_lds;  // per group: most LDS bytes of one workgroup
     std::vector<uint32_t> group_c0;   // per group: first colour; group g runs colours [c0[g], c0[g + 1])
+    // per group: where its regions start and the fixed stride between two
+    // patches' regions (its largest region; shorter ones are padded with
+    // their first slot), so a launch computes every region's address and
+    // every step's from the patch number alone
+    std::vector<uint32_t> group_roff, group_rstride;
+    std::vector<uint32_t> group_nsmax;   // per group: most slots with frame state in one region
     PatchStats stats;
 };
 
@@ -146,12 +152,16 @@ inline bool patch_build_host(const PatchSource& src, uint32_t P, uint32_t F, uin
     stats.colours_per_launch = F;
     stats.launches_per_sweep = ngroups;
 
+    out.group_roff.assign(ngroups, 0);
+    out.group_rstride.assign(ngroups, 0);
+    out.group_nsmax.assign(ngroups, 0);
     std::vector<uint32_t> mark_need(V, 0), mark_reg(V, 0), loc(V, 0);
     uint32_t token = 0;
-    std::vector<std::vector<uint32_t>> need(kMaxPatchSteps);
+    std::vector<std::vector<uint32_t>> need(kMaxPatchSteps), regs(P);
     std::vector<uint32_t> extra;
     for (uint32_t g = 0; g < ngroups; g++) {
         const uint32_t c0 = out.group_c0[g], nst = out.group_c0[g + 1] - c0;
+        const size_t head0 = out.heads.size();
         auto step_of = [&](uint32_t s) -> int {
             const uint32_t k = col[s];
             return (k >= c0 && k < c0 + nst) ? (int)(k - c0) : -1;
@@ -183,15 +193,16 @@ inline bool patch_build_host(const PatchSource& src, uint32_t P, uint32_t F, uin
                 }
             for (uint32_t j = 0; j < nst; j++) std::sort(need[j].begin(), need[j].end());
             // region: owned slots first, then every other slot read, ascending
+            std::vector<uint32_t>& reg = regs[q];
+            reg.clear();
             PatchHead hd{};
-            hd.roff = (uint32_t)out.region.size();
             hd.owned = own_off[q + 1] - own_off[q];
             hd.nsteps = nst;
             hd.step_off = (uint32_t)out.steps.size();
             for (uint32_t r = own_off[q]; r < own_off[q + 1]; r++) {
                 mark_reg[own[r]] = token;
-                loc[own[r]] = (uint32_t)(out.region.size() - hd.roff);
-                out.region.push_back(own[r]);
+                loc[own[r]] = (uint32_t)reg.size();
+                reg.push_back(own[r]);
             }
             // then the halo the patch updates (frame state staged too), then
             // the slots only read
@@ -204,10 +215,10 @@ inline bool patch_build_host(const PatchSource& src, uint32_t P, uint32_t F, uin
                     }
             std::sort(extra.begin(), extra.end());
             for (uint32_t s : extra) {
-                loc[s] = (uint32_t)(out.region.size() - hd.roff);
-                out.region.push_back(s);
+                loc[s] = (uint32_t)reg.size();
+                reg.push_back(s);
             }
-            hd.nstate = (uint32_t)(out.region.size() - hd.roff);
+            hd.nstate = (uint32_t)reg.size();
             extra.clear();
             for (uint32_t j = 0; j < nst; j++)
                 for (uint32_t s : need[j])
@@ -218,18 +229,19 @@ inline bool patch_build_host(const PatchSource& src, uint32_t P, uint32_t F, uin
                         }
             std::sort(extra.begin(), extra.end());
             for (uint32_t s : extra) {
-                loc[s] = (uint32_t)(out.region.size() - hd.roff);
-                out.region.push_back(s);
+                loc[s] = (uint32_t)reg.size();
+                reg.push_back(s);
             }
-            hd.rsize = (uint32_t)(out.region.size() - hd.roff);
-            if (patch_lds_bytes(hd.rsize, hd.nstate) > max_lds) {
+            hd.rsize = (uint32_t)reg.size();
+            if (patch_lds_bytes(hd.rsize, hd.nstate, hd.owned) > max_lds) {
                 err = "patch strategy: a region of " + std::to_string(hd.rsize) + " slots (" +
                       std::to_string(hd.nstate) + " updated) needs " +
-                      std::to_string(patch_lds_bytes(hd.rsize, hd.nstate)) + " bytes of LDS, more than one workgroup's " +
+                      std::to_string(patch_lds_bytes(hd.rsize, hd.nstate, hd.owned)) + " bytes of LDS, more than one workgroup's " +
                       std::to_string(max_lds) + "; use more patches or fewer colours per launch";
                 return false;
             }
-            out.group_lds[g] = std::max(out.group_lds[g], patch_lds_bytes(hd.rsize, hd.nstate));
+            out.group_lds[g] = std::max(out.group_lds[g], patch_lds_bytes(hd.rsize, hd.nstate, hd.owned));
+            out.group_nsmax[g] = std::max(out.group_nsmax[g], hd.nstate);
             stats.max_region = std::max(stats.max_region, hd.rsize);
             stats.region_slots += hd.rsize;
             // steps: update lists and local ELLs (entries hold LDS indices;
@@ -249,8 +261,7 @@ inline bool patch_build_host(const PatchSource& src, uint32_t P, uint32_t F, uin
                 ps.boff = (uint32_t)out.ell_bq.size();
                 if ((uint64_t)out.ell_e.size() + (uint64_t)ew * ps.n >= (1ull << 32) ||
                     (uint64_t)out.ell_bq.size() + (uint64_t)bw * ps.n >= (1ull << 32) ||
-                    (uint64_t)out.entries.size() + ps.n >= (1ull << 32) ||
-                    (uint64_t)out.region.size() >= (1ull << 32)) {
+                    (uint64_t)out.entries.size() + ps.n >= (1ull << 32)) {
                     err = "patch strategy: local adjacency above 2^32 entries";
                     return false;
                 }
@@ -279,7 +290,25 @@ inline bool patch_build_host(const PatchSource& src, uint32_t P, uint32_t F, uin
                 stats.ell_entries += (uint64_t)ps.n * (ew + bw);
                 out.steps.push_back(ps);
             }
+            // F records per patch: a group of fewer colours ends in empty
+            // steps, which no launch reaches (it runs the group's nst steps)
+            out.steps.resize(out.steps.size() + (F - nst), PatchStep{0u, 0u, 0u, 0u, 0u, 0u});
             out.heads.push_back(hd);
+        }
+        // the group's regions at a fixed stride, padded with their first slot
+        uint32_t rs = 1;
+        for (uint32_t q = 0; q < P; q++) rs = std::max(rs, (uint32_t)regs[q].size());
+        if ((uint64_t)out.region.size() + (uint64_t)P * rs >= (1ull << 32)) {
+            err = "patch strategy: regions above 2^32 slots";
+            return false;
+        }
+        out.group_roff[g] = (uint32_t)out.region.size();
+        out.group_rstride[g] = rs;
+        for (uint32_t q = 0; q < P; q++) {
+            PatchHead& hd = out.heads[head0 + q];
+            hd.roff = out.group_roff[g] + q * rs;
+            out.region.insert(out.region.end(), regs[q].begin(), regs[q].end());
+            out.region.resize((size_t)hd.roff + rs, regs[q].empty() ? 0u : regs[q][0]);
         }
     }
     // (a zero-width ELL reads entry 0: keep every array non-empty)

@@ -138,13 +138,23 @@ static bool check_launch(const PatchHost& h, uint32_t g, uint32_t P, uint32_t V,
         const size_t hi = (size_t)g * P + p;
         CHECK(hi < h.heads.size(), "head %zu of %zu", hi, h.heads.size());
         const PatchHead& hd = h.heads[hi];
-        CHECK((size_t)hd.roff + hd.rsize <= h.region.size(), "region range");
+        // the kernel addresses the region and the steps by stride: patch p's
+        // region entries [0, rstride) and step records [0, F) are read
+        const uint32_t F = h.stats.colours_per_launch, rs = h.group_rstride[g];
+        const uint32_t nst = h.group_c0[g + 1] - h.group_c0[g];
+        CHECK(hd.roff == h.group_roff[g] + p * rs, "region address %u", hd.roff);
+        CHECK(hd.rsize <= rs && rs >= 1, "region stride %u below its size %u", rs, hd.rsize);
+        // frame state is read for entries [0, nsmax) of the region (in region order)
+        CHECK(hd.nstate <= h.group_nsmax[g] && h.group_nsmax[g] <= rs && h.group_nsmax[g] >= 1, "state bound");
+        CHECK((size_t)hd.roff + rs <= h.region.size(), "region range");
         CHECK(hd.owned <= hd.rsize, "owned > region");
-        for (uint32_t r = 0; r < hd.rsize; r++) CHECK(h.region[hd.roff + r] < V, "region slot out of range");
-        CHECK((size_t)hd.step_off + hd.nsteps <= h.steps.size(), "step range");
+        for (uint32_t r = 0; r < rs; r++) CHECK(h.region[hd.roff + r] < V, "region slot out of range");
+        CHECK(hd.step_off == (uint32_t)(hi * F), "step address %u", hd.step_off);
+        CHECK(hd.nsteps == nst && nst <= F, "steps %u of the group's %u", hd.nsteps, nst);
+        CHECK((size_t)hd.step_off + F <= h.steps.size(), "step range");
         CHECK(hd.nsteps >= 1, "a launch without steps");
         CHECK(hd.nstate <= hd.rsize && hd.owned <= hd.nstate, "state prefix");
-        CHECK(patch_lds_bytes(hd.rsize, hd.nstate) <= h.group_lds[g], "LDS above the launch's size");
+        CHECK(patch_lds_bytes(hd.rsize, hd.nstate, hd.owned) <= h.group_lds[g], "LDS above the launch's size");
         for (uint32_t j = 0; j < hd.nsteps; j++) {
             const PatchStep& st = h.steps[hd.step_off + j];
             // every step's first entry and ELL entry 0 are loaded, empty or
