@@ -61,6 +61,29 @@ struct PatchHead {
 inline uint32_t patch_lds_bytes(uint32_t rsize, uint32_t nstate, uint32_t owned) {
     return 16u * (rsize + 2u * nstate) + 4u * owned;
 }
+// LDS bytes a launch adds ahead of that for the updates' staged sums: with
+// 8 lanes per vertex, kStagedWords words per group of 8 threads.
+__host__ __device__ inline uint32_t patch_staged_bytes(uint32_t lanes, uint32_t threads) {
+    return lanes == 8u ? threads / 8u * kStagedWords * 4u : 0u;
+}
+// Byte offsets of one workgroup's LDS sections: staged sums at 0, then the
+// region's positions, the frame state (x_start | mass, v | meta) of the
+// first nstate slots, the global slots of the first `owned`; `end` = bytes
+// used.  The kernel places its sections with it and the host sizes its
+// launches with it.
+struct PatchLds {
+    uint32_t pos, x0m, vm, slot, end;
+};
+__host__ __device__ inline PatchLds patch_lds(uint32_t lanes, uint32_t threads, uint32_t rsize, uint32_t nstate,
+                                              uint32_t owned) {
+    PatchLds l;
+    l.pos = patch_staged_bytes(lanes, threads);
+    l.x0m = l.pos + 16u * rsize;
+    l.vm = l.x0m + 16u * nstate;
+    l.slot = l.vm + 16u * nstate;
+    l.end = l.slot + 4u * owned;
+    return l;
+}
 
 struct PatchArgs {
     const PatchHead* heads;   // the group's P heads

@@ -55,10 +55,11 @@ __device__ __forceinline__ PatchStep step_at(Uniform<PatchStep> p, uint32_t j) {
 // One launch: workgroup b runs patch (b % 8) * per_xcd + b / 8 through the
 // group's colour steps.  B threads, G lanes per vertex update, so B / G
 // updates per round; a step takes ceil(n / (B / G)) rounds.
-// LDS: positions of the region (rsize float4), then the frame state of the
-// first nstate slots (x_start | mass, v | meta bits), then the global slots of
-// the first `owned` (the write-back reads them there, not from the region
-// list in memory again).  The only global loads after the region are a
+// LDS: with 8 lanes per vertex, the staged sums of every group of 8 threads
+// (kStagedWords words each), then positions of the region (rsize float4),
+// the frame state of the first nstate slots (x_start | mass, v | meta bits),
+// the global slots of the first `owned` (the write-back reads them there,
+// not from the region list in memory again).  The only global loads after the region are a
 // round's update entries and ELL entries, which no step writes: those of the
 // next round are issued before the current round's gathers and arithmetic,
 // so their latency overlaps them.
@@ -74,10 +75,13 @@ __global__ __launch_bounds__(B) void k_vbd_patch(PatchArgs k, VbdArgs a) {
     const uint32_t* const region = k.region + p * k.rstride;
     const float4* const state = k.state + 2u * p * k.rstride;
     const uint32_t nsteps = k.nsteps;
-    float4* const s_pos = s_lds;
-    float4* const s_x0m = s_lds + hd.rsize;
-    float4* const s_vm = s_x0m + hd.nstate;
-    uint32_t* const s_slot = reinterpret_cast<uint32_t*>(s_vm + hd.nstate);
+    const PatchLds lay = patch_lds(G, B, hd.rsize, hd.nstate, hd.owned);
+    char* const s_base = reinterpret_cast<char*>(s_lds);
+    float* const s_scr = reinterpret_cast<float*>(s_base);
+    float4* const s_pos = reinterpret_cast<float4*>(s_base + lay.pos);
+    float4* const s_x0m = reinterpret_cast<float4*>(s_base + lay.x0m);
+    float4* const s_vm = reinterpret_cast<float4*>(s_base + lay.vm);
+    uint32_t* const s_slot = reinterpret_cast<uint32_t*>(s_base + lay.slot);
     const PatchMem mem{s_pos};
     constexpr uint32_t S = B / G;   // updates per round
     const uint32_t lane = threadIdx.x % G, gi = threadIdx.x / G;
@@ -168,8 +172,12 @@ __global__ __launch_bounds__(B) void k_vbd_patch(PatchArgs k, VbdArgs a) {
             cur.v0 = F3{vm.x, vm.y, vm.z};
             cur.meta = __float_as_uint(vm.w);
             const F3 xk = mem.ldx(nullptr, ecur.y);
-            group_finish<G>(a, cur, xk, ecur.x, ecur.y, valid ? i : 0u, st.n, st.eoff, st.boff, ext, lane, valid,
-                            mem);
+            if constexpr (G == 8)
+                group_finish_staged8(a, cur, xk, ecur.x, ecur.y, valid ? i : 0u, st.n, st.eoff, st.boff, ext, lane,
+                                     valid, mem, s_scr + gi * kStagedWords);
+            else
+                group_finish<G>(a, cur, xk, ecur.x, ecur.y, valid ? i : 0u, st.n, st.eoff, st.boff, ext, lane, valid,
+                                mem);
         }
         if (j2 != j) __syncthreads();   // the step's LDS writes before the next step's reads
         st = st2;
@@ -209,6 +217,11 @@ const void* patch_kernel_b(uint32_t threads) {
             if constexpr (G == 4) return (const void*)k_vbd_patch<G, 512>;
             else return (const void*)k_vbd_patch<G, 1024>;
     }
+}
+// Threads per workgroup of the kernel patch_kernel(lanes, threads) returns.
+uint32_t patch_kernel_threads(uint32_t lanes, uint32_t threads) {
+    if (threads == 256 || threads == 512) return threads;
+    return lanes == 4 ? 512u : 1024u;
 }
 const void* patch_kernel(uint32_t lanes, uint32_t threads) {
     switch (lanes) {
@@ -263,7 +276,9 @@ int patch_build(lats_vbd* c, uint32_t P, uint32_t F) {
     src.ell_bi = ell_bi.data();
     PatchHost h;
     std::string err;
-    if (!patch_build_host(src, P, F, (uint32_t)lds_limit, h, err))
+    // (room for the staged sums of the largest workgroup)
+    const uint32_t staged = patch_staged_bytes(8, 1024);
+    if (!patch_build_host(src, P, F, (uint32_t)lds_limit > staged ? (uint32_t)lds_limit - staged : 0u, h, err))
         return fail(err.rfind("patch", 0) == 0 ? LATS_ERR_UNSUPPORTED : LATS_ERR_INVALID_ARGUMENT, err);
     HIPCHK(c->pt_heads.upload(h.heads));
     HIPCHK(c->pt_steps.upload(h.steps));
@@ -310,7 +325,7 @@ int patch_prepare(lats_vbd* c) {
     for (uint32_t lanes : {4u, 8u, 16u})
         for (uint32_t threads : {256u, 512u, 1024u})
             HIPCHK(hipFuncSetAttribute(patch_kernel(lanes, threads), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)most));
+                                       (int)(most + patch_staged_bytes(lanes, patch_kernel_threads(lanes, threads)))));
     return LATS_OK;
 }
 
@@ -336,8 +351,8 @@ void enqueue_patch_launch(lats_vbd* c, uint32_t g, const float* xin, float* xout
     a.ell_e = c->pt_ell_e.p;
     a.ell_bq = c->pt_ell_bq.p;
     a.ell_bi = c->pt_ell_bi.p;
-    const size_t shm = c->pt_group_lds[g];
     const uint32_t threads = patch_block(c);
+    const size_t shm = c->pt_group_lds[g] + patch_staged_bytes(patch_lanes(c), threads);
     const dim3 grid(P), block(threads);
 #define LATS_PATCH(Gv, Bv) hipLaunchKernelGGL((k_vbd_patch<Gv, Bv>), grid, block, shm, c->stream, k, a)
     switch (patch_lanes(c) * 10000 + threads) {

@@ -155,6 +155,20 @@ static bool check_launch(const PatchHost& h, uint32_t g, uint32_t P, uint32_t V,
         CHECK(hd.nsteps >= 1, "a launch without steps");
         CHECK(hd.nstate <= hd.rsize && hd.owned <= hd.nstate, "state prefix");
         CHECK(patch_lds_bytes(hd.rsize, hd.nstate, hd.owned) <= h.group_lds[g], "LDS above the launch's size");
+        // the kernel's LDS sections: staged sums [0, pos) of exactly the
+        // staged size, then positions, state, slots in order, disjoint, and
+        // inside what the launch allocates (group_lds + staged bytes)
+        {
+            const PatchLds l = patch_lds(G, B, hd.rsize, hd.nstate, hd.owned);
+            const uint32_t staged = patch_staged_bytes(G, B);
+            CHECK(l.pos == staged && (G != 8 || staged == B / 8 * kStagedWords * 4), "staged section");
+            CHECK(l.x0m == l.pos + 16 * hd.rsize && l.vm == l.x0m + 16 * hd.nstate &&
+                      l.slot == l.vm + 16 * hd.nstate && l.end == l.slot + 4 * hd.owned,
+                  "LDS sections out of order");
+            CHECK(l.pos % 16 == 0, "positions not 16-byte aligned");
+            CHECK(l.end <= h.group_lds[g] + staged, "LDS end %u above the allocation %u", l.end,
+                  h.group_lds[g] + staged);
+        }
         for (uint32_t j = 0; j < hd.nsteps; j++) {
             const PatchStep& st = h.steps[hd.step_off + j];
             // every step's first entry and ELL entry 0 are loaded, empty or
