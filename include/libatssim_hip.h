@@ -163,9 +163,10 @@ int lats_vbd_set_strategy(lats_vbd* ctx, int strategy);
 /* Patch strategy geometry.  patches: a multiple of 8 (0: about 256 vertices
  * per patch, 64..1024 patches); colours_per_launch: colour passes one launch
  * runs (0 or >= the colour count: a whole sweep per launch); threads per
- * workgroup: 256, 512 or 1024.  Lanes per vertex come from
- * lats_vbd_set_lanes_per_vertex (4, 8, 16; 1 runs as 8).  The layout is
- * built on first use; a region that does not fit one workgroup's LDS gives
+ * workgroup: 256, 512 or 1024.  Patch updates run on 8 lanes per vertex
+ * whatever lats_vbd_set_lanes_per_vertex sets (the other strategies use
+ * that).  The layout is built on first use; a region that does not fit one
+ * workgroup's LDS gives
  * LATS_ERR_UNSUPPORTED.  No counterpart in the reference (its colour loop
  * dispatches one pass per colour, example3_hybrid_cpu_gpu.cpp:1217-1348). */
 typedef struct lats_patch_info {

@@ -136,7 +136,7 @@ struct lats_vbd {
     lats::DBuf<float4> pt_state;
     lats::DBuf<uint2> pt_entries, pt_ell_e;
     lats::DBuf<float4> pt_ell_bq;
-    lats::DBuf<lats::PairIds> pt_ell_bi;
+    lats::DBuf<lats::PatchPair> pt_ell_bi;
     lats::DBuf<float> xalt;
 
     // graph per (record) flag; graphs of k colour sweeps (lats_vbd_iterate)

@@ -39,8 +39,16 @@ constexpr uint32_t kMaxPatchSteps = 16;
 constexpr uint32_t kXcds = 8;
 
 // One colour step of one patch: its updates are entries [off, off + n) of
-// the entry list ({slot, LDS index}), with a local ELL (ELL entries hold LDS
-// indices) at bases eoff / boff and widths ew / bw.
+// the entry list ({slot, LDS index}), with a local ELL at bases eoff / boff
+// and widths ew / bw.  Local ELL entries hold LDS indices: a spring entry
+// {LDS index of the other end, rest length}; a pair entry the vertex's row
+// of Q (as in the global layout) and a PatchPair.
+struct PatchPair {
+    uint32_t i01;   // LDS indices of the pair's points 0 | 1 << 16 (the vertex itself among them)
+    uint32_t i23;   // points 2 | 3 << 16
+    uint32_t qoo;   // bits of Q[o][o], o = the vertex's position in the pair
+};
+static_assert(sizeof(PatchPair) == sizeof(PairIds), "local pair entries take the global entries' place");
 struct PatchStep {
     uint32_t off, n, eoff, boff, ew, bw;
 };
@@ -61,10 +69,16 @@ struct PatchHead {
 inline uint32_t patch_lds_bytes(uint32_t rsize, uint32_t nstate, uint32_t owned) {
     return 16u * (rsize + 2u * nstate) + 4u * owned;
 }
-// LDS bytes a launch adds ahead of that for the updates' staged sums: with
-// 8 lanes per vertex, kStagedWords words per group of 8 threads.
-__host__ __device__ inline uint32_t patch_staged_bytes(uint32_t lanes, uint32_t threads) {
-    return lanes == 8u ? threads / 8u * kStagedWords * 4u : 0u;
+// A patch update runs on 8 lanes (one group); its sums are formed one value
+// per lane through LDS words of its group: the spring terms of a round (8
+// lanes x 8 values) and the pair terms of the two prefetched rounds (2 x 8
+// lanes x 4 values), plus 8 words that keep the groups of a wave on distinct
+// banks.
+constexpr uint32_t kPatchLanes = 8;
+constexpr uint32_t kStagedWords = 136;
+// LDS bytes a launch of `threads` threads adds for the staged sums.
+__host__ __device__ inline uint32_t patch_staged_bytes(uint32_t threads) {
+    return threads / kPatchLanes * kStagedWords * 4u;
 }
 // Byte offsets of one workgroup's LDS sections: staged sums at 0, then the
 // region's positions, the frame state (x_start | mass, v | meta) of the
@@ -74,10 +88,9 @@ __host__ __device__ inline uint32_t patch_staged_bytes(uint32_t lanes, uint32_t 
 struct PatchLds {
     uint32_t pos, x0m, vm, slot, end;
 };
-__host__ __device__ inline PatchLds patch_lds(uint32_t lanes, uint32_t threads, uint32_t rsize, uint32_t nstate,
-                                              uint32_t owned) {
+__host__ __device__ inline PatchLds patch_lds(uint32_t threads, uint32_t rsize, uint32_t nstate, uint32_t owned) {
     PatchLds l;
-    l.pos = patch_staged_bytes(lanes, threads);
+    l.pos = patch_staged_bytes(threads);
     l.x0m = l.pos + 16u * rsize;
     l.vm = l.x0m + 16u * nstate;
     l.slot = l.vm + 16u * nstate;

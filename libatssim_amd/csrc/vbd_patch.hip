@@ -52,19 +52,166 @@ __device__ __forceinline__ PatchStep step_at(Uniform<PatchStep> p, uint32_t j) {
     return PatchStep{p[j].off, p[j].n, p[j].eoff, p[j].boff, p[j].ew, p[j].bw};
 }
 
+// Orders one wave's LDS writes and reads of the staged terms: the lanes of
+// a wave run in lockstep and a wave's LDS accesses complete in issue order,
+// so a wavefront-scope fence (which keeps the compiler from moving the
+// accesses across it) is all the exchange needs.
+__device__ __forceinline__ void staged_exchange() {
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// The pair's four points (pair order) of a local pair entry.
+__device__ __forceinline__ void patch_pair_points(const PatchMem& mem, const PatchPair& e, F3& p0, F3& p1, F3& p2,
+                                                  F3& p3) {
+    p0 = mem.ldx(nullptr, e.i01 & 0xffffu);
+    p1 = mem.ldx(nullptr, e.i01 >> 16);
+    p2 = mem.ldx(nullptr, e.i23 & 0xffffu);
+    p3 = mem.ldx(nullptr, e.i23 >> 16);
+}
+
+// The arithmetic of one patch update on its group of 8 lanes (lane = 0..7):
+// vbd_vertex_group's terms -- VBD::compute_inertia, compute_stretch_mass_spring,
+// compute_bending_quadratic (xpbd_constraints.h:4027-4205) -- summed as the
+// reference sums them (example2_async_iter.cpp:939-992), then the Newton
+// step of vbd_step (:993-1014) on lane 0.  Lane j computes the terms of
+// incidences j, j + 8, ...; every lane writes its terms to the group's LDS
+// words `scr`, then lane v adds value v of lanes 0, 1, ..., 7 (round after
+// round) in that order -- the additions lane 0 makes in vbd_vertex_group, in
+// the same order, so every sum is bit-identical -- and lane 0 collects the
+// finished sums with DPP moves.  Spring values: lane v < 6 sums S's entry v
+// (xx, xy, xz, yy, yz, zz), lanes 6 / 7 fs.x / fs.y, and lane 0 fs.z with
+// DPP adds; pair values: lanes 0, 3, 5 sum hb (the diagonal they hold), 6 / 7
+// fb.x / fb.y, lane 1 fb.z.  Lanes past a degree hold +0 terms, which leave
+// a sum started at +0 unchanged.  A pair entry names the pair's four points
+// (the vertex's own among them: the same LDS word as xk) and carries
+// Q[o][o], so no lane selects points by the vertex's position in the pair.
+// valid == false: the group has no vertex; it still takes part in the DPP
+// moves and stores nothing.
+__device__ __forceinline__ void patch_update(const VbdArgs& a, const GroupLoads<8>& L, F3 xk, uint32_t s,
+                                             uint32_t sx, uint32_t i, uint32_t n, uint32_t eoff, uint32_t boff,
+                                             F3 ext, uint32_t lane, bool valid, const PatchMem& mem, float* scr) {
+    constexpr uint32_t G = kPatchLanes, PE = GroupLoads<8>::PE, PB = GroupLoads<8>::PB;
+    static_assert(PE == 1 && PB == 2, "one spring round and two pair rounds are staged");
+    const uint2* pe = L.pe;
+    const float4* pq = L.pq;
+    const PatchPair* pp = reinterpret_cast<const PatchPair*>(L.pi);
+    const uint32_t meta = valid ? L.meta : 0u;
+    const uint32_t edeg = meta & 0xffu, bdeg = (meta >> 8) & 0xffu;
+    float4* const s4 = reinterpret_cast<float4*>(scr);
+    const PatchPair* const ell_pair = reinterpret_cast<const PatchPair*>(a.ell_bi);
+
+    const F3 pxo = mem.ldx(nullptr, pe[0].x);
+    F3 pg[PB][4];
+#pragma unroll
+    for (uint32_t k = 0; k < PB; k++) patch_pair_points(mem, pp[k], pg[k][0], pg[k][1], pg[k][2], pg[k][3]);
+    __builtin_amdgcn_sched_barrier(0);   // every gather issued before the arithmetic
+
+    Sym3 Hin;
+    F3 fin;
+    inertia_term(xk, L.x0, L.v0, L.m, (meta >> 16) & 1u, a.g, ext, a.h, a.h2inv, Hin, fin);
+
+    // the terms of the prefetched rounds, staged
+    float fsz = 0.f;
+    {
+        Sym3 t;
+        F3 tf;
+        spring_term_masked(a, xk, lane < edeg, pe[0], pxo, t, tf);
+        s4[2 * lane] = float4{t.xx, t.xy, t.xz, t.yy};
+        s4[2 * lane + 1] = float4{t.yz, t.zz, tf.x, tf.y};
+#pragma unroll
+        for (int q = 0; q < (int)G; q++) fsz = fsz + lane_plus(tf.z, q);
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < PB; k++) {
+        float th = 0.f;
+        F3 tf = F3{0.f, 0.f, 0.f};
+        if (k * G + lane < bdeg) {
+            tf = bending_force(pq[k], pg[k][0], pg[k][1], pg[k][2], pg[k][3], a.kb);
+            th = a.kb * __uint_as_float(pp[k].qoo);
+        }
+        s4[16 + 8 * k + lane] = float4{th, tf.x, tf.y, tf.z};
+    }
+    staged_exchange();
+    // spring chain of value `lane` over the prefetched round
+    float acc = 0.f;
+#pragma unroll
+    for (uint32_t q = 0; q < G; q++) acc = acc + scr[8 * q + lane];
+    // spring rounds past the prefetched one (degrees above 8): the same
+    // staging, one round at a time, after the reads above
+    for (uint32_t r = PE * G; r < edeg; r += G) {
+        const uint32_t j = r + lane;
+        const bool on = j < edeg;
+        const uint2 e = on ? a.ell_e[eoff + j * n + i] : uint2{0u, 0u};
+        Sym3 t;
+        F3 tf;
+        spring_term_of(a, xk, on, e, on ? mem.ldx(nullptr, e.x) : F3{0.f, 0.f, 0.f}, t, tf);
+        staged_exchange();
+        s4[2 * lane] = float4{t.xx, t.xy, t.xz, t.yy};
+        s4[2 * lane + 1] = float4{t.yz, t.zz, tf.x, tf.y};
+#pragma unroll
+        for (int q = 0; q < (int)G; q++) fsz = fsz + lane_plus(tf.z, q);
+        staged_exchange();
+#pragma unroll
+        for (uint32_t q = 0; q < G; q++) acc = acc + scr[8 * q + lane];
+    }
+    // pair chains over the prefetched rounds
+    const uint32_t bv = lane == 6u ? 1u : (lane == 7u ? 2u : (lane == 1u ? 3u : 0u));
+    float accb = 0.f;
+#pragma unroll
+    for (uint32_t k = 0; k < PB; k++)
+#pragma unroll
+        for (uint32_t q = 0; q < G; q++) accb = accb + scr[64 + 32 * k + 4 * q + bv];
+    // pair rounds past the prefetched ones (degrees above 16)
+    for (uint32_t r = PB * G; r < bdeg; r += G) {
+        const uint32_t j = r + lane;
+        float th = 0.f;
+        F3 tf = F3{0.f, 0.f, 0.f};
+        if (j < bdeg) {
+            const uint32_t k = boff + j * n + i;
+            const PatchPair e = ell_pair[k];
+            F3 p0, p1, p2, p3;
+            patch_pair_points(mem, e, p0, p1, p2, p3);
+            tf = bending_force(a.ell_bq[k], p0, p1, p2, p3, a.kb);
+            th = a.kb * __uint_as_float(e.qoo);
+        }
+        staged_exchange();
+        s4[16 + lane] = float4{th, tf.x, tf.y, tf.z};
+        staged_exchange();
+#pragma unroll
+        for (uint32_t q = 0; q < G; q++) accb = accb + scr[64 + 4 * q + bv];
+    }
+    staged_exchange();   // this group's reads before its next writes
+
+    // lane v's value: H = inertia + S (then + hb on the diagonal), f = fin +
+    // fs (then + fb)
+    const bool diag = lane == 0u || lane == 3u || lane == 5u;
+    const float base = lane < 6u ? (diag ? Hin.xx : 0.f) : (lane == 6u ? fin.x : fin.y);
+    float val = base + acc;
+    if (diag || lane >= 6u) val = val + accb;
+    float fz = fin.z + fsz;
+    fz = fz + lane_plus(accb, 1);
+    const Sym3 H = Sym3{val, lane_plus(val, 1), lane_plus(val, 2), lane_plus(val, 3), lane_plus(val, 4),
+                        lane_plus(val, 5)};
+    const F3 f = F3{lane_plus(val, 6), lane_plus(val, 7), fz};
+    if (!valid || lane != 0) return;
+    vertex_solve(a, H, f, xk, s, sx, mem);
+}
+
 // One launch: workgroup b runs patch (b % 8) * per_xcd + b / 8 through the
-// group's colour steps.  B threads, G lanes per vertex update, so B / G
-// updates per round; a step takes ceil(n / (B / G)) rounds.
-// LDS: with 8 lanes per vertex, the staged sums of every group of 8 threads
-// (kStagedWords words each), then positions of the region (rsize float4),
-// the frame state of the first nstate slots (x_start | mass, v | meta bits),
-// the global slots of the first `owned` (the write-back reads them there,
-// not from the region list in memory again).  The only global loads after the region are a
-// round's update entries and ELL entries, which no step writes: those of the
-// next round are issued before the current round's gathers and arithmetic,
-// so their latency overlaps them.
-template <int G, int B>
+// group's colour steps.  B threads, 8 lanes per vertex update, so B / 8
+// updates per round; a step takes ceil(n / (B / 8)) rounds.
+// LDS (patch_lds): the staged sums of every group of 8 threads, positions
+// of the region (rsize float4), the frame state of the first nstate slots
+// (x_start | mass, v | meta bits), the global slots of the first `owned`
+// (the write-back reads them there, not from the region list in memory
+// again).  The only global loads after the region are a round's update
+// entries and ELL entries, which no step writes: those of the next round
+// are issued before the current round's gathers and arithmetic, so their
+// latency overlaps them.
+template <int B>
 __global__ __launch_bounds__(B) void k_vbd_patch(PatchArgs k, VbdArgs a) {
+    constexpr uint32_t G = kPatchLanes;
     extern __shared__ float4 s_lds[];
     const uint32_t b = blockIdx.x;
     const uint32_t p = (b % kXcds) * k.per_xcd + b / kXcds;
@@ -75,7 +222,7 @@ __global__ __launch_bounds__(B) void k_vbd_patch(PatchArgs k, VbdArgs a) {
     const uint32_t* const region = k.region + p * k.rstride;
     const float4* const state = k.state + 2u * p * k.rstride;
     const uint32_t nsteps = k.nsteps;
-    const PatchLds lay = patch_lds(G, B, hd.rsize, hd.nstate, hd.owned);
+    const PatchLds lay = patch_lds(B, hd.rsize, hd.nstate, hd.owned);
     char* const s_base = reinterpret_cast<char*>(s_lds);
     float* const s_scr = reinterpret_cast<float*>(s_base);
     float4* const s_pos = reinterpret_cast<float4*>(s_base + lay.pos);
@@ -86,7 +233,6 @@ __global__ __launch_bounds__(B) void k_vbd_patch(PatchArgs k, VbdArgs a) {
     constexpr uint32_t S = B / G;   // updates per round
     const uint32_t lane = threadIdx.x % G, gi = threadIdx.x / G;
     const uint32_t wave_first = (threadIdx.x / 64) * (64 / G);   // first update index of this wave in a round
-
     auto next_step = [&](uint32_t j) {
         while (j < nsteps && steps[j].n == 0) j++;
         return j;
@@ -172,12 +318,8 @@ __global__ __launch_bounds__(B) void k_vbd_patch(PatchArgs k, VbdArgs a) {
             cur.v0 = F3{vm.x, vm.y, vm.z};
             cur.meta = __float_as_uint(vm.w);
             const F3 xk = mem.ldx(nullptr, ecur.y);
-            if constexpr (G == 8)
-                group_finish_staged8(a, cur, xk, ecur.x, ecur.y, valid ? i : 0u, st.n, st.eoff, st.boff, ext, lane,
-                                     valid, mem, s_scr + gi * kStagedWords);
-            else
-                group_finish<G>(a, cur, xk, ecur.x, ecur.y, valid ? i : 0u, st.n, st.eoff, st.boff, ext, lane, valid,
-                                mem);
+            patch_update(a, cur, xk, ecur.x, ecur.y, valid ? i : 0u, st.n, st.eoff, st.boff, ext, lane, valid, mem,
+                         s_scr + gi * kStagedWords);
         }
         if (j2 != j) __syncthreads();   // the step's LDS writes before the next step's reads
         st = st2;
@@ -208,26 +350,11 @@ __global__ __launch_bounds__(256) void k_patch_stage(const uint32_t* region, uin
     st[2u * e + 1u] = float4{w.x, w.y, w.z, __uint_as_float(meta[s])};
 }
 
-template <int G>
-const void* patch_kernel_b(uint32_t threads) {
+const void* patch_kernel(uint32_t threads) {
     switch (threads) {
-        case 256: return (const void*)k_vbd_patch<G, 256>;
-        case 512: return (const void*)k_vbd_patch<G, 512>;
-        default:
-            if constexpr (G == 4) return (const void*)k_vbd_patch<G, 512>;
-            else return (const void*)k_vbd_patch<G, 1024>;
-    }
-}
-// Threads per workgroup of the kernel patch_kernel(lanes, threads) returns.
-uint32_t patch_kernel_threads(uint32_t lanes, uint32_t threads) {
-    if (threads == 256 || threads == 512) return threads;
-    return lanes == 4 ? 512u : 1024u;
-}
-const void* patch_kernel(uint32_t lanes, uint32_t threads) {
-    switch (lanes) {
-        case 4: return patch_kernel_b<4>(threads);
-        case 16: return patch_kernel_b<16>(threads);
-        default: return patch_kernel_b<8>(threads);
+        case 256: return (const void*)k_vbd_patch<256>;
+        case 512: return (const void*)k_vbd_patch<512>;
+        default: return (const void*)k_vbd_patch<1024>;
     }
 }
 
@@ -235,11 +362,12 @@ const void* patch_kernel(uint32_t lanes, uint32_t threads) {
 
 namespace lats {
 
-uint32_t patch_lanes(const lats_vbd* c) { return (c->lanes == 4 || c->lanes == 16) ? c->lanes : 8u; }
-// Threads per workgroup of the launches (4 lanes at 1024 threads would spill
-// registers: 512 instead).
+// Patch updates run on 8 lanes whatever the context's lanes per vertex (which
+// the other strategies use).
+uint32_t patch_lanes(const lats_vbd*) { return kPatchLanes; }
+// Threads per workgroup of the launches (256, 512 or 1024).
 uint32_t patch_block(const lats_vbd* c) {
-    return (patch_lanes(c) == 4 && c->patch_threads == 1024) ? 512u : c->patch_threads;
+    return (c->patch_threads == 256 || c->patch_threads == 512) ? c->patch_threads : 1024u;
 }
 
 // Builds the patch layout of context c for P patches and F colours per
@@ -277,7 +405,7 @@ int patch_build(lats_vbd* c, uint32_t P, uint32_t F) {
     PatchHost h;
     std::string err;
     // (room for the staged sums of the largest workgroup)
-    const uint32_t staged = patch_staged_bytes(8, 1024);
+    const uint32_t staged = patch_staged_bytes(1024);
     if (!patch_build_host(src, P, F, (uint32_t)lds_limit > staged ? (uint32_t)lds_limit - staged : 0u, h, err))
         return fail(err.rfind("patch", 0) == 0 ? LATS_ERR_UNSUPPORTED : LATS_ERR_INVALID_ARGUMENT, err);
     HIPCHK(c->pt_heads.upload(h.heads));
@@ -322,10 +450,9 @@ int patch_prepare(lats_vbd* c) {
     if (rc) return rc;
     // dynamic LDS above the default limit must be allowed per kernel
     const uint32_t most = *std::max_element(c->pt_group_lds.begin(), c->pt_group_lds.end());
-    for (uint32_t lanes : {4u, 8u, 16u})
-        for (uint32_t threads : {256u, 512u, 1024u})
-            HIPCHK(hipFuncSetAttribute(patch_kernel(lanes, threads), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)(most + patch_staged_bytes(lanes, patch_kernel_threads(lanes, threads)))));
+    for (uint32_t threads : {256u, 512u, 1024u})
+        HIPCHK(hipFuncSetAttribute(patch_kernel(threads), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)(most + patch_staged_bytes(threads))));
     return LATS_OK;
 }
 
@@ -350,22 +477,16 @@ void enqueue_patch_launch(lats_vbd* c, uint32_t g, const float* xin, float* xout
     VbdArgs a = c->args(nullptr);
     a.ell_e = c->pt_ell_e.p;
     a.ell_bq = c->pt_ell_bq.p;
-    a.ell_bi = c->pt_ell_bi.p;
+    // (local pair entries: PatchPair records in the global entries' place)
+    a.ell_bi = reinterpret_cast<const PairIds*>(c->pt_ell_bi.p);
     const uint32_t threads = patch_block(c);
-    const size_t shm = c->pt_group_lds[g] + patch_staged_bytes(patch_lanes(c), threads);
+    const size_t shm = c->pt_group_lds[g] + patch_staged_bytes(threads);
     const dim3 grid(P), block(threads);
-#define LATS_PATCH(Gv, Bv) hipLaunchKernelGGL((k_vbd_patch<Gv, Bv>), grid, block, shm, c->stream, k, a)
-    switch (patch_lanes(c) * 10000 + threads) {
-        case 40256: LATS_PATCH(4, 256); break;
-        case 40512: LATS_PATCH(4, 512); break;
-        case 80256: LATS_PATCH(8, 256); break;
-        case 80512: LATS_PATCH(8, 512); break;
-        case 160256: LATS_PATCH(16, 256); break;
-        case 160512: LATS_PATCH(16, 512); break;
-        case 161024: LATS_PATCH(16, 1024); break;
-        default: LATS_PATCH(8, 1024); break;
+    switch (threads) {
+        case 256: hipLaunchKernelGGL(k_vbd_patch<256>, grid, block, shm, c->stream, k, a); break;
+        case 512: hipLaunchKernelGGL(k_vbd_patch<512>, grid, block, shm, c->stream, k, a); break;
+        default: hipLaunchKernelGGL(k_vbd_patch<1024>, grid, block, shm, c->stream, k, a); break;
     }
-#undef LATS_PATCH
 }
 
 // The frame state of every region entry, for the patch launches that follow

@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstring>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -35,7 +36,7 @@ struct PatchHost {
     std::vector<uint32_t> region;     // patch q of group g: group_roff[g] + q * group_rstride[g]
     std::vector<uint2> entries, ell_e;
     std::vector<float4> ell_bq;
-    std::vector<PairIds> ell_bi;
+    std::vector<PatchPair> ell_bi;
     std::vector<uint32_t> group_lds;  // per group: most LDS bytes of one workgroup
     std::vector<uint32_t> group_c0;   // per group: first colour; group g runs colours [c0[g], c0[g + 1])
     // per group: where its regions start and the fixed stride between two
@@ -233,6 +234,10 @@ inline bool patch_build_host(const PatchSource& src, uint32_t P, uint32_t F, uin
                 reg.push_back(s);
             }
             hd.rsize = (uint32_t)reg.size();
+            if (hd.rsize > 65536u) {   // (local pair entries hold 16-bit LDS indices)
+                err = "patch strategy: a region of " + std::to_string(hd.rsize) + " slots, above 65536";
+                return false;
+            }
             if (patch_lds_bytes(hd.rsize, hd.nstate, hd.owned) > max_lds) {
                 err = "patch strategy: a region of " + std::to_string(hd.rsize) + " slots (" +
                       std::to_string(hd.nstate) + " updated) needs " +
@@ -267,7 +272,7 @@ inline bool patch_build_host(const PatchSource& src, uint32_t P, uint32_t F, uin
                 }
                 out.ell_e.resize(out.ell_e.size() + (size_t)ew * ps.n, uint2{0u, 0u});
                 out.ell_bq.resize(out.ell_bq.size() + (size_t)bw * ps.n, float4{0.f, 0.f, 0.f, 0.f});
-                out.ell_bi.resize(out.ell_bi.size() + (size_t)bw * ps.n, PairIds{0u, 0u, 0u});
+                out.ell_bi.resize(out.ell_bi.size() + (size_t)bw * ps.n, PatchPair{0u, 0u, 0u});
                 for (uint32_t i = 0; i < ps.n; i++) {
                     const uint32_t s = need[j][i];
                     out.entries.push_back(uint2{s, loc[s]});
@@ -281,9 +286,17 @@ inline bool patch_build_host(const PatchSource& src, uint32_t P, uint32_t F, uin
                         const size_t at = ci.boff + (size_t)row * ci.count + ig;
                         const PairIds& id = src.ell_bi[at];
                         const size_t to = ps.boff + (size_t)row * ps.n + i;
-                        out.ell_bq[to] = src.ell_bq[at];
-                        out.ell_bi[to] =
-                            PairIds{loc[id.a & 0x3fffffffu] | (id.a & 0xc0000000u), loc[id.b], loc[id.c]};
+                        const float4 q = src.ell_bq[at];
+                        out.ell_bq[to] = q;
+                        // the pair's four points in pair order, the vertex at o
+                        const uint32_t o = id.a >> 30;
+                        const uint32_t g0 = loc[id.a & 0x3fffffffu], g1 = loc[id.b], g2 = loc[id.c], me = loc[s];
+                        const uint32_t pt[4] = {o == 0 ? me : g0, o == 1 ? me : (o == 0 ? g0 : g1),
+                                                o == 2 ? me : (o <= 1 ? g1 : g2), o == 3 ? me : g2};
+                        const float qo[4] = {q.x, q.y, q.z, q.w};
+                        uint32_t qoo;
+                        std::memcpy(&qoo, &qo[o], 4);
+                        out.ell_bi[to] = PatchPair{pt[0] | (pt[1] << 16), pt[2] | (pt[3] << 16), qoo};
                     }
                 }
                 stats.updates_per_sweep += ps.n;
@@ -315,7 +328,7 @@ inline bool patch_build_host(const PatchSource& src, uint32_t P, uint32_t F, uin
     if (out.ell_e.empty()) out.ell_e.push_back(uint2{0u, 0u});
     if (out.ell_bq.empty()) {
         out.ell_bq.push_back(float4{0.f, 0.f, 0.f, 0.f});
-        out.ell_bi.push_back(PairIds{0u, 0u, 0u});
+        out.ell_bi.push_back(PatchPair{0u, 0u, 0u});
     }
     // a sentinel entry: every step's first entry is readable, empty steps too
     out.entries.push_back(uint2{0u, 0u});

@@ -7,8 +7,10 @@
 //    update entries, local ELL rows for every lane of every round (lanes past
 //    a step's end, rows clamped to the width, zero widths, the tail rows past
 //    the prefetched rounds) and every LDS index gathered through them -- lies
-//    inside the arrays the launch reads, for 4, 8 and 16 lanes per update and
-//    256 / 512 / 1024 threads;
+//    inside the arrays the launch reads, for 256 / 512 / 1024 threads (8
+//    lanes per update), and the launch's LDS sections fit its allocation;
+//  * every local pair entry names the pair's four points in pair order (the
+//    vertex's own LDS word at its position o) and carries Q[o][o];
 //  * workgroup b -> patch (b % 8) * (P / 8) + b / 8 covers every patch once
 //    and the owned slots cover every slot once per launch;
 //  * emulating the launches (region -> LDS, colour steps on LDS, owned slots
@@ -19,6 +21,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -124,10 +127,10 @@ static bool make_layout(const Cloth& cl, Layout& out) {
     return true;
 }
 
-// Bounds of every index one launch of group g computes (k_vbd_patch with G
-// lanes and B threads).
-template <int G>
+// Bounds of every index one launch of group g computes (k_vbd_patch with B
+// threads, 8 lanes per update).
 static bool check_launch(const PatchHost& h, uint32_t g, uint32_t P, uint32_t V, uint32_t B) {
+    constexpr int G = kPatchLanes;
     constexpr uint32_t PE = GroupLoads<G>::PE, PB = GroupLoads<G>::PB;
     const uint32_t S = B / G;
     std::vector<int> seen(P, 0);
@@ -159,9 +162,9 @@ static bool check_launch(const PatchHost& h, uint32_t g, uint32_t P, uint32_t V,
         // staged size, then positions, state, slots in order, disjoint, and
         // inside what the launch allocates (group_lds + staged bytes)
         {
-            const PatchLds l = patch_lds(G, B, hd.rsize, hd.nstate, hd.owned);
-            const uint32_t staged = patch_staged_bytes(G, B);
-            CHECK(l.pos == staged && (G != 8 || staged == B / 8 * kStagedWords * 4), "staged section");
+            const PatchLds l = patch_lds(B, hd.rsize, hd.nstate, hd.owned);
+            const uint32_t staged = patch_staged_bytes(B);
+            CHECK(l.pos == staged && staged == B / 8 * kStagedWords * 4, "staged section");
             CHECK(l.x0m == l.pos + 16 * hd.rsize && l.vm == l.x0m + 16 * hd.nstate &&
                       l.slot == l.vm + 16 * hd.nstate && l.end == l.slot + 4 * hd.owned,
                   "LDS sections out of order");
@@ -198,8 +201,9 @@ static bool check_launch(const PatchHost& h, uint32_t g, uint32_t P, uint32_t V,
                             const uint32_t jj = k * G + lane, row = jj < st.bw ? jj : st.bw - 1;
                             const size_t at = (size_t)(st.boff + row * st.n + ii) * b_on;
                             CHECK(at < h.ell_bq.size() && at < h.ell_bi.size(), "pair ELL %zu", at);
-                            const PairIds q = h.ell_bi[at];
-                            CHECK((q.a & 0x3fffffffu) < hd.rsize && q.b < hd.rsize && q.c < hd.rsize,
+                            const PatchPair q = h.ell_bi[at];
+                            CHECK((q.i01 & 0xffffu) < hd.rsize && (q.i01 >> 16) < hd.rsize &&
+                                      (q.i23 & 0xffffu) < hd.rsize && (q.i23 >> 16) < hd.rsize,
                                   "pair LDS index");
                         }
                         if (i >= st.n) continue;
@@ -211,8 +215,9 @@ static bool check_launch(const PatchHost& h, uint32_t g, uint32_t P, uint32_t V,
                         for (uint32_t jj = PB * G + lane; jj < st.bw; jj += G) {
                             const size_t at = (size_t)st.boff + jj * st.n + i;
                             CHECK(at < h.ell_bi.size(), "tail pair");
-                            const PairIds q = h.ell_bi[at];
-                            CHECK((q.a & 0x3fffffffu) < hd.rsize && q.b < hd.rsize && q.c < hd.rsize,
+                            const PatchPair q = h.ell_bi[at];
+                            CHECK((q.i01 & 0xffffu) < hd.rsize && (q.i01 >> 16) < hd.rsize &&
+                                      (q.i23 & 0xffffu) < hd.rsize && (q.i23 >> 16) < hd.rsize,
                                   "tail pair LDS index");
                         }
                     }
@@ -230,7 +235,11 @@ static bool check_launch(const PatchHost& h, uint32_t g, uint32_t P, uint32_t V,
 
 // Stand-in for the vertex update: reads the own position, every incident
 // spring (other position, rest length) and every incident pair (Q row, the
-// other three positions) in incidence order, as the kernel does.
+// pair's four points in pair order, Q[o][o]) in incidence order, as the
+// kernel does.  ell.pair gives the points and Q[o][o] from the layout's own
+// entry format: the global layout's (three other slots, the vertex at o) or
+// the patch's (four LDS indices, Q[o][o] stored).
+using PosFn = std::function<const double*(uint32_t)>;
 template <class Pos, class Ell>
 static void update(uint32_t meta, double* own, Pos pos, Ell ell) {
     const uint32_t ed = meta & 0xffu, bd = (meta >> 8) & 0xffu;
@@ -246,17 +255,11 @@ static void update(uint32_t meta, double* own, Pos pos, Ell ell) {
     }
     for (uint32_t row = 0; row < bd; row++) {
         float4 q;
-        PairIds id;
-        ell.pair(row, q, id);
-        const uint32_t o = id.a >> 30;
-        const double* g0 = pos(id.a & 0x3fffffffu);
-        const double* g1 = pos(id.b);
-        const double* g2 = pos(id.c);
-        const double* p0 = o == 0 ? own : g0;
-        const double* p1 = o == 1 ? own : (o == 0 ? g0 : g1);
-        const double* p2 = o == 2 ? own : (o <= 1 ? g1 : g2);
-        const double* p3 = o == 3 ? own : g2;
-        for (int d = 0; d < 3; d++) acc[d] = acc[d] + 1e-4 * (q.x * p0[d] + q.y * p1[d] + q.z * p2[d] + q.w * p3[d]);
+        const double* pt[4];
+        float qoo;
+        ell.pair(row, own, PosFn(pos), q, pt, qoo);
+        for (int d = 0; d < 3; d++)
+            acc[d] = acc[d] + 1e-4 * (q.x * pt[0][d] + q.y * pt[1][d] + q.z * pt[2][d] + q.w * pt[3][d]) + 1e-5 * qoo;
     }
     for (int d = 0; d < 3; d++) own[d] = acc[d];
 }
@@ -277,9 +280,20 @@ static bool emulate(const Layout& lay, const PatchHost& h, uint32_t sweeps) {
                     const ColourInfo& ci;
                     uint32_t i;
                     void spring(uint32_t row, uint2& e) const { e = L.ell_e[ci.eoff + (size_t)row * ci.count + i]; }
-                    void pair(uint32_t row, float4& q, PairIds& id) const {
+                    void pair(uint32_t row, const double* own, const PosFn& pos, float4& q, const double* (&pt)[4],
+                              float& qoo) const {
                         q = L.ell_bq[ci.boff + (size_t)row * ci.count + i];
-                        id = L.ell_bi[ci.boff + (size_t)row * ci.count + i];
+                        const PairIds id = L.ell_bi[ci.boff + (size_t)row * ci.count + i];
+                        const uint32_t o = id.a >> 30;
+                        const double* g0 = pos(id.a & 0x3fffffffu);
+                        const double* g1 = pos(id.b);
+                        const double* g2 = pos(id.c);
+                        pt[0] = o == 0 ? own : g0;
+                        pt[1] = o == 1 ? own : (o == 0 ? g0 : g1);
+                        pt[2] = o == 2 ? own : (o <= 1 ? g1 : g2);
+                        pt[3] = o == 3 ? own : g2;
+                        const float qs[4] = {q.x, q.y, q.z, q.w};
+                        qoo = qs[o];
                     }
                 } ell{L, ci, i};
                 update(L.meta[s], &plain[3 * (size_t)s], [&](uint32_t u) { return &plain[3 * (size_t)u]; }, ell);
@@ -305,9 +319,15 @@ static bool emulate(const Layout& lay, const PatchHost& h, uint32_t sweeps) {
                             const PatchStep& st;
                             uint32_t i;
                             void spring(uint32_t row, uint2& e) const { e = h.ell_e[st.eoff + (size_t)row * st.n + i]; }
-                            void pair(uint32_t row, float4& q, PairIds& id) const {
+                            void pair(uint32_t row, const double*, const PosFn& pos, float4& q, const double* (&pt)[4],
+                                      float& qoo) const {
                                 q = h.ell_bq[st.boff + (size_t)row * st.n + i];
-                                id = h.ell_bi[st.boff + (size_t)row * st.n + i];
+                                const PatchPair e = h.ell_bi[st.boff + (size_t)row * st.n + i];
+                                pt[0] = pos(e.i01 & 0xffffu);
+                                pt[1] = pos(e.i01 >> 16);
+                                pt[2] = pos(e.i23 & 0xffffu);
+                                pt[3] = pos(e.i23 >> 16);
+                                std::memcpy(&qoo, &e.qoo, 4);
                             }
                         } ell{h, st, i};
                         update(L.meta[ent.x], &lds[3 * (size_t)ent.y], [&](uint32_t r) { return &lds[3 * (size_t)r]; },
@@ -352,8 +372,7 @@ static bool run_case(const char* name, const Cloth& cl, std::vector<std::pair<ui
         bool ok = true;
         for (uint32_t g = 0; g < h.group_lds.size() && ok; g++)
             for (uint32_t B : {256u, 512u, 1024u})
-                ok = ok && check_launch<4>(h, g, P, lay.V, B) && check_launch<8>(h, g, P, lay.V, B) &&
-                     check_launch<16>(h, g, P, lay.V, B);
+                ok = ok && check_launch(h, g, P, lay.V, B);
         ok = ok && emulate(lay, h, 3);
         std::printf("%s %s V=%u C=%u P=%u F=%u launches/sweep=%u max_region=%u updates/sweep=%llu (%.2fx)\n",
                     ok ? "ok  " : "FAIL", name, lay.V, lay.C, P, h.stats.colours_per_launch,
@@ -392,7 +411,7 @@ int main() {
             }
             bool ok = true;
             for (uint32_t g = 0; g < h.group_lds.size() && ok; g++)
-                ok = check_launch<8>(h, g, P, lay.V, 1024) && check_launch<4>(h, g, P, lay.V, 512);
+                ok = check_launch(h, g, P, lay.V, 1024) && check_launch(h, g, P, lay.V, 512);
             std::printf("%s grid256 P=%u F=%u launches/sweep=%u max_region=%u updates/sweep=%llu (%.2fx) regions "
                         "%.2fx\n",
                         ok ? "ok  " : "FAIL", P, h.stats.colours_per_launch, h.stats.launches_per_sweep,

@@ -4,7 +4,8 @@ recomputed in LDS) against the graph strategy and the oracle.
 The patch launches read one position buffer and write the other, and every
 update is the graph strategy's arithmetic on the same inputs, so positions and
 velocities are expected to be bit-identical for every geometry (patches,
-colours per launch, lanes, threads).  The layout itself -- bounds of every
+colours per launch, threads; patch updates always run on 8 lanes, the graph
+strategy compared against runs on 8, 4 or 16).  The layout itself -- bounds of every
 index a launch computes, and Gauss-Seidel order under emulation -- is checked
 on the CPU by tests/cpp/test_patch_layout.hip.
 """
@@ -23,13 +24,15 @@ def _print_case(name):
 
 
 def _pair(cloth, params, patches=0, colours=0, threads=1024, lanes=8):
+    """A graph-strategy context with `lanes` lanes per vertex and a patch one."""
     a = GpuVBD(cloth, params)
     a.set_strategy(_lib.LATS_STRATEGY_GRAPH)
-    a.set_lanes_per_vertex(8)
+    a.set_lanes_per_vertex(lanes)
     b = GpuVBD(cloth, params)
-    b.set_lanes_per_vertex(lanes)
+    b.set_lanes_per_vertex(lanes)   # (patch updates run on 8 lanes regardless)
     b.set_patches(patches, colours, threads)
     b.set_strategy(_lib.LATS_STRATEGY_PATCH)
+    assert b.patch_info()["lanes"] == 8
     return a, b
 
 
