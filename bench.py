@@ -309,9 +309,13 @@ def main():
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(cloth, params, wind)
     if not args.no_warm_start:
+        patch = None
+        if strategy == _lib.LATS_STRATEGY_PATCH:
+            pi = g.patch_info()
+            patch = (pi["patches"], pi["colours_per_launch"], pi["threads"])
         out["warm_start"] = bench_warm_start(args, cloth, params, wind, plain_ms=gpu_ms / K,
                                              lanes=g.lanes_per_vertex(), regions=g.regions(),
-                                             colour_block=g.colour_block())
+                                             colour_block=g.colour_block(), patch=patch)
     if others is not None:
         out["other_workloads"] = others
     print(json.dumps(out), flush=True)
@@ -384,7 +388,7 @@ def bench_mlp(args):
                       "train_step_ms": round(step_ms, 4), "iters": args.steps}), flush=True)
 
 
-def bench_warm_start(args, cloth, params, wind, plain_ms, lanes=8, regions=1, colour_block=256):
+def bench_warm_start(args, cloth, params, wind, plain_ms, lanes=8, regions=1, colour_block=256, patch=None):
     """Learned warm start on the same cloth and wind, against the plain frame
     graph, per frame, measured two ways:
 
@@ -396,7 +400,11 @@ def bench_warm_start(args, cloth, params, wind, plain_ms, lanes=8, regions=1, co
     deployable: the target-free stopping rule (energy decrease per sweep <=
       tol * |E|) with tol and the check schedule calibrated on four training
       scenarios; the plain energy is computed only to report how many frames
-      reach it."""
+      reach it.
+
+    The warm-started frames run the plain frame's configuration: its lanes,
+    bands and block size, and with patch = (patches, colours per launch,
+    threads) its patch strategy."""
     import numpy as np
     from libatssim_amd import configs
     from libatssim_amd.warm_start import WarmStartMLP, calibrate_stop_schedule, evaluate, evaluate_rule, train_recipe
@@ -415,8 +423,16 @@ def bench_warm_start(args, cloth, params, wind, plain_ms, lanes=8, regions=1, co
                  record_every=args.warm_record_every)
     cal_frames = min(30, args.warm_frames)
     cal_scen = min(4, args.warm_scenarios)
+    from libatssim_amd import _lib
+
+    def setup(g):
+        if patch is not None:
+            g.set_patches(*patch)
+            g.set_strategy(_lib.LATS_STRATEGY_PATCH)
+
     cal = calibrate_stop_schedule(m, [configs.warm_c3_scenario(k, n=256, frames=cal_frames) for k in range(cal_scen)],
-                                  params, cal_frames, lanes=lanes, regions=regions, colour_block=colour_block)
+                                  params, cal_frames, lanes=lanes, regions=regions, colour_block=colour_block,
+                                  setup=setup)
     tol, fc, ce = cal["tol"], cal["first_check"], cal["check_every"]
     train_s = time.perf_counter() - t0
     full_wind = configs.grid_c1(frames=60)[3]
@@ -434,11 +450,11 @@ def bench_warm_start(args, cloth, params, wind, plain_ms, lanes=8, regions=1, co
 
     # the warm-started frames run the plain frame's colour-pass configuration
     oracle = summary(evaluate(m, cloth, params, full_wind, frames, lanes=lanes, regions=regions,
-                              colour_block=colour_block))
+                              colour_block=colour_block, setup=setup))
     oracle["stop"] = ("first check at or below the plain 100-iteration energy of the same frame start, computed "
                       "beforehand on a second context (not known to a deployed solver)")
     deploy = summary(evaluate_rule(m, cloth, params, full_wind, frames, tol, first_check=fc, check_every=ce,
-                                   lanes=lanes, regions=regions, colour_block=colour_block))
+                                   lanes=lanes, regions=regions, colour_block=colour_block, setup=setup))
     deploy["stop"] = "energy decrease per sweep <= tol * |E| since the previous check (no reference solve)"
     deploy["tol"] = tol
     deploy["first_check"] = fc
@@ -451,6 +467,7 @@ def bench_warm_start(args, cloth, params, wind, plain_ms, lanes=8, regions=1, co
     fwd_ms, step_ms = m.time_kernels(cloth.num_verts, 20)   # after the evaluation: timing steps train
     return {
         "ms_per_frame_plain": round(plain_ms, 4),
+        "strategy": "patch" if patch is not None else "graph",
         "frames": frames,
         "deployable": deploy,
         "oracle_stopped": oracle,

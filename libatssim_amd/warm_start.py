@@ -361,6 +361,9 @@ def evaluate(mlp, cloth, params, wind, frames, first_check=FIRST_CHECK, check_ev
     if setup:
         setup(run)
         setup(ref)
+    for g in (run, ref):
+        if g.strategy()[0] == _lib.LATS_STRATEGY_PATCH:
+            g.patch_info()   # the patch layout is built here, not inside the first timed frame
     mlp.bind(run)
     out = []
     for f in range(frames):

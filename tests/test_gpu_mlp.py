@@ -287,6 +287,16 @@ def test_trained_warm_start_needs_fewer_sweeps(gpu_available):
     assert all(r["e_final"] <= r["target"] for r in learned)
     assert all(r["e_final"] <= r["target"] for r in plain)
     assert sl.mean() <= 0.6 * sp.mean(), (sl.mean(), sp.mean())
+    # the same frames on the patch strategy: identical sweeps give identical
+    # energies, hence the same decisions and sweep counts, frame for frame
+
+    def patch(g):
+        g.set_patches(16, 4, 512)
+        g.set_strategy(_lib.LATS_STRATEGY_PATCH)
+
+    learned_p = evaluate(m, cloth, params, wind, 30, use_model=True, setup=patch)
+    for a, b in zip(learned, learned_p):
+        assert (a["sweeps"], a["kept"], a["e_final"]) == (b["sweeps"], b["kept"], b["e_final"])
 
 
 def test_guided_recording_is_deterministic(gpu_available):
