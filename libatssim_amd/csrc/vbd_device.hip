@@ -1734,10 +1734,14 @@ int lats_vbd_autotune(lats_vbd* c, uint32_t frames, int* chosen, uint32_t* chose
     if (c->V >= 64 && c->V <= (1u << 18)) {
         auto round8 = [](uint32_t n) { return std::max(8u, std::min(4096u, n / 8u * 8u)); };
         const uint32_t pa = round8(c->V / 256u), pb = round8(c->V / 64u);
+        // (and one patch per CU, 256, for cloths too small for that at 64
+        // vertices per patch: more halo, every CU busy)
+        const uint32_t pc = round8(std::min(256u, c->V / 16u));
         for (const Cand& k : {Cand{LATS_STRATEGY_PATCH, pa, 8, 0, 1024}, Cand{LATS_STRATEGY_PATCH, pa, 8, 0, 512},
                               Cand{LATS_STRATEGY_PATCH, pb, 8, 0, 256}, Cand{LATS_STRATEGY_PATCH, pb, 8, 0, 512},
                               Cand{LATS_STRATEGY_PATCH, pa, 8, 4, 1024}, Cand{LATS_STRATEGY_PATCH, pa, 8, 4, 512}})
             if (k.blocks <= c->V) cands.push_back(k);
+        if (pc != pa && pc != pb && pc <= c->V) cands.push_back(Cand{LATS_STRATEGY_PATCH, pc, 8, 0, 512});
     }
     float best = 3.4e38f;
     Cand win = cands[0];
