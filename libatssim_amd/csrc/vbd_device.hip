@@ -78,8 +78,15 @@ __global__ __launch_bounds__(B) void k_vbd_colour(const uint2* ell_e, const floa
         if (first >= hi) return;                  // the whole wave is past the band
         const uint32_t i = lo + j * (B / G) + threadIdx.x / G, lane = threadIdx.x % G;
         const bool valid = i < hi;
-        vbd_vertex_group<G>(a, start + (valid ? i : lo), valid ? i : lo, count, eoff, boff, ew, bw, ext, lane,
-                            valid);
+        if constexpr (G == 8) {
+            // (8 lanes: the sums formed one value per lane through LDS)
+            __shared__ float4 scr[B / 8 * kGroupStagedWords / 4];
+            vbd_vertex_group_staged8(a, start + (valid ? i : lo), valid ? i : lo, count, eoff, boff, ew, bw, ext,
+                                     lane, valid, reinterpret_cast<float*>(scr) + threadIdx.x / 8 * kGroupStagedWords);
+        } else {
+            vbd_vertex_group<G>(a, start + (valid ? i : lo), valid ? i : lo, count, eoff, boff, ew, bw, ext, lane,
+                                valid);
+        }
     }
 }
 

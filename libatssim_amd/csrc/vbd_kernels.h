@@ -552,4 +552,167 @@ __device__ __forceinline__ void group_finish(const VbdArgs& a, const GroupLoads<
     vertex_solve(a, H, f, xk, s, sx, mem);
 }
 
+// Words of LDS one group of 8 lanes uses in group_finish_staged8 (the
+// colour pass's 8-lane update; the patch strategy has its own): the
+// spring terms of a round (8 lanes x 8 values) and the bending terms of the
+// two prefetched rounds (2 x 8 lanes x 4 values), plus 8 words that keep
+// the groups of a wave on distinct banks.
+constexpr uint32_t kGroupStagedWords = 136;
+// Orders one wave's LDS writes and reads of the staged terms: the lanes of
+// a wave run in lockstep and a wave's LDS accesses complete in issue order,
+// so a wavefront-scope fence (which keeps the compiler from moving the
+// accesses across it) is all the exchange needs.
+__device__ __forceinline__ void group_staged_exchange() {
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// group_finish for 8 lanes per vertex with the sums formed one value per
+// lane: every lane writes its terms to the group's LDS words `scr`, then
+// lane v adds value v of lanes 0, 1, ..., 7 (round after round) in that
+// order -- the same additions, in the same order, as lane 0 forms in
+// group_finish, so every sum is bit-identical -- and lane 0 collects the
+// finished sums with DPP moves.  Eight chains run side by side instead of
+// one lane adding every value (8 adds per value instead of 8 per value and
+// lane-0 only).  Spring values: lane v < 6 sums S's entry v (xx, xy, xz, yy,
+// yz, zz), lanes 6 / 7 fs.x / fs.y, and fs.z is summed in lane 0 with DPP
+// adds.  Bending values: lanes 0, 3, 5 sum hb (the diagonal they hold), 6 /
+// 7 fb.x / fb.y, lane 1 fb.z.  scr: kGroupStagedWords words, 16-byte aligned,
+// used by this group only.
+template <class Mem>
+__device__ __forceinline__ void group_finish_staged8(const VbdArgs& a, const GroupLoads<8>& L, F3 xk, uint32_t s,
+                                                     uint32_t sx, uint32_t i, uint32_t n, uint32_t eoff,
+                                                     uint32_t boff, F3 ext, uint32_t lane, bool valid, Mem mem,
+                                                     float* scr) {
+    constexpr uint32_t G = 8, PE = GroupLoads<8>::PE, PB = GroupLoads<8>::PB;
+    static_assert(PE == 1 && PB == 2, "one spring round and two pair rounds are staged");
+    const uint2* pe = L.pe;
+    const float4* pq = L.pq;
+    const PairIds* pi = L.pi;
+    const F3 x0 = L.x0, v0 = L.v0;
+    const float m = L.m;
+    const uint32_t meta = valid ? L.meta : 0u;
+    const uint32_t edeg = meta & 0xffu, bdeg = (meta >> 8) & 0xffu;
+    float4* const s4 = reinterpret_cast<float4*>(scr);
+
+    const F3 pxo = mem.ldx(a.x, pe[0].x);
+    F3 pg[PB][3];
+#pragma unroll
+    for (uint32_t k = 0; k < PB; k++) {
+        pg[k][0] = mem.ldx(a.x, pi[k].a & 0x3fffffffu);
+        pg[k][1] = mem.ldx(a.x, pi[k].b);
+        pg[k][2] = mem.ldx(a.x, pi[k].c);
+    }
+    __builtin_amdgcn_sched_barrier(0);   // every gather issued before the arithmetic
+
+    Sym3 Hin;
+    F3 fin;
+    inertia_term(xk, x0, v0, m, (meta >> 16) & 1u, a.g, ext, a.h, a.h2inv, Hin, fin);
+
+    // the terms of the prefetched rounds (+0 past the degree), staged
+    float fsz = 0.f;
+    {
+        Sym3 t;
+        F3 tf;
+        spring_term_masked(a, xk, lane < edeg, pe[0], pxo, t, tf);
+        s4[2 * lane] = float4{t.xx, t.xy, t.xz, t.yy};
+        s4[2 * lane + 1] = float4{t.yz, t.zz, tf.x, tf.y};
+#pragma unroll
+        for (int q = 0; q < (int)G; q++) fsz = fsz + lane_plus(tf.z, q);
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < PB; k++) {
+        float th = 0.f;
+        F3 tf = F3{0.f, 0.f, 0.f};
+        if (k * G + lane < bdeg) {
+            const uint32_t o = pi[k].a >> 30;
+            F3 p0, p1, p2, p3;
+            pair_points(o, xk, pg[k][0], pg[k][1], pg[k][2], p0, p1, p2, p3);
+            bending_parts(pq[k], o, p0, p1, p2, p3, a.kb, th, tf);
+        }
+        s4[16 + 8 * k + lane] = float4{th, tf.x, tf.y, tf.z};
+    }
+    group_staged_exchange();
+    // spring chain of value `lane` over the prefetched round
+    float acc = 0.f;
+#pragma unroll
+    for (uint32_t q = 0; q < G; q++) acc = acc + scr[8 * q + lane];
+    // spring rounds past the prefetched one (degrees above 8): the same
+    // staging, one round at a time, after the reads above
+    for (uint32_t r = PE * G; r < edeg; r += G) {
+        const uint32_t j = r + lane;
+        const bool on = j < edeg;
+        const uint2 e = on ? a.ell_e[eoff + j * n + i] : uint2{0u, 0u};
+        Sym3 t;
+        F3 tf;
+        spring_term_of(a, xk, on, e, on ? mem.ldx(a.x, e.x) : F3{0.f, 0.f, 0.f}, t, tf);
+        group_staged_exchange();
+        s4[2 * lane] = float4{t.xx, t.xy, t.xz, t.yy};
+        s4[2 * lane + 1] = float4{t.yz, t.zz, tf.x, tf.y};
+#pragma unroll
+        for (int q = 0; q < (int)G; q++) fsz = fsz + lane_plus(tf.z, q);
+        group_staged_exchange();
+#pragma unroll
+        for (uint32_t q = 0; q < G; q++) acc = acc + scr[8 * q + lane];
+    }
+    // bending chains over the prefetched rounds
+    const uint32_t bv = lane == 6u ? 1u : (lane == 7u ? 2u : (lane == 1u ? 3u : 0u));
+    float accb = 0.f;
+#pragma unroll
+    for (uint32_t k = 0; k < PB; k++)
+#pragma unroll
+        for (uint32_t q = 0; q < G; q++) accb = accb + scr[64 + 32 * k + 4 * q + bv];
+    // pair rounds past the prefetched ones (degrees above 16)
+    for (uint32_t r = PB * G; r < bdeg; r += G) {
+        const uint32_t j = r + lane;
+        float th = 0.f;
+        F3 tf = F3{0.f, 0.f, 0.f};
+        if (j < bdeg) {
+            const uint32_t k = boff + j * n + i;
+            const float4 q = a.ell_bq[k];
+            const PairIds id = a.ell_bi[k];
+            const uint32_t o = id.a >> 30;
+            F3 p0, p1, p2, p3;
+            pair_points(o, xk, mem.ldx(a.x, id.a & 0x3fffffffu), mem.ldx(a.x, id.b), mem.ldx(a.x, id.c), p0, p1, p2,
+                        p3);
+            bending_parts(q, o, p0, p1, p2, p3, a.kb, th, tf);
+        }
+        group_staged_exchange();
+        s4[16 + lane] = float4{th, tf.x, tf.y, tf.z};
+        group_staged_exchange();
+#pragma unroll
+        for (uint32_t q = 0; q < G; q++) accb = accb + scr[64 + 4 * q + bv];
+    }
+    group_staged_exchange();   // this group's reads before its next writes
+
+    // lane v's value: H = inertia + S (then + hb on the diagonal), f = fin +
+    // fs (then + fb)
+    const bool diag = lane == 0u || lane == 3u || lane == 5u;
+    const float base = lane < 6u ? (diag ? Hin.xx : 0.f) : (lane == 6u ? fin.x : fin.y);
+    float val = base + acc;
+    if (diag || lane >= 6u) val = val + accb;
+    float fz = fin.z + fsz;
+    fz = fz + lane_plus(accb, 1);
+    const Sym3 H = Sym3{val, lane_plus(val, 1), lane_plus(val, 2), lane_plus(val, 3), lane_plus(val, 4),
+                        lane_plus(val, 5)};
+    const F3 f = F3{lane_plus(val, 6), lane_plus(val, 7), fz};
+    if (!valid || lane != 0) return;
+    vertex_solve(a, H, f, xk, s, sx, mem);
+}
+
+// vbd_vertex_group<8> with the sums formed through the group's LDS words
+// `scr` (kGroupStagedWords of them): bit-identical results.
+template <class Mem = PlainMem>
+__device__ __forceinline__ void vbd_vertex_group_staged8(const VbdArgs& a, uint32_t s, uint32_t i, uint32_t n,
+                                                         uint32_t eoff, uint32_t boff, uint32_t ew, uint32_t bw, F3 ext,
+                                                         uint32_t lane, bool valid, float* scr, Mem mem = Mem{}) {
+    GroupLoads<8> L;
+    group_load_ell<8>(a, i, n, eoff, boff, ew, bw, lane, L);
+    __builtin_amdgcn_sched_barrier(0);
+    const F3 xk = mem.ldx(a.x, s);
+    group_load_own<8>(a, s, L, mem);
+    __builtin_amdgcn_sched_barrier(0);
+    group_finish_staged8(a, L, xk, s, s, i, n, eoff, boff, ext, lane, valid, mem, scr);
+}
+
 }  // namespace lats
