@@ -1,7 +1,8 @@
 """Parity over the configured runs themselves, not a slice of them:
 
 * the 256 x 256 windy cloth: all 60 frames of 100 iterations, positions and
-  velocities bit-identical to the oracle after every frame;
+  velocities bit-identical to the oracle after every frame, with one kernel
+  per colour and with the patch launches the benchmark measures;
 * the 512-cloth batch: every cloth bit-identical to its own oracle run after
   two frames of 100 iterations (per-cloth stiffnesses and pins);
 * the learned warm start on the 256 x 256 cloth: the MFMA forward on the
@@ -30,11 +31,17 @@ def _case(name):
     print(f"[case] {name}", flush=True)
 
 
-def test_c1_all_frames_bit_exact(gpu_available):
-    _case("grid_c1 256x256, all 60 windy frames vs oracle")
+@pytest.mark.parametrize("strategy", ["graph", "patch"])
+def test_c1_all_frames_bit_exact(gpu_available, strategy):
+    """patch: the geometry the autotuner picks for this cloth (256 patches,
+    4 colours per launch, 512 threads), the one the benchmark measures."""
+    _case(f"grid_c1 256x256, all 60 windy frames vs oracle, {strategy} strategy")
     cloth, params, frames, wind = configs.grid_c1(frames=60)
     assert frames == 60
     g = GpuVBD(cloth, params)
+    if strategy == "patch":
+        g.set_patches(256, 4, 512)
+        g.set_strategy(_lib.LATS_STRATEGY_PATCH)
     o = oracle_sim(cloth, params, threads=O.max_threads())
     for fr in range(frames):
         g.set_external_force(wind[fr])
